@@ -1,0 +1,146 @@
+/*
+ * onionpir_gpu.h -- C ABI of the MI355X (gfx950) OnionPIRv2 server library.
+ *
+ * The host-side PirServer (onionpirv2_amd/host/server.h) calls only these
+ * entry points; any other host language binds them the same way (ctypes,
+ * cgo, JNI). Plain pointers and sizes only. Every call returns an int status
+ * (OPIR_OK = 0, negative on error); opir_last_error() gives the message.
+ * The status maps onto the reference's exceptions:
+ *   OPIR_EINVAL -> std::invalid_argument, OPIR_ERANGE -> std::out_of_range,
+ *   others      -> std::runtime_error.
+ *
+ * Data formats at this boundary are the reference's own:
+ *   - ciphertext: c0 then c1, each N uint64 values mod Q (limb-major,
+ *     K = 1 logical limb for CONFIG_N2048_K1_COMP), src/includes/rlwe.h:17-31;
+ *   - NTT-form values use the reference's NTT evaluation order;
+ *   - RGSW rows: 2*l rows of [c0 N][c1 N], src/includes/gsw.h:8-17;
+ *   - BV keys: [h keys][L_KS rows][a, b][N], galois k = N/2^i + 1,
+ *     src/includes/bv_keyswitch.h:45-66;
+ *   - response: c0 then c1, N uint64 values mod small_q.
+ * Host buffers are owned by the caller; device buffers by the context.
+ * A context is not thread-safe (the reference's PirServer is not either,
+ * src/includes/gsw.h:25-30); one context serves one host thread.
+ */
+#ifndef ONIONPIR_GPU_H
+#define ONIONPIR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OPIR_OK 0
+#define OPIR_EINVAL (-1)
+#define OPIR_ERANGE (-2)
+#define OPIR_ERUNTIME (-3)
+#define OPIR_EHIP (-4)
+#define OPIR_ENOMEM (-5)
+
+typedef struct opir_ctx opir_ctx;
+typedef struct opir_db opir_db;
+
+/* Scheme description: the reference's compile-time DBConsts plus the
+ * PirParams-derived values (src/includes/database_constants.h:77-94,
+ * src/pir.cpp:14-98), as a runtime value. */
+typedef struct opir_scheme {
+  uint32_t poly_degree;     /* N (2048) */
+  uint32_t composite;       /* 1: Q = q1*q2 composite single limb */
+  uint64_t q1, q2;          /* RNS primes of Q */
+  uint64_t ntt_root;        /* primitive 2N-th root mod Q (w_crt) */
+  uint64_t plain_mod;       /* t */
+  uint64_t small_q;         /* response modulus q' */
+  uint32_t l_ep, base_log2;         /* data selector gadget */
+  uint32_t l_key, base_log2_key;    /* RGSW(s) completion gadget */
+  uint32_t l_ks, bv_base_log2;      /* BV keyswitch gadget */
+} opir_scheme;
+
+/* Replaces the scheme set-up of PirParams::PirParams / init_composite_rns
+ * (src/pir.cpp:14-98). device = HIP device ordinal. */
+int opir_ctx_create(const opir_scheme *scheme, int device, opir_ctx **out);
+void opir_ctx_destroy(opir_ctx *ctx);
+/* Message of the last failed call on ctx (or of the last failed
+ * opir_ctx_create when ctx is NULL). Valid until the next call. */
+const char *opir_last_error(const opir_ctx *ctx);
+/* Use an existing HIP stream (hipStream_t) for all work of this context;
+ * NULL reverts to the context's own stream. */
+int opir_ctx_set_stream(opir_ctx *ctx, void *hip_stream);
+/* Blocks until all work queued by the context has finished. */
+int opir_ctx_synchronize(opir_ctx *ctx);
+
+/* Replaces PirServer::PirServer(const PirParams&) database allocation
+ * (src/server.cpp:29-46) for a layout PirParams::with_layout produced
+ * (src/pir.cpp:100-134): N0 = fst_dim_sz, Nrest = other_dim_sz. */
+int opir_db_create(opir_ctx *ctx, size_t fst_dim_sz, size_t other_dim_sz, size_t num_dims,
+                   size_t expansion_height, opir_db **out);
+void opir_db_destroy(opir_db *db);
+/* Replaces PirServer::load_data's NTT + transpose (src/server.cpp:118-171):
+ * `count` coefficient-form plaintexts (N values < t each, pt-major),
+ * logical indices [first_pt, first_pt + count). Validation of the values
+ * (< t) is the caller's; this call rejects ranges outside the layout. */
+int opir_db_load_plain(opir_db *db, const uint16_t *coeffs, size_t first_pt, size_t count);
+/* Loads an already preprocessed database in the reference's composite
+ * layout: lo[coeff * num_pt + pt] = NTT_Q(pt)[coeff] mod q1, hi the same
+ * mod q2 (src/server.cpp:162-170). */
+int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi);
+
+/* Replaces PirServer::set_client_bv_galois_key + set_client_gsw_key /
+ * set_client_session_keys (src/server.cpp:857-875). bv: [h][l_ks][2: a, b][N]
+ * mod Q in NTT form, h >= expansion height of every db used with it;
+ * gsw: [2*l_key][2][N] mod Q in NTT form. Re-setting an id replaces the keys. */
+int opir_client_set_keys(opir_ctx *ctx, uint64_t client_id, const uint64_t *bv, size_t h, const uint64_t *gsw);
+int opir_client_remove(opir_ctx *ctx, uint64_t client_id);
+
+/* Replaces PirServer::make_query (src/server.cpp:895-947): query is the
+ * coefficient-form packed BFV query [2][N] mod Q; resp receives the
+ * mod-switched response [2][N] mod small_q. Unknown client -> OPIR_ERANGE. */
+int opir_answer(opir_db *db, uint64_t client_id, const uint64_t *query, uint64_t *resp);
+/* Batched form: B queries (distinct or repeated client ids) against one
+ * database in one pass; queries [B][2][N], resps [B][2][N]. */
+int opir_answer_batch(opir_db *db, size_t B, const uint64_t *client_ids, const uint64_t *queries,
+                      uint64_t *resps);
+/* Same as opir_answer_batch with queries/resps in device memory; work is
+ * queued on the context stream and this call returns without waiting. */
+int opir_answer_batch_device(opir_db *db, size_t B, const uint64_t *client_ids, const uint64_t *d_queries,
+                             uint64_t *d_resps);
+
+/* Per-stage device time of the last answer call under the reference's stage
+ * names (src/includes/logging.h:122-134): [0] Expand, [1] Convert,
+ * [2] First dim, [3] Other dim, [4] Modulus switching, [5] scan kernel only.
+ * Requires opir_ctx_set_profiling(ctx, 1) before the answer call. */
+int opir_ctx_set_profiling(opir_ctx *ctx, int on);
+int opir_last_stage_ms(opir_ctx *ctx, float *out6);
+
+/* ---- Stage entry points (host buffers), one per reference stage ---- */
+
+/* utils::ntt_fwd / ntt_inv over Q (src/utils.cpp:84-90), in place on
+ * `count` polynomials of N values. */
+int opir_ntt_fwd(opir_ctx *ctx, uint64_t *data, size_t count);
+int opir_ntt_inv(opir_ctx *ctx, uint64_t *data, size_t count);
+/* PirServer::fast_expand_qry (src/server.cpp:731-816): out receives the
+ * u = N0 + l_ep*(dims-1) leaves, [u][2][N] coefficient form. */
+int opir_expand(opir_db *db, uint64_t client_id, const uint64_t *query, uint64_t *out);
+/* PirServer::complete_selectors (src/server.cpp:823-855): expanded holds
+ * the u leaves; sel receives [dims-1][2*l_ep][2][N] NTT form. */
+int opir_complete(opir_db *db, uint64_t client_id, const uint64_t *expanded, uint64_t *sel);
+/* PirServer::evaluate_first_dim (src/server.cpp:313-361, 529-597): query
+ * [N0][2][N] coefficient form; out [Nrest][2][N] coefficient form. */
+int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out);
+/* PirServer::evaluate_other_dim (src/server.cpp:599-722): mid [Nrest][2][N]
+ * coefficient form, sel [dims-1][2*l_ep][2][N] NTT form; out [2][N]
+ * coefficient form. */
+int opir_other_dim(opir_db *db, const uint64_t *mid, const uint64_t *sel, uint64_t *out);
+/* GSWEval::external_product (src/gsw.cpp:119-182): gsw [2l][2][N] NTT form,
+ * ct [2][N] coefficient form -> out [2][N] NTT form. */
+int opir_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t base_log2, const uint64_t *ct,
+                  uint64_t *out);
+/* PirServer::mod_switch_inplace (src/server.cpp:1025-1043): ct [2][N]
+ * coefficient form mod Q -> out [2][N] mod small_q. */
+int opir_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ONIONPIR_GPU_H */

@@ -1,0 +1,60 @@
+"""Build the native parts of onionpirv2_amd in-tree.
+
+  libonionpir_gpu.so   HIP kernels + C ABI (include/onionpir_gpu.h), gfx950
+  _opir_host*.so       C++ host mirror of the reference PirParams/PirClient/
+                       PirServer API with its pybind11 binding
+
+Run `python -m onionpirv2_amd.build` (or __graft_entry__.build()).
+"""
+import os
+import subprocess
+import sys
+import sysconfig
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIBDIR = os.path.join(PKG, "lib")
+GPU_LIB = os.path.join(LIBDIR, "libonionpir_gpu.so")
+HOST_EXT = os.path.join(LIBDIR, "_opir_host" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed FP32 ops stay off for every HIP translation unit
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+GPU_SRC = ["csrc/opir_kernels.hip", "csrc/opir_capi.hip"]
+HOST_SRC = ["host/params.cpp", "host/ring.cpp", "host/client.cpp", "host/server.cpp", "host/bindings.cpp"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd, cwd=PKG)
+
+
+def build_gpu():
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = GPU_LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + GPU_SRC + ["-o", tmp])
+    os.replace(tmp, GPU_LIB)
+
+
+def build_host():
+    os.makedirs(LIBDIR, exist_ok=True)
+    import pybind11
+    inc = ["-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"], "-I" + os.path.join(ROOT, "include")]
+    tmp = HOST_EXT + ".tmp"
+    _run(["g++", "-O3", "-march=x86-64-v3", "-std=c++20", "-fPIC", "-shared", "-Wall", "-Wno-unused-parameter"]
+         + inc + HOST_SRC + ["-L" + LIBDIR, "-lonionpir_gpu", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, HOST_EXT)
+
+
+def build_all():
+    build_gpu()
+    build_host()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    {"gpu": build_gpu, "host": build_host, "all": build_all}[what]()

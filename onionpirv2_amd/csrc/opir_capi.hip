@@ -1,0 +1,815 @@
+// C ABI of the OnionPIRv2 MI355X server library (include/onionpir_gpu.h).
+// Orchestrates the kernels of opir_kernels.hip along PirServer::make_query
+// (src/server.cpp:895-947).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/onionpir_gpu.h"
+#include "opir_kernels.h"
+
+using namespace opir;
+typedef unsigned __int128 u128;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct ClientKeys {
+  u32 *bv = nullptr;   // [h][l_ks][2 (b,a)][2 limbs][N]
+  size_t h = 0;
+  u32 *gsw = nullptr;  // [2 l_key][2][2 limbs][N]
+};
+
+// Small ring of pinned staging slots for per-batch key-pointer tables.
+constexpr int kRing = 8;
+
+}  // namespace
+
+struct opir_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  opir_scheme sch{};
+  SchemeConst S{};
+  GadgetCfg G{};
+  u32 *d_tables = nullptr;
+  std::unordered_map<uint64_t, ClientKeys> clients;
+  std::string err;
+  bool profiling = false;
+  hipEvent_t ev[8] = {};
+  // pointer-table ring
+  const u32 **pin_ptrs[kRing] = {};
+  const u32 **dev_ptrs[kRing] = {};
+  hipEvent_t ring_ev[kRing] = {};
+  size_t ring_cap = 0;
+  int ring_pos = 0;
+  // host staging for synchronous calls
+  void *pin = nullptr;
+  size_t pin_bytes = 0;
+};
+
+struct opir_db {
+  opir_ctx *ctx = nullptr;
+  size_t N0 = 0, N0p = 0, Nrest = 0, dims = 0, h = 0, num_pt = 0, u = 0;
+  u32 *d_db = nullptr;
+  // workspace for up to ws_B queries
+  size_t ws_B = 0;
+  u32 *exp_a = nullptr, *exp_b = nullptr, *qntt = nullptr, *sel = nullptr, *mid_a = nullptr, *mid_b = nullptr;
+  u64 *q_u64 = nullptr, *r_u64 = nullptr;
+  size_t exp_cts = 0;
+};
+
+namespace {
+
+int set_err(opir_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                         \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return set_err((ctx), OPIR_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCH_CHECK(ctx)                                                                          \
+  do {                                                                                             \
+    hipError_t e_ = hipGetLastError();                                                             \
+    if (e_ != hipSuccess) return set_err((ctx), OPIR_EHIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+u64 mulmod_h(u64 a, u64 b, u64 m) { return (u64)((u128)a * b % m); }
+u64 powmod_h(u64 b, u64 e, u64 m) {
+  u64 r = 1 % m;
+  b %= m;
+  while (e) {
+    if (e & 1) r = mulmod_h(r, b, m);
+    b = mulmod_h(b, b, m);
+    e >>= 1;
+  }
+  return r;
+}
+bool inv_h(u64 a, u64 m, u64 &out) {
+  int64_t s0 = 1, s1 = 0, r0 = (int64_t)(a % m), r1 = (int64_t)m;
+  if (r0 == 0) return false;
+  while (r1) {
+    int64_t q = r0 / r1, t = r1;
+    r1 = r0 - q * r1; r0 = t;
+    t = s1; s1 = s0 - q * s1; s0 = t;
+  }
+  if (r0 != 1) return false;
+  out = (u64)(s0 < 0 ? s0 + (int64_t)m : s0);
+  return true;
+}
+u32 shoup_h(u32 w, u32 q) { return (u32)(((u64)w << 32) / q); }
+size_t brv(size_t x, int logn) {
+  size_t y = 0;
+  for (int i = 0; i < logn; i++) { y = (y << 1) | (x & 1); x >>= 1; }
+  return y;
+}
+
+int ensure_pin(opir_ctx *ctx, size_t bytes) {
+  if (ctx->pin_bytes >= bytes) return OPIR_OK;
+  if (ctx->pin) (void)hipHostFree(ctx->pin);
+  ctx->pin = nullptr;
+  ctx->pin_bytes = 0;
+  HIP_TRY(ctx, hipHostMalloc(&ctx->pin, bytes, hipHostMallocDefault));
+  ctx->pin_bytes = bytes;
+  return OPIR_OK;
+}
+
+// Upload per-query key pointers for a batch through the pinned ring.
+int upload_key_ptrs(opir_ctx *ctx, const uint64_t *ids, size_t B, BatchPtrs &P) {
+  if (ctx->ring_cap < B) {
+    for (int i = 0; i < kRing; i++) {
+      if (ctx->ring_ev[i]) (void)hipEventSynchronize(ctx->ring_ev[i]);
+      if (ctx->pin_ptrs[i]) (void)hipHostFree(ctx->pin_ptrs[i]);
+      if (ctx->dev_ptrs[i]) (void)hipFree(ctx->dev_ptrs[i]);
+      ctx->pin_ptrs[i] = nullptr;
+      ctx->dev_ptrs[i] = nullptr;
+    }
+    size_t cap = 64;
+    while (cap < B) cap *= 2;
+    for (int i = 0; i < kRing; i++) {
+      HIP_TRY(ctx, hipHostMalloc((void **)&ctx->pin_ptrs[i], 2 * cap * sizeof(u32 *), hipHostMallocDefault));
+      HIP_TRY(ctx, hipMalloc((void **)&ctx->dev_ptrs[i], 2 * cap * sizeof(u32 *)));
+      if (!ctx->ring_ev[i]) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ring_ev[i], hipEventDisableTiming));
+    }
+    ctx->ring_cap = cap;
+  }
+  const int slot = ctx->ring_pos;
+  ctx->ring_pos = (ctx->ring_pos + 1) % kRing;
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ring_ev[slot]));
+  const u32 **hp = ctx->pin_ptrs[slot];
+  for (size_t i = 0; i < B; i++) {
+    auto it = ctx->clients.find(ids[i]);
+    if (it == ctx->clients.end())
+      return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(ids[i]));
+    hp[i] = it->second.bv;
+    hp[B + i] = it->second.gsw;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->dev_ptrs[slot], hp, 2 * B * sizeof(u32 *), hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->ring_ev[slot], ctx->stream));
+  P.bv_keys = ctx->dev_ptrs[slot];
+  P.gsw_keys = ctx->dev_ptrs[slot] + B;
+  return OPIR_OK;
+}
+
+size_t nodes_at_level(size_t u, size_t h, size_t L) {
+  const size_t cap = size_t{1} << h;
+  const size_t n = (u * (size_t{1} << L) + cap - 1) / cap;
+  const size_t full = size_t{1} << L;
+  return n < full ? n : full;
+}
+
+int ensure_ws(opir_db *db, size_t B) {
+  opir_ctx *ctx = db->ctx;
+  if (db->ws_B >= B) return OPIR_OK;
+  u32 **bufs[] = {&db->exp_a, &db->exp_b, &db->qntt, &db->sel, &db->mid_a, &db->mid_b};
+  for (auto b : bufs) {
+    if (*b) (void)hipFree(*b);
+    *b = nullptr;
+  }
+  if (db->q_u64) (void)hipFree(db->q_u64);
+  if (db->r_u64) (void)hipFree(db->r_u64);
+  db->q_u64 = db->r_u64 = nullptr;
+  db->ws_B = 0;
+  const size_t ct = kCtWords * sizeof(u32);
+  const size_t nsel = db->dims - 1;
+  HIP_TRY(ctx, hipMalloc((void **)&db->exp_a, B * db->exp_cts * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->exp_b, B * db->exp_cts * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->qntt, B * db->N0 * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->sel, B * (nsel ? nsel : 1) * 2 * ctx->G.l_ep * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->mid_a, B * db->Nrest * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->mid_b, B * db->Nrest * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->q_u64, B * 2 * kN * sizeof(u64)));
+  HIP_TRY(ctx, hipMalloc((void **)&db->r_u64, B * 2 * kN * sizeof(u64)));
+  db->ws_B = B;
+  return OPIR_OK;
+}
+
+void rec(opir_ctx *ctx, int i) {
+  if (ctx->profiling) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+}
+
+// Expansion (server.cpp:731-816) for B queries whose root cts are already in
+// exp_a[q][0] (RNS coefficient form). Returns the buffer holding the leaves.
+u32 *run_expand(opir_db *db, size_t B, const BatchPtrs &P) {
+  opir_ctx *ctx = db->ctx;
+  const size_t stride = db->exp_cts * kCtWords;
+  u32 *in = db->exp_a, *out = db->exp_b;
+  for (size_t L = 0; L < db->h; L++) {
+    const size_t n = nodes_at_level(db->u, db->h, L);
+    launch_expand_level(in, out, stride, stride, (int)n, (int)B, (int)L, P, ctx->G, ctx->S, ctx->stream);
+    std::swap(in, out);
+  }
+  return in;
+}
+
+// MUX tree (server.cpp:599-649) over NTT-form candidates in mid_a.
+// Returns the buffer whose slot 0 (per query) holds the result.
+u32 *run_other_dim(opir_db *db, size_t B, const u32 *sel) {
+  opir_ctx *ctx = db->ctx;
+  const size_t stride = db->Nrest * kCtWords;
+  if (db->dims == 1) return db->mid_a;
+  const size_t hs = db->dims - 1;
+  const size_t sel_words = 2 * (size_t)ctx->G.l_ep * kCtWords;  // one selector
+  const size_t sel_stride = (db->dims - 1) * sel_words;
+  const size_t last = 2 * db->Nrest - (size_t{1} << hs);
+  const size_t offset = db->Nrest - last;
+  const size_t perfect = size_t{1} << (hs - 1);
+  u32 *in = db->mid_a, *out = db->mid_b;
+  launch_mux(in, out, stride, (int)perfect, (int)offset, 0, (int)B, sel, sel_stride, ctx->G, ctx->S, ctx->stream);
+  std::swap(in, out);
+  for (size_t a = 1; a < hs; a++) {
+    const size_t half = (size_t{1} << (hs - a)) >> 1;
+    launch_mux(in, out, stride, (int)half, -1, (int)half, (int)B, sel + a * sel_words, sel_stride, ctx->G, ctx->S,
+               ctx->stream);
+    std::swap(in, out);
+  }
+  return in;
+}
+
+int validate_mod(opir_ctx *ctx, const uint64_t *v, size_t n, uint64_t mod, const char *what) {
+  for (size_t i = 0; i < n; i++)
+    if (v[i] >= mod) return set_err(ctx, OPIR_EINVAL, std::string(what) + " value not reduced mod Q");
+  return OPIR_OK;
+}
+
+// Full answer pipeline for B queries already in db->q_u64 (device).
+int check_answerable(opir_db *db) {
+  // server.cpp:613-622: the selector tree must not be taller than the candidates.
+  if (db->dims > 1 && db->Nrest < (size_t{1} << (db->dims - 2)))
+    return set_err(db->ctx, OPIR_EINVAL,
+                   "evaluate_other_dim: layout has fewer candidates than its selector tree; query-shape views "
+                   "cannot answer make_query");
+  return OPIR_OK;
+}
+
+int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u64 *d_r) {
+  opir_ctx *ctx = db->ctx;
+  int rc = check_answerable(db);
+  if (rc) return rc;
+  rc = ensure_ws(db, B);
+  if (rc) return rc;
+  BatchPtrs P;
+  rc = upload_key_ptrs(ctx, ids, B, P);
+  if (rc) return rc;
+  for (size_t i = 0; i < B; i++) {
+    const ClientKeys &k = ctx->clients[ids[i]];
+    if (k.h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
+  }
+  const size_t exp_stride = db->exp_cts * kCtWords;
+  rec(ctx, 0);
+  // root ct per query -> exp_a[q][0]
+  for (size_t i = 0; i < B; i++)
+    launch_split_u64(d_q + i * 2 * kN, db->exp_a + i * exp_stride, 2, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  u32 *leaves = run_expand(db, B, P);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 1);
+  const size_t nsel = db->dims - 1;
+  const size_t sel_stride = nsel * 2 * (size_t)ctx->G.l_ep * kCtWords;
+  launch_complete(leaves + db->N0 * kCtWords, db->sel, exp_stride, sel_stride, (int)(nsel * ctx->G.l_ep), (int)B, P,
+                  ctx->G, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 2);
+  launch_ct_ntt(leaves, db->qntt, exp_stride, db->N0 * kCtWords, (int)db->N0, (int)B, ctx->S, ctx->stream);
+  rec(ctx, 5);
+  launch_scan(db->d_db, db->qntt, db->mid_a, db->N0 * kCtWords, db->Nrest * kCtWords, (int)B, db->N0, db->N0p,
+              db->Nrest, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 6);
+  rec(ctx, 3);
+  u32 *res = run_other_dim(db, B, db->sel);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 4);
+  launch_final(res, d_r, db->Nrest * kCtWords, (int)B, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 7);
+  return OPIR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *opir_last_error(const opir_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
+  if (!sch || !out) return set_err(nullptr, OPIR_EINVAL, "null argument");
+  *out = nullptr;
+  if (sch->poly_degree != (uint32_t)kN || !sch->composite)
+    return set_err(nullptr, OPIR_EINVAL, "this build supports the composite N=2048 scheme (CONFIG_N2048_K1_COMP)");
+  const u64 q1 = sch->q1, q2 = sch->q2;
+  if (q1 >= (1ULL << 30) || q2 >= (1ULL << 30) || q1 < 3 || q2 < 3 || q1 >= 2 * q2 || q2 >= 2 * q1 ||
+      (q1 - 1) % (2 * kN) || (q2 - 1) % (2 * kN))
+    return set_err(nullptr, OPIR_EINVAL, "limb primes must be < 2^30, within a factor 2, and 1 mod 2N");
+  if (sch->l_ep == 0 || sch->l_key == 0 || sch->l_ks == 0 || sch->base_log2 == 0 || sch->base_log2 > 32 ||
+      sch->base_log2_key == 0 || sch->base_log2_key > 32 || sch->bv_base_log2 == 0 || sch->bv_base_log2 > 32)
+    return set_err(nullptr, OPIR_EINVAL, "invalid gadget parameters");
+  if (sch->small_q < 2 || sch->small_q >= (1ULL << 32))
+    return set_err(nullptr, OPIR_EINVAL, "small_q must fit 32 bits");
+  opir_ctx *ctx = new (std::nothrow) opir_ctx();
+  if (!ctx) return set_err(nullptr, OPIR_ENOMEM, "out of host memory");
+  ctx->sch = *sch;
+  ctx->device = device;
+  auto fail = [&](const std::string &m) {
+    g_create_error = m;
+    opir_ctx_destroy(ctx);
+    return OPIR_EHIP;
+  };
+  if (hipSetDevice(device) != hipSuccess) return fail("hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) return fail("stream create");
+  ctx->stream = ctx->own_stream;
+  for (auto &e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail("event create");
+  // twiddle tables per limb
+  std::vector<u32> tab(2 * 4 * kN);
+  const u64 Q = q1 * q2;
+  for (int l = 0; l < 2; l++) {
+    const u64 q = l == 0 ? q1 : q2;
+    const u64 w = sch->ntt_root % q;
+    if (powmod_h(w, kN, q) != q - 1) {
+      opir_ctx_destroy(ctx);
+      return set_err(nullptr, OPIR_EINVAL, "ntt_root is not a primitive 2N-th root of unity");
+    }
+    u64 wi;
+    inv_h(w, q, wi);
+    u32 *psi = tab.data() + (size_t)l * 4 * kN, *psi_s = psi + kN, *ipsi = psi + 2 * kN, *ipsi_s = psi + 3 * kN;
+    u64 p = 1, pi = 1;
+    for (size_t i = 0; i < (size_t)kN; i++) {
+      const size_t r = brv(i, kLogN);
+      psi[r] = (u32)p; psi_s[r] = shoup_h((u32)p, (u32)q);
+      ipsi[r] = (u32)pi; ipsi_s[r] = shoup_h((u32)pi, (u32)q);
+      p = mulmod_h(p, w, q);
+      pi = mulmod_h(pi, wi, q);
+    }
+    LimbConst &L = ctx->S.limb[l];
+    L.q = (u32)q; L.q2 = (u32)(2 * q);
+    L.r32 = (u32)((1ULL << 32) % q); L.r32s = shoup_h(L.r32, (u32)q);
+    L.one_s = (u32)((1ULL << 32) / q);
+    u64 ni;
+    inv_h(kN, q, ni);
+    L.ninv = (u32)ni; L.ninv_s = shoup_h((u32)ni, (u32)q);
+  }
+  if (hipMalloc((void **)&ctx->d_tables, tab.size() * sizeof(u32)) != hipSuccess) return fail("hipMalloc tables");
+  if (hipMemcpy(ctx->d_tables, tab.data(), tab.size() * sizeof(u32), hipMemcpyHostToDevice) != hipSuccess)
+    return fail("hipMemcpy tables");
+  for (int l = 0; l < 2; l++) {
+    LimbConst &L = ctx->S.limb[l];
+    const u32 *base = ctx->d_tables + (size_t)l * 4 * kN;
+    L.psi = base; L.psi_s = base + kN; L.ipsi = base + 2 * kN; L.ipsi_s = base + 3 * kN;
+  }
+  ctx->S.Q = Q;
+  ctx->S.half_Q = Q >> 1;
+  ctx->S.q1_mod_q2 = (u32)(q1 % q2);
+  u64 q1inv;
+  if (!inv_h(q1 % q2, q2, q1inv)) {
+    opir_ctx_destroy(ctx);
+    return set_err(nullptr, OPIR_EINVAL, "q1, q2 not coprime");
+  }
+  ctx->S.q1inv = (u32)q1inv;
+  ctx->S.q1inv_s = shoup_h((u32)q1inv, (u32)q2);
+  ctx->S.q2_mod_q1 = (u32)(q2 % q1);
+  ctx->S.small_q = sch->small_q;
+  ctx->G = GadgetCfg{(int)sch->l_ep, (int)sch->base_log2, (int)sch->l_key, (int)sch->base_log2_key, (int)sch->l_ks,
+                     (int)sch->bv_base_log2};
+  *out = ctx;
+  return OPIR_OK;
+}
+
+void opir_ctx_destroy(opir_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->clients) {
+    (void)hipFree(kv.second.bv);
+    (void)hipFree(kv.second.gsw);
+  }
+  for (int i = 0; i < kRing; i++) {
+    if (ctx->ring_ev[i]) (void)hipEventDestroy(ctx->ring_ev[i]);
+    if (ctx->pin_ptrs[i]) (void)hipHostFree(ctx->pin_ptrs[i]);
+    if (ctx->dev_ptrs[i]) (void)hipFree(ctx->dev_ptrs[i]);
+  }
+  for (auto &e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->pin) (void)hipHostFree(ctx->pin);
+  if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+}
+
+int opir_ctx_set_stream(opir_ctx *ctx, void *s) {
+  if (!ctx) return OPIR_EINVAL;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  return OPIR_OK;
+}
+
+int opir_ctx_synchronize(opir_ctx *ctx) {
+  if (!ctx) return OPIR_EINVAL;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return OPIR_OK;
+}
+
+int opir_ctx_set_profiling(opir_ctx *ctx, int on) {
+  if (!ctx) return OPIR_EINVAL;
+  ctx->profiling = on != 0;
+  return OPIR_OK;
+}
+
+int opir_last_stage_ms(opir_ctx *ctx, float *o) {
+  if (!ctx || !o) return OPIR_EINVAL;
+  if (!ctx->profiling) return set_err(ctx, OPIR_EINVAL, "profiling is off");
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipEventElapsedTime(&o[0], ctx->ev[0], ctx->ev[1]));  // Expand
+  HIP_TRY(ctx, hipEventElapsedTime(&o[1], ctx->ev[1], ctx->ev[2]));  // Convert
+  HIP_TRY(ctx, hipEventElapsedTime(&o[2], ctx->ev[2], ctx->ev[3]));  // First dim (query NTT + scan)
+  HIP_TRY(ctx, hipEventElapsedTime(&o[3], ctx->ev[3], ctx->ev[4]));  // Other dim
+  HIP_TRY(ctx, hipEventElapsedTime(&o[4], ctx->ev[4], ctx->ev[7]));  // Modulus switching
+  HIP_TRY(ctx, hipEventElapsedTime(&o[5], ctx->ev[5], ctx->ev[6]));  // scan kernel
+  return OPIR_OK;
+}
+
+int opir_db_create(opir_ctx *ctx, size_t N0, size_t Nrest, size_t dims, size_t h, opir_db **out) {
+  if (!ctx || !out) return OPIR_EINVAL;
+  *out = nullptr;
+  if (N0 == 0 || Nrest == 0 || dims == 0 || h > 11) return set_err(ctx, OPIR_EINVAL, "invalid layout");
+  const size_t cap = size_t{1} << h;
+  const size_t u = N0 + (size_t)ctx->G.l_ep * (dims - 1);
+  if (u > cap) return set_err(ctx, OPIR_EINVAL, "layout does not fit the expansion capacity");
+  if (dims > 1 && (dims - 1 > 40 || Nrest > (size_t{1} << (dims - 1))))
+    return set_err(ctx, OPIR_EINVAL, "more candidates than the selector tree addresses");
+  opir_db *db = new (std::nothrow) opir_db();
+  if (!db) return set_err(ctx, OPIR_ENOMEM, "out of host memory");
+  db->ctx = ctx;
+  db->N0 = N0; db->N0p = (N0 + 3) & ~size_t{3}; db->Nrest = Nrest; db->dims = dims; db->h = h;
+  db->num_pt = N0 * Nrest;
+  db->u = u;
+  db->exp_cts = u + 2;
+  const size_t bytes = 2 * (size_t)kN * db->N0p * Nrest * sizeof(u32);
+  if (hipMalloc((void **)&db->d_db, bytes) != hipSuccess) {
+    delete db;
+    return set_err(ctx, OPIR_ENOMEM, "hipMalloc database (" + std::to_string(bytes) + " bytes)");
+  }
+  if (hipMemsetAsync(db->d_db, 0, bytes, ctx->stream) != hipSuccess) {
+    (void)hipFree(db->d_db);
+    delete db;
+    return set_err(ctx, OPIR_EHIP, "hipMemset database");
+  }
+  *out = db;
+  return OPIR_OK;
+}
+
+void opir_db_destroy(opir_db *db) {
+  if (!db) return;
+  (void)hipStreamSynchronize(db->ctx->stream);
+  for (u32 *p : {db->d_db, db->exp_a, db->exp_b, db->qntt, db->sel, db->mid_a, db->mid_b})
+    if (p) (void)hipFree(p);
+  if (db->q_u64) (void)hipFree(db->q_u64);
+  if (db->r_u64) (void)hipFree(db->r_u64);
+  delete db;
+}
+
+int opir_db_load_plain(opir_db *db, const uint16_t *coeffs, size_t first_pt, size_t count) {
+  if (!db || (!coeffs && count)) return OPIR_EINVAL;
+  opir_ctx *ctx = db->ctx;
+  if (first_pt > db->num_pt || count > db->num_pt - first_pt)
+    return set_err(ctx, OPIR_EINVAL, "PirServer::load_data logical plaintext count exceeds its layout");
+  const size_t chunk = 8192;
+  uint16_t *d_pts = nullptr;
+  u32 *d_tmp = nullptr;
+  HIP_TRY(ctx, hipMalloc((void **)&d_pts, chunk * kN * sizeof(uint16_t)));
+  if (hipMalloc((void **)&d_tmp, 2 * chunk * kN * sizeof(u32)) != hipSuccess) {
+    (void)hipFree(d_pts);
+    return set_err(ctx, OPIR_ENOMEM, "hipMalloc staging");
+  }
+  int rc = OPIR_OK;
+  for (size_t off = 0; off < count && rc == OPIR_OK; off += chunk) {
+    const size_t n = std::min(chunk, count - off);
+    hipError_t e = hipMemcpyAsync(d_pts, coeffs + off * kN, n * kN * sizeof(uint16_t), hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e != hipSuccess) { rc = set_err(ctx, OPIR_EHIP, "upload plaintexts"); break; }
+    launch_db_ntt_plain(d_pts, d_tmp, n, ctx->S, ctx->stream);
+    launch_db_scatter(d_tmp, db->d_db, first_pt + off, n, db->N0, db->N0p, db->Nrest, ctx->stream);
+    e = hipGetLastError();
+    if (e != hipSuccess) { rc = set_err(ctx, OPIR_EHIP, std::string("db ingest: ") + hipGetErrorString(e)); break; }
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { rc = set_err(ctx, OPIR_EHIP, std::string("db ingest: ") + hipGetErrorString(e)); break; }
+  }
+  (void)hipFree(d_pts);
+  (void)hipFree(d_tmp);
+  return rc;
+}
+
+int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi) {
+  if (!db || !lo || !hi) return OPIR_EINVAL;
+  opir_ctx *ctx = db->ctx;
+  const size_t n = (size_t)kN * db->num_pt;
+  u32 *d_ref = nullptr;
+  HIP_TRY(ctx, hipMalloc((void **)&d_ref, 2 * n * sizeof(u32)));
+  int rc = OPIR_OK;
+  if (hipMemcpy(d_ref, lo, n * sizeof(u32), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_ref + n, hi, n * sizeof(u32), hipMemcpyHostToDevice) != hipSuccess)
+    rc = set_err(ctx, OPIR_EHIP, "upload reference-layout database");
+  if (!rc) {
+    launch_db_from_ref(d_ref, db->d_db, db->N0, db->N0p, db->Nrest, ctx->stream);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = set_err(ctx, OPIR_EHIP, std::string("db transpose: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(d_ref);
+  return rc;
+}
+
+int opir_client_set_keys(opir_ctx *ctx, uint64_t id, const uint64_t *bv, size_t h, const uint64_t *gsw) {
+  if (!ctx) return OPIR_EINVAL;
+  if ((!bv && h) || !gsw) return set_err(ctx, OPIR_EINVAL, "PirServer session key bundle cannot be null");
+  if (h > 11) return set_err(ctx, OPIR_EINVAL, "BV key count out of range");
+  const size_t L = ctx->G.l_ks, lk = ctx->G.l_key;
+  const u64 Q = ctx->S.Q, q1 = ctx->S.limb[0].q, q2 = ctx->S.limb[1].q;
+  int rc = h ? validate_mod(ctx, bv, h * L * 2 * kN, Q, "BV key") : OPIR_OK;
+  if (rc) return rc;
+  rc = validate_mod(ctx, gsw, 2 * lk * 2 * kN, Q, "RGSW key");
+  if (rc) return rc;
+  std::vector<u32> hbv(h * L * kCtWords), hg(2 * lk * kCtWords);
+  for (size_t r = 0; r < h * L; r++)
+    for (int ab = 0; ab < 2; ab++) {      // input order a, b
+      const int out = ab == 0 ? 1 : 0;    // device order b (-> c0), a (-> c1)
+      const uint64_t *src = bv + (r * 2 + ab) * kN;
+      u32 *dst = hbv.data() + r * kCtWords + (size_t)out * 2 * kN;
+      for (size_t j = 0; j < (size_t)kN; j++) {
+        dst[j] = (u32)(src[j] % q1);
+        dst[kN + j] = (u32)(src[j] % q2);
+      }
+    }
+  for (size_t r = 0; r < 2 * lk; r++)
+    for (int p = 0; p < 2; p++) {
+      const uint64_t *src = gsw + (r * 2 + p) * kN;
+      u32 *dst = hg.data() + r * kCtWords + (size_t)p * 2 * kN;
+      for (size_t j = 0; j < (size_t)kN; j++) {
+        dst[j] = (u32)(src[j] % q1);
+        dst[kN + j] = (u32)(src[j] % q2);
+      }
+    }
+  ClientKeys k;
+  k.h = h;
+  if (h) HIP_TRY(ctx, hipMalloc((void **)&k.bv, hbv.size() * sizeof(u32)));
+  if (hipMalloc((void **)&k.gsw, hg.size() * sizeof(u32)) != hipSuccess) {
+    (void)hipFree(k.bv);
+    return set_err(ctx, OPIR_ENOMEM, "hipMalloc keys");
+  }
+  if ((h && hipMemcpy(k.bv, hbv.data(), hbv.size() * sizeof(u32), hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(k.gsw, hg.data(), hg.size() * sizeof(u32), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(k.bv);
+    (void)hipFree(k.gsw);
+    return set_err(ctx, OPIR_EHIP, "upload keys");
+  }
+  auto it = ctx->clients.find(id);
+  if (it != ctx->clients.end()) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(it->second.bv);
+    (void)hipFree(it->second.gsw);
+  }
+  ctx->clients[id] = k;
+  return OPIR_OK;
+}
+
+int opir_client_remove(opir_ctx *ctx, uint64_t id) {
+  if (!ctx) return OPIR_EINVAL;
+  auto it = ctx->clients.find(id);
+  if (it == ctx->clients.end()) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  (void)hipFree(it->second.bv);
+  (void)hipFree(it->second.gsw);
+  ctx->clients.erase(it);
+  return OPIR_OK;
+}
+
+int opir_answer_batch_device(opir_db *db, size_t B, const uint64_t *ids, const uint64_t *d_q, uint64_t *d_r) {
+  if (!db || !ids || !d_q || !d_r) return OPIR_EINVAL;
+  if (B == 0) return OPIR_OK;
+  return answer_device(db, B, ids, d_q, d_r);
+}
+
+int opir_answer_batch(opir_db *db, size_t B, const uint64_t *ids, const uint64_t *queries, uint64_t *resps) {
+  if (!db || !ids || !queries || !resps) return OPIR_EINVAL;
+  if (B == 0) return OPIR_OK;
+  opir_ctx *ctx = db->ctx;
+  int rc = validate_mod(ctx, queries, B * 2 * kN, ctx->S.Q, "query");
+  if (rc) return rc;
+  for (size_t i = 0; i < B; i++)
+    if (!ctx->clients.count(ids[i])) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(ids[i]));
+  rc = ensure_ws(db, B);
+  if (rc) return rc;
+  const size_t bytes = B * 2 * kN * sizeof(u64);
+  rc = ensure_pin(ctx, bytes);
+  if (rc) return rc;
+  std::memcpy(ctx->pin, queries, bytes);
+  HIP_TRY(ctx, hipMemcpyAsync(db->q_u64, ctx->pin, bytes, hipMemcpyHostToDevice, ctx->stream));
+  rc = answer_device(db, B, ids, db->q_u64, db->r_u64);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->pin, db->r_u64, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(resps, ctx->pin, bytes);
+  return OPIR_OK;
+}
+
+int opir_answer(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *resp) {
+  return opir_answer_batch(db, 1, &id, query, resp);
+}
+
+// ---------------------------------------------------------------------------
+// Stage entry points (host buffers)
+// ---------------------------------------------------------------------------
+namespace {
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+}  // namespace
+
+#define DEV_ALLOC(ctx, buf, bytes) HIP_TRY(ctx, hipMalloc(&(buf).p, (bytes)))
+
+static int ntt_entry(opir_ctx *ctx, uint64_t *data, size_t count, bool inverse) {
+  if (!ctx || (!data && count)) return OPIR_EINVAL;
+  if (!count) return OPIR_OK;
+  int rc = validate_mod(ctx, data, count * kN, ctx->S.Q, "NTT input");
+  if (rc) return rc;
+  DevBuf d;
+  DEV_ALLOC(ctx, d, count * kN * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(d.p, data, count * kN * sizeof(u64), hipMemcpyHostToDevice));
+  launch_ntt_u64((u64 *)d.p, count, inverse, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(data, d.p, count * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int opir_ntt_fwd(opir_ctx *ctx, uint64_t *data, size_t count) { return ntt_entry(ctx, data, count, false); }
+int opir_ntt_inv(opir_ctx *ctx, uint64_t *data, size_t count) { return ntt_entry(ctx, data, count, true); }
+
+int opir_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out) {
+  if (!db || !query || !out) return OPIR_EINVAL;
+  opir_ctx *ctx = db->ctx;
+  int rc = validate_mod(ctx, query, 2 * kN, ctx->S.Q, "query");
+  if (rc) return rc;
+  if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  if (ctx->clients[id].h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
+  rc = ensure_ws(db, 1);
+  if (rc) return rc;
+  BatchPtrs P;
+  rc = upload_key_ptrs(ctx, &id, 1, P);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpy(db->q_u64, query, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  launch_split_u64(db->q_u64, db->exp_a, 2, ctx->S, ctx->stream);
+  u32 *leaves = run_expand(db, 1, P);
+  LAUNCH_CHECK(ctx);
+  DevBuf o;
+  DEV_ALLOC(ctx, o, db->u * 2 * kN * sizeof(u64));
+  launch_join_u64(leaves, (u64 *)o.p, db->u * 2, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, db->u * 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int opir_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *sel) {
+  if (!db || !expanded || !sel) return OPIR_EINVAL;
+  opir_ctx *ctx = db->ctx;
+  const size_t nsel = db->dims - 1;
+  if (!nsel) return OPIR_OK;
+  if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  int rc = validate_mod(ctx, expanded, db->u * 2 * kN, ctx->S.Q, "expanded");
+  if (rc) return rc;
+  rc = ensure_ws(db, 1);
+  if (rc) return rc;
+  BatchPtrs P;
+  rc = upload_key_ptrs(ctx, &id, 1, P);
+  if (rc) return rc;
+  DevBuf e;
+  DEV_ALLOC(ctx, e, db->u * 2 * kN * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(e.p, expanded, db->u * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  launch_split_u64((u64 *)e.p, db->exp_a, db->u * 2, ctx->S, ctx->stream);
+  const size_t sel_stride = nsel * 2 * (size_t)ctx->G.l_ep * kCtWords;
+  launch_complete(db->exp_a + db->N0 * kCtWords, db->sel, 0, sel_stride, (int)(nsel * ctx->G.l_ep), 1, P, ctx->G,
+                  ctx->S, ctx->stream);
+  const size_t npoly = nsel * 2 * ctx->G.l_ep * 2;
+  DevBuf o;
+  DEV_ALLOC(ctx, o, npoly * kN * sizeof(u64));
+  launch_join_u64(db->sel, (u64 *)o.p, npoly, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(sel, o.p, npoly * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
+  if (!db || !query || !out) return OPIR_EINVAL;
+  opir_ctx *ctx = db->ctx;
+  int rc = validate_mod(ctx, query, db->N0 * 2 * kN, ctx->S.Q, "query");
+  if (rc) return rc;
+  rc = ensure_ws(db, 1);
+  if (rc) return rc;
+  DevBuf e;
+  DEV_ALLOC(ctx, e, db->N0 * 2 * kN * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(e.p, query, db->N0 * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  launch_split_u64((u64 *)e.p, db->exp_a, db->N0 * 2, ctx->S, ctx->stream);
+  launch_ct_ntt(db->exp_a, db->qntt, 0, 0, (int)db->N0, 1, ctx->S, ctx->stream);
+  launch_scan(db->d_db, db->qntt, db->mid_a, 0, 0, 1, db->N0, db->N0p, db->Nrest, ctx->S, ctx->stream);
+  DevBuf o;
+  DEV_ALLOC(ctx, o, db->Nrest * 2 * kN * sizeof(u64));
+  launch_ct_to_coef_u64(db->mid_a, (u64 *)o.p, db->Nrest, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, db->Nrest * 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int opir_other_dim(opir_db *db, const uint64_t *mid, const uint64_t *sel, uint64_t *out) {
+  if (!db || !mid || !out || (db->dims > 1 && !sel)) return OPIR_EINVAL;
+  opir_ctx *ctx = db->ctx;
+  if (int e = check_answerable(db)) return e;
+  const size_t nsel = db->dims - 1;
+  const size_t sel_polys = nsel * 2 * ctx->G.l_ep * 2;
+  int rc = validate_mod(ctx, mid, db->Nrest * 2 * kN, ctx->S.Q, "candidates");
+  if (rc) return rc;
+  if (nsel) rc = validate_mod(ctx, sel, sel_polys * kN, ctx->S.Q, "selectors");
+  if (rc) return rc;
+  rc = ensure_ws(db, 1);
+  if (rc) return rc;
+  DevBuf m;
+  DEV_ALLOC(ctx, m, db->Nrest * 2 * kN * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(m.p, mid, db->Nrest * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  // stage coefficient-form candidates in mid_b (Nrest cts), NTT into mid_a
+  launch_split_u64((u64 *)m.p, db->mid_b, db->Nrest * 2, ctx->S, ctx->stream);
+  launch_ct_ntt(db->mid_b, db->mid_a, 0, 0, (int)db->Nrest, 1, ctx->S, ctx->stream);
+  DevBuf s;
+  if (nsel) {
+    DEV_ALLOC(ctx, s, sel_polys * kN * sizeof(u64));
+    HIP_TRY(ctx, hipMemcpy(s.p, sel, sel_polys * kN * sizeof(u64), hipMemcpyHostToDevice));
+    launch_split_u64((u64 *)s.p, db->sel, sel_polys, ctx->S, ctx->stream);
+  }
+  u32 *res = run_other_dim(db, 1, db->sel);
+  DevBuf o;
+  DEV_ALLOC(ctx, o, 2 * kN * sizeof(u64));
+  launch_ct_to_coef_u64(res, (u64 *)o.p, 1, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int opir_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const uint64_t *ct, uint64_t *out) {
+  if (!ctx || !gsw || !ct || !out) return OPIR_EINVAL;
+  if (l == 0 || l > 64 || b == 0 || b > 32) return set_err(ctx, OPIR_EINVAL, "invalid gadget");
+  int rc = validate_mod(ctx, gsw, 2 * l * 2 * kN, ctx->S.Q, "RGSW");
+  if (rc) return rc;
+  rc = validate_mod(ctx, ct, 2 * kN, ctx->S.Q, "ciphertext");
+  if (rc) return rc;
+  DevBuf g64, g32, c64, c32, o32, o64;
+  DEV_ALLOC(ctx, g64, 2 * l * 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, g32, 2 * l * kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, c64, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, c32, kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, o32, kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, o64, 2 * kN * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(g64.p, gsw, 2 * l * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(c64.p, ct, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  launch_split_u64((u64 *)g64.p, (u32 *)g32.p, 2 * l * 2, ctx->S, ctx->stream);
+  launch_split_u64((u64 *)c64.p, (u32 *)c32.p, 2, ctx->S, ctx->stream);
+  launch_ext_prod((u32 *)g32.p, (int)l, (int)b, (u32 *)c32.p, (u32 *)o32.p, ctx->S, ctx->stream);
+  launch_join_u64((u32 *)o32.p, (u64 *)o64.p, 2, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int opir_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out) {
+  if (!ctx || !ct || !out) return OPIR_EINVAL;
+  int rc = validate_mod(ctx, ct, 2 * kN, ctx->S.Q, "ciphertext");
+  if (rc) return rc;
+  DevBuf c64, c32, n32, o64;
+  DEV_ALLOC(ctx, c64, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, c32, kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, n32, kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, o64, 2 * kN * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(c64.p, ct, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  launch_split_u64((u64 *)c64.p, (u32 *)c32.p, 2, ctx->S, ctx->stream);
+  launch_ct_ntt((u32 *)c32.p, (u32 *)n32.p, 0, 0, 1, 1, ctx->S, ctx->stream);
+  launch_final((u32 *)n32.p, (u64 *)o64.p, 0, 1, ctx->S, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// Internal launch interface between the C-ABI layer and the HIP kernels.
+// Ciphertext layout on device ("RNS ct"): [poly p][limb][N] u32, 4*N words;
+// NTT-form polynomials are stored in HEXL evaluation order (bit-reversed).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "opir_device.h"
+
+namespace opir {
+
+constexpr size_t kCtWords = 4 * kN;  // one RNS ciphertext
+
+struct GadgetCfg {
+  int l_ep, b_ep;     // data selectors (external product in the MUX tree)
+  int l_key, b_key;   // RGSW(s) completion key
+  int l_ks, b_ks;     // BV keyswitch
+};
+
+// Per-query pointers for batched launches (device arrays of length B).
+struct BatchPtrs {
+  const u32 *const *bv_keys;   // client BV keys: [h][l_ks][2 (b,a)][2 limbs][N]
+  const u32 *const *gsw_keys;  // client RGSW(s): [2 l_key][2][2][N]
+};
+
+void launch_split_u64(const u64 *in, u32 *out, size_t npoly, const SchemeConst &S, hipStream_t st);
+void launch_join_u64(const u32 *in, u64 *out, size_t npoly, const SchemeConst &S, hipStream_t st);
+void launch_ntt_u64(u64 *data, size_t npoly, bool inverse, const SchemeConst &S, hipStream_t st);
+void launch_ntt_rns(const u32 *in, u32 *out, size_t npoly_rns, bool inverse, const SchemeConst &S,
+                    hipStream_t st);
+
+// DB preprocessing. Device DB layout: [limb][level][kq][row][4] with
+// pt = row * N0 + 4*kq + j; N0p = N0 rounded up to 4 (padding is zero).
+void launch_db_ntt_plain(const uint16_t *pts, u32 *tmp, size_t count, const SchemeConst &S, hipStream_t st);
+void launch_db_scatter(const u32 *tmp, u32 *db, size_t first_pt, size_t count, size_t N0, size_t N0p,
+                       size_t Nrest, hipStream_t st);
+void launch_db_from_ref(const u32 *ref, u32 *db, size_t N0, size_t N0p, size_t Nrest, hipStream_t st);
+
+// Expansion level: nodes [0, n) of input buffer -> children 2r, 2r+1 of output buffer.
+void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int nnodes, int B,
+                         int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st);
+// Forward NTT of ciphertexts (both polys): in coef RNS cts -> out NTT RNS cts.
+void launch_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int count, int B,
+                   const SchemeConst &S, hipStream_t st);
+// Selector completion: rows = nsel * l_ep expanded leaves (coef form) -> sel [nsel][2 l_ep][2][2][N].
+void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
+                     const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st);
+// First-dim scan: qntt [B][N0][2][2][N] -> mid [B][Nrest][2][2][N] (NTT form).
+void launch_scan(const u32 *db, const u32 *qntt, u32 *mid, size_t q_stride, size_t mid_stride, int B,
+                 size_t N0, size_t N0p, size_t Nrest, const SchemeConst &S, hipStream_t st);
+// MUX level. ragged: slot s < offset copies in[s]; s >= offset muxes in[off+2j], in[off+2j+1].
+// perfect (offset < 0): slot s muxes in[s], in[s+half].
+void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
+                const u32 *sel, size_t sel_stride, const GadgetCfg &G, const SchemeConst &S, hipStream_t st);
+// INTT + CRT + centred rescale to small_q: res [B][ct] NTT form -> out [B][2][N] u64.
+void launch_final(const u32 *res, u64 *out, size_t res_stride, int B, const SchemeConst &S, hipStream_t st);
+// NTT-form RNS ct -> coefficient-form u64 mod Q ct (test entry points).
+void launch_ct_to_coef_u64(const u32 *in, u64 *out, size_t count, const SchemeConst &S, hipStream_t st);
+// Stand-alone external product (test entry point).
+void launch_ext_prod(const u32 *gsw, int l, int b, const u32 *ct_coef, u32 *out_ntt, const SchemeConst &S,
+                     hipStream_t st);
+
+}  // namespace opir

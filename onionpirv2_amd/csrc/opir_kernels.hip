@@ -1,0 +1,668 @@
+// HIP kernels of the OnionPIRv2 answer path for CDNA4 (gfx950).
+// Reference algorithm: src/server.cpp:731-947 (make_query and its stages),
+// src/gsw.cpp:119-182,252-357 (external product, completion),
+// src/bv_keyswitch.cpp:296-367 (BV keyswitch), src/matrix.cpp:232-350 (scan).
+#include "opir_kernels.h"
+
+namespace opir {
+
+namespace {
+
+struct Barrier {
+  __device__ void operator()() const { __syncthreads(); }
+};
+
+// Gadget-decomposition MAC shared by the external product, the completion
+// and the BV keyswitch: for every digit of every input polynomial, forward
+// NTT the digit (limb of this lane block) and accumulate digit * key row
+// into two u64 accumulators per value. Digits are produced LSB-first from
+// the running centred value D, exactly like decomp_rlwe_single_mod
+// (src/gsw.cpp:285-293) and signed_gadget_decompose (bv_keyswitch.cpp:32-39).
+// GSW_ROWS: key row for digit i of poly p is p*l + (l-1-i) (MSB-first rows,
+// gsw.cpp:286); otherwise row i (BV keys, bv_keyswitch.cpp:344-356).
+// Key row layout: [2 outputs][2 limbs][N] u32, NTT form.
+template <int NP, bool GSW_ROWS>
+__device__ __forceinline__ void gadget_mac(i64 (&D)[NP][8], int l, int b, const u32 *__restrict__ key, int sb,
+                                           int t, const LimbConst &L, u32 *nttlds, u64 (&acc0)[8],
+                                           u64 (&acc1)[8]) {
+  Barrier bar;
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    for (int i = 0; i < l; i++) {
+      u32 v[8];
+#pragma unroll
+      for (int c = 0; c < 8; c++) v[c] = digit_mod(next_digit(D[p][c], b), L.q);
+      ntt_fwd_2048(v, nttlds, t, L, bar);
+      const int row = GSW_ROWS ? (p * l + (l - 1 - i)) : i;
+      const u32 *r = key + (size_t)row * (4 * kN);
+      u32 k0[8], k1[8];
+      load_map3(k0, r + (0 * 2 + sb) * kN, t);
+      load_map3(k1, r + (1 * 2 + sb) * kN, t);
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        acc0[c] += (u64)v[c] * k0[c];
+        acc1[c] += (u64)v[c] * k1[c];
+      }
+    }
+  }
+}
+
+// Compose both limbs of a coefficient-form polynomial held in LDS (natural
+// order) at this thread's map0 indices and centre the result.
+__device__ __forceinline__ void compose_centre(const u32 *lo, const u32 *hi, int t, const SchemeConst &S,
+                                               i64 (&D)[8]) {
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int j = map0(t, c);
+    D[c] = centre(crt_compose(lo[j], hi[j], S), S);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Elementwise helpers.
+// ---------------------------------------------------------------------------
+__global__ void k_split_u64(const u64 *__restrict__ in, u32 *__restrict__ out, size_t npoly, SchemeConst S) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npoly * kN) return;
+  const size_t p = i / kN, j = i % kN;
+  const u64 x = in[i];
+  out[(2 * p + 0) * kN + j] = reduce_u64(x, S.limb[0]);
+  out[(2 * p + 1) * kN + j] = reduce_u64(x, S.limb[1]);
+}
+
+__global__ void k_join_u64(const u32 *__restrict__ in, u64 *__restrict__ out, size_t npoly, SchemeConst S) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npoly * kN) return;
+  const size_t p = i / kN, j = i % kN;
+  out[i] = crt_compose(in[(2 * p) * kN + j], in[(2 * p + 1) * kN + j], S);
+}
+
+// In-place NTT of u64 (mod Q) polynomials; one workgroup (2 lane blocks,
+// one per limb) per polynomial.
+__global__ __launch_bounds__(512) void k_ntt_u64(u64 *data, int inverse, SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ u32 res[2][kN];
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  u64 *poly = data + (size_t)blockIdx.x * kN;
+  u32 v[8];
+  Barrier bar;
+  if (!inverse) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = reduce_u64(poly[map0(t, c)], L);
+    ntt_fwd_2048(v, nlds[sb], t, L, bar);
+#pragma unroll
+    for (int c = 0; c < 8; c++) res[sb][map3(t, c)] = v[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = reduce_u64(poly[map3(t, c)], L);
+    ntt_inv_2048(v, nlds[sb], t, L, bar);
+#pragma unroll
+    for (int c = 0; c < 8; c++) res[sb][map0(t, c)] = v[c];
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kN; j += 512) poly[j] = crt_compose(res[0][j], res[1][j], S);
+}
+
+// RNS polys [npoly][2 limbs][N]: coef (map0) <-> NTT (map3).
+__global__ __launch_bounds__(512) void k_ntt_rns(const u32 *in, u32 *out, int inverse, SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  const size_t off = ((size_t)blockIdx.x * 2 + sb) * kN;
+  u32 v[8];
+  Barrier bar;
+  if (!inverse) {
+    load_map0(v, in + off, t);
+    ntt_fwd_2048(v, nlds[sb], t, L, bar);
+    store_map3(out + off, v, t);
+  } else {
+    load_map3(v, in + off, t);
+    ntt_inv_2048(v, nlds[sb], t, L, bar);
+    store_map0(out + off, v, t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DB preprocessing (src/server.cpp:80-203). Plaintext coefficients are < t,
+// so NTT_Q(pt) mod q_i = NTT_{q_i}(pt): each limb is transformed directly.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void k_db_ntt_plain(const uint16_t *__restrict__ pts, u32 *__restrict__ tmp,
+                                                      SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  const uint16_t *p = pts + (size_t)blockIdx.x * kN;
+  u32 v[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = p[map0(t, c)];
+  ntt_fwd_2048(v, nlds[sb], t, L, Barrier());
+  store_map3(tmp + ((size_t)sb * gridDim.x + blockIdx.x) * kN, v, t);
+}
+
+// tmp [limb][count][N] (pt-major) -> db [limb][level][kq][row][4].
+// Tile: 32 levels x 64 pts through LDS.
+__global__ __launch_bounds__(256) void k_db_scatter(const u32 *__restrict__ tmp, u32 *__restrict__ db,
+                                                    size_t first_pt, size_t count, size_t N0, size_t N0p,
+                                                    size_t Nrest) {
+  __shared__ u32 tile[32][65];
+  const int limb = blockIdx.z;
+  const size_t pt0 = (size_t)blockIdx.x * 64;
+  const int lv0 = blockIdx.y * 32;
+  const u32 *src = tmp + (size_t)limb * count * kN;
+  const size_t level_words = Nrest * N0p;
+  // load: 64 pts x 32 levels (each pt row contributes 32 consecutive words)
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int pl = e / 32, ll = e % 32;
+    const size_t pt = pt0 + pl;
+    tile[ll][pl] = pt < count ? src[pt * kN + lv0 + ll] : 0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int ll = e / 64, pl = e % 64;
+    const size_t pt = pt0 + pl;
+    if (pt >= count) continue;
+    const size_t g = first_pt + pt;
+    const size_t row = g / N0, k = g % N0;
+    const size_t dst = (size_t)limb * kN * level_words + (size_t)(lv0 + ll) * level_words +
+                       (k >> 2) * (Nrest * 4) + row * 4 + (k & 3);
+    db[dst] = tile[ll][pl];
+  }
+}
+
+// Reference layout ref [limb][coeff][num_pt] (server.cpp:162-170) -> device layout.
+__global__ void k_db_from_ref(const u32 *__restrict__ ref, u32 *__restrict__ db, size_t N0, size_t N0p,
+                              size_t Nrest) {
+  const size_t num_pt = N0 * Nrest;
+  const size_t total = 2 * (size_t)kN * num_pt;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const size_t pt = i % num_pt, lv = (i / num_pt) % kN, limb = i / (num_pt * kN);
+  const size_t row = pt / N0, k = pt % N0;
+  const size_t level_words = Nrest * N0p;
+  db[limb * kN * level_words + lv * level_words + (k >> 2) * (Nrest * 4) + row * 4 + (k & 3)] = ref[i];
+}
+
+// ---------------------------------------------------------------------------
+// Expansion level (src/server.cpp:786-810 with bv_keyswitch.cpp:296-367).
+// One workgroup per (node, query): lane block sb computes limb sb.
+//   c' = BVKS(sigma_k(c));  out[2r] = c + c';  out[2r+1] = (c - c') * X^-k
+// ---------------------------------------------------------------------------
+struct ExpandArgs {
+  const u32 *in;
+  u32 *out;
+  size_t in_stride, out_stride;  // words between queries
+  const u32 *const *bv_keys;
+  size_t key_level_words;        // words per level in a client's BV key
+  int level;                     // k = 2^level, galois = N/k + 1
+  int kinv;                      // galois^-1 mod 2N
+  int l_ks, b_ks;
+};
+
+__global__ __launch_bounds__(512) void k_expand_level(ExpandArgs A, SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ u32 sh[2][2][kN];  // [poly][limb][N]
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  const int node = blockIdx.x, qi = blockIdx.y;
+  const u32 *c = A.in + qi * A.in_stride + (size_t)node * kCtWords;
+  u32 c0[8], c1[8];
+  load_map0(c0, c + (0 * 2 + sb) * kN, t);
+  load_map0(c1, c + (1 * 2 + sb) * kN, t);
+  store_map0(sh[0][sb], c0, t);
+  store_map0(sh[1][sb], c1, t);
+  __syncthreads();
+  // sigma_k gather (utils.cpp:58-72 inverted): out[j] = in[i0] or -in[i0-N], i0 = j*kinv mod 2N
+  u32 s0[8], s1[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const int j = map0(t, cc);
+    const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+    if (i0 < kN) {
+      s0[cc] = sh[0][sb][i0];
+      s1[cc] = sh[1][sb][i0];
+    } else {
+      const u32 a = sh[0][sb][i0 - kN], b = sh[1][sb][i0 - kN];
+      s0[cc] = a ? L.q - a : 0;
+      s1[cc] = b ? L.q - b : 0;
+    }
+  }
+  __syncthreads();
+  store_map0(sh[1][sb], s1, t);
+  __syncthreads();
+  i64 D[1][8];
+  compose_centre(sh[1][0], sh[1][1], t, S, D[0]);
+  u64 accb[8], acca[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
+  const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
+  gadget_mac<1, false>(D, A.l_ks, A.b_ks, key, sb, t, L, nlds[sb], accb, acca);
+  u32 db[8], da[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    db[cc] = reduce_u64(accb[cc], L);
+    da[cc] = reduce_u64(acca[cc], L);
+  }
+  Barrier bar;
+  ntt_inv_2048(db, nlds[sb], t, L, bar);
+  ntt_inv_2048(da, nlds[sb], t, L, bar);
+  // c' = (sigma(c0) + delta_b, delta_a); children (server.cpp:803-808)
+  u32 *left = A.out + qi * A.out_stride + (size_t)(2 * node) * kCtWords;
+  u32 *right = left + kCtWords;
+  u32 l0[8], l1[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const u32 p0 = add_mod(s0[cc], db[cc], L.q), p1 = da[cc];
+    l0[cc] = add_mod(c0[cc], p0, L.q);
+    l1[cc] = add_mod(c1[cc], p1, L.q);
+    c0[cc] = sub_mod(c0[cc], p0, L.q);
+    c1[cc] = sub_mod(c1[cc], p1, L.q);
+  }
+  store_map0(left + (0 * 2 + sb) * kN, l0, t);
+  store_map0(left + (1 * 2 + sb) * kN, l1, t);
+  // negacyclic shift by -k (utils.cpp:166-186 with shift = -k):
+  // out[j] = d[j+k] if j+k < N else -d[j+k-N]
+  store_map0(sh[0][sb], c0, t);
+  store_map0(sh[1][sb], c1, t);
+  __syncthreads();
+  const int k = 1 << A.level;
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const int j = map0(t, cc);
+    const int s = j + k;
+    u32 a, b;
+    if (s < kN) {
+      a = sh[0][sb][s];
+      b = sh[1][sb][s];
+    } else {
+      a = sh[0][sb][s - kN];
+      b = sh[1][sb][s - kN];
+      a = a ? L.q - a : 0;
+      b = b ? L.q - b : 0;
+    }
+    right[(0 * 2 + sb) * kN + j] = a;
+    right[(1 * 2 + sb) * kN + j] = b;
+  }
+}
+
+// Forward NTT of both polys of `count` cts per query (evaluate_first_dim's
+// query transform, server.cpp:331-337). grid (count*2, B).
+__global__ __launch_bounds__(512) void k_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride,
+                                                SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  const size_t poly = blockIdx.x;  // ct*2 + p
+  const size_t off = (poly * 2 + sb) * kN;
+  u32 v[8];
+  load_map0(v, in + blockIdx.y * in_stride + off, t);
+  ntt_fwd_2048(v, nlds[sb], t, L, Barrier());
+  store_map3(out + blockIdx.y * out_stride + off, v, t);
+}
+
+// ---------------------------------------------------------------------------
+// Selector completion (server.cpp:823-855, gsw.cpp:317-357). One workgroup
+// per expanded selector row: top row = NTT(row); bottom row =
+// external_product(RGSW(s), row) with the key gadget (l_key, b_key).
+// sel layout: [selector][2*l_ep rows][2 polys][2 limbs][N].
+// ---------------------------------------------------------------------------
+struct CompleteArgs {
+  const u32 *leaves;
+  u32 *sel;
+  size_t leaf_stride, sel_stride;
+  const u32 *const *gsw_keys;
+  int l_ep, l_key, b_key;
+};
+
+__global__ __launch_bounds__(512) void k_complete(CompleteArgs A, SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ u32 sh[2][2][kN];
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  const int g = blockIdx.x, qi = blockIdx.y;
+  const int a = g / A.l_ep, r = g % A.l_ep;
+  const u32 *row = A.leaves + qi * A.leaf_stride + (size_t)g * kCtWords;
+  u32 *selb = A.sel + qi * A.sel_stride + (size_t)a * (2 * A.l_ep) * kCtWords;
+  Barrier bar;
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    u32 v[8];
+    load_map0(v, row + (p * 2 + sb) * kN, t);
+    store_map0(sh[p][sb], v, t);
+    ntt_fwd_2048(v, nlds[sb], t, L, bar);
+    store_map3(selb + (size_t)r * kCtWords + (p * 2 + sb) * kN, v, t);
+  }
+  __syncthreads();
+  i64 D[2][8];
+  compose_centre(sh[0][0], sh[0][1], t, S, D[0]);
+  compose_centre(sh[1][0], sh[1][1], t, S, D[1]);
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  gadget_mac<2, true>(D, A.l_key, A.b_key, A.gsw_keys[qi], sb, t, L, nlds[sb], acc0, acc1);
+  u32 o0[8], o1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    o0[c] = reduce_u64(acc0[c], L);
+    o1[c] = reduce_u64(acc1[c], L);
+  }
+  u32 *bot = selb + (size_t)(A.l_ep + r) * kCtWords;
+  store_map3(bot + (0 * 2 + sb) * kN, o0, t);
+  store_map3(bot + (1 * 2 + sb) * kN, o1, t);
+}
+
+// ---------------------------------------------------------------------------
+// First-dimension scan (server.cpp:313-361 / matrix.cpp:232-350), plain
+// path. For each limb and NTT level l:
+//   mid[i][p][limb][l] = sum_k db[limb][l][i][k] * qntt[k][p][limb][l] mod q
+// Workgroup: 256 rows x LV levels of one limb; query columns for those LV
+// levels are staged in LDS; the DB streams once from HBM with 16-byte loads.
+// u64 accumulators take 64 products (< 2^64 for 29-bit q) between reductions.
+// ---------------------------------------------------------------------------
+constexpr int kScanRows = 256;
+constexpr int kScanLdsWords = 8192;
+
+__global__ __launch_bounds__(256) void k_scan(const u32 *__restrict__ db, const u32 *__restrict__ qntt,
+                                              u32 *__restrict__ mid, int N0, int N0p, int Nrest, int LV,
+                                              int nrowblk, size_t q_stride, size_t mid_stride, SchemeConst S) {
+  __shared__ __align__(16) u32 bt[kScanLdsWords];  // [lv][N0p][2]
+  const int limb = blockIdx.z;
+  const LimbConst &L = S.limb[limb];
+  const int lvblk = blockIdx.x / nrowblk, rowblk = blockIdx.x % nrowblk;
+  const int lv0 = lvblk * LV;
+  const int qi = blockIdx.y;
+  const u32 *qn = qntt + qi * q_stride;
+  // stage B tile: bt[lv][k][p] = qntt[k][p][limb][lv0 + lv]
+  for (int e = threadIdx.x; e < LV * N0p * 2; e += 256) {
+    const int lv = e % LV, kp = e / LV;
+    const int k = kp >> 1, p = kp & 1;
+    bt[(lv * N0p + k) * 2 + p] = k < N0 ? qn[((size_t)k * 2 + p) * 2 * kN + (size_t)limb * kN + lv0 + lv] : 0;
+  }
+  __syncthreads();
+  const int row = rowblk * kScanRows + threadIdx.x;
+  if (row >= Nrest) return;
+  const size_t level_words = (size_t)Nrest * N0p;
+  const u32 *dbl = db + (size_t)limb * kN * level_words;
+  u32 *out = mid + qi * mid_stride + (size_t)row * kCtWords;
+  const int nkq = N0p >> 2;
+  for (int lv = 0; lv < LV; lv++) {
+    const uint4 *a4 = reinterpret_cast<const uint4 *>(dbl + (size_t)(lv0 + lv) * level_words) + row;
+    const uint4 *b4 = reinterpret_cast<const uint4 *>(bt + lv * N0p * 2);
+    u64 acc0 = 0, acc1 = 0;
+    int kq = 0;
+    while (kq < nkq) {
+      const int end = min(nkq, kq + 16);
+#pragma unroll 4
+      for (; kq < end; kq++) {
+        const uint4 a = a4[(size_t)kq * Nrest];
+        const uint4 b01 = b4[2 * kq], b23 = b4[2 * kq + 1];
+        acc0 += (u64)a.x * b01.x + (u64)a.y * b01.z;
+        acc1 += (u64)a.x * b01.y + (u64)a.y * b01.w;
+        acc0 += (u64)a.z * b23.x + (u64)a.w * b23.z;
+        acc1 += (u64)a.z * b23.y + (u64)a.w * b23.w;
+      }
+      acc0 = reduce_u64(acc0, L);
+      acc1 = reduce_u64(acc1, L);
+    }
+    out[(0 * 2 + limb) * kN + lv0 + lv] = (u32)acc0;
+    out[(1 * 2 + limb) * kN + lv0 + lv] = (u32)acc1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MUX level (server.cpp:599-722) on NTT-form candidates:
+//   res = x + EP(sel, INTT(y - x))     (EP output is NTT form)
+// which equals the reference's coefficient-form x + INTT(EP(sel, y - x)).
+// ---------------------------------------------------------------------------
+struct MuxArgs {
+  const u32 *in;
+  u32 *out;
+  size_t stride;       // words between queries (in and out)
+  const u32 *sel;      // this level's selector for query 0
+  size_t sel_stride;   // words between queries
+  int offset, half;    // offset >= 0: ragged level; else perfect level with `half`
+  int l, b;
+};
+
+__global__ __launch_bounds__(512) void k_mux(MuxArgs A, SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ u32 sh[2][2][kN];
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  const int s = blockIdx.x, qi = blockIdx.y;
+  const u32 *in = A.in + qi * A.stride;
+  u32 *dst = A.out + qi * A.stride + (size_t)s * kCtWords;
+  int xi, yi;
+  if (A.offset >= 0) {
+    if (s < A.offset) {  // unpaired candidate of the ragged level stays in place
+      const u32 *src = in + (size_t)s * kCtWords;
+      for (int e = threadIdx.x; e < (int)kCtWords / 4; e += 512)
+        reinterpret_cast<uint4 *>(dst)[e] = reinterpret_cast<const uint4 *>(src)[e];
+      return;
+    }
+    xi = A.offset + 2 * (s - A.offset);
+    yi = xi + 1;
+  } else {
+    xi = s;
+    yi = s + A.half;
+  }
+  const u32 *x = in + (size_t)xi * kCtWords, *y = in + (size_t)yi * kCtWords;
+  Barrier bar;
+  u32 xv[2][8];
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    u32 yv[8];
+    load_map3(xv[p], x + (p * 2 + sb) * kN, t);
+    load_map3(yv, y + (p * 2 + sb) * kN, t);
+#pragma unroll
+    for (int c = 0; c < 8; c++) yv[c] = sub_mod(yv[c], xv[p][c], L.q);
+    ntt_inv_2048(yv, nlds[sb], t, L, bar);
+    store_map0(sh[p][sb], yv, t);
+  }
+  __syncthreads();
+  i64 D[2][8];
+  compose_centre(sh[0][0], sh[0][1], t, S, D[0]);
+  compose_centre(sh[1][0], sh[1][1], t, S, D[1]);
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  gadget_mac<2, true>(D, A.l, A.b, A.sel + qi * A.sel_stride, sb, t, L, nlds[sb], acc0, acc1);
+  u32 o0[8], o1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    o0[c] = add_mod(xv[0][c], reduce_u64(acc0[c], L), L.q);
+    o1[c] = add_mod(xv[1][c], reduce_u64(acc1[c], L), L.q);
+  }
+  store_map3(dst + (0 * 2 + sb) * kN, o0, t);
+  store_map3(dst + (1 * 2 + sb) * kN, o1, t);
+}
+
+// ---------------------------------------------------------------------------
+// Final stage: INTT, CRT compose, centred rescale Q -> small_q
+// (server.cpp:1025-1043 with utils.cpp:445-462). grid (2 polys, B).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u64 rescale_to_small(u64 x, const SchemeConst &S) {
+  i64 v = (i64)x;
+  if (v >= (i64)(S.Q / 2)) v -= (i64)S.Q;
+  const bool neg = v < 0;
+  const u64 a = neg ? (u64)(-v) : (u64)v;
+  // floor((a * small_q + floor(Q/2)) / Q) by restoring division (quotient < 2^32)
+  unsigned __int128 num = (unsigned __int128)a * S.small_q + (S.Q >> 1);
+  u64 quo = 0;
+  for (int bit = 31; bit >= 0; bit--) {
+    const unsigned __int128 d = (unsigned __int128)S.Q << bit;
+    if (num >= d) {
+      num -= d;
+      quo |= (u64)1 << bit;
+    }
+  }
+  const i64 r = neg ? -(i64)quo : (i64)quo;
+  const i64 sq = (i64)S.small_q;
+  i64 m = r % sq;
+  if (m < 0) m += sq;
+  return (u64)m;
+}
+
+__global__ __launch_bounds__(512) void k_final(const u32 *res, u64 *out, size_t res_stride, SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ u32 sh[2][kN];
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  const int p = blockIdx.x, qi = blockIdx.y;
+  u32 v[8];
+  load_map3(v, res + qi * res_stride + (p * 2 + sb) * kN, t);
+  ntt_inv_2048(v, nlds[sb], t, L, Barrier());
+  store_map0(sh[sb], v, t);
+  __syncthreads();
+  u64 *o = out + ((size_t)qi * 2 + p) * kN;
+  for (int j = threadIdx.x; j < kN; j += 512) o[j] = rescale_to_small(crt_compose(sh[0][j], sh[1][j], S), S);
+}
+
+// NTT-form RNS cts -> coefficient-form u64 mod Q. grid (count*2).
+__global__ __launch_bounds__(512) void k_ct_to_coef_u64(const u32 *in, u64 *out, SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ u32 sh[2][kN];
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  const size_t poly = blockIdx.x;
+  u32 v[8];
+  load_map3(v, in + (poly * 2 + sb) * kN, t);
+  ntt_inv_2048(v, nlds[sb], t, L, Barrier());
+  store_map0(sh[sb], v, t);
+  __syncthreads();
+  for (int j = threadIdx.x; j < kN; j += 512) out[poly * kN + j] = crt_compose(sh[0][j], sh[1][j], S);
+}
+
+// Stand-alone external product (gsw.cpp:119-182): ct coef RNS -> out NTT RNS.
+__global__ __launch_bounds__(512) void k_ext_prod(const u32 *gsw, int l, int b, const u32 *ct, u32 *out,
+                                                  SchemeConst S) {
+  __shared__ u32 nlds[2][kLdsPoly];
+  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[sb];
+  i64 D[2][8];
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int j = map0(t, c);
+      D[p][c] = centre(crt_compose(ct[(p * 2 + 0) * kN + j], ct[(p * 2 + 1) * kN + j], S), S);
+    }
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  gadget_mac<2, true>(D, l, b, gsw, sb, t, L, nlds[sb], acc0, acc1);
+  u32 o0[8], o1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    o0[c] = reduce_u64(acc0[c], L);
+    o1[c] = reduce_u64(acc1[c], L);
+  }
+  store_map3(out + (0 * 2 + sb) * kN, o0, t);
+  store_map3(out + (1 * 2 + sb) * kN, o1, t);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+void launch_split_u64(const u64 *in, u32 *out, size_t npoly, const SchemeConst &S, hipStream_t st) {
+  k_split_u64<<<cdiv(npoly * kN, 256), 256, 0, st>>>(in, out, npoly, S);
+}
+void launch_join_u64(const u32 *in, u64 *out, size_t npoly, const SchemeConst &S, hipStream_t st) {
+  k_join_u64<<<cdiv(npoly * kN, 256), 256, 0, st>>>(in, out, npoly, S);
+}
+void launch_ntt_u64(u64 *data, size_t npoly, bool inverse, const SchemeConst &S, hipStream_t st) {
+  if (npoly) k_ntt_u64<<<(unsigned)npoly, 512, 0, st>>>(data, inverse ? 1 : 0, S);
+}
+void launch_ntt_rns(const u32 *in, u32 *out, size_t npoly_rns, bool inverse, const SchemeConst &S,
+                    hipStream_t st) {
+  if (npoly_rns) k_ntt_rns<<<(unsigned)npoly_rns, 512, 0, st>>>(in, out, inverse ? 1 : 0, S);
+}
+void launch_db_ntt_plain(const uint16_t *pts, u32 *tmp, size_t count, const SchemeConst &S, hipStream_t st) {
+  if (count) k_db_ntt_plain<<<(unsigned)count, 512, 0, st>>>(pts, tmp, S);
+}
+void launch_db_scatter(const u32 *tmp, u32 *db, size_t first_pt, size_t count, size_t N0, size_t N0p,
+                       size_t Nrest, hipStream_t st) {
+  if (!count) return;
+  dim3 grid(cdiv(count, 64), kN / 32, 2);
+  k_db_scatter<<<grid, 256, 0, st>>>(tmp, db, first_pt, count, N0, N0p, Nrest);
+}
+void launch_db_from_ref(const u32 *ref, u32 *db, size_t N0, size_t N0p, size_t Nrest, hipStream_t st) {
+  const size_t total = 2 * (size_t)kN * N0 * Nrest;
+  k_db_from_ref<<<cdiv(total, 256), 256, 0, st>>>(ref, db, N0, N0p, Nrest);
+}
+
+void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int nnodes, int B,
+                         int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
+  ExpandArgs A;
+  A.in = in; A.out = out; A.in_stride = in_stride; A.out_stride = out_stride;
+  A.bv_keys = P.bv_keys;
+  A.key_level_words = (size_t)G.l_ks * kCtWords;
+  A.level = level;
+  const unsigned gk = (unsigned)(kN >> level) + 1;
+  unsigned inv = 1;  // galois^-1 mod 2N by exponentiation: odd g has order dividing N
+  for (int e = 0; e < kN - 1; e++) inv = (inv * gk) & (2 * kN - 1);
+  A.kinv = (int)inv;
+  A.l_ks = G.l_ks; A.b_ks = G.b_ks;
+  dim3 grid(nnodes, B);
+  k_expand_level<<<grid, 512, 0, st>>>(A, S);
+}
+
+void launch_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int count, int B,
+                   const SchemeConst &S, hipStream_t st) {
+  if (count <= 0) return;
+  dim3 grid(count * 2, B);
+  k_ct_ntt<<<grid, 512, 0, st>>>(in, out, in_stride, out_stride, S);
+}
+
+void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
+                     const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
+  if (nrows <= 0) return;
+  CompleteArgs A;
+  A.leaves = leaves; A.sel = sel; A.leaf_stride = leaf_stride; A.sel_stride = sel_stride;
+  A.gsw_keys = P.gsw_keys; A.l_ep = G.l_ep; A.l_key = G.l_key; A.b_key = G.b_key;
+  dim3 grid(nrows, B);
+  k_complete<<<grid, 512, 0, st>>>(A, S);
+}
+
+void launch_scan(const u32 *db, const u32 *qntt, u32 *mid, size_t q_stride, size_t mid_stride, int B,
+                 size_t N0, size_t N0p, size_t Nrest, const SchemeConst &S, hipStream_t st) {
+  int LV = (int)(kScanLdsWords / (2 * N0p));
+  if (LV > 16) LV = 16;
+  if (LV < 1) LV = 1;
+  while (kN % LV) LV--;
+  const int nrowblk = (int)cdiv(Nrest, kScanRows);
+  dim3 grid((kN / LV) * nrowblk, B, 2);
+  k_scan<<<grid, 256, 0, st>>>(db, qntt, mid, (int)N0, (int)N0p, (int)Nrest, LV, nrowblk, q_stride, mid_stride,
+                               S);
+}
+
+void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
+                const u32 *sel, size_t sel_stride, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
+  if (nslots <= 0) return;
+  MuxArgs A;
+  A.in = in; A.out = out; A.stride = stride; A.sel = sel; A.sel_stride = sel_stride;
+  A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
+  dim3 grid(nslots, B);
+  k_mux<<<grid, 512, 0, st>>>(A, S);
+}
+
+void launch_final(const u32 *res, u64 *out, size_t res_stride, int B, const SchemeConst &S, hipStream_t st) {
+  dim3 grid(2, B);
+  k_final<<<grid, 512, 0, st>>>(res, out, res_stride, S);
+}
+
+void launch_ct_to_coef_u64(const u32 *in, u64 *out, size_t count, const SchemeConst &S, hipStream_t st) {
+  if (count) k_ct_to_coef_u64<<<(unsigned)(count * 2), 512, 0, st>>>(in, out, S);
+}
+
+void launch_ext_prod(const u32 *gsw, int l, int b, const u32 *ct_coef, u32 *out_ntt, const SchemeConst &S,
+                     hipStream_t st) {
+  k_ext_prod<<<1, 512, 0, st>>>(gsw, l, b, ct_coef, out_ntt, S);
+}
+
+
+}  // namespace opir

@@ -1,0 +1,275 @@
+"""ctypes binding of the C ABI in include/onionpir_gpu.h.
+
+This is the boundary the GPU parity tests and bench.py call. Loading fails
+loudly when libonionpir_gpu.so has not been built: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libonionpir_gpu.so")
+
+OPIR_OK = 0
+OPIR_EINVAL = -1
+OPIR_ERANGE = -2
+OPIR_ERUNTIME = -3
+OPIR_EHIP = -4
+OPIR_ENOMEM = -5
+
+# Every symbol include/onionpir_gpu.h declares.
+EXPORTED = [
+    "opir_ctx_create", "opir_ctx_destroy", "opir_last_error", "opir_ctx_set_stream", "opir_ctx_synchronize",
+    "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt",
+    "opir_client_set_keys", "opir_client_remove",
+    "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
+    "opir_ctx_set_profiling", "opir_last_stage_ms",
+    "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
+    "opir_ext_prod", "opir_mod_switch",
+]
+
+
+class InvalidArgument(ValueError):
+    """OPIR_EINVAL: the reference throws std::invalid_argument."""
+
+
+class OutOfRange(IndexError):
+    """OPIR_ERANGE: the reference throws std::out_of_range."""
+
+
+class OpirRuntimeError(RuntimeError):
+    """Any other failure (std::runtime_error in the reference)."""
+
+
+class OpirScheme(ctypes.Structure):
+    _fields_ = [
+        ("poly_degree", ctypes.c_uint32), ("composite", ctypes.c_uint32),
+        ("q1", ctypes.c_uint64), ("q2", ctypes.c_uint64), ("ntt_root", ctypes.c_uint64),
+        ("plain_mod", ctypes.c_uint64), ("small_q", ctypes.c_uint64),
+        ("l_ep", ctypes.c_uint32), ("base_log2", ctypes.c_uint32),
+        ("l_key", ctypes.c_uint32), ("base_log2_key", ctypes.c_uint32),
+        ("l_ks", ctypes.c_uint32), ("bv_base_log2", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def load_library():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(f"{_LIB_PATH} is not built; run `python -m onionpirv2_amd.build gpu`")
+    lib = ctypes.CDLL(_LIB_PATH)
+    P = ctypes.c_void_p
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    sz = ctypes.c_size_t
+    sig = {
+        "opir_ctx_create": ([ctypes.POINTER(OpirScheme), ctypes.c_int, ctypes.POINTER(P)], ctypes.c_int),
+        "opir_ctx_destroy": ([P], None),
+        "opir_last_error": ([P], ctypes.c_char_p),
+        "opir_ctx_set_stream": ([P, P], ctypes.c_int),
+        "opir_ctx_synchronize": ([P], ctypes.c_int),
+        "opir_db_create": ([P, sz, sz, sz, sz, ctypes.POINTER(P)], ctypes.c_int),
+        "opir_db_destroy": ([P], None),
+        "opir_db_load_plain": ([P, P, sz, sz], ctypes.c_int),
+        "opir_db_load_ntt": ([P, P, P], ctypes.c_int),
+        "opir_client_set_keys": ([P, ctypes.c_uint64, P, sz, P], ctypes.c_int),
+        "opir_client_remove": ([P, ctypes.c_uint64], ctypes.c_int),
+        "opir_answer": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
+        "opir_answer_batch": ([P, sz, P, P, P], ctypes.c_int),
+        "opir_answer_batch_device": ([P, sz, P, P, P], ctypes.c_int),
+        "opir_ctx_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
+        "opir_last_stage_ms": ([P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "opir_ntt_fwd": ([P, P, sz], ctypes.c_int),
+        "opir_ntt_inv": ([P, P, sz], ctypes.c_int),
+        "opir_expand": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
+        "opir_complete": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
+        "opir_first_dim": ([P, P, P], ctypes.c_int),
+        "opir_other_dim": ([P, P, P, P], ctypes.c_int),
+        "opir_ext_prod": ([P, P, sz, sz, P, P], ctypes.c_int),
+        "opir_mod_switch": ([P, P, P], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    del u64p
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _u64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    if shape is not None and a.shape != tuple(shape):
+        a = a.reshape(shape)
+    return a
+
+
+class GpuContext:
+    """One opir_ctx: scheme tables, client keys and a HIP stream on one device."""
+
+    def __init__(self, scheme: dict, device: int = 0):
+        self.lib = load_library()
+        s = OpirScheme(**scheme)
+        self.scheme = dict(scheme)
+        self.N = scheme["poly_degree"]
+        h = ctypes.c_void_p()
+        rc = self.lib.opir_ctx_create(ctypes.byref(s), device, ctypes.byref(h))
+        if rc != OPIR_OK:
+            self._raise(rc, self.lib.opir_last_error(None))
+        self.h = h
+
+    def _raise(self, rc, msg):
+        msg = msg.decode() if isinstance(msg, bytes) else str(msg)
+        if rc == OPIR_EINVAL:
+            raise InvalidArgument(msg)
+        if rc == OPIR_ERANGE:
+            raise OutOfRange(msg)
+        raise OpirRuntimeError(f"opir error {rc}: {msg}")
+
+    def check(self, rc):
+        if rc != OPIR_OK:
+            self._raise(rc, self.lib.opir_last_error(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.opir_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, hip_stream_ptr):
+        self.check(self.lib.opir_ctx_set_stream(self.h, ctypes.c_void_p(hip_stream_ptr or 0)))
+
+    def synchronize(self):
+        self.check(self.lib.opir_ctx_synchronize(self.h))
+
+    def set_profiling(self, on=True):
+        self.check(self.lib.opir_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def last_stage_ms(self):
+        out = (ctypes.c_float * 6)()
+        self.check(self.lib.opir_last_stage_ms(self.h, out))
+        names = ["Expand", "Convert", "First dim", "Other dim", "Modulus switching", "scan kernel"]
+        return dict(zip(names, [float(x) for x in out]))
+
+    def set_client_keys(self, client_id, bv, gsw):
+        bv = _u64(bv)
+        gsw = _u64(gsw)
+        self.check(self.lib.opir_client_set_keys(self.h, client_id, _ptr(bv), bv.shape[0], _ptr(gsw)))
+
+    def remove_client(self, client_id):
+        self.check(self.lib.opir_client_remove(self.h, client_id))
+
+    def ntt_fwd(self, polys):
+        a = _u64(polys).copy()
+        self.check(self.lib.opir_ntt_fwd(self.h, _ptr(a), a.size // self.N))
+        return a
+
+    def ntt_inv(self, polys):
+        a = _u64(polys).copy()
+        self.check(self.lib.opir_ntt_inv(self.h, _ptr(a), a.size // self.N))
+        return a
+
+    def ext_prod(self, gsw, l, base_log2, ct):
+        g = _u64(gsw)
+        c = _u64(ct)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        self.check(self.lib.opir_ext_prod(self.h, _ptr(g), l, base_log2, _ptr(c), _ptr(out)))
+        return out
+
+    def mod_switch(self, ct):
+        c = _u64(ct)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        self.check(self.lib.opir_mod_switch(self.h, _ptr(c), _ptr(out)))
+        return out
+
+
+class GpuDatabase:
+    """One opir_db: a resident NTT-form database of one layout."""
+
+    def __init__(self, ctx: GpuContext, fst_dim_sz, other_dim_sz, num_dims, expansion_height):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.N = ctx.N
+        self.N0, self.Nrest, self.dims, self.h = fst_dim_sz, other_dim_sz, num_dims, expansion_height
+        self.u = fst_dim_sz + ctx.scheme["l_ep"] * (num_dims - 1)
+        h = ctypes.c_void_p()
+        ctx.check(self.lib.opir_db_create(ctx.h, fst_dim_sz, other_dim_sz, num_dims, expansion_height,
+                                          ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.opir_db_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_plain(self, pts, first_pt=0):
+        a = np.ascontiguousarray(pts, dtype=np.uint16)
+        self.ctx.check(self.lib.opir_db_load_plain(self.h, _ptr(a), first_pt, a.size // self.N))
+
+    def load_ntt(self, lo, hi):
+        lo = np.ascontiguousarray(lo, dtype=np.uint32)
+        hi = np.ascontiguousarray(hi, dtype=np.uint32)
+        self.ctx.check(self.lib.opir_db_load_ntt(self.h, _ptr(lo), _ptr(hi)))
+
+    def answer(self, client_id, query):
+        q = _u64(query)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        self.ctx.check(self.lib.opir_answer(self.h, client_id, _ptr(q), _ptr(out)))
+        return out
+
+    def answer_batch(self, client_ids, queries):
+        ids = np.ascontiguousarray(client_ids, dtype=np.uint64)
+        q = _u64(queries)
+        out = np.zeros((len(ids), 2, self.N), dtype=np.uint64)
+        self.ctx.check(self.lib.opir_answer_batch(self.h, len(ids), _ptr(ids), _ptr(q), _ptr(out)))
+        return out
+
+    def answer_batch_device(self, client_ids, d_queries_ptr, d_resps_ptr):
+        ids = np.ascontiguousarray(client_ids, dtype=np.uint64)
+        self.ctx.check(self.lib.opir_answer_batch_device(self.h, len(ids), _ptr(ids),
+                                                         ctypes.c_void_p(d_queries_ptr),
+                                                         ctypes.c_void_p(d_resps_ptr)))
+
+    def expand(self, client_id, query):
+        q = _u64(query)
+        out = np.zeros((self.u, 2, self.N), dtype=np.uint64)
+        self.ctx.check(self.lib.opir_expand(self.h, client_id, _ptr(q), _ptr(out)))
+        return out
+
+    def complete(self, client_id, expanded):
+        e = _u64(expanded)
+        ns = self.dims - 1
+        out = np.zeros((ns, 2 * self.ctx.scheme["l_ep"], 2 * self.N), dtype=np.uint64)
+        self.ctx.check(self.lib.opir_complete(self.h, client_id, _ptr(e), _ptr(out)))
+        return out
+
+    def first_dim(self, query_cts):
+        q = _u64(query_cts)
+        out = np.zeros((self.Nrest, 2, self.N), dtype=np.uint64)
+        self.ctx.check(self.lib.opir_first_dim(self.h, _ptr(q), _ptr(out)))
+        return out
+
+    def other_dim(self, mid, sel):
+        m = _u64(mid)
+        s = _u64(sel) if sel is not None else np.zeros(1, dtype=np.uint64)
+        out = np.zeros((2, self.N), dtype=np.uint64)
+        self.ctx.check(self.lib.opir_other_dim(self.h, _ptr(m), _ptr(s), _ptr(out)))
+        return out

@@ -1,0 +1,190 @@
+"""GPU parity: every stage of the HIP answer path against the CPU oracle,
+bit-exact, through the C ABI (onionpirv2_amd/gpu.py -> libonionpir_gpu.so).
+
+Layouts are small enough for the oracle to finish in seconds; each covers a
+shape feature of the reference's evaluate_other_dim (src/server.cpp:599-649):
+  (43, 5)        N0=16,  dims=3, Nrest=3   ragged deepest level (offset 1)
+  (700, 8)       N0=128, dims=4, Nrest=6   ragged with offset 2
+  (4096, 9)      N0=256, dims=5, Nrest=16  cfg0-like, perfect tree
+  (43, 5, False) N0=26,  dims=2, Nrest=2   non-power-of-two first dimension
+  (1, 0)         singleton: no expansion, no selectors
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import scheme_from_oracle
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = [(43, 5, True), (700, 8, True), (4096, 9, True), (43, 5, False), (1, 0, True)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gpu_ctx(comp_params):
+    from onionpirv2_amd.gpu import GpuContext
+    say("[gpu] creating context")
+    ctx = GpuContext(scheme_from_oracle(comp_params))
+    yield ctx
+    ctx.close()
+
+
+class Setup:
+    """Oracle server+client and a GPU database holding the same plaintexts."""
+
+    def __init__(self, O, base, layout, seed, gpu_ctx, cid=11):
+        self.p = base.with_layout(*layout)
+        p = self.p
+        self.cid = cid
+        self.client = O.PirClient(p, cid, seed)
+        self.bv = self.client.create_bv_galois_keys()
+        self.gsw = self.client.generate_gsw_from_key()
+        self.server = O.PirServer(p)
+        logical = p.target_num_pt
+        self.indices = sorted({0, logical - 1, (seed * 7919) % logical})
+        self.server.load_data(logical, "random", seed, self.indices)
+        self.server.set_client_keys(cid, self.bv, self.gsw)
+        from onionpirv2_amd.gpu import GpuDatabase
+        self.db = GpuDatabase(gpu_ctx, p.fst_dim_sz, p.other_dim_sz, p.num_dims, p.expansion_height)
+        draws = O.mt19937_64_draws(seed, logical * p.N)
+        pts = (draws % np.uint64(p.plain_mod)).astype(np.uint16).reshape(logical, p.N)
+        self.db.load_plain(pts)
+        gpu_ctx.set_client_keys(cid, self.bv, self.gsw)
+
+
+@pytest.mark.parametrize("nq", [1, 7])
+def test_ntt_fwd_inv_match_oracle(O, comp_params, gpu_ctx, nq):
+    say(f"[gpu] ntt fwd/inv nq={nq}")
+    Q = comp_params.rns_mods[0]
+    rng = np.random.default_rng(100 + nq)
+    polys = rng.integers(0, Q, size=(nq, comp_params.N), dtype=np.uint64)
+    exp = O.ntt_fwd(polys.ravel(), comp_params.N, Q).reshape(nq, -1)
+    got = gpu_ctx.ntt_fwd(polys)
+    assert np.array_equal(got, exp)
+    back = gpu_ctx.ntt_inv(got)
+    assert np.array_equal(back, polys)
+    assert np.array_equal(gpu_ctx.ntt_inv(polys), O.ntt_inv(polys.ravel(), comp_params.N, Q).reshape(nq, -1))
+
+
+def test_ntt_kat_product(comp_params, gpu_ctx):
+    """tests/test_hexl_ntt.cpp:41-59: (1+2x+3x^2)(4+5x) = 4+13x+22x^2+15x^3."""
+    say("[gpu] ntt product KAT")
+    N, Q = comp_params.N, comp_params.rns_mods[0]
+    a = np.zeros(N, np.uint64); a[:3] = [1, 2, 3]
+    b = np.zeros(N, np.uint64); b[:2] = [4, 5]
+    fa, fb = gpu_ctx.ntt_fwd(a), gpu_ctx.ntt_fwd(b)
+    prod = np.array([(int(x) * int(y)) % Q for x, y in zip(fa.ravel(), fb.ravel())], dtype=np.uint64)
+    c = gpu_ctx.ntt_inv(prod).ravel()
+    assert list(c[:5]) == [4, 13, 22, 15, 0]
+
+
+@pytest.mark.parametrize("which", ["key", "data"])
+def test_external_product_matches_oracle(O, comp_params, gpu_ctx, which):
+    say(f"[gpu] external product ({which} gadget)")
+    p = comp_params
+    l, b = (p.l_key, p.base_log2_key) if which == "key" else (p.l_ep, p.base_log2)
+    rng = np.random.default_rng(7)
+    sk, _ = O.gen_sk_and_encrypt(p, np.zeros(p.N, np.uint64), 5)
+    one = np.zeros(p.N, np.uint64); one[0] = 1
+    gsw = O.plain_to_gsw(p, one, sk, l, b, 9)
+    ct = rng.integers(0, p.rns_mods[0], size=(2, p.N), dtype=np.uint64)
+    exp = O.external_product(p, gsw, ct, l, b)
+    got = gpu_ctx.ext_prod(gsw, l, b, ct)
+    assert np.array_equal(got, exp)
+
+
+def test_mod_switch_matches_oracle(O, comp_params, gpu_ctx):
+    say("[gpu] mod switch")
+    p = comp_params
+    rng = np.random.default_rng(3)
+    ct = rng.integers(0, p.rns_mods[0], size=(2, p.N), dtype=np.uint64)
+    # include the centring boundary values
+    Q = p.rns_mods[0]
+    ct[0, :6] = [0, 1, Q // 2 - 1, Q // 2, Q // 2 + 1, Q - 1]
+    srv = O.PirServer(p.with_layout(1, 0))
+    assert np.array_equal(gpu_ctx.mod_switch(ct), srv.mod_switch(ct))
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: "x".join(map(str, l)))
+def test_stages_match_oracle(O, comp_params, gpu_ctx, layout):
+    say(f"[gpu] stages layout={layout}")
+    s = Setup(O, comp_params, layout, seed=1000 + layout[0], gpu_ctx=gpu_ctx)
+    p = s.p
+    idx = s.indices[-1]
+    query = s.client.fast_generate_query(p, idx)
+    say("  expand")
+    exp_o = s.server.fast_expand_qry(s.cid, query)
+    exp_g = s.db.expand(s.cid, query)
+    assert np.array_equal(exp_g, exp_o)
+    if p.num_dims > 1:
+        say("  complete")
+        sel_o = s.server.complete_selectors(s.cid, exp_o)
+        sel_g = s.db.complete(s.cid, exp_o)
+        assert np.array_equal(sel_g, sel_o)
+    else:
+        sel_o = None
+    say("  first dim")
+    fq = exp_o[: p.fst_dim_sz]
+    mid_o = s.server.evaluate_first_dim(fq)
+    mid_g = s.db.first_dim(fq)
+    assert np.array_equal(mid_g, mid_o)
+    say("  other dim")
+    if p.num_dims > 1:
+        res_o = s.server.evaluate_other_dim(mid_o, sel_o)
+        res_g = s.db.other_dim(mid_o, sel_o)
+        assert np.array_equal(res_g, res_o)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: "x".join(map(str, l)))
+def test_answer_bit_exact_and_decrypts(O, comp_params, gpu_ctx, layout):
+    say(f"[gpu] answer layout={layout}")
+    s = Setup(O, comp_params, layout, seed=2000 + layout[0], gpu_ctx=gpu_ctx)
+    p = s.p
+    for idx in s.indices:
+        query = s.client.fast_generate_query(p, idx)
+        resp_o = s.server.make_query(s.cid, query)
+        resp_g = s.db.answer(s.cid, query)
+        assert np.array_equal(resp_g, resp_o), f"index {idx}"
+        wire = s.server.save_resp_to_stream(resp_g)
+        assert len(wire) == 11264
+        pt, budget = s.client.decrypt_mod_q(s.client.load_resp_from_stream(wire))
+        assert np.array_equal(pt, s.server.direct_get_original_plaintext(idx))
+        assert budget > 0
+
+
+def test_batch_equals_single(O, comp_params, gpu_ctx):
+    """Each query's response is the same alone and inside a batch."""
+    say("[gpu] batch vs single")
+    s = Setup(O, comp_params, (4096, 9, True), seed=77, gpu_ctx=gpu_ctx, cid=21)
+    other = O.PirClient(s.p, 22, 78)
+    gpu_ctx.set_client_keys(22, other.create_bv_galois_keys(), other.generate_gsw_from_key())
+    qs, ids = [], []
+    for j, idx in enumerate([5, 4095, 777, 3000, 5]):
+        c = s.client if j % 2 == 0 else other
+        qs.append(c.fast_generate_query(s.p, idx))
+        ids.append(c.client_id)
+    batch = s.db.answer_batch(ids, np.stack(qs))
+    for j in range(len(qs)):
+        assert np.array_equal(batch[j], s.db.answer(ids[j], qs[j]))
+    assert np.array_equal(batch[0], s.server.make_query(21, qs[0]))
+
+
+def test_errors_map_to_reference_exceptions(comp_params, gpu_ctx):
+    from onionpirv2_amd.gpu import GpuDatabase, InvalidArgument, OutOfRange
+    say("[gpu] error paths")
+    p = comp_params.with_layout(43, 5)
+    db = GpuDatabase(gpu_ctx, p.fst_dim_sz, p.other_dim_sz, p.num_dims, p.expansion_height)
+    q = np.zeros((2, p.N), np.uint64)
+    with pytest.raises(OutOfRange):
+        db.answer(987654, q)
+    bad = np.full((2, p.N), p.rns_mods[0], np.uint64)
+    with pytest.raises(InvalidArgument):
+        db.answer(987654, bad)
+    with pytest.raises(InvalidArgument):
+        db.load_plain(np.zeros((p.num_pt + 1, p.N), np.uint16))
