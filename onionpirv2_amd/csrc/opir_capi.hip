@@ -171,6 +171,7 @@ size_t nodes_at_level(size_t u, size_t h, size_t L) {
 int ensure_ws(opir_db *db, size_t B) {
   opir_ctx *ctx = db->ctx;
   if (db->ws_B >= B) return OPIR_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // queued work may still use the old buffers
   u32 **bufs[] = {&db->exp_a, &db->exp_b, &db->qntt, &db->sel, &db->mid_a, &db->mid_b};
   for (auto b : bufs) {
     if (*b) (void)hipFree(*b);

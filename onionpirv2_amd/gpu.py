@@ -273,3 +273,21 @@ class GpuDatabase:
         out = np.zeros((2, self.N), dtype=np.uint64)
         self.ctx.check(self.lib.opir_other_dim(self.h, _ptr(m), _ptr(s), _ptr(out)))
         return out
+
+
+def answer_batch_device_raw(db_handle, client_ids, d_queries_ptr, d_resps_ptr):
+    """opir_answer_batch_device on a database handle owned by the host API."""
+    lib = load_library()
+    ids = np.ascontiguousarray(client_ids, dtype=np.uint64)
+    rc = lib.opir_answer_batch_device(ctypes.c_void_p(db_handle), len(ids), _ptr(ids),
+                                      ctypes.c_void_p(d_queries_ptr), ctypes.c_void_p(d_resps_ptr))
+    return rc
+
+
+def ctx_call_raw(name, ctx_handle, *args):
+    lib = load_library()
+    return getattr(lib, name)(ctypes.c_void_p(ctx_handle), *args)
+
+
+def last_error_raw(ctx_handle):
+    return load_library().opir_last_error(ctypes.c_void_p(ctx_handle)).decode()

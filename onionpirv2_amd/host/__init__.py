@@ -1,0 +1,15 @@
+"""Host API (C++ via pybind11): PirParams, PirClient, PirServer, GpuScheme.
+
+Importing fails loudly if the native module has not been built
+(`python -m onionpirv2_amd.build`).
+"""
+import os as _os
+import sys as _sys
+
+_LIB = _os.path.join(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__))), "lib")
+if _LIB not in _sys.path:
+    _sys.path.insert(0, _LIB)
+try:
+    from _opir_host import GpuScheme, PirClient, PirParams, PirServer  # noqa: F401
+except ImportError as e:  # pragma: no cover
+    raise ImportError(f"onionpirv2_amd native host module is not built in {_LIB}: {e}") from e

@@ -1,0 +1,227 @@
+// pybind11 module `_opir_host`: Python view of the host C++ API
+// (PirParams / PirClient / PirServer) with ciphertexts as numpy arrays
+// [2, N] (c0, c1), RGSW as [2l, 2N] and BV keys as [h, l_ks, 2 (a, b), N].
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstring>
+
+#include "onionpir_host.h"
+
+namespace py = pybind11;
+using namespace onionpir;
+using Arr = py::array_t<uint64_t, py::array::c_style | py::array::forcecast>;
+
+static Arr ct_arr(const RlweCt &ct) {
+  const size_t n = ct.c0.size();
+  Arr a({(py::ssize_t)2, (py::ssize_t)n});
+  std::memcpy(a.mutable_data(), ct.c0.data(), n * 8);
+  std::memcpy(a.mutable_data() + n, ct.c1.data(), n * 8);
+  return a;
+}
+static RlweCt arr_ct(const Arr &a) {
+  if (a.ndim() != 2 || a.shape(0) != 2) throw std::invalid_argument("ciphertext array must be [2, N]");
+  const size_t n = a.shape(1);
+  RlweCt ct;
+  ct.c0.assign(a.data(), a.data() + n);
+  ct.c1.assign(a.data() + n, a.data() + 2 * n);
+  return ct;
+}
+static Arr gsw_arr(const GSWCt &g) {
+  const size_t r = g.size(), w = r ? g[0].size() : 0;
+  Arr a({(py::ssize_t)r, (py::ssize_t)w});
+  for (size_t i = 0; i < r; i++) std::memcpy(a.mutable_data() + i * w, g[i].data(), w * 8);
+  return a;
+}
+static GSWCt arr_gsw(const Arr &a) {
+  GSWCt g(a.shape(0));
+  const size_t w = a.shape(1);
+  for (size_t i = 0; i < g.size(); i++) g[i].assign(a.data() + i * w, a.data() + (i + 1) * w);
+  return g;
+}
+static Arr bv_arr(const bvks::BvGaloisKeys &k) {
+  const size_t h = k.keys.size(), L = h ? k.keys[0].cts.size() : 0, N = (h && L) ? k.keys[0].cts[0].a.size() : 0;
+  Arr a({(py::ssize_t)h, (py::ssize_t)L, (py::ssize_t)2, (py::ssize_t)N});
+  for (size_t i = 0; i < h; i++)
+    for (size_t r = 0; r < L; r++) {
+      std::memcpy(a.mutable_data() + ((i * L + r) * 2 + 0) * N, k.keys[i].cts[r].a.data(), N * 8);
+      std::memcpy(a.mutable_data() + ((i * L + r) * 2 + 1) * N, k.keys[i].cts[r].b.data(), N * 8);
+    }
+  return a;
+}
+static bvks::BvGaloisKeys arr_bv(const Arr &a, size_t N) {
+  bvks::BvGaloisKeys k;
+  if (a.ndim() != 4) throw std::invalid_argument("BV keys must be [h, l_ks, 2, N]");
+  const size_t h = a.shape(0), L = a.shape(1), n = a.shape(3);
+  for (size_t i = 0; i < h; i++) {
+    bvks::BvKeySwitchKey key;
+    key.galois_k = (uint32_t)((N >> i) + 1);
+    key.cts.resize(L);
+    for (size_t r = 0; r < L; r++) {
+      key.cts[r].a.assign(a.data() + ((i * L + r) * 2) * n, a.data() + ((i * L + r) * 2 + 1) * n);
+      key.cts[r].b.assign(a.data() + ((i * L + r) * 2 + 1) * n, a.data() + ((i * L + r) * 2 + 2) * n);
+    }
+    k.keys.push_back(std::move(key));
+  }
+  return k;
+}
+
+PYBIND11_MODULE(_opir_host, m) {
+  m.doc() = "Host API of the MI355X OnionPIRv2 server (PirParams, PirClient, PirServer)";
+  py::register_exception_translator([](std::exception_ptr p) {
+    try {
+      if (p) std::rethrow_exception(p);
+    } catch (const std::out_of_range &e) {
+      PyErr_SetString(PyExc_IndexError, e.what());
+    }
+  });
+
+  py::class_<PirParams>(m, "PirParams")
+      .def(py::init([](const std::string &name) { return PirParams(SchemeConfig::by_name(name)); }),
+           py::arg("config") = "CONFIG_N2048_K1_COMP")
+      .def("with_layout",
+           [](const PirParams &p, size_t t, size_t h, bool pow2) { return p.with_layout({t, h, pow2}); },
+           py::arg("target_num_pt"), py::arg("expansion_height"), py::arg("fst_dim_pow2") = true)
+      .def("with_query_shape", &PirParams::with_query_shape)
+      .def("scheme_compatible", &PirParams::scheme_compatible)
+      .def_property_readonly("N", &PirParams::get_poly_degree)
+      .def_property_readonly("rns_mods", &PirParams::get_rns_mods)
+      .def_property_readonly("q1", &PirParams::q1)
+      .def_property_readonly("q2", &PirParams::q2)
+      .def_property_readonly("ntt_root", &PirParams::ntt_root)
+      .def_property_readonly("plain_mod", &PirParams::get_plain_mod)
+      .def_property_readonly("small_q", &PirParams::get_small_q)
+      .def_property_readonly("base_log2", &PirParams::get_base_log2)
+      .def_property_readonly("base_log2_key", &PirParams::get_base_log2_key)
+      .def_property_readonly("bv_base_log2", &PirParams::get_bv_base_log2)
+      .def_property_readonly("l_ep", &PirParams::get_l)
+      .def_property_readonly("l_key", &PirParams::get_l_key)
+      .def_property_readonly("l_ks", &PirParams::get_l_ks)
+      .def_property_readonly("num_pt", &PirParams::get_num_pt)
+      .def_property_readonly("target_num_pt", &PirParams::get_target_num_pt)
+      .def_property_readonly("fst_dim_sz", &PirParams::get_fst_dim_sz)
+      .def_property_readonly("other_dim_sz", &PirParams::get_other_dim_sz)
+      .def_property_readonly("num_dims", &PirParams::get_num_dims)
+      .def_property_readonly("expansion_height", &PirParams::get_expan_height)
+      .def_property_readonly("pt_size", &PirParams::get_pt_size)
+      .def_property_readonly("db_size_mb", &PirParams::get_DBSize_MB)
+      .def_property_readonly("physical_storage_mb", &PirParams::get_physical_storage_MB)
+      .def("get_BFV_size", &PirParams::get_BFV_size, py::arg("use_seed") = true)
+      .def("get_gsw_key_size", &PirParams::get_gsw_key_size, py::arg("use_seed") = true)
+      .def("get_bv_galois_key_size", &PirParams::get_bv_galois_key_size, py::arg("use_seed") = true);
+
+  py::class_<PirClient>(m, "PirClient")
+      .def(py::init<const PirParams &, size_t, uint64_t>(), py::arg("params"), py::arg("client_id"), py::arg("seed"))
+      .def_property_readonly("client_id", &PirClient::get_client_id)
+      .def("secret_key", [](const PirClient &c) {
+        const auto &d = c.secret_key().data;
+        Arr a((py::ssize_t)d.size());
+        std::memcpy(a.mutable_data(), d.data(), d.size() * 8);
+        return a;
+      })
+      .def("get_query_indices", &PirClient::get_query_indices)
+      .def("fast_generate_query", [](PirClient &c, const PirParams &p, size_t i) { return ct_arr(c.fast_generate_query(p, i)); })
+      .def("create_bv_galois_keys", [](PirClient &c) { return bv_arr(c.create_bv_galois_keys()); })
+      .def("generate_gsw_from_key", [](PirClient &c) { return gsw_arr(c.generate_gsw_from_key()); })
+      .def("load_resp_from_stream",
+           [](const PirClient &c, py::bytes b) {
+             std::stringstream ss(std::string(b), std::ios::in | std::ios::binary);
+             return ct_arr(c.load_resp_from_stream(ss));
+           })
+      .def("decrypt_mod_q", [](const PirClient &c, Arr ct) {
+        int budget = 0;
+        RlwePt pt = c.decrypt_mod_q(arr_ct(ct), &budget);
+        Arr a((py::ssize_t)pt.data.size());
+        std::memcpy(a.mutable_data(), pt.data.data(), pt.data.size() * 8);
+        return py::make_tuple(a, budget);
+      });
+
+  py::class_<GpuScheme, std::shared_ptr<GpuScheme>>(m, "GpuScheme")
+      .def(py::init<const PirParams &, int>(), py::arg("params"), py::arg("device") = 0)
+      .def("ctx_handle", [](const GpuScheme &s) { return (uintptr_t)s.ctx(); });
+
+  py::class_<PirServer>(m, "PirServer")
+      .def(py::init<const PirParams &, int>(), py::arg("params"), py::arg("device") = 0)
+      .def(py::init<const PirParams &, std::shared_ptr<GpuScheme>>(), py::arg("params"), py::arg("scheme"))
+      .def("gen_data", &PirServer::gen_data, py::arg("record_indices") = std::vector<size_t>{}, py::arg("seed") = 0,
+           py::call_guard<py::gil_scoped_release>())
+      .def("load_data",
+           [](PirServer &s, size_t logical, py::function src, std::vector<size_t> rec) {
+             const size_t N = s.get_params().get_poly_degree();
+             s.load_data(logical, [&](size_t i, RlwePt &out) {
+               Arr a = src(i).cast<Arr>();
+               out.data.assign(a.data(), a.data() + a.size());
+               (void)N;
+             }, rec);
+           },
+           py::arg("logical_num_pt"), py::arg("source"), py::arg("record_indices") = std::vector<size_t>{})
+      .def("load_data_array",
+           [](PirServer &s, Arr pts, std::vector<size_t> rec) {
+             const size_t N = s.get_params().get_poly_degree(), n = pts.size() / N;
+             const uint64_t *d = pts.data();
+             py::gil_scoped_release nogil;
+             s.load_data(n, [&](size_t i, RlwePt &out) { out.data.assign(d + i * N, d + (i + 1) * N); }, rec);
+           },
+           py::arg("pts"), py::arg("record_indices") = std::vector<size_t>{})
+      .def("set_client_bv_galois_key",
+           [](PirServer &s, size_t id, Arr bv) { s.set_client_bv_galois_key(id, arr_bv(bv, s.get_params().get_poly_degree())); })
+      .def("set_client_gsw_key", [](PirServer &s, size_t id, Arr g) { s.set_client_gsw_key(id, arr_gsw(g)); })
+      .def("set_client_session_keys",
+           [](PirServer &s, size_t id, Arr bv, Arr g) {
+             auto k = std::make_shared<PirSessionKeys>();
+             k->bv_galois_keys = arr_bv(bv, s.get_params().get_poly_degree());
+             k->gsw_key = arr_gsw(g);
+             s.set_client_session_keys(id, k);
+           })
+      .def("make_query",
+           [](PirServer &s, size_t id, Arr q) {
+             RlweCt ct = arr_ct(q);
+             RlweCt r;
+             {
+               py::gil_scoped_release nogil;
+               r = s.make_query(id, ct);
+             }
+             return ct_arr(r);
+           })
+      .def("make_queries",
+           [](PirServer &s, std::vector<size_t> ids, Arr qs) {
+             std::vector<RlweCt> v;
+             const size_t n = qs.shape(0), N = qs.shape(2);
+             for (size_t i = 0; i < n; i++) {
+               RlweCt ct;
+               ct.c0.assign(qs.data() + i * 2 * N, qs.data() + i * 2 * N + N);
+               ct.c1.assign(qs.data() + i * 2 * N + N, qs.data() + (i + 1) * 2 * N);
+               v.push_back(std::move(ct));
+             }
+             std::vector<RlweCt> r;
+             {
+               py::gil_scoped_release nogil;
+               r = s.make_queries(ids, v);
+             }
+             Arr a({(py::ssize_t)n, (py::ssize_t)2, (py::ssize_t)N});
+             for (size_t i = 0; i < n; i++) {
+               std::memcpy(a.mutable_data() + i * 2 * N, r[i].c0.data(), N * 8);
+               std::memcpy(a.mutable_data() + i * 2 * N + N, r[i].c1.data(), N * 8);
+             }
+             return a;
+           })
+      .def("save_resp_to_stream",
+           [](PirServer &s, Arr ct) {
+             std::stringstream ss;
+             s.save_resp_to_stream(arr_ct(ct), ss);
+             return py::bytes(ss.str());
+           })
+      .def("direct_get_original_plaintext",
+           [](const PirServer &s, size_t i) {
+             RlwePt p = s.direct_get_original_plaintext(i);
+             Arr a((py::ssize_t)p.data.size());
+             std::memcpy(a.mutable_data(), p.data.data(), p.data.size() * 8);
+             return a;
+           })
+      .def("db_handle", [](const PirServer &s) { return (uintptr_t)s.gpu_db(); })
+      .def("ctx_handle", [](const PirServer &s) { return (uintptr_t)s.gpu_scheme()->ctx(); })
+      .def("gpu_scheme", &PirServer::gpu_scheme)
+      .def_property_readonly("params", &PirServer::get_params);
+}

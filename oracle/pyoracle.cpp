@@ -244,6 +244,23 @@ PYBIND11_MODULE(onionpir_oracle, m) {
     decrypt_rns(arr_to_ct(ct, ntt), s, p.N(), p.rns_mods, p.plain_mod, p.q0_inv_mod_q1, pt);
     return vec_to_arr(pt);
   });
+  m.def("phase", [](const PirParams &p, U64Arr sk, U64Arr ct, bool ntt) {
+    // c0 + c1*s mod Q (K = 1), centred to (-Q/2, Q/2]; coefficient form.
+    const size_t N = p.N();
+    const u64 Q = p.rns_mods[0];
+    RlweCt c = arr_to_ct(ct, ntt);
+    std::vector<u64> c0(c.c0), c1(c.c1), ph(N);
+    if (ntt) ntt_inv(c0.data(), N, Q);
+    else ntt_fwd(c1.data(), N, Q);
+    for (size_t i = 0; i < N; i++) ph[i] = mulmod(c1[i], sk.data()[i], Q);
+    ntt_inv(ph.data(), N, Q);
+    py::array_t<int64_t> out((py::ssize_t)N);
+    for (size_t i = 0; i < N; i++) {
+      const u64 v = (ph[i] + c0[i]) % Q;
+      out.mutable_data()[i] = v > Q / 2 ? (int64_t)v - (int64_t)Q : (int64_t)v;
+    }
+    return out;
+  });
   m.def("decrypt_and_budget", [](const PirParams &p, U64Arr sk, U64Arr ct, bool ntt) {
     RlweSk s{arr_to_vec(sk)};
     std::vector<u64> pt;
@@ -300,6 +317,15 @@ PYBIND11_MODULE(onionpir_oracle, m) {
              return py::make_tuple(vec_to_arr(pt), b);
            })
       .def("decrypt_ct", [](const PirClient &c, U64Arr ct) { return vec_to_arr(c.decrypt_ct(arr_to_ct(ct))); })
+      .def("encrypt_bfv",
+           [](const PirClient &c, U64Arr msg, u64 seed) {
+             // encrypt_bfv_rns (rlwe.cpp:278-308) under this client's key, own RNG
+             std::mt19937_64 rng(seed);
+             RlweCt ct;
+             encrypt_bfv_rns(arr_to_vec(msg), c.sk, c.params.N(), c.params.rns_mods, c.params.plain_mod,
+                             c.params.sigma(), rng, ct);
+             return ct_to_arr(ct);
+           })
       .def("load_resp_from_stream",
            [](const PirClient &c, py::bytes b) { return ct_to_arr(c.load_resp_from_stream(std::string(b))); });
 
@@ -315,6 +341,14 @@ PYBIND11_MODULE(onionpir_oracle, m) {
            },
            py::arg("logical_num_pt"), py::arg("kind"), py::arg("seed") = 0, py::arg("record") = std::vector<size_t>{},
            py::arg("array") = py::none())
+      .def("fill_random_ntt",
+           [](PirServer &s, u64 seed) {
+             // Uniform NTT-domain residues for timing only: the answer path's cost
+             // does not depend on the database values.
+             std::mt19937_64 r(seed);
+             for (auto &x : s.db_lo) x = (uint32_t)(r() % s.params.q1);
+             for (auto &x : s.db_hi) x = (uint32_t)(r() % s.params.q2);
+           })
       .def("set_client_keys",
            [](PirServer &s, size_t id, U64Arr bv, U64Arr gsw) {
              s.set_client_keys(id, arr_to_bv(bv, s.params.N()), arr_to_gsw(gsw));
