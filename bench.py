@@ -135,7 +135,12 @@ def main():
         log(f"[check] query {j} decrypts to plaintext {idxs[j]}, noise budget {budget}")
 
     db_h, ctx_h = server.db_handle(), server.ctx_handle()
-    stream = torch.cuda.current_stream()
+    # One explicit stream (default priority) shared by torch's events and the
+    # library: torch's default stream is the NULL stream, which the C ABI
+    # would not adopt.
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    assert stream.cuda_stream, "expected a non-NULL HIP stream"
     assert G.ctx_call_raw("opir_ctx_set_stream", ctx_h, __import__("ctypes").c_void_p(stream.cuda_stream)) == 0
     d_q = torch.from_numpy(queries.view(np.int64)).to(dev)
     d_r = torch.empty_like(d_q)

@@ -55,11 +55,13 @@ struct opir_ctx {
 
 struct opir_db {
   opir_ctx *ctx = nullptr;
-  size_t N0 = 0, N0p = 0, Nrest = 0, dims = 0, h = 0, num_pt = 0, u = 0;
-  u32 *d_db = nullptr;
+  size_t N0 = 0, Nrest = 0, dims = 0, h = 0, num_pt = 0, u = 0;
+  DbGeom geom;
+  int8_t *d_db = nullptr;  // digit planes in MFMA fragment order (opir_scan.hip)
   // workspace for up to ws_B queries
   size_t ws_B = 0;
   u32 *exp_a = nullptr, *exp_b = nullptr, *qntt = nullptr, *sel = nullptr, *mid_a = nullptr, *mid_b = nullptr;
+  int8_t *qfrag = nullptr;
   u64 *q_u64 = nullptr, *r_u64 = nullptr;
   size_t exp_cts = 0;
 };
@@ -179,8 +181,11 @@ int ensure_ws(opir_db *db, size_t B) {
   }
   if (db->q_u64) (void)hipFree(db->q_u64);
   if (db->r_u64) (void)hipFree(db->r_u64);
+  if (db->qfrag) (void)hipFree(db->qfrag);
   db->q_u64 = db->r_u64 = nullptr;
+  db->qfrag = nullptr;
   db->ws_B = 0;
+  HIP_TRY(ctx, hipMalloc((void **)&db->qfrag, qfrag_bytes(make_qfrag_geom(db->N0, (int)B))));
   const size_t ct = kCtWords * sizeof(u32);
   const size_t nsel = db->dims - 1;
   HIP_TRY(ctx, hipMalloc((void **)&db->exp_a, B * db->exp_cts * ct));
@@ -283,8 +288,8 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   rec(ctx, 2);
   launch_ct_ntt(leaves, db->qntt, exp_stride, db->N0 * kCtWords, (int)db->N0, (int)B, ctx->S, ctx->stream);
   rec(ctx, 5);
-  launch_scan(db->d_db, db->qntt, db->mid_a, db->N0 * kCtWords, db->Nrest * kCtWords, (int)B, db->N0, db->N0p,
-              db->Nrest, ctx->S, ctx->stream);
+  launch_scan(db->d_db, db->qntt, db->qfrag, db->mid_a, db->N0 * kCtWords, db->Nrest * kCtWords, (int)B, db->geom,
+              ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   rec(ctx, 6);
   rec(ctx, 3);
@@ -450,11 +455,12 @@ int opir_db_create(opir_ctx *ctx, size_t N0, size_t Nrest, size_t dims, size_t h
   opir_db *db = new (std::nothrow) opir_db();
   if (!db) return set_err(ctx, OPIR_ENOMEM, "out of host memory");
   db->ctx = ctx;
-  db->N0 = N0; db->N0p = (N0 + 3) & ~size_t{3}; db->Nrest = Nrest; db->dims = dims; db->h = h;
+  db->N0 = N0; db->Nrest = Nrest; db->dims = dims; db->h = h;
+  db->geom = make_db_geom(N0, Nrest);
   db->num_pt = N0 * Nrest;
   db->u = u;
   db->exp_cts = u + 2;
-  const size_t bytes = 2 * (size_t)kN * db->N0p * Nrest * sizeof(u32);
+  const size_t bytes = db_bytes(db->geom);
   if (hipMalloc((void **)&db->d_db, bytes) != hipSuccess) {
     delete db;
     return set_err(ctx, OPIR_ENOMEM, "hipMalloc database (" + std::to_string(bytes) + " bytes)");
@@ -471,8 +477,10 @@ int opir_db_create(opir_ctx *ctx, size_t N0, size_t Nrest, size_t dims, size_t h
 void opir_db_destroy(opir_db *db) {
   if (!db) return;
   (void)hipStreamSynchronize(db->ctx->stream);
-  for (u32 *p : {db->d_db, db->exp_a, db->exp_b, db->qntt, db->sel, db->mid_a, db->mid_b})
+  for (u32 *p : {db->exp_a, db->exp_b, db->qntt, db->sel, db->mid_a, db->mid_b})
     if (p) (void)hipFree(p);
+  if (db->d_db) (void)hipFree(db->d_db);
+  if (db->qfrag) (void)hipFree(db->qfrag);
   if (db->q_u64) (void)hipFree(db->q_u64);
   if (db->r_u64) (void)hipFree(db->r_u64);
   delete db;
@@ -498,7 +506,7 @@ int opir_db_load_plain(opir_db *db, const uint16_t *coeffs, size_t first_pt, siz
                                   ctx->stream);
     if (e != hipSuccess) { rc = set_err(ctx, OPIR_EHIP, "upload plaintexts"); break; }
     launch_db_ntt_plain(d_pts, d_tmp, n, ctx->S, ctx->stream);
-    launch_db_scatter(d_tmp, db->d_db, first_pt + off, n, db->N0, db->N0p, db->Nrest, ctx->stream);
+    launch_db_scatter(d_tmp, db->d_db, first_pt + off, n, db->geom, ctx->stream);
     e = hipGetLastError();
     if (e != hipSuccess) { rc = set_err(ctx, OPIR_EHIP, std::string("db ingest: ") + hipGetErrorString(e)); break; }
     e = hipStreamSynchronize(ctx->stream);
@@ -520,7 +528,7 @@ int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi) {
       hipMemcpy(d_ref + n, hi, n * sizeof(u32), hipMemcpyHostToDevice) != hipSuccess)
     rc = set_err(ctx, OPIR_EHIP, "upload reference-layout database");
   if (!rc) {
-    launch_db_from_ref(d_ref, db->d_db, db->N0, db->N0p, db->Nrest, ctx->stream);
+    launch_db_from_ref(d_ref, db->d_db, db->geom, ctx->stream);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = set_err(ctx, OPIR_EHIP, std::string("db transpose: ") + hipGetErrorString(e));
   }
@@ -724,7 +732,7 @@ int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
   HIP_TRY(ctx, hipMemcpy(e.p, query, db->N0 * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
   launch_split_u64((u64 *)e.p, db->exp_a, db->N0 * 2, ctx->S, ctx->stream);
   launch_ct_ntt(db->exp_a, db->qntt, 0, 0, (int)db->N0, 1, ctx->S, ctx->stream);
-  launch_scan(db->d_db, db->qntt, db->mid_a, 0, 0, 1, db->N0, db->N0p, db->Nrest, ctx->S, ctx->stream);
+  launch_scan(db->d_db, db->qntt, db->qfrag, db->mid_a, 0, 0, 1, db->geom, ctx->S, ctx->stream);
   DevBuf o;
   DEV_ALLOC(ctx, o, db->Nrest * 2 * kN * sizeof(u64));
   launch_ct_to_coef_u64(db->mid_a, (u64 *)o.p, db->Nrest, ctx->S, ctx->stream);

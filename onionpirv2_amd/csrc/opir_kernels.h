@@ -30,12 +30,24 @@ void launch_ntt_u64(u64 *data, size_t npoly, bool inverse, const SchemeConst &S,
 void launch_ntt_rns(const u32 *in, u32 *out, size_t npoly_rns, bool inverse, const SchemeConst &S,
                     hipStream_t st);
 
-// DB preprocessing. Device DB layout: [limb][level][kq][row][4] with
-// pt = row * N0 + 4*kq + j; N0p = N0 rounded up to 4 (padding is zero).
+// Database geometry: pt = row * N0 + k; rows padded to 16 (nrt tiles), k to
+// 64 (nkb blocks). Device layout: balanced int8 digit planes in MFMA
+// A-fragment order, see opir_scan.hip.
+struct DbGeom {
+  size_t N0 = 0, Nrest = 0, nrt = 0, nkb = 0;
+};
+struct QfragGeom {
+  int N0 = 0, nkb = 0, nct = 0, ncm = 0;  // column tiles of 16, stored columns per tile
+};
+DbGeom make_db_geom(size_t N0, size_t Nrest);
+size_t db_bytes(const DbGeom &g);
+QfragGeom make_qfrag_geom(size_t N0, int B);
+size_t qfrag_bytes(const QfragGeom &q);
+
+// DB preprocessing (NTT of plaintexts, then scatter into the scan layout).
 void launch_db_ntt_plain(const uint16_t *pts, u32 *tmp, size_t count, const SchemeConst &S, hipStream_t st);
-void launch_db_scatter(const u32 *tmp, u32 *db, size_t first_pt, size_t count, size_t N0, size_t N0p,
-                       size_t Nrest, hipStream_t st);
-void launch_db_from_ref(const u32 *ref, u32 *db, size_t N0, size_t N0p, size_t Nrest, hipStream_t st);
+void launch_db_scatter(const u32 *tmp, int8_t *db, size_t first_pt, size_t count, const DbGeom &g, hipStream_t st);
+void launch_db_from_ref(const u32 *ref, int8_t *db, const DbGeom &g, hipStream_t st);
 
 // Expansion level: nodes [0, n) of input buffer -> children 2r, 2r+1 of output buffer.
 void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int nnodes, int B,
@@ -46,9 +58,10 @@ void launch_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride,
 // Selector completion: rows = nsel * l_ep expanded leaves (coef form) -> sel [nsel][2 l_ep][2][2][N].
 void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
                      const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st);
-// First-dim scan: qntt [B][N0][2][2][N] -> mid [B][Nrest][2][2][N] (NTT form).
-void launch_scan(const u32 *db, const u32 *qntt, u32 *mid, size_t q_stride, size_t mid_stride, int B,
-                 size_t N0, size_t N0p, size_t Nrest, const SchemeConst &S, hipStream_t st);
+// First-dim scan: qntt [B][N0][2][2][N] -> mid [B][Nrest][2][2][N] (NTT form);
+// qfrag is scratch of qfrag_bytes(make_qfrag_geom(N0, B)) bytes.
+void launch_scan(const int8_t *db, const u32 *qntt, int8_t *qfrag, u32 *mid, size_t q_stride, size_t mid_stride,
+                 int B, const DbGeom &g, const SchemeConst &S, hipStream_t st);
 // MUX level. ragged: slot s < offset copies in[s]; s >= offset muxes in[off+2j], in[off+2j+1].
 // perfect (offset < 0): slot s muxes in[s], in[s+half].
 void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,

@@ -142,49 +142,6 @@ __global__ __launch_bounds__(512) void k_db_ntt_plain(const uint16_t *__restrict
   store_map3(tmp + ((size_t)sb * gridDim.x + blockIdx.x) * kN, v, t);
 }
 
-// tmp [limb][count][N] (pt-major) -> db [limb][level][kq][row][4].
-// Tile: 32 levels x 64 pts through LDS.
-__global__ __launch_bounds__(256) void k_db_scatter(const u32 *__restrict__ tmp, u32 *__restrict__ db,
-                                                    size_t first_pt, size_t count, size_t N0, size_t N0p,
-                                                    size_t Nrest) {
-  __shared__ u32 tile[32][65];
-  const int limb = blockIdx.z;
-  const size_t pt0 = (size_t)blockIdx.x * 64;
-  const int lv0 = blockIdx.y * 32;
-  const u32 *src = tmp + (size_t)limb * count * kN;
-  const size_t level_words = Nrest * N0p;
-  // load: 64 pts x 32 levels (each pt row contributes 32 consecutive words)
-  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-    const int pl = e / 32, ll = e % 32;
-    const size_t pt = pt0 + pl;
-    tile[ll][pl] = pt < count ? src[pt * kN + lv0 + ll] : 0;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-    const int ll = e / 64, pl = e % 64;
-    const size_t pt = pt0 + pl;
-    if (pt >= count) continue;
-    const size_t g = first_pt + pt;
-    const size_t row = g / N0, k = g % N0;
-    const size_t dst = (size_t)limb * kN * level_words + (size_t)(lv0 + ll) * level_words +
-                       (k >> 2) * (Nrest * 4) + row * 4 + (k & 3);
-    db[dst] = tile[ll][pl];
-  }
-}
-
-// Reference layout ref [limb][coeff][num_pt] (server.cpp:162-170) -> device layout.
-__global__ void k_db_from_ref(const u32 *__restrict__ ref, u32 *__restrict__ db, size_t N0, size_t N0p,
-                              size_t Nrest) {
-  const size_t num_pt = N0 * Nrest;
-  const size_t total = 2 * (size_t)kN * num_pt;
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const size_t pt = i % num_pt, lv = (i / num_pt) % kN, limb = i / (num_pt * kN);
-  const size_t row = pt / N0, k = pt % N0;
-  const size_t level_words = Nrest * N0p;
-  db[limb * kN * level_words + lv * level_words + (k >> 2) * (Nrest * 4) + row * 4 + (k & 3)] = ref[i];
-}
-
 // ---------------------------------------------------------------------------
 // Expansion level (src/server.cpp:786-810 with bv_keyswitch.cpp:296-367).
 // One workgroup per (node, query): lane block sb computes limb sb.
@@ -351,64 +308,6 @@ __global__ __launch_bounds__(512) void k_complete(CompleteArgs A, SchemeConst S)
   u32 *bot = selb + (size_t)(A.l_ep + r) * kCtWords;
   store_map3(bot + (0 * 2 + sb) * kN, o0, t);
   store_map3(bot + (1 * 2 + sb) * kN, o1, t);
-}
-
-// ---------------------------------------------------------------------------
-// First-dimension scan (server.cpp:313-361 / matrix.cpp:232-350), plain
-// path. For each limb and NTT level l:
-//   mid[i][p][limb][l] = sum_k db[limb][l][i][k] * qntt[k][p][limb][l] mod q
-// Workgroup: 256 rows x LV levels of one limb; query columns for those LV
-// levels are staged in LDS; the DB streams once from HBM with 16-byte loads.
-// u64 accumulators take 64 products (< 2^64 for 29-bit q) between reductions.
-// ---------------------------------------------------------------------------
-constexpr int kScanRows = 256;
-constexpr int kScanLdsWords = 8192;
-
-__global__ __launch_bounds__(256) void k_scan(const u32 *__restrict__ db, const u32 *__restrict__ qntt,
-                                              u32 *__restrict__ mid, int N0, int N0p, int Nrest, int LV,
-                                              int nrowblk, size_t q_stride, size_t mid_stride, SchemeConst S) {
-  __shared__ __align__(16) u32 bt[kScanLdsWords];  // [lv][N0p][2]
-  const int limb = blockIdx.z;
-  const LimbConst &L = S.limb[limb];
-  const int lvblk = blockIdx.x / nrowblk, rowblk = blockIdx.x % nrowblk;
-  const int lv0 = lvblk * LV;
-  const int qi = blockIdx.y;
-  const u32 *qn = qntt + qi * q_stride;
-  // stage B tile: bt[lv][k][p] = qntt[k][p][limb][lv0 + lv]
-  for (int e = threadIdx.x; e < LV * N0p * 2; e += 256) {
-    const int lv = e % LV, kp = e / LV;
-    const int k = kp >> 1, p = kp & 1;
-    bt[(lv * N0p + k) * 2 + p] = k < N0 ? qn[((size_t)k * 2 + p) * 2 * kN + (size_t)limb * kN + lv0 + lv] : 0;
-  }
-  __syncthreads();
-  const int row = rowblk * kScanRows + threadIdx.x;
-  if (row >= Nrest) return;
-  const size_t level_words = (size_t)Nrest * N0p;
-  const u32 *dbl = db + (size_t)limb * kN * level_words;
-  u32 *out = mid + qi * mid_stride + (size_t)row * kCtWords;
-  const int nkq = N0p >> 2;
-  for (int lv = 0; lv < LV; lv++) {
-    const uint4 *a4 = reinterpret_cast<const uint4 *>(dbl + (size_t)(lv0 + lv) * level_words) + row;
-    const uint4 *b4 = reinterpret_cast<const uint4 *>(bt + lv * N0p * 2);
-    u64 acc0 = 0, acc1 = 0;
-    int kq = 0;
-    while (kq < nkq) {
-      const int end = min(nkq, kq + 16);
-#pragma unroll 4
-      for (; kq < end; kq++) {
-        const uint4 a = a4[(size_t)kq * Nrest];
-        const uint4 b01 = b4[2 * kq], b23 = b4[2 * kq + 1];
-        acc0 += (u64)a.x * b01.x + (u64)a.y * b01.z;
-        acc1 += (u64)a.x * b01.y + (u64)a.y * b01.w;
-        acc0 += (u64)a.z * b23.x + (u64)a.w * b23.z;
-        acc1 += (u64)a.z * b23.y + (u64)a.w * b23.w;
-      }
-      acc0 = reduce_u64(acc0, L);
-      acc1 = reduce_u64(acc1, L);
-    }
-    out[(0 * 2 + limb) * kN + lv0 + lv] = (u32)acc0;
-    out[(1 * 2 + limb) * kN + lv0 + lv] = (u32)acc1;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -584,17 +483,6 @@ void launch_ntt_rns(const u32 *in, u32 *out, size_t npoly_rns, bool inverse, con
 void launch_db_ntt_plain(const uint16_t *pts, u32 *tmp, size_t count, const SchemeConst &S, hipStream_t st) {
   if (count) k_db_ntt_plain<<<(unsigned)count, 512, 0, st>>>(pts, tmp, S);
 }
-void launch_db_scatter(const u32 *tmp, u32 *db, size_t first_pt, size_t count, size_t N0, size_t N0p,
-                       size_t Nrest, hipStream_t st) {
-  if (!count) return;
-  dim3 grid(cdiv(count, 64), kN / 32, 2);
-  k_db_scatter<<<grid, 256, 0, st>>>(tmp, db, first_pt, count, N0, N0p, Nrest);
-}
-void launch_db_from_ref(const u32 *ref, u32 *db, size_t N0, size_t N0p, size_t Nrest, hipStream_t st) {
-  const size_t total = 2 * (size_t)kN * N0 * Nrest;
-  k_db_from_ref<<<cdiv(total, 256), 256, 0, st>>>(ref, db, N0, N0p, Nrest);
-}
-
 void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int nnodes, int B,
                          int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
   ExpandArgs A;
@@ -626,18 +514,6 @@ void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel
   A.gsw_keys = P.gsw_keys; A.l_ep = G.l_ep; A.l_key = G.l_key; A.b_key = G.b_key;
   dim3 grid(nrows, B);
   k_complete<<<grid, 512, 0, st>>>(A, S);
-}
-
-void launch_scan(const u32 *db, const u32 *qntt, u32 *mid, size_t q_stride, size_t mid_stride, int B,
-                 size_t N0, size_t N0p, size_t Nrest, const SchemeConst &S, hipStream_t st) {
-  int LV = (int)(kScanLdsWords / (2 * N0p));
-  if (LV > 16) LV = 16;
-  if (LV < 1) LV = 1;
-  while (kN % LV) LV--;
-  const int nrowblk = (int)cdiv(Nrest, kScanRows);
-  dim3 grid((kN / LV) * nrowblk, B, 2);
-  k_scan<<<grid, 256, 0, st>>>(db, qntt, mid, (int)N0, (int)N0p, (int)Nrest, LV, nrowblk, q_stride, mid_stride,
-                               S);
 }
 
 void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
