@@ -287,9 +287,9 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   LAUNCH_CHECK(ctx);
   rec(ctx, 2);
   launch_ct_ntt(leaves, db->qntt, exp_stride, db->N0 * kCtWords, (int)db->N0, (int)B, ctx->S, ctx->stream);
+  launch_qfrag(db->qntt, db->qfrag, db->N0 * kCtWords, (int)B, db->geom, ctx->stream);
   rec(ctx, 5);
-  launch_scan(db->d_db, db->qntt, db->qfrag, db->mid_a, db->N0 * kCtWords, db->Nrest * kCtWords, (int)B, db->geom,
-              ctx->S, ctx->stream);
+  launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   rec(ctx, 6);
   rec(ctx, 3);
@@ -732,7 +732,8 @@ int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
   HIP_TRY(ctx, hipMemcpy(e.p, query, db->N0 * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
   launch_split_u64((u64 *)e.p, db->exp_a, db->N0 * 2, ctx->S, ctx->stream);
   launch_ct_ntt(db->exp_a, db->qntt, 0, 0, (int)db->N0, 1, ctx->S, ctx->stream);
-  launch_scan(db->d_db, db->qntt, db->qfrag, db->mid_a, 0, 0, 1, db->geom, ctx->S, ctx->stream);
+  launch_qfrag(db->qntt, db->qfrag, 0, 1, db->geom, ctx->stream);
+  launch_scan(db->d_db, db->qfrag, db->mid_a, 0, 1, db->geom, ctx->S, ctx->stream);
   DevBuf o;
   DEV_ALLOC(ctx, o, db->Nrest * 2 * kN * sizeof(u64));
   launch_ct_to_coef_u64(db->mid_a, (u64 *)o.p, db->Nrest, ctx->S, ctx->stream);

@@ -18,6 +18,35 @@ struct GadgetCfg {
   int l_ks, b_ks;     // BV keyswitch
 };
 
+struct ExpandArgs {
+  const u32 *in;
+  u32 *out;
+  size_t in_stride, out_stride;  // words between queries
+  const u32 *const *bv_keys;     // per query
+  size_t key_level_words;        // words per level in a client's BV key
+  int level;                     // k = 2^level, galois = N/k + 1
+  int kinv;                      // galois^-1 mod 2N
+  int l_ks, b_ks;
+};
+
+struct CompleteArgs {
+  const u32 *leaves;
+  u32 *sel;
+  size_t leaf_stride, sel_stride;
+  const u32 *const *gsw_keys;
+  int l_ep, l_key, b_key;
+};
+
+struct MuxArgs {
+  const u32 *in;
+  u32 *out;
+  size_t stride;       // words between queries (in and out)
+  const u32 *sel;      // this level's selector for query 0
+  size_t sel_stride;   // words between queries
+  int offset, half;    // offset >= 0: ragged level; else perfect level with `half`
+  int l, b;
+};
+
 // Per-query pointers for batched launches (device arrays of length B).
 struct BatchPtrs {
   const u32 *const *bv_keys;   // client BV keys: [h][l_ks][2 (b,a)][2 limbs][N]
@@ -58,10 +87,13 @@ void launch_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride,
 // Selector completion: rows = nsel * l_ep expanded leaves (coef form) -> sel [nsel][2 l_ep][2][2][N].
 void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
                      const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st);
-// First-dim scan: qntt [B][N0][2][2][N] -> mid [B][Nrest][2][2][N] (NTT form);
-// qfrag is scratch of qfrag_bytes(make_qfrag_geom(N0, B)) bytes.
-void launch_scan(const int8_t *db, const u32 *qntt, int8_t *qfrag, u32 *mid, size_t q_stride, size_t mid_stride,
-                 int B, const DbGeom &g, const SchemeConst &S, hipStream_t st);
+// First-dim scan in two launches: launch_qfrag turns the selectors qntt
+// [B][N0][2][2][N] into MFMA B fragments (qfrag_bytes(make_qfrag_geom(N0, B))
+// bytes), launch_scan multiplies the DB by them -> mid [B][Nrest][2][2][N]
+// (NTT form).
+void launch_qfrag(const u32 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st);
+void launch_scan(const int8_t *db, const int8_t *qfrag, u32 *mid, size_t mid_stride, int B, const DbGeom &g,
+                 const SchemeConst &S, hipStream_t st);
 // MUX level. ragged: slot s < offset copies in[s]; s >= offset muxes in[off+2j], in[off+2j+1].
 // perfect (offset < 0): slot s muxes in[s], in[s+half].
 void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,

@@ -142,108 +142,6 @@ __global__ __launch_bounds__(512) void k_db_ntt_plain(const uint16_t *__restrict
   store_map3(tmp + ((size_t)sb * gridDim.x + blockIdx.x) * kN, v, t);
 }
 
-// ---------------------------------------------------------------------------
-// Expansion level (src/server.cpp:786-810 with bv_keyswitch.cpp:296-367).
-// One workgroup per (node, query): lane block sb computes limb sb.
-//   c' = BVKS(sigma_k(c));  out[2r] = c + c';  out[2r+1] = (c - c') * X^-k
-// ---------------------------------------------------------------------------
-struct ExpandArgs {
-  const u32 *in;
-  u32 *out;
-  size_t in_stride, out_stride;  // words between queries
-  const u32 *const *bv_keys;
-  size_t key_level_words;        // words per level in a client's BV key
-  int level;                     // k = 2^level, galois = N/k + 1
-  int kinv;                      // galois^-1 mod 2N
-  int l_ks, b_ks;
-};
-
-__global__ __launch_bounds__(512) void k_expand_level(ExpandArgs A, SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
-  __shared__ u32 sh[2][2][kN];  // [poly][limb][N]
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
-  const LimbConst &L = S.limb[sb];
-  const int node = blockIdx.x, qi = blockIdx.y;
-  const u32 *c = A.in + qi * A.in_stride + (size_t)node * kCtWords;
-  u32 c0[8], c1[8];
-  load_map0(c0, c + (0 * 2 + sb) * kN, t);
-  load_map0(c1, c + (1 * 2 + sb) * kN, t);
-  store_map0(sh[0][sb], c0, t);
-  store_map0(sh[1][sb], c1, t);
-  __syncthreads();
-  // sigma_k gather (utils.cpp:58-72 inverted): out[j] = in[i0] or -in[i0-N], i0 = j*kinv mod 2N
-  u32 s0[8], s1[8];
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) {
-    const int j = map0(t, cc);
-    const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
-    if (i0 < kN) {
-      s0[cc] = sh[0][sb][i0];
-      s1[cc] = sh[1][sb][i0];
-    } else {
-      const u32 a = sh[0][sb][i0 - kN], b = sh[1][sb][i0 - kN];
-      s0[cc] = a ? L.q - a : 0;
-      s1[cc] = b ? L.q - b : 0;
-    }
-  }
-  __syncthreads();
-  store_map0(sh[1][sb], s1, t);
-  __syncthreads();
-  i64 D[1][8];
-  compose_centre(sh[1][0], sh[1][1], t, S, D[0]);
-  u64 accb[8], acca[8];
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
-  const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
-  gadget_mac<1, false>(D, A.l_ks, A.b_ks, key, sb, t, L, nlds[sb], accb, acca);
-  u32 db[8], da[8];
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) {
-    db[cc] = reduce_u64(accb[cc], L);
-    da[cc] = reduce_u64(acca[cc], L);
-  }
-  Barrier bar;
-  ntt_inv_2048(db, nlds[sb], t, L, bar);
-  ntt_inv_2048(da, nlds[sb], t, L, bar);
-  // c' = (sigma(c0) + delta_b, delta_a); children (server.cpp:803-808)
-  u32 *left = A.out + qi * A.out_stride + (size_t)(2 * node) * kCtWords;
-  u32 *right = left + kCtWords;
-  u32 l0[8], l1[8];
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) {
-    const u32 p0 = add_mod(s0[cc], db[cc], L.q), p1 = da[cc];
-    l0[cc] = add_mod(c0[cc], p0, L.q);
-    l1[cc] = add_mod(c1[cc], p1, L.q);
-    c0[cc] = sub_mod(c0[cc], p0, L.q);
-    c1[cc] = sub_mod(c1[cc], p1, L.q);
-  }
-  store_map0(left + (0 * 2 + sb) * kN, l0, t);
-  store_map0(left + (1 * 2 + sb) * kN, l1, t);
-  // negacyclic shift by -k (utils.cpp:166-186 with shift = -k):
-  // out[j] = d[j+k] if j+k < N else -d[j+k-N]
-  store_map0(sh[0][sb], c0, t);
-  store_map0(sh[1][sb], c1, t);
-  __syncthreads();
-  const int k = 1 << A.level;
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) {
-    const int j = map0(t, cc);
-    const int s = j + k;
-    u32 a, b;
-    if (s < kN) {
-      a = sh[0][sb][s];
-      b = sh[1][sb][s];
-    } else {
-      a = sh[0][sb][s - kN];
-      b = sh[1][sb][s - kN];
-      a = a ? L.q - a : 0;
-      b = b ? L.q - b : 0;
-    }
-    right[(0 * 2 + sb) * kN + j] = a;
-    right[(1 * 2 + sb) * kN + j] = b;
-  }
-}
-
 // Forward NTT of both polys of `count` cts per query (evaluate_first_dim's
 // query transform, server.cpp:331-337). grid (count*2, B).
 __global__ __launch_bounds__(512) void k_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride,
@@ -257,125 +155,6 @@ __global__ __launch_bounds__(512) void k_ct_ntt(const u32 *in, u32 *out, size_t 
   load_map0(v, in + blockIdx.y * in_stride + off, t);
   ntt_fwd_2048(v, nlds[sb], t, L, Barrier());
   store_map3(out + blockIdx.y * out_stride + off, v, t);
-}
-
-// ---------------------------------------------------------------------------
-// Selector completion (server.cpp:823-855, gsw.cpp:317-357). One workgroup
-// per expanded selector row: top row = NTT(row); bottom row =
-// external_product(RGSW(s), row) with the key gadget (l_key, b_key).
-// sel layout: [selector][2*l_ep rows][2 polys][2 limbs][N].
-// ---------------------------------------------------------------------------
-struct CompleteArgs {
-  const u32 *leaves;
-  u32 *sel;
-  size_t leaf_stride, sel_stride;
-  const u32 *const *gsw_keys;
-  int l_ep, l_key, b_key;
-};
-
-__global__ __launch_bounds__(512) void k_complete(CompleteArgs A, SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
-  __shared__ u32 sh[2][2][kN];
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
-  const LimbConst &L = S.limb[sb];
-  const int g = blockIdx.x, qi = blockIdx.y;
-  const int a = g / A.l_ep, r = g % A.l_ep;
-  const u32 *row = A.leaves + qi * A.leaf_stride + (size_t)g * kCtWords;
-  u32 *selb = A.sel + qi * A.sel_stride + (size_t)a * (2 * A.l_ep) * kCtWords;
-  Barrier bar;
-#pragma unroll
-  for (int p = 0; p < 2; p++) {
-    u32 v[8];
-    load_map0(v, row + (p * 2 + sb) * kN, t);
-    store_map0(sh[p][sb], v, t);
-    ntt_fwd_2048(v, nlds[sb], t, L, bar);
-    store_map3(selb + (size_t)r * kCtWords + (p * 2 + sb) * kN, v, t);
-  }
-  __syncthreads();
-  i64 D[2][8];
-  compose_centre(sh[0][0], sh[0][1], t, S, D[0]);
-  compose_centre(sh[1][0], sh[1][1], t, S, D[1]);
-  u64 acc0[8], acc1[8];
-#pragma unroll
-  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
-  gadget_mac<2, true>(D, A.l_key, A.b_key, A.gsw_keys[qi], sb, t, L, nlds[sb], acc0, acc1);
-  u32 o0[8], o1[8];
-#pragma unroll
-  for (int c = 0; c < 8; c++) {
-    o0[c] = reduce_u64(acc0[c], L);
-    o1[c] = reduce_u64(acc1[c], L);
-  }
-  u32 *bot = selb + (size_t)(A.l_ep + r) * kCtWords;
-  store_map3(bot + (0 * 2 + sb) * kN, o0, t);
-  store_map3(bot + (1 * 2 + sb) * kN, o1, t);
-}
-
-// ---------------------------------------------------------------------------
-// MUX level (server.cpp:599-722) on NTT-form candidates:
-//   res = x + EP(sel, INTT(y - x))     (EP output is NTT form)
-// which equals the reference's coefficient-form x + INTT(EP(sel, y - x)).
-// ---------------------------------------------------------------------------
-struct MuxArgs {
-  const u32 *in;
-  u32 *out;
-  size_t stride;       // words between queries (in and out)
-  const u32 *sel;      // this level's selector for query 0
-  size_t sel_stride;   // words between queries
-  int offset, half;    // offset >= 0: ragged level; else perfect level with `half`
-  int l, b;
-};
-
-__global__ __launch_bounds__(512) void k_mux(MuxArgs A, SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
-  __shared__ u32 sh[2][2][kN];
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
-  const LimbConst &L = S.limb[sb];
-  const int s = blockIdx.x, qi = blockIdx.y;
-  const u32 *in = A.in + qi * A.stride;
-  u32 *dst = A.out + qi * A.stride + (size_t)s * kCtWords;
-  int xi, yi;
-  if (A.offset >= 0) {
-    if (s < A.offset) {  // unpaired candidate of the ragged level stays in place
-      const u32 *src = in + (size_t)s * kCtWords;
-      for (int e = threadIdx.x; e < (int)kCtWords / 4; e += 512)
-        reinterpret_cast<uint4 *>(dst)[e] = reinterpret_cast<const uint4 *>(src)[e];
-      return;
-    }
-    xi = A.offset + 2 * (s - A.offset);
-    yi = xi + 1;
-  } else {
-    xi = s;
-    yi = s + A.half;
-  }
-  const u32 *x = in + (size_t)xi * kCtWords, *y = in + (size_t)yi * kCtWords;
-  Barrier bar;
-  u32 xv[2][8];
-#pragma unroll
-  for (int p = 0; p < 2; p++) {
-    u32 yv[8];
-    load_map3(xv[p], x + (p * 2 + sb) * kN, t);
-    load_map3(yv, y + (p * 2 + sb) * kN, t);
-#pragma unroll
-    for (int c = 0; c < 8; c++) yv[c] = sub_mod(yv[c], xv[p][c], L.q);
-    ntt_inv_2048(yv, nlds[sb], t, L, bar);
-    store_map0(sh[p][sb], yv, t);
-  }
-  __syncthreads();
-  i64 D[2][8];
-  compose_centre(sh[0][0], sh[0][1], t, S, D[0]);
-  compose_centre(sh[1][0], sh[1][1], t, S, D[1]);
-  u64 acc0[8], acc1[8];
-#pragma unroll
-  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
-  gadget_mac<2, true>(D, A.l, A.b, A.sel + qi * A.sel_stride, sb, t, L, nlds[sb], acc0, acc1);
-  u32 o0[8], o1[8];
-#pragma unroll
-  for (int c = 0; c < 8; c++) {
-    o0[c] = add_mod(xv[0][c], reduce_u64(acc0[c], L), L.q);
-    o1[c] = add_mod(xv[1][c], reduce_u64(acc1[c], L), L.q);
-  }
-  store_map3(dst + (0 * 2 + sb) * kN, o0, t);
-  store_map3(dst + (1 * 2 + sb) * kN, o1, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -483,47 +262,11 @@ void launch_ntt_rns(const u32 *in, u32 *out, size_t npoly_rns, bool inverse, con
 void launch_db_ntt_plain(const uint16_t *pts, u32 *tmp, size_t count, const SchemeConst &S, hipStream_t st) {
   if (count) k_db_ntt_plain<<<(unsigned)count, 512, 0, st>>>(pts, tmp, S);
 }
-void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int nnodes, int B,
-                         int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
-  ExpandArgs A;
-  A.in = in; A.out = out; A.in_stride = in_stride; A.out_stride = out_stride;
-  A.bv_keys = P.bv_keys;
-  A.key_level_words = (size_t)G.l_ks * kCtWords;
-  A.level = level;
-  const unsigned gk = (unsigned)(kN >> level) + 1;
-  unsigned inv = 1;  // galois^-1 mod 2N by exponentiation: odd g has order dividing N
-  for (int e = 0; e < kN - 1; e++) inv = (inv * gk) & (2 * kN - 1);
-  A.kinv = (int)inv;
-  A.l_ks = G.l_ks; A.b_ks = G.b_ks;
-  dim3 grid(nnodes, B);
-  k_expand_level<<<grid, 512, 0, st>>>(A, S);
-}
-
 void launch_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int count, int B,
                    const SchemeConst &S, hipStream_t st) {
   if (count <= 0) return;
   dim3 grid(count * 2, B);
   k_ct_ntt<<<grid, 512, 0, st>>>(in, out, in_stride, out_stride, S);
-}
-
-void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
-                     const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
-  if (nrows <= 0) return;
-  CompleteArgs A;
-  A.leaves = leaves; A.sel = sel; A.leaf_stride = leaf_stride; A.sel_stride = sel_stride;
-  A.gsw_keys = P.gsw_keys; A.l_ep = G.l_ep; A.l_key = G.l_key; A.b_key = G.b_key;
-  dim3 grid(nrows, B);
-  k_complete<<<grid, 512, 0, st>>>(A, S);
-}
-
-void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
-                const u32 *sel, size_t sel_stride, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
-  if (nslots <= 0) return;
-  MuxArgs A;
-  A.in = in; A.out = out; A.stride = stride; A.sel = sel; A.sel_stride = sel_stride;
-  A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
-  dim3 grid(nslots, B);
-  k_mux<<<grid, 512, 0, st>>>(A, S);
 }
 
 void launch_final(const u32 *res, u64 *out, size_t res_stride, int B, const SchemeConst &S, hipStream_t st) {

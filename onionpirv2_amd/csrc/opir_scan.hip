@@ -125,9 +125,9 @@ __global__ __launch_bounds__(256) void k_qfrag(const u32 *__restrict__ qntt, int
 }
 
 // The scan. Workgroup = 4 waves = 4 row tiles (64 rows) x LV levels x one
-// column tile, one limb. blockIdx.x is remapped so that the row blocks of one
-// (limb, level block, column tile) run on the same XCD and share its L2 copy
-// of the query fragments.
+// column tile, one limb. blockIdx.x is remapped so that consecutive logical
+// blocks run on the same XCD: the column tiles of one DB tile, then the row
+// blocks of one level block (which share the query fragments).
 constexpr int kScanLV = 16;
 
 __global__ __launch_bounds__(256) void k_scan_mfma(const int8_t *__restrict__ db, const int8_t *__restrict__ qfrag,
@@ -135,10 +135,12 @@ __global__ __launch_bounds__(256) void k_scan_mfma(const int8_t *__restrict__ db
                                                    int nrb, int B, SchemeConst S) {
   const unsigned W = gridDim.x, p = blockIdx.x;
   const unsigned L = (p % 8) * (W / 8) + p / 8;  // XCD-aware: consecutive L on one XCD
-  const int rb = L % nrb;
-  unsigned rest = L / nrb;
-  const int ct = rest % qg.nct;
-  rest /= qg.nct;
+  // column tiles innermost: the WGs reading the same DB tile run together on
+  // one XCD, so the tile comes from HBM once and from L2 for the others
+  const int ct = L % qg.nct;
+  unsigned rest = L / qg.nct;
+  const int rb = rest % nrb;
+  rest /= nrb;
   const int lb = rest % (kN / kScanLV);
   const int limb = rest / (kN / kScanLV);
   const LimbConst &Lc = S.limb[limb];
@@ -243,13 +245,17 @@ void launch_db_from_ref(const u32 *ref, int8_t *db, const DbGeom &g, hipStream_t
   k_db_from_ref<<<cdivu(total, 256), 256, 0, st>>>(ref, db, g);
 }
 
-void launch_scan(const int8_t *db, const u32 *qntt, int8_t *qfrag, u32 *mid, size_t q_stride, size_t mid_stride,
-                 int B, const DbGeom &g, const SchemeConst &S, hipStream_t st) {
+void launch_qfrag(const u32 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st) {
   const QfragGeom qg = make_qfrag_geom(g.N0, B);
   // k_qfrag writes every stored column and every k block (zeros for k >= N0);
   // the scan never reads the unstored columns of a partial last tile.
   dim3 pg(2 * B, cdivu(qg.nkb * 64, 16), (kN / 64) * 2);
   k_qfrag<<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg);
+}
+
+void launch_scan(const int8_t *db, const int8_t *qfrag, u32 *mid, size_t mid_stride, int B, const DbGeom &g,
+                 const SchemeConst &S, hipStream_t st) {
+  const QfragGeom qg = make_qfrag_geom(g.N0, B);
   const int nrb = (int)cdivu(g.nrt, 4);
   const size_t W = (size_t)2 * (kN / kScanLV) * qg.nct * nrb;
   k_scan_mfma<<<(unsigned)W, 256, 0, st>>>(db, qfrag, mid, mid_stride, g, qg, nrb, B, S);

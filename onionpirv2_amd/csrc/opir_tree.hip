@@ -1,0 +1,333 @@
+// Query expansion, selector completion and the MUX tree of the answer path
+// (src/server.cpp:599-855, src/gsw.cpp:119-182,317-357,
+// src/bv_keyswitch.cpp:296-367).
+//
+// Each tree node (BV keyswitch, completion row, MUX) is one 1024-thread
+// workgroup of four 256-thread lane blocks. A lane block runs 2048-point
+// transforms of one RNS limb; the four blocks split a node's gadget digits so
+// that its sequential chain of transforms is half as long as with one block
+// per limb. Partial digit sums of two blocks are combined through LDS.
+//
+// Digits: the reference extracts balanced digits LSB-first with
+//   r = sign-extended low b bits of d,  d <- (d - r) >> b
+// (gsw.cpp:285-293, bv_keyswitch.cpp:32-39). Unrolled, digit i of the centred
+// value D is r_i = sext_b(floor((D + H_i) / 2^(b*i))) with
+// H_i = 2^(b-1) * (1 + 2^b + ... + 2^(b(i-1))), so any block can compute any
+// digit directly.
+#include "opir_kernels.h"
+
+namespace opir {
+
+namespace {
+
+struct Barrier {
+  __device__ void operator()() const { __syncthreads(); }
+};
+
+__device__ __forceinline__ i64 digit_closed(i64 D, int i, int b) {
+  i64 H = 0;
+  for (int j = 0; j < i; j++) H += ((i64)1 << (b - 1)) << (b * j);
+  const i64 di = (D + H) >> (b * i);
+  const i64 half = (i64)1 << (b - 1);
+  return ((di + half) & (((i64)1 << b) - 1)) - half;
+}
+
+__device__ __forceinline__ void compose_centre8(const u32 *lo, const u32 *hi, int t, const SchemeConst &S,
+                                                i64 (&D)[8]) {
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int j = map0(t, c);
+    D[c] = centre(crt_compose(lo[j], hi[j], S), S);
+  }
+}
+
+// Digits [i0, i1) of one polynomial: NTT each digit (this block's limb) and
+// accumulate digit x key row into two u64 accumulators per value.
+// row(i) = row_base + (rev ? l-1-i : i); key rows are [2 outputs][2 limbs][N].
+__device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int l, int b, bool rev, int row_base,
+                                          const u32 *__restrict__ key, int limb, int t_in, const LimbConst &L,
+                                          u32 *nttlds, u64 (&acc0)[8], u64 (&acc1)[8]) {
+  Barrier bar;
+#pragma unroll 1
+  for (int i = i0; i < i1; i++) {
+    // An opaque copy of t keeps the per-thread twiddle and key addresses
+    // inside the loop; hoisted, they do not fit in 128 registers.
+    int t = t_in;
+    asm volatile("" : "+v"(t));
+    u32 v[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = digit_mod(digit_closed(D[c], i, b), L.q);
+    ntt_fwd_2048(v, nttlds, t, L, bar);
+    const u32 *r = key + (size_t)(row_base + (rev ? l - 1 - i : i)) * kCtWords;
+    u32 k0[8], k1[8];
+    load_map3(k0, r + (0 * 2 + limb) * kN, t);
+    load_map3(k1, r + (1 * 2 + limb) * kN, t);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      acc0[c] += (u64)v[c] * k0[c];
+      acc1[c] += (u64)v[c] * k1[c];
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Expansion level (server.cpp:786-810 with bv_keyswitch.cpp:296-367):
+//   c' = (sigma(c0) + sum d_i*b_i, sum d_i*a_i), d_i digits of sigma(c1)
+//   out[2r] = c + c',  out[2r+1] = (c - c') * X^-k
+// Lane block (limb, h): digits [h*ceil(L/2), ...) of limb `limb`; afterwards
+// block h finishes output polynomial h (h = 0: c0 side, h = 1: c1 side).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_expand_level(ExpandArgs A, SchemeConst S) {
+  __shared__ u32 nlds[4][kLdsPoly];
+  __shared__ u32 sh[2][2][kN];  // [poly][limb]: c, later partial sums, later shift staging
+  __shared__ u32 sig[2][kN];    // sigma(c1) per limb
+  const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // lb is wave-uniform
+  const int limb = lb & 1, h = lb >> 1;
+  const LimbConst &L = S.limb[limb];
+  const int node = blockIdx.x, qi = blockIdx.y;
+  const u32 *c = A.in + qi * A.in_stride + (size_t)node * kCtWords;
+  {
+    u32 cv[8];  // this block's polynomial h of the input, limb `limb`
+    load_map0(cv, c + (h * 2 + limb) * kN, t);
+    store_map0(sh[h][limb], cv, t);
+  }
+  __syncthreads();
+  // sigma_k gather: out[j] = in[i0] or -in[i0 - N], i0 = j * kinv mod 2N
+  // (only sigma(c1) feeds the digits; sigma(c0) is re-gathered at the end so
+  // that neither stays in registers across the transforms)
+  if (h == 1) {
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      const int j = map0(t, cc);
+      const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+      const u32 x = sh[1][limb][i0 < kN ? i0 : i0 - kN];
+      sig[limb][j] = (i0 < kN || !x) ? x : L.q - x;
+    }
+  }
+  __syncthreads();
+  i64 D[8];
+  compose_centre8(sig[0], sig[1], t, S, D);
+  u64 accb[8], acca[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
+  const int half = (A.l_ks + 1) / 2;
+  const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
+  digit_mac(D, h * half, h ? A.l_ks : half, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[lb], accb, acca);
+  // exchange partial sums: block h keeps output h (0: delta_b, 1: delta_a)
+  u32 keep[8], give[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const u32 b = reduce_u64(accb[cc], L), a = reduce_u64(acca[cc], L);
+    keep[cc] = h ? a : b;
+    give[cc] = h ? b : a;
+  }
+  __syncthreads();  // all reads of sh (gather) are long done; reuse it
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) sh[1 - h][limb][map3(t, cc)] = give[cc];
+  __syncthreads();
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) keep[cc] = add_mod(keep[cc], sh[h][limb][map3(t, cc)], L.q);
+  ntt_inv_2048(keep, nlds[lb], t, L, Barrier());  // delta_b (h=0) or delta_a (h=1), coefficient form
+  // c'_h = sigma(c0) + delta_b (h=0) or delta_a (h=1)
+  u32 *left = A.out + qi * A.out_stride + (size_t)(2 * node) * kCtWords;
+  u32 *right = left + kCtWords;
+  const u32 *ch = c + (h * 2 + limb) * kN;
+  u32 cv[8];
+  load_map0(cv, ch, t);
+  if (h == 0) {  // c'_0 = sigma(c0) + delta_b
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      const int j = map0(t, cc);
+      const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+      const u32 x = ch[i0 < kN ? i0 : i0 - kN];
+      keep[cc] = add_mod(keep[cc], (i0 < kN || !x) ? x : L.q - x, L.q);
+    }
+  }
+  u32 l_out[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    l_out[cc] = add_mod(cv[cc], keep[cc], L.q);
+    cv[cc] = sub_mod(cv[cc], keep[cc], L.q);
+  }
+  store_map0(left + (h * 2 + limb) * kN, l_out, t);
+  // right = (c - c') * X^-k: out[j] = d[j+k] if j+k < N else -d[j+k-N]
+  __syncthreads();
+  store_map0(sh[h][limb], cv, t);
+  __syncthreads();
+  const int k = 1 << A.level;
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const int j = map0(t, cc), s = j + k;
+    const u32 x = sh[h][limb][s < kN ? s : s - kN];
+    right[(h * 2 + limb) * kN + j] = (s < kN || !x) ? x : L.q - x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Selector completion (server.cpp:823-855, gsw.cpp:317-357): top row r =
+// NTT(row); bottom row = external_product(RGSW(s), row), key gadget.
+// Lane block (limb, p): poly p of the row, all l_key digits of it.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_complete(CompleteArgs A, SchemeConst S) {
+  __shared__ u32 nlds[4][kLdsPoly];
+  __shared__ u32 sh[2][2][kN];
+  const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // lb is wave-uniform
+  const int limb = lb & 1, p = lb >> 1;
+  const LimbConst &L = S.limb[limb];
+  const int g = blockIdx.x, qi = blockIdx.y;
+  const int a = g / A.l_ep, r = g % A.l_ep;
+  const u32 *row = A.leaves + qi * A.leaf_stride + (size_t)g * kCtWords;
+  u32 *selb = A.sel + qi * A.sel_stride + (size_t)a * (2 * A.l_ep) * kCtWords;
+  u32 v[8];
+  load_map0(v, row + (p * 2 + limb) * kN, t);
+  store_map0(sh[p][limb], v, t);
+  ntt_fwd_2048(v, nlds[lb], t, L, Barrier());  // its barriers also publish sh
+  store_map3(selb + (size_t)r * kCtWords + (p * 2 + limb) * kN, v, t);
+  i64 D[8];
+  compose_centre8(sh[p][0], sh[p][1], t, S, D);
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  digit_mac(D, 0, A.l_key, A.l_key, A.b_key, true, p * A.l_key, A.gsw_keys[qi], limb, t, L, nlds[lb], acc0, acc1);
+  u32 o0[8], o1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    o0[c] = reduce_u64(acc0[c], L);
+    o1[c] = reduce_u64(acc1[c], L);
+  }
+  __syncthreads();
+  if (p == 1) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      sh[0][limb][map3(t, c)] = o0[c];
+      sh[1][limb][map3(t, c)] = o1[c];
+    }
+  }
+  __syncthreads();
+  if (p == 0) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      o0[c] = add_mod(o0[c], sh[0][limb][map3(t, c)], L.q);
+      o1[c] = add_mod(o1[c], sh[1][limb][map3(t, c)], L.q);
+    }
+    u32 *bot = selb + (size_t)(A.l_ep + r) * kCtWords;
+    store_map3(bot + (0 * 2 + limb) * kN, o0, t);
+    store_map3(bot + (1 * 2 + limb) * kN, o1, t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MUX level (server.cpp:599-722) on NTT-form candidates:
+//   res = x + EP(sel, INTT(y - x))   (EP output in NTT form)
+// equal to the reference's coefficient-form x + INTT(EP(sel, y - x)).
+// Lane block (limb, p): INTT of poly p of y - x, then the l digits of poly p.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_mux(MuxArgs A, SchemeConst S) {
+  __shared__ u32 nlds[4][kLdsPoly];
+  __shared__ u32 sh[2][2][kN];
+  const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // lb is wave-uniform
+  const int limb = lb & 1, p = lb >> 1;
+  const LimbConst &L = S.limb[limb];
+  const int s = blockIdx.x, qi = blockIdx.y;
+  const u32 *in = A.in + qi * A.stride;
+  u32 *dst = A.out + qi * A.stride + (size_t)s * kCtWords;
+  int xi, yi;
+  if (A.offset >= 0) {
+    if (s < A.offset) {  // unpaired candidate of the ragged level stays in place
+      const u32 *src = in + (size_t)s * kCtWords;
+      for (int e = threadIdx.x; e < (int)kCtWords / 4; e += 1024)
+        reinterpret_cast<uint4 *>(dst)[e] = reinterpret_cast<const uint4 *>(src)[e];
+      return;
+    }
+    xi = A.offset + 2 * (s - A.offset);
+    yi = xi + 1;
+  } else {
+    xi = s;
+    yi = s + A.half;
+  }
+  const u32 *x = in + (size_t)xi * kCtWords, *y = in + (size_t)yi * kCtWords;
+  u32 v[8], w[8];
+  load_map3(v, y + (p * 2 + limb) * kN, t);
+  load_map3(w, x + (p * 2 + limb) * kN, t);
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = sub_mod(v[c], w[c], L.q);
+  ntt_inv_2048(v, nlds[lb], t, L, Barrier());
+  store_map0(sh[p][limb], v, t);
+  __syncthreads();
+  i64 D[8];
+  compose_centre8(sh[p][0], sh[p][1], t, S, D);
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  digit_mac(D, 0, A.l, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[lb], acc0, acc1);
+  u32 o0[8], o1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    o0[c] = reduce_u64(acc0[c], L);
+    o1[c] = reduce_u64(acc1[c], L);
+  }
+  __syncthreads();
+  if (p == 1) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      sh[0][limb][map3(t, c)] = o0[c];
+      sh[1][limb][map3(t, c)] = o1[c];
+    }
+  }
+  __syncthreads();
+  if (p == 0) {
+    u32 x0[8], x1[8];
+    load_map3(x0, x + (0 * 2 + limb) * kN, t);
+    load_map3(x1, x + (1 * 2 + limb) * kN, t);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      x0[c] = add_mod(x0[c], add_mod(o0[c], sh[0][limb][map3(t, c)], L.q), L.q);
+      x1[c] = add_mod(x1[c], add_mod(o1[c], sh[1][limb][map3(t, c)], L.q), L.q);
+    }
+    store_map3(dst + (0 * 2 + limb) * kN, x0, t);
+    store_map3(dst + (1 * 2 + limb) * kN, x1, t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int nnodes, int B,
+                         int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
+  ExpandArgs A;
+  A.in = in; A.out = out; A.in_stride = in_stride; A.out_stride = out_stride;
+  A.bv_keys = P.bv_keys;
+  A.key_level_words = (size_t)G.l_ks * kCtWords;
+  A.level = level;
+  const unsigned gk = (unsigned)(kN >> level) + 1;
+  unsigned inv = 1;  // gk^-1 mod 2N = gk^(N-1): the unit group mod 2N = 2^12 has order N
+  for (int e = 0; e < kN - 1; e++) inv = (inv * gk) & (2 * kN - 1);
+  A.kinv = (int)inv;
+  A.l_ks = G.l_ks; A.b_ks = G.b_ks;
+  dim3 grid(nnodes, B);
+  k_expand_level<<<grid, 1024, 0, st>>>(A, S);
+}
+
+void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
+                     const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
+  if (nrows <= 0) return;
+  CompleteArgs A;
+  A.leaves = leaves; A.sel = sel; A.leaf_stride = leaf_stride; A.sel_stride = sel_stride;
+  A.gsw_keys = P.gsw_keys; A.l_ep = G.l_ep; A.l_key = G.l_key; A.b_key = G.b_key;
+  dim3 grid(nrows, B);
+  k_complete<<<grid, 1024, 0, st>>>(A, S);
+}
+
+void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
+                const u32 *sel, size_t sel_stride, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
+  if (nslots <= 0) return;
+  MuxArgs A;
+  A.in = in; A.out = out; A.stride = stride; A.sel = sel; A.sel_stride = sel_stride;
+  A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
+  dim3 grid(nslots, B);
+  k_mux<<<grid, 1024, 0, st>>>(A, S);
+}
+
+}  // namespace opir
