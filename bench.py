@@ -39,7 +39,12 @@ def parse():
     ap.add_argument("--height", type=int, default=9)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--sweep", default="1,8,32,64,128",
+                    help="batch sizes of the batch sweep (comma list; empty to skip)")
     ap.add_argument("--seed", type=int, default=20251142)
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="run a plain device copy of this many GiB first (calibrates rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE against known bytes)")
     return ap.parse_args()
 
 
@@ -66,6 +71,19 @@ def pct(xs, p):
     lo = int(k)
     hi = min(lo + 1, len(xs) - 1)
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def committed_traffic(args):
+    """HBM bytes per k_scan_mfma launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), when the
+    workload is the one it was measured on."""
+    path = os.path.join(ROOT, "profiles", "k_scan_mfma_traffic_256x1024.md")
+    if (args.target, args.height) != (262144, 9) or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(args, log):
@@ -97,8 +115,8 @@ def cpu_baseline(args, log):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path shards by "
-                                   "replicating the database per GPU, see DESIGN.md)"}))
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (across GPUs the "
+                                   "path shards by replicating the database, one process per GPU)"}))
         sys.exit(2)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     import numpy as np
@@ -110,22 +128,33 @@ def main():
     torch.cuda.set_device(dev)
     params = H.PirParams().with_layout(args.target, args.height, True)
     N, B = params.N, args.batch
+    sweep = [int(x) for x in args.sweep.split(",") if x] if not args.no_batch else []
+    nclients = max([B] + sweep)
     log(f"[setup] layout N0={params.fst_dim_sz} Nrest={params.other_dim_sz} dims={params.num_dims} "
         f"num_pt={params.num_pt} h={params.expansion_height}")
     t0 = time.time()
     server = H.PirServer(params)
-    idxs = distinct_indices(max(B, 8), params.num_pt, args.seed)
+    idxs = distinct_indices(max(nclients, 8), params.num_pt, args.seed)
     server.gen_data(idxs[:8], args.seed)
     log(f"[setup] database generated and loaded in {time.time() - t0:.1f}s")
     t0 = time.time()
     clients = []
-    for j in range(B):
+    for j in range(nclients):
         c = H.PirClient(params, 1000 + j, args.seed + 7 * j)
         server.set_client_bv_galois_key(1000 + j, c.create_bv_galois_keys())
         server.set_client_gsw_key(1000 + j, c.generate_gsw_from_key())
         clients.append(c)
-    queries = np.stack([clients[j].fast_generate_query(params, idxs[j]) for j in range(B)])
-    log(f"[setup] {B} clients keyed, queries built in {time.time() - t0:.1f}s")
+    queries = np.stack([clients[j].fast_generate_query(params, idxs[j]) for j in range(nclients)])
+    log(f"[setup] {nclients} clients keyed, queries built in {time.time() - t0:.1f}s")
+
+    # PCIe-inclusive plain rate: the host-buffer entry point (query upload,
+    # answer, response download, host conversions), for DESIGN notes only.
+    for j in range(2):
+        server.make_query(1000 + j, queries[j].copy())
+    t0 = time.perf_counter()
+    for j in range(10):
+        server.make_query(1000 + j % 2, queries[j % 2].copy())
+    host_ms = (time.perf_counter() - t0) * 100.0
 
     # correctness gate on the benchmarked database: decrypt two answers
     for j in range(2):
@@ -142,9 +171,18 @@ def main():
     torch.cuda.set_stream(stream)
     assert stream.cuda_stream, "expected a non-NULL HIP stream"
     assert G.ctx_call_raw("opir_ctx_set_stream", ctx_h, __import__("ctypes").c_void_p(stream.cuda_stream)) == 0
+    if args.copy_probe_gib > 0:
+        n = int(args.copy_probe_gib * 2**30) // 4
+        src = torch.ones(n, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[probe] copied {n * 4} bytes twice")
+        del src, dst
     d_q = torch.from_numpy(queries.view(np.int64)).to(dev)
     d_r = torch.empty_like(d_q)
-    ids = np.arange(1000, 1000 + B, dtype=np.uint64)
+    ids = np.arange(1000, 1000 + nclients, dtype=np.uint64)
 
     def answer(b, off=0):
         rc = G.answer_batch_device_raw(db_h, ids[off:off + b], d_q[off:off + b].data_ptr(), d_r[off:off + b].data_ptr())
@@ -152,14 +190,15 @@ def main():
             raise RuntimeError(G.last_error_raw(ctx_h))
 
     def timed(b, steps, warmup):
+        off = lambda i: (i * b) % nclients if b * 2 <= nclients else 0
         for i in range(warmup):
-            answer(b, (i * b) % B if b < B else 0)
+            answer(b, off(i))
         torch.cuda.synchronize()
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
         wall0 = time.perf_counter()
         evs[0].record()
         for i in range(steps):
-            answer(b, (i * b) % B if b < B else 0)
+            answer(b, off(i))
             evs[i + 1].record()
         torch.cuda.synchronize()
         wall = time.perf_counter() - wall0
@@ -191,6 +230,7 @@ def main():
     achieved = scan_bytes / (scan * 1e-3) / 1e9
     log(f"[plain] stages {stages} scan {scan:.3f} ms -> {achieved:.0f} GB/s")
 
+    traffic, traffic_src = committed_traffic(args)
     result = {
         "metric": "DB GiB/s & queries/s, 1GB n=2048, batched-64 vs plain GPU; p50/p99 ms",
         "value": round(gib_db / (ms_plain * 1e-3), 3),
@@ -198,7 +238,9 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_plain, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u32 (exact RNS mod 29-bit primes; no floating point)",
+        "dtype": "u32",
+        "arithmetic": "exact modular: two 29-bit RNS limbs on 32-bit VALU; scan on int8 MFMA digit planes "
+                      "with int32 accumulation; no floating point",
         "data": "synthetic (seeded DB coefficients < t, seeded client keys and queries)",
         "config": {"workload": "OnionPIRv2 make_query, CONFIG_N2048_K1_COMP, plain single-query path, "
                                "with_layout({262144,9}) = 256 x 1024 plaintexts (0.75 GiB logical, 4 GiB NTT-form)",
@@ -206,9 +248,11 @@ def main():
                    "target_num_pt": params.target_num_pt, "parallelism": "single GPU"},
         "queries_per_s_plain": round(1e3 / ms_plain, 2),
         "latency_ms_plain": {"p50": round(pct(per, 50), 4), "p99": round(pct(per, 99), 4)},
+        "plain_ms_pcie_inclusive": round(host_ms, 4),
         "stage_ms_plain": stages,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_scan (first-dimension DB scan)", "kernel_ms": round(scan, 4),
                      "algorithmic_bytes": scan_bytes},
     }
@@ -228,6 +272,17 @@ def main():
         rb = d_r.cpu().numpy().view(np.uint64)
         for j in [0, B - 1]:
             assert np.array_equal(rb[j], server.make_query(1000 + j, queries[j].copy())), "batch != plain"
+        # batch-size sweep: one fused pass per batch, distinct clients
+        rows = []
+        for b in sweep:
+            steps_s = max(3, min(args.steps, 256 // b))
+            total_ms, per_s, _ = timed(b, steps_s, 1)
+            ms_s = total_ms / steps_s
+            rows.append({"batch": b, "ms_per_batch": round(ms_s, 3), "queries_per_s": round(b * 1e3 / ms_s, 2),
+                         "p50_ms": round(pct(per_s, 50), 3), "p99_ms": round(pct(per_s, 99), 3), "steps": steps_s})
+            log(f"[sweep] B={b}: {ms_s:.2f} ms/batch -> {b * 1e3 / ms_s:.1f} q/s")
+        if rows:
+            result["batch_sweep"] = rows
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(args, log)

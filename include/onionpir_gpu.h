@@ -1,7 +1,7 @@
 /*
  * onionpir_gpu.h -- C ABI of the MI355X (gfx950) OnionPIRv2 server library.
  *
- * The host-side PirServer (onionpirv2_amd/host/server.h) calls only these
+ * The host-side PirServer (onionpirv2_amd/host/onionpir_host.h) calls only these
  * entry points; any other host language binds them the same way (ctypes,
  * cgo, JNI). Plain pointers and sizes only. Every call returns an int status
  * (OPIR_OK = 0, negative on error); opir_last_error() gives the message.

@@ -274,8 +274,7 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   const size_t exp_stride = db->exp_cts * kCtWords;
   rec(ctx, 0);
   // root ct per query -> exp_a[q][0]
-  for (size_t i = 0; i < B; i++)
-    launch_split_u64(d_q + i * 2 * kN, db->exp_a + i * exp_stride, 2, ctx->S, ctx->stream);
+  launch_split_u64_batch(d_q, 2 * kN, db->exp_a, exp_stride, 2, (int)B, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   u32 *leaves = run_expand(db, B, P);
   LAUNCH_CHECK(ctx);

@@ -54,6 +54,9 @@ struct BatchPtrs {
 };
 
 void launch_split_u64(const u64 *in, u32 *out, size_t npoly, const SchemeConst &S, hipStream_t st);
+// B groups of npoly polynomials: in + b*in_stride (u64 words) -> out + b*out_stride (u32 words).
+void launch_split_u64_batch(const u64 *in, size_t in_stride, u32 *out, size_t out_stride, size_t npoly, int B,
+                            const SchemeConst &S, hipStream_t st);
 void launch_join_u64(const u32 *in, u64 *out, size_t npoly, const SchemeConst &S, hipStream_t st);
 void launch_ntt_u64(u64 *data, size_t npoly, bool inverse, const SchemeConst &S, hipStream_t st);
 void launch_ntt_rns(const u32 *in, u32 *out, size_t npoly_rns, bool inverse, const SchemeConst &S,

@@ -63,9 +63,14 @@ __device__ __forceinline__ void compose_centre(const u32 *lo, const u32 *hi, int
 // ---------------------------------------------------------------------------
 // Elementwise helpers.
 // ---------------------------------------------------------------------------
-__global__ void k_split_u64(const u64 *__restrict__ in, u32 *__restrict__ out, size_t npoly, SchemeConst S) {
+// Group blockIdx.y: npoly polynomials at in + y*in_stride (u64 words) ->
+// out + y*out_stride (u32 words).
+__global__ void k_split_u64(const u64 *__restrict__ in, u32 *__restrict__ out, size_t npoly, size_t in_stride,
+                            size_t out_stride, SchemeConst S) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npoly * kN) return;
+  in += blockIdx.y * in_stride;
+  out += blockIdx.y * out_stride;
   const size_t p = i / kN, j = i % kN;
   const u64 x = in[i];
   out[(2 * p + 0) * kN + j] = reduce_u64(x, S.limb[0]);
@@ -82,7 +87,7 @@ __global__ void k_join_u64(const u32 *__restrict__ in, u64 *__restrict__ out, si
 // In-place NTT of u64 (mod Q) polynomials; one workgroup (2 lane blocks,
 // one per limb) per polynomial.
 __global__ __launch_bounds__(512) void k_ntt_u64(u64 *data, int inverse, SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 res[2][kN];
   const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
   const LimbConst &L = S.limb[sb];
@@ -108,7 +113,7 @@ __global__ __launch_bounds__(512) void k_ntt_u64(u64 *data, int inverse, SchemeC
 
 // RNS polys [npoly][2 limbs][N]: coef (map0) <-> NTT (map3).
 __global__ __launch_bounds__(512) void k_ntt_rns(const u32 *in, u32 *out, int inverse, SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
   const LimbConst &L = S.limb[sb];
   const size_t off = ((size_t)blockIdx.x * 2 + sb) * kN;
@@ -131,7 +136,7 @@ __global__ __launch_bounds__(512) void k_ntt_rns(const u32 *in, u32 *out, int in
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(512) void k_db_ntt_plain(const uint16_t *__restrict__ pts, u32 *__restrict__ tmp,
                                                       SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
   const LimbConst &L = S.limb[sb];
   const uint16_t *p = pts + (size_t)blockIdx.x * kN;
@@ -146,7 +151,7 @@ __global__ __launch_bounds__(512) void k_db_ntt_plain(const uint16_t *__restrict
 // query transform, server.cpp:331-337). grid (count*2, B).
 __global__ __launch_bounds__(512) void k_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride,
                                                 SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
   const LimbConst &L = S.limb[sb];
   const size_t poly = blockIdx.x;  // ct*2 + p
@@ -184,7 +189,7 @@ __device__ __forceinline__ u64 rescale_to_small(u64 x, const SchemeConst &S) {
 }
 
 __global__ __launch_bounds__(512) void k_final(const u32 *res, u64 *out, size_t res_stride, SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 sh[2][kN];
   const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
   const LimbConst &L = S.limb[sb];
@@ -200,7 +205,7 @@ __global__ __launch_bounds__(512) void k_final(const u32 *res, u64 *out, size_t 
 
 // NTT-form RNS cts -> coefficient-form u64 mod Q. grid (count*2).
 __global__ __launch_bounds__(512) void k_ct_to_coef_u64(const u32 *in, u64 *out, SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 sh[2][kN];
   const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
   const LimbConst &L = S.limb[sb];
@@ -216,7 +221,7 @@ __global__ __launch_bounds__(512) void k_ct_to_coef_u64(const u32 *in, u64 *out,
 // Stand-alone external product (gsw.cpp:119-182): ct coef RNS -> out NTT RNS.
 __global__ __launch_bounds__(512) void k_ext_prod(const u32 *gsw, int l, int b, const u32 *ct, u32 *out,
                                                   SchemeConst S) {
-  __shared__ u32 nlds[2][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
   const LimbConst &L = S.limb[sb];
   i64 D[2][8];
@@ -247,7 +252,14 @@ __global__ __launch_bounds__(512) void k_ext_prod(const u32 *gsw, int l, int b, 
 static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 void launch_split_u64(const u64 *in, u32 *out, size_t npoly, const SchemeConst &S, hipStream_t st) {
-  k_split_u64<<<cdiv(npoly * kN, 256), 256, 0, st>>>(in, out, npoly, S);
+  k_split_u64<<<cdiv(npoly * kN, 256), 256, 0, st>>>(in, out, npoly, 0, 0, S);
+}
+
+void launch_split_u64_batch(const u64 *in, size_t in_stride, u32 *out, size_t out_stride, size_t npoly, int B,
+                            const SchemeConst &S, hipStream_t st) {
+  if (B <= 0) return;
+  dim3 grid((unsigned)cdiv(npoly * kN, 256), (unsigned)B);
+  k_split_u64<<<grid, 256, 0, st>>>(in, out, npoly, in_stride, out_stride, S);
 }
 void launch_join_u64(const u32 *in, u64 *out, size_t npoly, const SchemeConst &S, hipStream_t st) {
   k_join_u64<<<cdiv(npoly * kN, 256), 256, 0, st>>>(in, out, npoly, S);

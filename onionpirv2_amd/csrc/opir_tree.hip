@@ -80,7 +80,7 @@ __device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int
 // block h finishes output polynomial h (h = 0: c0 side, h = 1: c1 side).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_expand_level(ExpandArgs A, SchemeConst S) {
-  __shared__ u32 nlds[4][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[4][kLdsPoly];
   __shared__ u32 sh[2][2][kN];  // [poly][limb]: c, later partial sums, later shift staging
   __shared__ u32 sig[2][kN];    // sigma(c1) per limb
   const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // lb is wave-uniform
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(1024) void k_expand_level(ExpandArgs A, SchemeConst
 // Lane block (limb, p): poly p of the row, all l_key digits of it.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_complete(CompleteArgs A, SchemeConst S) {
-  __shared__ u32 nlds[4][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[4][kLdsPoly];
   __shared__ u32 sh[2][2][kN];
   const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // lb is wave-uniform
   const int limb = lb & 1, p = lb >> 1;
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(1024) void k_complete(CompleteArgs A, SchemeConst S
 // Lane block (limb, p): INTT of poly p of y - x, then the l digits of poly p.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_mux(MuxArgs A, SchemeConst S) {
-  __shared__ u32 nlds[4][kLdsPoly];
+  __shared__ __align__(16) u32 nlds[4][kLdsPoly];
   __shared__ u32 sh[2][2][kN];
   const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // lb is wave-uniform
   const int limb = lb & 1, p = lb >> 1;
