@@ -1,6 +1,7 @@
 // C ABI of the OnionPIRv2 MI355X server library (include/onionpir_gpu.h).
 // Orchestrates the kernels of opir_kernels.hip along PirServer::make_query
 // (src/server.cpp:895-947).
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -319,6 +320,19 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   if (sch->l_ep == 0 || sch->l_key == 0 || sch->l_ks == 0 || sch->base_log2 == 0 || sch->base_log2 > 32 ||
       sch->base_log2_key == 0 || sch->base_log2_key > 32 || sch->bv_base_log2 == 0 || sch->bv_base_log2 > 32)
     return set_err(nullptr, OPIR_EINVAL, "invalid gadget parameters");
+  {
+    // Digits must be residues (|r| <= 2^(b-1) < q), their shifts must fit an
+    // int64, and each lane block's 64-bit digit x key sums (NTT outputs kept
+    // in [0, 4q)) must not overflow: terms * (4q - 1) * (q - 1) < 2^64.
+    const uint64_t qmin = q1 < q2 ? q1 : q2, qmax = q1 < q2 ? q2 : q1;
+    const uint64_t bmax = std::max({sch->base_log2, sch->base_log2_key, sch->bv_base_log2});
+    const uint64_t terms = std::max({(uint64_t)sch->l_ep, (uint64_t)sch->l_key, (uint64_t)(sch->l_ks + 1) / 2});
+    const unsigned __int128 worst = (unsigned __int128)terms * (4 * qmax - 1) * (qmax - 1);
+    if ((1ULL << (bmax - 1)) >= qmin || (uint64_t)sch->base_log2 * (sch->l_ep - 1) > 62 ||
+        (uint64_t)sch->base_log2_key * (sch->l_key - 1) > 62 || (uint64_t)sch->bv_base_log2 * (sch->l_ks - 1) > 62 ||
+        worst >> 64)
+      return set_err(nullptr, OPIR_EINVAL, "gadget too long or too wide for exact 64-bit digit sums");
+  }
   if (sch->small_q < 2 || sch->small_q >= (1ULL << 32))
     return set_err(nullptr, OPIR_EINVAL, "small_q must fit 32 bits");
   opir_ctx *ctx = new (std::nothrow) opir_ctx();

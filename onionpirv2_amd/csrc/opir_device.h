@@ -110,10 +110,13 @@ __device__ __forceinline__ int lds_idx(int i) { return i + ((i >> 5) << 2); }
 // bar() is the barrier for the lane block (whole workgroup barrier).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void bf_fwd(u32 &X, u32 &Y, u32 w, u32 ws, const LimbConst &L) {
+  // t = Y*w - hi*q lies in [0, 2q); tn = -t mod 2^32 lets both outputs take
+  // one instruction each (x - tn, and a three-input add for x + 2q - t).
   const u32 x = csub(X, L.q2);
-  const u32 t = mul_shoup(Y, w, ws, L.q);
-  X = x + t;
-  Y = x - t + L.q2;
+  const u32 hi = __umulhi(Y, ws);
+  const u32 tn = hi * L.q - Y * w;
+  X = x - tn;
+  Y = x + tn + L.q2;
 }
 
 __device__ __forceinline__ void bf_inv(u32 &X, u32 &Y, u32 w, u32 ws, const LimbConst &L) {
@@ -122,45 +125,60 @@ __device__ __forceinline__ void bf_inv(u32 &X, u32 &Y, u32 w, u32 ws, const Limb
   Y = mul_shoup(x - y + L.q2, w, ws, L.q);
 }
 
-// Forward radix-8 round on a column (indices base + c*T); twiddle groups g.
+// Twiddles of one radix-8 group b: w[b], w[2b..2b+1], w[4b..4b+3], loaded as
+// one dword, one dwordx2 and one dwordx4 (tables are 256-byte aligned).
+struct Tw8 {
+  u32 w1, s1;
+  uint2 w2, s2;
+  uint4 w4, s4;
+};
+// Load at a uniform base plus a 32-bit byte offset, so that the address is
+// formed by the load itself (scalar base + vector offset) instead of 64-bit
+// vector adds.
+template <typename T>
+__device__ __forceinline__ T ld_off(const u32 *__restrict__ base, unsigned byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+__device__ __forceinline__ Tw8 load_tw8(const u32 *__restrict__ w, const u32 *__restrict__ ws, unsigned b) {
+  Tw8 r;
+  r.w1 = ld_off<u32>(w, b << 2);
+  r.s1 = ld_off<u32>(ws, b << 2);
+  r.w2 = ld_off<uint2>(w, b << 3);
+  r.s2 = ld_off<uint2>(ws, b << 3);
+  r.w4 = ld_off<uint4>(w, b << 4);
+  r.s4 = ld_off<uint4>(ws, b << 4);
+  return r;
+}
+
+// Forward radix-8 round on a column (indices base + c*T); twiddle group
+// b = m0 + g (stage offsets m0, 2 m0, 4 m0).
 __device__ __forceinline__ void fwd_round8(u32 v[8], int m0, int g, const LimbConst &L) {
-  {
-    const u32 w = L.psi[m0 + g], ws = L.psi_s[m0 + g];
+  const Tw8 T = load_tw8(L.psi, L.psi_s, (unsigned)(m0 + g));
 #pragma unroll
-    for (int c = 0; c < 4; c++) bf_fwd(v[c], v[c + 4], w, ws, L);
-  }
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int idx = 2 * m0 + 2 * g + h;
-    const u32 w = L.psi[idx], ws = L.psi_s[idx];
-    bf_fwd(v[4 * h + 0], v[4 * h + 2], w, ws, L);
-    bf_fwd(v[4 * h + 1], v[4 * h + 3], w, ws, L);
-  }
-#pragma unroll
-  for (int h = 0; h < 4; h++) {
-    const int idx = 4 * m0 + 4 * g + h;
-    bf_fwd(v[2 * h], v[2 * h + 1], L.psi[idx], L.psi_s[idx], L);
-  }
+  for (int c = 0; c < 4; c++) bf_fwd(v[c], v[c + 4], T.w1, T.s1, L);
+  bf_fwd(v[0], v[2], T.w2.x, T.s2.x, L);
+  bf_fwd(v[1], v[3], T.w2.x, T.s2.x, L);
+  bf_fwd(v[4], v[6], T.w2.y, T.s2.y, L);
+  bf_fwd(v[5], v[7], T.w2.y, T.s2.y, L);
+  bf_fwd(v[0], v[1], T.w4.x, T.s4.x, L);
+  bf_fwd(v[2], v[3], T.w4.y, T.s4.y, L);
+  bf_fwd(v[4], v[5], T.w4.z, T.s4.z, L);
+  bf_fwd(v[6], v[7], T.w4.w, T.s4.w, L);
 }
 
 __device__ __forceinline__ void inv_round8(u32 v[8], int m0, int g, const LimbConst &L) {
+  const Tw8 T = load_tw8(L.ipsi, L.ipsi_s, (unsigned)(m0 + g));
+  bf_inv(v[0], v[1], T.w4.x, T.s4.x, L);
+  bf_inv(v[2], v[3], T.w4.y, T.s4.y, L);
+  bf_inv(v[4], v[5], T.w4.z, T.s4.z, L);
+  bf_inv(v[6], v[7], T.w4.w, T.s4.w, L);
+  bf_inv(v[0], v[2], T.w2.x, T.s2.x, L);
+  bf_inv(v[1], v[3], T.w2.x, T.s2.x, L);
+  bf_inv(v[4], v[6], T.w2.y, T.s2.y, L);
+  bf_inv(v[5], v[7], T.w2.y, T.s2.y, L);
 #pragma unroll
-  for (int h = 0; h < 4; h++) {
-    const int idx = 4 * m0 + 4 * g + h;
-    bf_inv(v[2 * h], v[2 * h + 1], L.ipsi[idx], L.ipsi_s[idx], L);
-  }
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int idx = 2 * m0 + 2 * g + h;
-    const u32 w = L.ipsi[idx], ws = L.ipsi_s[idx];
-    bf_inv(v[4 * h + 0], v[4 * h + 2], w, ws, L);
-    bf_inv(v[4 * h + 1], v[4 * h + 3], w, ws, L);
-  }
-  {
-    const u32 w = L.ipsi[m0 + g], ws = L.ipsi_s[m0 + g];
-#pragma unroll
-    for (int c = 0; c < 4; c++) bf_inv(v[c], v[c + 4], w, ws, L);
-  }
+  for (int c = 0; c < 4; c++) bf_inv(v[c], v[c + 4], T.w1, T.s1, L);
 }
 
 // Index of value c of thread t in radix-8 round r (r = 0, 1, 2).
@@ -170,67 +188,105 @@ __device__ __forceinline__ int r8_index(int r, int t, int c) {
   return g * 8 * T + u + c * T;
 }
 
-template <typename Bar>
-__device__ __forceinline__ void lds_exchange(u32 v[8], u32 *lds, int t, int from_r, int to_r, Bar bar) {
-  // from_r/to_r in {0,1,2,3}; 3 = consecutive mapping.
+// Padded LDS word of value c of thread t in mapping r, written as
+// lds_base<r>(t) + c * kLdsStride<r>: the padding i + (i>>5)*4 adds a
+// constant per c because every mapping's c-step is either a multiple of 32
+// (r = 0, 1) or stays inside one 32-word row (r = 2, 3). The constant
+// strides let every LDS access use an immediate offset.
+//   r = 0: i = t + 256c                  -> (t + 4(t>>5)) + 288c
+//   r = 1: i = 256(t>>5) + (t&31) + 32c  -> (288(t>>5) + (t&31)) + 36c
+//   r = 2: i = 32(t>>2) + (t&3) + 4c     -> (36(t>>2) + (t&3)) + 4c
+//   r = 3: i = 8t + c                    -> (8t + 4(t>>2)) + c
+template <int R> __device__ __forceinline__ int lds_base(int t) {
+  if constexpr (R == 0) return t + ((t >> 5) << 2);
+  else if constexpr (R == 1) return 288 * (t >> 5) + (t & 31);
+  else if constexpr (R == 2) return 36 * (t >> 2) + (t & 3);
+  else return 8 * t + ((t >> 2) << 2);
+}
+template <int R> constexpr int kLdsStride = R == 0 ? 288 : R == 1 ? 36 : R == 2 ? 4 : 1;
+
+template <int R> __device__ __forceinline__ void lds_put(u32 *lds, int t, const u32 v[8]) {
+  u32 *p = lds + lds_base<R>(t);
+  if constexpr (R == 3) {  // 8 consecutive words, 16-byte aligned
+    reinterpret_cast<uint4 *>(p)[0] = make_uint4(v[0], v[1], v[2], v[3]);
+    reinterpret_cast<uint4 *>(p)[1] = make_uint4(v[4], v[5], v[6], v[7]);
+  } else {
 #pragma unroll
-  for (int c = 0; c < 8; c++) {
-    const int i = from_r == 3 ? 8 * t + c : r8_index(from_r, t, c);
-    lds[lds_idx(i)] = v[c];
+    for (int c = 0; c < 8; c++) p[c * kLdsStride<R>] = v[c];
   }
+}
+template <int R> __device__ __forceinline__ void lds_get(const u32 *lds, int t, u32 v[8]) {
+  const u32 *p = lds + lds_base<R>(t);
+  if constexpr (R == 3) {
+    const uint4 a = reinterpret_cast<const uint4 *>(p)[0], b = reinterpret_cast<const uint4 *>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = p[c * kLdsStride<R>];
+  }
+}
+
+template <int FROM, int TO, typename Bar>
+__device__ __forceinline__ void lds_exchange(u32 v[8], u32 *lds, int t, Bar bar) {
+  lds_put<FROM>(lds, t, v);
   bar();
-#pragma unroll
-  for (int c = 0; c < 8; c++) {
-    const int i = to_r == 3 ? 8 * t + c : r8_index(to_r, t, c);
-    v[c] = lds[lds_idx(i)];
-  }
+  lds_get<TO>(lds, t, v);
   bar();
 }
 
-template <typename Bar>
+// kReduce = false leaves outputs lazily in [0, 4q) (< 2^31), for callers that
+// only multiply-accumulate them into 64-bit sums.
+template <bool kReduce = true, typename Bar>
 __device__ __forceinline__ void ntt_fwd_2048(u32 v[8], u32 *lds, int t, const LimbConst &L, Bar bar) {
   fwd_round8(v, 1, 0, L);                       // stages 0-2 (map0, g = 0)
-  lds_exchange(v, lds, t, 0, 1, bar);
+  lds_exchange<0, 1>(v, lds, t, bar);
   fwd_round8(v, 8, t / 32, L);                  // stages 3-5
-  lds_exchange(v, lds, t, 1, 2, bar);
+  lds_exchange<1, 2>(v, lds, t, bar);
   fwd_round8(v, 64, t / 4, L);                  // stages 6-8
-  lds_exchange(v, lds, t, 2, 3, bar);
-  // stages 9-10 on 8 consecutive values
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int idx = 512 + 2 * t + h;
-    const u32 w = L.psi[idx], ws = L.psi_s[idx];
-    bf_fwd(v[4 * h + 0], v[4 * h + 2], w, ws, L);
-    bf_fwd(v[4 * h + 1], v[4 * h + 3], w, ws, L);
+  lds_exchange<2, 3>(v, lds, t, bar);
+  // stages 9-10 on 8 consecutive values: twiddles w[512 + 2t + h], w[1024 + 4t + h]
+  {
+    const unsigned u = (unsigned)t;
+    const uint2 w2 = ld_off<uint2>(L.psi, (512 * 4) + (u << 3));
+    const uint2 s2 = ld_off<uint2>(L.psi_s, (512 * 4) + (u << 3));
+    const uint4 w4 = ld_off<uint4>(L.psi, (1024 * 4) + (u << 4));
+    const uint4 s4 = ld_off<uint4>(L.psi_s, (1024 * 4) + (u << 4));
+    bf_fwd(v[0], v[2], w2.x, s2.x, L);
+    bf_fwd(v[1], v[3], w2.x, s2.x, L);
+    bf_fwd(v[4], v[6], w2.y, s2.y, L);
+    bf_fwd(v[5], v[7], w2.y, s2.y, L);
+    bf_fwd(v[0], v[1], w4.x, s4.x, L);
+    bf_fwd(v[2], v[3], w4.y, s4.y, L);
+    bf_fwd(v[4], v[5], w4.z, s4.z, L);
+    bf_fwd(v[6], v[7], w4.w, s4.w, L);
   }
 #pragma unroll
-  for (int h = 0; h < 4; h++) {
-    const int idx = 1024 + 4 * t + h;
-    bf_fwd(v[2 * h], v[2 * h + 1], L.psi[idx], L.psi_s[idx], L);
-  }
-#pragma unroll
-  for (int c = 0; c < 8; c++) v[c] = reduce_4q(v[c], L.q, L.q2);
+  for (int c = 0; c < 8; c++) v[c] = kReduce ? reduce_4q(v[c], L.q, L.q2) : v[c];
 }
 
 template <typename Bar>
 __device__ __forceinline__ void ntt_inv_2048(u32 v[8], u32 *lds, int t, const LimbConst &L, Bar bar) {
-#pragma unroll
-  for (int h = 0; h < 4; h++) {
-    const int idx = 1024 + 4 * t + h;
-    bf_inv(v[2 * h], v[2 * h + 1], L.ipsi[idx], L.ipsi_s[idx], L);
+  {
+    const unsigned u = (unsigned)t;
+    const uint4 w4 = ld_off<uint4>(L.ipsi, (1024 * 4) + (u << 4));
+    const uint4 s4 = ld_off<uint4>(L.ipsi_s, (1024 * 4) + (u << 4));
+    const uint2 w2 = ld_off<uint2>(L.ipsi, (512 * 4) + (u << 3));
+    const uint2 s2 = ld_off<uint2>(L.ipsi_s, (512 * 4) + (u << 3));
+    bf_inv(v[0], v[1], w4.x, s4.x, L);
+    bf_inv(v[2], v[3], w4.y, s4.y, L);
+    bf_inv(v[4], v[5], w4.z, s4.z, L);
+    bf_inv(v[6], v[7], w4.w, s4.w, L);
+    bf_inv(v[0], v[2], w2.x, s2.x, L);
+    bf_inv(v[1], v[3], w2.x, s2.x, L);
+    bf_inv(v[4], v[6], w2.y, s2.y, L);
+    bf_inv(v[5], v[7], w2.y, s2.y, L);
   }
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int idx = 512 + 2 * t + h;
-    const u32 w = L.ipsi[idx], ws = L.ipsi_s[idx];
-    bf_inv(v[4 * h + 0], v[4 * h + 2], w, ws, L);
-    bf_inv(v[4 * h + 1], v[4 * h + 3], w, ws, L);
-  }
-  lds_exchange(v, lds, t, 3, 2, bar);
+  lds_exchange<3, 2>(v, lds, t, bar);
   inv_round8(v, 64, t / 4, L);
-  lds_exchange(v, lds, t, 2, 1, bar);
+  lds_exchange<2, 1>(v, lds, t, bar);
   inv_round8(v, 8, t / 32, L);
-  lds_exchange(v, lds, t, 1, 0, bar);
+  lds_exchange<1, 0>(v, lds, t, bar);
   inv_round8(v, 1, 0, L);
 #pragma unroll
   for (int c = 0; c < 8; c++) v[c] = csub(mul_shoup(v[c], L.ninv, L.ninv_s, L.q), L.q);

@@ -24,12 +24,16 @@ struct Barrier {
   __device__ void operator()() const { __syncthreads(); }
 };
 
-__device__ __forceinline__ i64 digit_closed(i64 D, int i, int b) {
+// Digit i of the balanced base-2^b expansion of D, as a residue mod q:
+// sext_b(floor((D + H_i) / 2^(b*i))), H_i = 2^(b-1) * sum_{j<i} 2^(b*j).
+__device__ __forceinline__ i64 digit_offset(int i, int b) {
   i64 H = 0;
   for (int j = 0; j < i; j++) H += ((i64)1 << (b - 1)) << (b * j);
-  const i64 di = (D + H) >> (b * i);
-  const i64 half = (i64)1 << (b - 1);
-  return ((di + half) & (((i64)1 << b) - 1)) - half;
+  return H;
+}
+__device__ __forceinline__ u32 digit_residue(i64 D, i64 H, int shift, int b, u32 q) {
+  const int r = __builtin_amdgcn_sbfe((int)(u32)((D + H) >> shift), 0, b);
+  return (u32)r + (q & (u32)(r >> 31));
 }
 
 __device__ __forceinline__ void compose_centre8(const u32 *lo, const u32 *hi, int t, const SchemeConst &S,
@@ -54,14 +58,18 @@ __device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int
     // inside the loop; hoisted, they do not fit in 128 registers.
     int t = t_in;
     asm volatile("" : "+v"(t));
+    const i64 H = digit_offset(i, b);
     u32 v[8];
 #pragma unroll
-    for (int c = 0; c < 8; c++) v[c] = digit_mod(digit_closed(D[c], i, b), L.q);
-    ntt_fwd_2048(v, nttlds, t, L, bar);
-    const u32 *r = key + (size_t)(row_base + (rev ? l - 1 - i : i)) * kCtWords;
-    u32 k0[8], k1[8];
-    load_map3(k0, r + (0 * 2 + limb) * kN, t);
-    load_map3(k1, r + (1 * 2 + limb) * kN, t);
+    for (int c = 0; c < 8; c++) v[c] = digit_residue(D[c], H, b * i, b, L.q);
+    ntt_fwd_2048<false>(v, nttlds, t, L, bar);  // [0, 4q): at most 10 products of < 2^60 per sum
+    // key row: [2 outputs][2 limbs][N] words; byte offsets fit in 32 bits
+    const unsigned row = (unsigned)(row_base + (rev ? l - 1 - i : i));
+    const unsigned off = row * (unsigned)(kCtWords * 4) + (unsigned)limb * (kN * 4) + ((unsigned)t << 5);
+    const uint4 a0 = ld_off<uint4>(key, off), a1 = ld_off<uint4>(key, off + 16);
+    const uint4 b0 = ld_off<uint4>(key, off + 2 * kN * 4), b1 = ld_off<uint4>(key, off + 2 * kN * 4 + 16);
+    const u32 k0[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    const u32 k1[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
     for (int c = 0; c < 8; c++) {
       acc0[c] += (u64)v[c] * k0[c];
