@@ -42,6 +42,12 @@ def parse():
     ap.add_argument("--sweep", default="1,8,32,64,128",
                     help="batch sizes of the batch sweep (comma list; empty to skip)")
     ap.add_argument("--seed", type=int, default=20251142)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
+    ap.add_argument("--serve-lambdas", default="50,200,1000,2000", help="Poisson arrival rates (1/s)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per rate")
+    ap.add_argument("--serve-clients", type=int, default=512)
+    ap.add_argument("--serve-budget-gib", type=float, default=48.0,
+                    help="HBM budget for client keys and batch workspaces of the serving loop")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="run a plain device copy of this many GiB first (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against known bytes)")
@@ -84,6 +90,67 @@ def committed_traffic(args):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":")[1]), os.path.relpath(path, ROOT)
     return None, None
+
+
+def serving(args, H, params, server, nkeyed, log):
+    """Continuous batching over two resident databases: the benchmark DB
+    (256 x 1024) and a 256 x 256 one (1 GiB NTT form), many clients with
+    their own keys on the device, Poisson arrivals swept over rates. Every
+    response is checked against the same query answered alone."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    p2 = H.PirParams().with_layout(65536, 9, True)
+    s2 = H.PirServer(p2)
+    s2.gen_data([], args.seed + 11)
+    servers, plist = [server, s2], [params, p2]
+    nc = args.serve_clients
+    t0 = time.time()
+
+    def mk(j):
+        c = H.PirClient(params, 5000 + j, args.seed + 1000003 * (j + 1))
+        return c, c.create_bv_galois_keys(), c.generate_gsw_from_key()
+
+    with ThreadPoolExecutor(8) as ex:
+        cl = list(ex.map(mk, range(nc)))
+    for j, (c, bv, gsw) in enumerate(cl):
+        for s in servers:
+            s.set_client_bv_galois_key(5000 + j, bv)
+            s.set_client_gsw_key(5000 + j, gsw)
+    per_db = 64
+    entries = [(d, (d * per_db + k) % nc) for d in range(2) for k in range(per_db)]
+
+    def mq(e):
+        d, j = e
+        idx = splitmix64(args.seed ^ (d * 1000 + j)) % plist[d].target_num_pt
+        return cl[j][0].fast_generate_query(plist[d], idx)
+
+    with ThreadPoolExecutor(8) as ex:
+        qs = list(ex.map(mq, entries))
+    alone = [servers[d].make_query(5000 + j, q.copy()) for (d, j), q in zip(entries, qs)]
+    log(f"[serve] {nc} clients keyed on 2 databases, {len(entries)}-query pool answered alone, "
+        f"{time.time() - t0:.1f}s")
+    loop = H.ServingLoop(servers, max_batch=64, hbm_budget=int(args.serve_budget_gib * 2**30),
+                         keyed_clients=nkeyed + nc)
+    rows = []
+    try:
+        for lam in [float(x) for x in args.serve_lambdas.split(",") if x]:
+            run = loop.run_poisson([d for d, _ in entries], [5000 + j for _, j in entries], np.stack(qs), lam,
+                                   args.serve_seconds, args.seed + int(lam))
+            n = len(run["pool_index"])
+            exact = sum(np.array_equal(run["resp"][i], alone[run["pool_index"][i]]) for i in range(n))
+            lat = run["latency_ms"].tolist()
+            rows.append({"lambda_qps": lam, "requests": n, "achieved_qps": round(run["achieved_qps"], 1),
+                         "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
+                         "mean_batch": round(float(run["batch"].mean()), 2), "bit_exact_vs_alone": f"{exact}/{n}",
+                         "errors": run["errors"]})
+            log(f"[serve] lambda {lam:.0f}/s: {n} requests, {run['achieved_qps']:.0f} q/s, p50 {pct(lat, 50):.2f} ms "
+                f"p99 {pct(lat, 99):.2f} ms, mean batch {run['batch'].mean():.1f}, exact {exact}/{n}")
+            assert exact == n and run["errors"] == 0, "served response differs from the query answered alone"
+        limits = list(loop.batch_limits)
+    finally:
+        loop.stop()
+    return {"databases": ["256x1024 (4 GiB NTT form)", "256x256 (1 GiB NTT form)"], "clients": nc,
+            "hbm_budget_gib": args.serve_budget_gib, "batch_limits": limits, "rates": rows}
 
 
 def cpu_baseline(args, log):
@@ -283,6 +350,9 @@ def main():
             log(f"[sweep] B={b}: {ms_s:.2f} ms/batch -> {b * 1e3 / ms_s:.1f} q/s")
         if rows:
             result["batch_sweep"] = rows
+    if not args.no_serve:
+        torch.cuda.synchronize()
+        result["serving"] = serving(args, H, params, server, nclients, log)
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(args, log)

@@ -105,6 +105,16 @@ int opir_answer_batch(opir_db *db, size_t B, const uint64_t *client_ids, const u
 int opir_answer_batch_device(opir_db *db, size_t B, const uint64_t *client_ids, const uint64_t *d_queries,
                              uint64_t *d_resps);
 
+/* Serving support. No counterpart in the reference, whose PirServer answers
+ * one query per make_query call; a serving loop uses these to admit batches
+ * by HBM budget. workspace_bytes: device bytes of the answer workspace for
+ * batches of up to B queries against db. reserve: allocate that workspace
+ * now (it is kept and only grows). key_bytes: device bytes of one client's
+ * keys at expansion height h. */
+int opir_db_workspace_bytes(const opir_db *db, size_t B, size_t *out);
+int opir_db_reserve(opir_db *db, size_t B);
+int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out);
+
 /* Per-stage device time of the last answer call under the reference's stage
  * names (src/includes/logging.h:122-134): [0] Expand, [1] Convert,
  * [2] First dim, [3] Other dim, [4] Modulus switching, [5] scan kernel only.

@@ -25,7 +25,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 GPU_SRC = ["csrc/opir_kernels.hip", "csrc/opir_tree.hip", "csrc/opir_scan.hip", "csrc/opir_capi.hip"]
-HOST_SRC = ["host/params.cpp", "host/ring.cpp", "host/client.cpp", "host/server.cpp", "host/bindings.cpp"]
+HOST_SRC = ["host/params.cpp", "host/ring.cpp", "host/client.cpp", "host/server.cpp", "host/serving.cpp",
+            "host/bindings.cpp"]
 
 
 def _run(cmd):
@@ -45,7 +46,7 @@ def build_host():
     import pybind11
     inc = ["-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"], "-I" + os.path.join(ROOT, "include")]
     tmp = HOST_EXT + ".tmp"
-    _run(["g++", "-O3", "-march=x86-64-v3", "-std=c++20", "-fPIC", "-shared", "-Wall", "-Wno-unused-parameter"]
+    _run(["g++", "-O3", "-march=x86-64-v3", "-std=c++20", "-fPIC", "-shared", "-Wall", "-Wno-unused-parameter", "-pthread"]
          + inc + HOST_SRC + ["-L" + LIBDIR, "-lonionpir_gpu", "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, HOST_EXT)
 

@@ -171,6 +171,15 @@ size_t nodes_at_level(size_t u, size_t h, size_t L) {
   return n < full ? n : full;
 }
 
+// Device bytes of the answer workspace for batches of up to B queries (the
+// buffers ensure_ws allocates).
+size_t ws_bytes(const opir_db *db, size_t B) {
+  const size_t ct = kCtWords * sizeof(u32);
+  const size_t nsel = db->dims - 1;
+  return qfrag_bytes(make_qfrag_geom(db->N0, (int)B)) + 2 * B * db->exp_cts * ct + B * db->N0 * ct +
+         B * (nsel ? nsel : 1) * 2 * db->ctx->G.l_ep * ct + 2 * B * db->Nrest * ct + 2 * B * 2 * kN * sizeof(u64);
+}
+
 int ensure_ws(opir_db *db, size_t B) {
   opir_ctx *ctx = db->ctx;
   if (db->ws_B >= B) return OPIR_OK;
@@ -610,6 +619,23 @@ int opir_client_remove(opir_ctx *ctx, uint64_t id) {
   (void)hipFree(it->second.bv);
   (void)hipFree(it->second.gsw);
   ctx->clients.erase(it);
+  return OPIR_OK;
+}
+
+int opir_db_workspace_bytes(const opir_db *db, size_t B, size_t *out) {
+  if (!db || !out) return OPIR_EINVAL;
+  *out = ws_bytes(db, B);
+  return OPIR_OK;
+}
+
+int opir_db_reserve(opir_db *db, size_t B) {
+  if (!db) return OPIR_EINVAL;
+  return B ? ensure_ws(db, B) : OPIR_OK;
+}
+
+int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out) {
+  if (!ctx || !out) return OPIR_EINVAL;
+  *out = (h * ctx->G.l_ks + 2 * (size_t)ctx->G.l_key) * kCtWords * sizeof(u32);
   return OPIR_OK;
 }
 

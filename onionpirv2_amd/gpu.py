@@ -23,6 +23,7 @@ EXPORTED = [
     "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt",
     "opir_client_set_keys", "opir_client_remove",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
+    "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes",
     "opir_ctx_set_profiling", "opir_last_stage_ms",
     "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
     "opir_ext_prod", "opir_mod_switch",
@@ -80,6 +81,9 @@ def load_library():
         "opir_answer": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
         "opir_answer_batch": ([P, sz, P, P, P], ctypes.c_int),
         "opir_answer_batch_device": ([P, sz, P, P, P], ctypes.c_int),
+        "opir_db_workspace_bytes": ([P, sz, ctypes.POINTER(sz)], ctypes.c_int),
+        "opir_db_reserve": ([P, sz], ctypes.c_int),
+        "opir_client_key_bytes": ([P, sz, ctypes.POINTER(sz)], ctypes.c_int),
         "opir_ctx_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
         "opir_last_stage_ms": ([P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "opir_ntt_fwd": ([P, P, sz], ctypes.c_int),

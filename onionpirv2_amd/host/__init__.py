@@ -1,4 +1,5 @@
-"""Host API (C++ via pybind11): PirParams, PirClient, PirServer, GpuScheme.
+"""Host API (C++ via pybind11): PirParams, PirClient, PirServer, GpuScheme, and the
+continuous-batching ServingLoop over several resident databases.
 
 Importing fails loudly if the native module has not been built
 (`python -m onionpirv2_amd.build`).
@@ -10,6 +11,6 @@ _LIB = _os.path.join(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__
 if _LIB not in _sys.path:
     _sys.path.insert(0, _LIB)
 try:
-    from _opir_host import GpuScheme, PirClient, PirParams, PirServer  # noqa: F401
+    from _opir_host import GpuScheme, PirClient, PirParams, PirServer, ServingLoop, plan_batch_limits  # noqa: F401
 except ImportError as e:  # pragma: no cover
     raise ImportError(f"onionpirv2_amd native host module is not built in {_LIB}: {e}") from e

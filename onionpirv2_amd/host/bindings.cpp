@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "onionpir_host.h"
+#include "serving.h"
 
 namespace py = pybind11;
 using namespace onionpir;
@@ -122,9 +123,33 @@ PYBIND11_MODULE(_opir_host, m) {
         return a;
       })
       .def("get_query_indices", &PirClient::get_query_indices)
-      .def("fast_generate_query", [](PirClient &c, const PirParams &p, size_t i) { return ct_arr(c.fast_generate_query(p, i)); })
-      .def("create_bv_galois_keys", [](PirClient &c) { return bv_arr(c.create_bv_galois_keys()); })
-      .def("generate_gsw_from_key", [](PirClient &c) { return gsw_arr(c.generate_gsw_from_key()); })
+      .def("fast_generate_query",
+           [](PirClient &c, const PirParams &p, size_t i) {
+             RlweCt q;
+             {
+               py::gil_scoped_release nogil;
+               q = c.fast_generate_query(p, i);
+             }
+             return ct_arr(q);
+           })
+      .def("create_bv_galois_keys",
+           [](PirClient &c) {
+             bvks::BvGaloisKeys k;
+             {
+               py::gil_scoped_release nogil;
+               k = c.create_bv_galois_keys();
+             }
+             return bv_arr(k);
+           })
+      .def("generate_gsw_from_key",
+           [](PirClient &c) {
+             GSWCt g;
+             {
+               py::gil_scoped_release nogil;
+               g = c.generate_gsw_from_key();
+             }
+             return gsw_arr(g);
+           })
       .def("load_resp_from_stream",
            [](const PirClient &c, py::bytes b) {
              std::stringstream ss(std::string(b), std::ios::in | std::ios::binary);
@@ -224,4 +249,93 @@ PYBIND11_MODULE(_opir_host, m) {
       .def("ctx_handle", [](const PirServer &s) { return (uintptr_t)s.gpu_scheme()->ctx(); })
       .def("gpu_scheme", &PirServer::gpu_scheme)
       .def_property_readonly("params", &PirServer::get_params);
+
+  m.def("plan_batch_limits", &plan_batch_limits, py::arg("ndb"), py::arg("ws"), py::arg("fixed"), py::arg("budget"),
+        py::arg("max_batch"));
+
+  py::class_<ServingLoop>(m, "ServingLoop")
+      .def(py::init([](std::vector<PirServer *> dbs, size_t max_batch, size_t hbm_budget, size_t keyed_clients) {
+             ServeConfig c;
+             c.max_batch = max_batch;
+             c.hbm_budget = hbm_budget;
+             c.keyed_clients = keyed_clients;
+             return std::make_unique<ServingLoop>(std::move(dbs), c);
+           }),
+           py::arg("dbs"), py::arg("max_batch") = 64, py::arg("hbm_budget") = 0, py::arg("keyed_clients") = 0,
+           py::keep_alive<1, 2>())
+      .def("submit",
+           [](ServingLoop &l, size_t db, size_t cid, Arr q) {
+             std::vector<u64> v(q.data(), q.data() + q.size());
+             return l.submit(db, cid, std::move(v));
+           })
+      .def("poll",
+           [](ServingLoop &l) {
+             std::vector<ServeResult> r;
+             l.poll(r);
+             py::list out;
+             for (auto &x : r) {
+               py::dict d;
+               d["ticket"] = x.ticket;
+               d["db"] = x.db;
+               d["client_id"] = x.client_id;
+               d["t_submit"] = x.t_submit;
+               d["t_start"] = x.t_start;
+               d["t_done"] = x.t_done;
+               d["batch"] = x.batch;
+               d["error"] = x.error;
+               const size_t n = x.resp.size() / 2;
+               Arr a({(py::ssize_t)2, (py::ssize_t)n});
+               if (n) std::memcpy(a.mutable_data(), x.resp.data(), 2 * n * 8);
+               d["resp"] = a;
+               out.append(d);
+             }
+             return out;
+           })
+      .def("wait_idle", &ServingLoop::wait_idle, py::call_guard<py::gil_scoped_release>())
+      .def("stop", &ServingLoop::stop, py::call_guard<py::gil_scoped_release>())
+      .def_property_readonly("batch_limits", &ServingLoop::batch_limits)
+      .def("run_poisson",
+           [](ServingLoop &l, std::vector<size_t> dbs, std::vector<size_t> cids, Arr qs, double lambda, double duration,
+              u64 seed) {
+             if (qs.ndim() != 3 || qs.shape(1) != 2 || (size_t)qs.shape(0) != dbs.size() || dbs.size() != cids.size())
+               throw std::invalid_argument("pool: dbs, client ids and queries [P, 2, N] must match");
+             const size_t P = qs.shape(0), N = qs.shape(2);
+             std::vector<PoolEntry> pool(P);
+             for (size_t i = 0; i < P; i++)
+               pool[i] = PoolEntry{dbs[i], cids[i], std::vector<u64>(qs.data() + i * 2 * N, qs.data() + (i + 1) * 2 * N)};
+             LoadRun run;
+             {
+               py::gil_scoped_release nogil;
+               run = run_poisson(l, pool, lambda, duration, seed);
+             }
+             const size_t n = run.results.size();
+             py::array_t<int64_t> idx((py::ssize_t)n), batch((py::ssize_t)n);
+             py::array_t<double> lat((py::ssize_t)n), wait((py::ssize_t)n);
+             Arr resp({(py::ssize_t)n, (py::ssize_t)2, (py::ssize_t)N});
+             size_t errors = 0;
+             for (size_t i = 0; i < n; i++) {
+               const ServeResult &x = run.results[i];
+               idx.mutable_data()[i] = (int64_t)run.pool_index[i];
+               batch.mutable_data()[i] = (int64_t)x.batch;
+               lat.mutable_data()[i] = (x.t_done - x.t_submit) * 1e3;
+               wait.mutable_data()[i] = (x.t_start - x.t_submit) * 1e3;
+               if (x.resp.size() == 2 * N)
+                 std::memcpy(resp.mutable_data() + i * 2 * N, x.resp.data(), 2 * N * 8);
+               else {
+                 std::memset(resp.mutable_data() + i * 2 * N, 0, 2 * N * 8);
+                 errors++;
+               }
+             }
+             py::dict d;
+             d["offered_qps"] = run.offered_qps;
+             d["achieved_qps"] = run.achieved_qps;
+             d["span_s"] = run.span_s;
+             d["pool_index"] = idx;
+             d["batch"] = batch;
+             d["latency_ms"] = lat;
+             d["queue_ms"] = wait;
+             d["resp"] = resp;
+             d["errors"] = errors;
+             return d;
+           });
 }

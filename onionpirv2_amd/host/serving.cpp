@@ -1,0 +1,206 @@
+// Continuous-batching serving loop (see serving.h).
+#include "serving.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <random>
+#include <stdexcept>
+
+#include "onionpir_gpu.h"
+
+namespace onionpir {
+
+std::vector<size_t> plan_batch_limits(size_t ndb, const std::function<size_t(size_t, size_t)> &ws, size_t fixed,
+                                      size_t budget, size_t max_batch) {
+  if (ndb == 0) throw std::invalid_argument("serving needs at least one database");
+  if (max_batch == 0) throw std::invalid_argument("max_batch must be positive");
+  std::vector<size_t> out(ndb, max_batch);
+  if (budget == 0) return out;
+  if (fixed >= budget) throw std::invalid_argument("HBM budget does not cover the resident client keys");
+  const size_t share = (budget - fixed) / ndb;
+  for (size_t d = 0; d < ndb; d++) {
+    if (ws(d, 1) > share) throw std::invalid_argument("HBM budget share too small for one query's workspace");
+    size_t lo = 1, hi = max_batch;  // largest B in [1, max_batch] with ws(d, B) <= share
+    while (lo < hi) {
+      const size_t mid = lo + (hi - lo + 1) / 2;
+      if (ws(d, mid) <= share)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    out[d] = lo;
+  }
+  return out;
+}
+
+ServingLoop::ServingLoop(std::vector<PirServer *> dbs, const ServeConfig &cfg) {
+  t0_ = std::chrono::steady_clock::now();
+  // one mutex per distinct context: a context serves one host thread at a time
+  std::map<opir_ctx *, std::mutex *> by_ctx;
+  size_t fixed = 0;
+  for (PirServer *s : dbs) {
+    if (!s) throw std::invalid_argument("null PirServer");
+    opir_ctx *ctx = s->gpu_scheme()->ctx();
+    if (!by_ctx.count(ctx)) {
+      ctx_mus_.push_back(std::make_unique<std::mutex>());
+      by_ctx[ctx] = ctx_mus_.back().get();
+      size_t kb = 0;
+      s->gpu_scheme()->check(opir_client_key_bytes(ctx, s->get_params().get_expan_height(), &kb));
+      fixed += cfg.keyed_clients * kb;
+    }
+    auto lane = std::make_unique<DbLane>();
+    lane->srv = s;
+    lane->ctx_mu = by_ctx[ctx];
+    lanes_.push_back(std::move(lane));
+  }
+  auto ws = [&](size_t d, size_t B) {
+    size_t b = 0;
+    lanes_[d]->srv->gpu_scheme()->check(opir_db_workspace_bytes(lanes_[d]->srv->gpu_db(), B, &b));
+    return b;
+  };
+  limits_ = plan_batch_limits(lanes_.size(), ws, fixed, cfg.hbm_budget, cfg.max_batch);
+  for (size_t d = 0; d < lanes_.size(); d++) {  // reserve every workspace before the first request
+    std::lock_guard<std::mutex> g(*lanes_[d]->ctx_mu);
+    lanes_[d]->srv->gpu_scheme()->check(opir_db_reserve(lanes_[d]->srv->gpu_db(), limits_[d]));
+  }
+  for (size_t d = 0; d < lanes_.size(); d++) lanes_[d]->th = std::thread(&ServingLoop::worker, this, d);
+}
+
+ServingLoop::~ServingLoop() { stop(); }
+
+double ServingLoop::now() const {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+uint64_t ServingLoop::submit(size_t db, size_t client_id, std::vector<u64> query) {
+  if (db >= lanes_.size()) throw std::out_of_range("unknown database index");
+  const size_t N = lanes_[db]->srv->get_params().get_poly_degree();
+  if (query.size() != 2 * N) throw std::invalid_argument("query must hold [c0 N][c1 N]");
+  std::lock_guard<std::mutex> g(mu_);
+  if (stop_) throw std::runtime_error("serving loop is stopped");
+  const uint64_t t = next_ticket_++;
+  lanes_[db]->queue.push_back(Req{t, client_id, std::move(query), now()});
+  submitted_++;
+  cv_work_.notify_all();
+  return t;
+}
+
+size_t ServingLoop::poll(std::vector<ServeResult> &out) {
+  std::lock_guard<std::mutex> g(mu_);
+  const size_t n = done_.size();
+  for (auto &r : done_) out.push_back(std::move(r));
+  done_.clear();
+  return n;
+}
+
+void ServingLoop::wait_idle() {
+  std::unique_lock<std::mutex> lk(mu_);
+  cv_done_.wait(lk, [&] { return completed_ == submitted_; });
+}
+
+void ServingLoop::stop() {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    stop_ = true;
+    cv_work_.notify_all();
+  }
+  for (auto &l : lanes_)
+    if (l->th.joinable()) l->th.join();
+}
+
+void ServingLoop::worker(size_t d) {
+  DbLane &lane = *lanes_[d];
+  PirServer &srv = *lane.srv;
+  const size_t N = srv.get_params().get_poly_degree();
+  const size_t limit = limits_[d];
+  std::vector<uint64_t> ids, q, r;
+  for (;;) {
+    std::vector<Req> batch;
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_work_.wait(lk, [&] { return stop_ || !lane.queue.empty(); });
+      if (lane.queue.empty()) return;  // stopping, nothing left for this database
+      const size_t b = std::min(lane.queue.size(), limit);
+      for (size_t i = 0; i < b; i++) {
+        batch.push_back(std::move(lane.queue.front()));
+        lane.queue.pop_front();
+      }
+    }
+    const size_t B = batch.size();
+    ids.resize(B);
+    q.resize(B * 2 * N);
+    r.resize(B * 2 * N);
+    for (size_t i = 0; i < B; i++) {
+      ids[i] = batch[i].client_id;
+      std::memcpy(&q[i * 2 * N], batch[i].query.data(), 2 * N * sizeof(u64));
+    }
+    const double ts = now();
+    int rc;
+    std::string err;
+    {
+      std::lock_guard<std::mutex> g(*lane.ctx_mu);
+      rc = opir_answer_batch(srv.gpu_db(), B, ids.data(), q.data(), r.data());
+      if (rc != OPIR_OK) err = opir_last_error(srv.gpu_scheme()->ctx());
+    }
+    const double td = now();
+    std::vector<ServeResult> res(B);
+    for (size_t i = 0; i < B; i++) {
+      ServeResult &x = res[i];
+      x.ticket = batch[i].ticket;
+      x.db = d;
+      x.client_id = batch[i].client_id;
+      x.t_submit = batch[i].t_submit;
+      x.t_start = ts;
+      x.t_done = td;
+      x.batch = B;
+      if (rc == OPIR_OK)
+        x.resp.assign(r.begin() + i * 2 * N, r.begin() + (i + 1) * 2 * N);
+      else
+        x.error = err;
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto &x : res) done_.push_back(std::move(x));
+    completed_ += B;
+    cv_done_.notify_all();
+  }
+}
+
+LoadRun run_poisson(ServingLoop &loop, const std::vector<PoolEntry> &pool, double lambda, double duration, u64 seed) {
+  if (pool.empty()) throw std::invalid_argument("empty request pool");
+  if (!(lambda > 0) || !(duration > 0)) throw std::invalid_argument("lambda and duration must be positive");
+  std::mt19937_64 rng(seed);
+  std::exponential_distribution<double> gap(lambda);
+  std::uniform_int_distribution<size_t> pick(0, pool.size() - 1);
+  std::map<uint64_t, size_t> origin;
+  const auto start = std::chrono::steady_clock::now();
+  double t = gap(rng);
+  size_t sent = 0;
+  while (t < duration) {
+    std::this_thread::sleep_until(start + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                              std::chrono::duration<double>(t)));
+    const size_t k = pick(rng);
+    origin[loop.submit(pool[k].db, pool[k].client_id, pool[k].query)] = k;
+    sent++;
+    t += gap(rng);
+  }
+  loop.wait_idle();
+  LoadRun run;
+  std::vector<ServeResult> all;
+  loop.poll(all);
+  double first = 1e300, last = 0;
+  for (auto &x : all) {
+    auto it = origin.find(x.ticket);
+    if (it == origin.end()) continue;  // a request submitted outside this run
+    first = std::min(first, x.t_submit);
+    last = std::max(last, x.t_done);
+    run.pool_index.push_back(it->second);
+    run.results.push_back(std::move(x));
+  }
+  run.offered_qps = sent / duration;
+  run.span_s = run.results.empty() ? 0 : last - first;
+  run.achieved_qps = run.span_s > 0 ? run.results.size() / run.span_s : 0;
+  return run;
+}
+
+}  // namespace onionpir

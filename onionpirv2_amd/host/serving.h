@@ -1,0 +1,110 @@
+// Continuous-batching serving loop over several resident PIR databases.
+//
+// The reference's PirServer answers one query per make_query call
+// (src/server.cpp:895-947) and has no serving layer. This loop is the system
+// form on top of the same answer path: requests from many clients arrive at
+// any time for any of several databases; each database has a worker thread
+// that takes whatever is queued for it (up to its admitted batch size) as one
+// fused batch (opir_answer_batch), so requests that arrive while a batch is
+// on the GPU join the next one. Databases built on their own GpuScheme have
+// their own HIP stream, so batches of different databases overlap on the
+// device. Batch sizes are admitted against an HBM budget for client keys and
+// per-batch workspaces (expanded queries, selectors, first-dimension
+// candidates), which are reserved once up front.
+#pragma once
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "onionpir_host.h"
+
+namespace onionpir {
+
+struct ServeConfig {
+  size_t max_batch = 64;      // upper bound per database and batch
+  size_t hbm_budget = 0;      // bytes for keys + workspaces; 0 = bounded by max_batch only
+  size_t keyed_clients = 0;   // clients whose keys are resident, counted against the budget
+};
+
+struct ServeResult {
+  uint64_t ticket = 0;
+  size_t db = 0, client_id = 0;
+  std::vector<u64> resp;      // [c0 N][c1 N] mod small_q; empty when error is set
+  double t_submit = 0, t_start = 0, t_done = 0;  // seconds since the loop started
+  size_t batch = 0;           // size of the batch that answered it
+  std::string error;
+};
+
+// Largest admitted batch per database: an equal share of (budget - fixed)
+// per database, and the largest B <= max_batch whose workspace ws(d, B) fits
+// that share (ws is non-decreasing in B). budget == 0 admits max_batch.
+// Throws std::invalid_argument if some database cannot take even B = 1.
+std::vector<size_t> plan_batch_limits(size_t ndb, const std::function<size_t(size_t, size_t)> &ws, size_t fixed,
+                                      size_t budget, size_t max_batch);
+
+class ServingLoop {
+ public:
+  ServingLoop(std::vector<PirServer *> dbs, const ServeConfig &cfg);
+  ~ServingLoop();
+  ServingLoop(const ServingLoop &) = delete;
+  ServingLoop &operator=(const ServingLoop &) = delete;
+
+  // Thread-safe. query: [c0 N][c1 N] coefficient form mod Q.
+  uint64_t submit(size_t db, size_t client_id, std::vector<u64> query);
+  // Moves completed results into out; returns how many.
+  size_t poll(std::vector<ServeResult> &out);
+  // Blocks until every submitted request has completed.
+  void wait_idle();
+  void stop();
+  const std::vector<size_t> &batch_limits() const { return limits_; }
+  double now() const;
+
+ private:
+  struct Req {
+    uint64_t ticket;
+    size_t client_id;
+    std::vector<u64> query;
+    double t_submit;
+  };
+  struct DbLane {
+    PirServer *srv = nullptr;
+    std::mutex *ctx_mu = nullptr;  // serialises databases that share one context
+    std::deque<Req> queue;
+    std::thread th;
+  };
+  void worker(size_t d);
+
+  std::vector<std::unique_ptr<DbLane>> lanes_;
+  std::vector<std::unique_ptr<std::mutex>> ctx_mus_;
+  std::vector<size_t> limits_;
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::vector<ServeResult> done_;
+  uint64_t next_ticket_ = 0, submitted_ = 0, completed_ = 0;
+  bool stop_ = false;
+  std::chrono::steady_clock::time_point t0_;
+};
+
+// Open-loop load: Poisson arrivals at rate lambda (1/s) for `duration`
+// seconds, each request a uniformly drawn entry of the pool (db, client,
+// query). Returns every result (with the pool index it came from) once all
+// have completed.
+struct PoolEntry {
+  size_t db, client_id;
+  std::vector<u64> query;
+};
+struct LoadRun {
+  std::vector<ServeResult> results;
+  std::vector<size_t> pool_index;  // per result
+  double offered_qps = 0, achieved_qps = 0, span_s = 0;
+};
+LoadRun run_poisson(ServingLoop &loop, const std::vector<PoolEntry> &pool, double lambda, double duration, u64 seed);
+
+}  // namespace onionpir

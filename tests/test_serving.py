@@ -1,0 +1,112 @@
+"""Continuous-batching serving loop (onionpirv2_amd/host/serving.cpp).
+
+CPU: the HBM-budget admission planner.
+GPU: requests for two resident databases from several clients, submitted as
+a burst and as Poisson arrivals, are answered in batches of whatever is
+queued; every response equals the same query answered alone
+(PirServer::make_query), bit for bit, and decrypts to the stored plaintext.
+"""
+import numpy as np
+import pytest
+
+
+def test_plan_batch_limits_budget_shares():
+    import onionpirv2_amd.host as H
+    # workspace of database d at batch B: 100 B + 10 d bytes
+    ws = lambda d, B: 100 * B + 10 * d
+    assert H.plan_batch_limits(2, ws, 50, 1050, 64) == [5, 4]
+    assert H.plan_batch_limits(2, ws, 0, 0, 64) == [64, 64]          # no budget: max_batch
+    assert H.plan_batch_limits(1, ws, 0, 10**9, 8) == [8]            # capped by max_batch
+    # a workspace with steps (column tiles of 8 queries), like the scan fragments
+    step = lambda d, B: 1000 * ((B + 7) // 8) + B
+    assert H.plan_batch_limits(1, step, 0, 2000 + 16, 64) == [16]
+    with pytest.raises(ValueError):
+        H.plan_batch_limits(1, ws, 1000, 1050, 64)                   # keys leave no room for one query
+    with pytest.raises(ValueError):
+        H.plan_batch_limits(1, ws, 2000, 1000, 64)                   # keys alone exceed the budget
+    with pytest.raises(ValueError):
+        H.plan_batch_limits(0, ws, 0, 0, 64)
+
+
+@pytest.fixture(scope="module")
+def served():
+    import onionpirv2_amd.host as H
+    print("[gpu] serving: building two databases and 4 clients", flush=True)
+    layouts = [(4096, 9), (700, 8)]
+    params = [H.PirParams().with_layout(t, h, True) for t, h in layouts]
+    servers = []
+    for j, p in enumerate(params):
+        s = H.PirServer(p)  # own GPU context and stream per database
+        s.gen_data([5, 17, 300, 699], 900 + j)
+        servers.append(s)
+    clients = {}
+    for c in range(4):
+        cid = 500 + c
+        cl = H.PirClient(params[0], cid, 77 + c)
+        bv, gsw = cl.create_bv_galois_keys(), cl.generate_gsw_from_key()
+        for s in servers:
+            s.set_client_bv_galois_key(cid, bv)
+            s.set_client_gsw_key(cid, gsw)
+        clients[cid] = cl
+    pool = []  # (db, client, index, query)
+    for d, p in enumerate(params):
+        for k, idx in enumerate([5, 17, 300, 699]):
+            cid = 500 + (k + d) % 4
+            pool.append((d, cid, idx, clients[cid].fast_generate_query(p, idx)))
+    alone = [servers[d].make_query(cid, q.copy()) for d, cid, _, q in pool]
+    yield H, servers, clients, pool, alone
+
+
+@pytest.mark.gpu
+def test_burst_batches_equal_alone(served):
+    H, servers, clients, pool, alone = served
+    print("[gpu] serving burst", flush=True)
+    loop = H.ServingLoop(servers, max_batch=6)
+    try:
+        tickets = {}
+        for rep in range(3):
+            for k, (d, cid, _, q) in enumerate(pool):
+                tickets[loop.submit(d, cid, q)] = k
+        loop.wait_idle()
+        res = loop.poll()
+    finally:
+        loop.stop()
+    assert len(res) == len(tickets)
+    assert max(r["batch"] for r in res) > 1, "the burst should have been batched"
+    assert all(r["batch"] <= 6 for r in res)
+    for r in res:
+        assert r["error"] == ""
+        assert np.array_equal(r["resp"], alone[tickets[r["ticket"]]])
+
+
+@pytest.mark.gpu
+def test_poisson_mix_equal_alone_and_decrypts(served):
+    H, servers, clients, pool, alone = served
+    print("[gpu] serving poisson", flush=True)
+    from onionpirv2_amd.gpu import load_library  # noqa: F401  (the HIP library must be the one answering)
+    # budget: keys of 4 clients on each context plus room for batches of a few queries
+    loop = H.ServingLoop(servers, max_batch=8, hbm_budget=2 << 30, keyed_clients=4)
+    try:
+        assert all(1 <= b <= 8 for b in loop.batch_limits)
+        dbs = [d for d, _, _, _ in pool]
+        cids = [c for _, c, _, _ in pool]
+        qs = np.stack([q for _, _, _, q in pool])
+        run = loop.run_poisson(dbs, cids, qs, 2000.0, 0.4, 12345)
+    finally:
+        loop.stop()
+    n = len(run["pool_index"])
+    assert n > 100 and run["errors"] == 0
+    for i in range(n):
+        assert np.array_equal(run["resp"][i], alone[run["pool_index"][i]])
+    for k in sorted(set(run["pool_index"].tolist()))[:4]:
+        d, cid, idx, _ = pool[k]
+        pt, budget = clients[cid].decrypt_mod_q(alone[k])
+        assert np.array_equal(pt, servers[d].direct_get_original_plaintext(idx)) and budget > 0
+    assert np.all(run["latency_ms"] >= run["queue_ms"])
+
+
+@pytest.mark.gpu
+def test_budget_too_small_is_rejected(served):
+    H, servers, _, _, _ = served
+    with pytest.raises(ValueError):
+        H.ServingLoop(servers, max_batch=8, hbm_budget=1 << 20, keyed_clients=4)
