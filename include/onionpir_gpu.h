@@ -9,14 +9,18 @@
  *   OPIR_EINVAL -> std::invalid_argument, OPIR_ERANGE -> std::out_of_range,
  *   others      -> std::runtime_error.
  *
- * Data formats at this boundary are the reference's own:
- *   - ciphertext: c0 then c1, each N uint64 values mod Q (limb-major,
- *     K = 1 logical limb for CONFIG_N2048_K1_COMP), src/includes/rlwe.h:17-31;
+ * Data formats at this boundary are the reference's own, with K RNS limbs
+ * (K = 1 logical limb mod Q = q1*q2 for CONFIG_N2048_K1_COMP, K = 2 limbs
+ * mod q1 and q2 for CONFIG_N4096_K2_MP):
+ *   - ciphertext: c0 then c1, each K*N uint64 values, limb-major
+ *     ([limb 0 N][limb 1 N] when K = 2), src/includes/rlwe.h:17-31;
  *   - NTT-form values use the reference's NTT evaluation order;
- *   - RGSW rows: 2*l rows of [c0 N][c1 N], src/includes/gsw.h:8-17;
- *   - BV keys: [h keys][L_KS rows][a, b][N], galois k = N/2^i + 1,
+ *   - RGSW rows: 2*l rows of [c0 K*N][c1 K*N], src/includes/gsw.h:8-17;
+ *   - BV keys: [h keys][L_KS rows][a, b][K*N], galois k = N/2^i + 1,
  *     src/includes/bv_keyswitch.h:45-66;
- *   - response: c0 then c1, N uint64 values mod small_q.
+ *   - response: c0 then c1, N uint64 values mod small_q (one limb after the
+ *     mod switch);
+ *   - the stage entry points below take the same K*N-value polynomials.
  * Host buffers are owned by the caller; device buffers by the context.
  * A context is not thread-safe (the reference's PirServer is not either,
  * src/includes/gsw.h:25-30); one context serves one host thread.
@@ -45,15 +49,17 @@ typedef struct opir_db opir_db;
  * PirParams-derived values (src/includes/database_constants.h:77-94,
  * src/pir.cpp:14-98), as a runtime value. */
 typedef struct opir_scheme {
-  uint32_t poly_degree;     /* N (2048) */
-  uint32_t composite;       /* 1: Q = q1*q2 composite single limb */
+  uint32_t poly_degree;     /* N: 2048 (composite) or 4096 (two primes) */
+  uint32_t composite;       /* 1: Q = q1*q2 composite single limb (CONFIG_N2048_K1_COMP);
+                               0: two RNS limbs q1, q2 (CONFIG_N4096_K2_MP, database_constants.h:111-124) */
   uint64_t q1, q2;          /* RNS primes of Q */
-  uint64_t ntt_root;        /* primitive 2N-th root mod Q (w_crt) */
+  uint64_t ntt_root;        /* composite: primitive 2N-th root mod Q (w_crt); two primes: root mod q1 */
   uint64_t plain_mod;       /* t */
   uint64_t small_q;         /* response modulus q' */
   uint32_t l_ep, base_log2;         /* data selector gadget */
   uint32_t l_key, base_log2_key;    /* RGSW(s) completion gadget */
   uint32_t l_ks, bv_base_log2;      /* BV keyswitch gadget */
+  uint64_t ntt_root2;       /* two primes: primitive 2N-th root mod q2 (ignored when composite) */
 } opir_scheme;
 
 /* Replaces the scheme set-up of PirParams::PirParams / init_composite_rns
@@ -84,6 +90,9 @@ int opir_db_load_plain(opir_db *db, const uint16_t *coeffs, size_t first_pt, siz
  * layout: lo[coeff * num_pt + pt] = NTT_Q(pt)[coeff] mod q1, hi the same
  * mod q2 (src/server.cpp:162-170). */
 int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi);
+/* Same as opir_db_load_plain with 64-bit coefficients (< t), for plaintext
+ * moduli beyond 16 bits (CONFIG_N4096_K2_MP: t ~ 2^39). */
+int opir_db_load_plain_u64(opir_db *db, const uint64_t *coeffs, size_t first_pt, size_t count);
 
 /* Replaces PirServer::set_client_bv_galois_key + set_client_gsw_key /
  * set_client_session_keys (src/server.cpp:857-875). bv: [h][l_ks][2: a, b][N]

@@ -1,174 +1,21 @@
 // C ABI of the OnionPIRv2 MI355X server library (include/onionpir_gpu.h).
 // Orchestrates the kernels of opir_kernels.hip along PirServer::make_query
 // (src/server.cpp:895-947).
-#include <algorithm>
-#include <hip/hip_runtime.h>
+#include "opir_capi_internal.h"
 
-#include <cstring>
-#include <mutex>
-#include <new>
-#include <string>
-#include <unordered_map>
-#include <vector>
+using namespace capi;
 
-#include "../../include/onionpir_gpu.h"
-#include "opir_kernels.h"
-
-using namespace opir;
-typedef unsigned __int128 u128;
+thread_local std::string capi::g_create_error;
 
 namespace {
 
-thread_local std::string g_create_error;
-
-struct ClientKeys {
-  u32 *bv = nullptr;   // [h][l_ks][2 (b,a)][2 limbs][N]
-  size_t h = 0;
-  u32 *gsw = nullptr;  // [2 l_key][2][2 limbs][N]
-};
-
-// Small ring of pinned staging slots for per-batch key-pointer tables.
-constexpr int kRing = 8;
-
-}  // namespace
-
-struct opir_ctx {
-  int device = 0;
-  hipStream_t own_stream = nullptr, stream = nullptr;
-  opir_scheme sch{};
-  SchemeConst S{};
-  GadgetCfg G{};
-  u32 *d_tables = nullptr;
-  std::unordered_map<uint64_t, ClientKeys> clients;
-  std::string err;
-  bool profiling = false;
-  hipEvent_t ev[8] = {};
-  // pointer-table ring
-  const u32 **pin_ptrs[kRing] = {};
-  const u32 **dev_ptrs[kRing] = {};
-  hipEvent_t ring_ev[kRing] = {};
-  size_t ring_cap = 0;
-  int ring_pos = 0;
-  // host staging for synchronous calls
-  void *pin = nullptr;
-  size_t pin_bytes = 0;
-};
-
-struct opir_db {
-  opir_ctx *ctx = nullptr;
-  size_t N0 = 0, Nrest = 0, dims = 0, h = 0, num_pt = 0, u = 0;
-  DbGeom geom;
-  int8_t *d_db = nullptr;  // digit planes in MFMA fragment order (opir_scan.hip)
-  // workspace for up to ws_B queries
-  size_t ws_B = 0;
-  u32 *exp_a = nullptr, *exp_b = nullptr, *qntt = nullptr, *sel = nullptr, *mid_a = nullptr, *mid_b = nullptr;
-  int8_t *qfrag = nullptr;
-  u64 *q_u64 = nullptr, *r_u64 = nullptr;
-  size_t exp_cts = 0;
-};
-
-namespace {
-
-int set_err(opir_ctx *ctx, int code, const std::string &msg) {
-  if (ctx) ctx->err = msg;
-  else g_create_error = msg;
-  return code;
-}
-
-#define HIP_TRY(ctx, expr)                                                                         \
-  do {                                                                                             \
-    hipError_t e_ = (expr);                                                                        \
-    if (e_ != hipSuccess) return set_err((ctx), OPIR_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-#define LAUNCH_CHECK(ctx)                                                                          \
-  do {                                                                                             \
-    hipError_t e_ = hipGetLastError();                                                             \
-    if (e_ != hipSuccess) return set_err((ctx), OPIR_EHIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
-  } while (0)
-
-u64 mulmod_h(u64 a, u64 b, u64 m) { return (u64)((u128)a * b % m); }
-u64 powmod_h(u64 b, u64 e, u64 m) {
-  u64 r = 1 % m;
-  b %= m;
-  while (e) {
-    if (e & 1) r = mulmod_h(r, b, m);
-    b = mulmod_h(b, b, m);
-    e >>= 1;
-  }
-  return r;
-}
-bool inv_h(u64 a, u64 m, u64 &out) {
-  int64_t s0 = 1, s1 = 0, r0 = (int64_t)(a % m), r1 = (int64_t)m;
-  if (r0 == 0) return false;
-  while (r1) {
-    int64_t q = r0 / r1, t = r1;
-    r1 = r0 - q * r1; r0 = t;
-    t = s1; s1 = s0 - q * s1; s0 = t;
-  }
-  if (r0 != 1) return false;
-  out = (u64)(s0 < 0 ? s0 + (int64_t)m : s0);
-  return true;
-}
-u32 shoup_h(u32 w, u32 q) { return (u32)(((u64)w << 32) / q); }
-size_t brv(size_t x, int logn) {
-  size_t y = 0;
-  for (int i = 0; i < logn; i++) { y = (y << 1) | (x & 1); x >>= 1; }
-  return y;
-}
-
-int ensure_pin(opir_ctx *ctx, size_t bytes) {
-  if (ctx->pin_bytes >= bytes) return OPIR_OK;
-  if (ctx->pin) (void)hipHostFree(ctx->pin);
-  ctx->pin = nullptr;
-  ctx->pin_bytes = 0;
-  HIP_TRY(ctx, hipHostMalloc(&ctx->pin, bytes, hipHostMallocDefault));
-  ctx->pin_bytes = bytes;
+int upload_k1(opir_ctx *ctx, const uint64_t *ids, size_t B, BatchPtrs &P) {
+  const void *const *t = nullptr;
+  const int rc = upload_key_ptrs(ctx, ids, B, &t);
+  if (rc) return rc;
+  P.bv_keys = reinterpret_cast<const u32 *const *>(t);
+  P.gsw_keys = reinterpret_cast<const u32 *const *>(t + B);
   return OPIR_OK;
-}
-
-// Upload per-query key pointers for a batch through the pinned ring.
-int upload_key_ptrs(opir_ctx *ctx, const uint64_t *ids, size_t B, BatchPtrs &P) {
-  if (ctx->ring_cap < B) {
-    for (int i = 0; i < kRing; i++) {
-      if (ctx->ring_ev[i]) (void)hipEventSynchronize(ctx->ring_ev[i]);
-      if (ctx->pin_ptrs[i]) (void)hipHostFree(ctx->pin_ptrs[i]);
-      if (ctx->dev_ptrs[i]) (void)hipFree(ctx->dev_ptrs[i]);
-      ctx->pin_ptrs[i] = nullptr;
-      ctx->dev_ptrs[i] = nullptr;
-    }
-    size_t cap = 64;
-    while (cap < B) cap *= 2;
-    for (int i = 0; i < kRing; i++) {
-      HIP_TRY(ctx, hipHostMalloc((void **)&ctx->pin_ptrs[i], 2 * cap * sizeof(u32 *), hipHostMallocDefault));
-      HIP_TRY(ctx, hipMalloc((void **)&ctx->dev_ptrs[i], 2 * cap * sizeof(u32 *)));
-      if (!ctx->ring_ev[i]) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ring_ev[i], hipEventDisableTiming));
-    }
-    ctx->ring_cap = cap;
-  }
-  const int slot = ctx->ring_pos;
-  ctx->ring_pos = (ctx->ring_pos + 1) % kRing;
-  HIP_TRY(ctx, hipEventSynchronize(ctx->ring_ev[slot]));
-  const u32 **hp = ctx->pin_ptrs[slot];
-  for (size_t i = 0; i < B; i++) {
-    auto it = ctx->clients.find(ids[i]);
-    if (it == ctx->clients.end())
-      return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(ids[i]));
-    hp[i] = it->second.bv;
-    hp[B + i] = it->second.gsw;
-  }
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->dev_ptrs[slot], hp, 2 * B * sizeof(u32 *), hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(ctx, hipEventRecord(ctx->ring_ev[slot], ctx->stream));
-  P.bv_keys = ctx->dev_ptrs[slot];
-  P.gsw_keys = ctx->dev_ptrs[slot] + B;
-  return OPIR_OK;
-}
-
-size_t nodes_at_level(size_t u, size_t h, size_t L) {
-  const size_t cap = size_t{1} << h;
-  const size_t n = (u * (size_t{1} << L) + cap - 1) / cap;
-  const size_t full = size_t{1} << L;
-  return n < full ? n : full;
 }
 
 // Device bytes of the answer workspace for batches of up to B queries (the
@@ -252,22 +99,7 @@ u32 *run_other_dim(opir_db *db, size_t B, const u32 *sel) {
   return in;
 }
 
-int validate_mod(opir_ctx *ctx, const uint64_t *v, size_t n, uint64_t mod, const char *what) {
-  for (size_t i = 0; i < n; i++)
-    if (v[i] >= mod) return set_err(ctx, OPIR_EINVAL, std::string(what) + " value not reduced mod Q");
-  return OPIR_OK;
-}
-
 // Full answer pipeline for B queries already in db->q_u64 (device).
-int check_answerable(opir_db *db) {
-  // server.cpp:613-622: the selector tree must not be taller than the candidates.
-  if (db->dims > 1 && db->Nrest < (size_t{1} << (db->dims - 2)))
-    return set_err(db->ctx, OPIR_EINVAL,
-                   "evaluate_other_dim: layout has fewer candidates than its selector tree; query-shape views "
-                   "cannot answer make_query");
-  return OPIR_OK;
-}
-
 int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u64 *d_r) {
   opir_ctx *ctx = db->ctx;
   int rc = check_answerable(db);
@@ -275,7 +107,7 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   rc = ensure_ws(db, B);
   if (rc) return rc;
   BatchPtrs P;
-  rc = upload_key_ptrs(ctx, ids, B, P);
+  rc = upload_k1(ctx, ids, B, P);
   if (rc) return rc;
   for (size_t i = 0; i < B; i++) {
     const ClientKeys &k = ctx->clients[ids[i]];
@@ -317,11 +149,38 @@ extern "C" {
 
 const char *opir_last_error(const opir_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
+static int ctx_create_k2(const opir_scheme *sch, int device, opir_ctx **out) {
+  if (sch->poly_degree != (uint32_t)k2::kN)
+    return set_err(nullptr, OPIR_EINVAL, "the two-prime scheme runs at N=4096 (CONFIG_N4096_K2_MP)");
+  if (int rc = k2_validate_scheme(sch)) return rc;
+  opir_ctx *ctx = new (std::nothrow) opir_ctx();
+  if (!ctx) return set_err(nullptr, OPIR_ENOMEM, "out of host memory");
+  ctx->sch = *sch;
+  ctx->device = device;
+  auto fail = [&](int code, const std::string &m) {
+    opir_ctx_destroy(ctx);
+    return set_err(nullptr, code, m);
+  };
+  if (hipSetDevice(device) != hipSuccess) return fail(OPIR_EHIP, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) return fail(OPIR_EHIP, "stream create");
+  ctx->stream = ctx->own_stream;
+  for (auto &e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(OPIR_EHIP, "event create");
+  const int rc = k2_ctx_init(ctx, sch);
+  if (rc) {
+    const std::string m = g_create_error;
+    return fail(rc, m);
+  }
+  *out = ctx;
+  return OPIR_OK;
+}
+
 int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   if (!sch || !out) return set_err(nullptr, OPIR_EINVAL, "null argument");
   *out = nullptr;
-  if (sch->poly_degree != (uint32_t)kN || !sch->composite)
-    return set_err(nullptr, OPIR_EINVAL, "this build supports the composite N=2048 scheme (CONFIG_N2048_K1_COMP)");
+  if (!sch->composite) return ctx_create_k2(sch, device, out);
+  if (sch->poly_degree != (uint32_t)kN)
+    return set_err(nullptr, OPIR_EINVAL, "the composite scheme runs at N=2048 (CONFIG_N2048_K1_COMP)");
   const u64 q1 = sch->q1, q2 = sch->q2;
   if (q1 >= (1ULL << 30) || q2 >= (1ULL << 30) || q1 < 3 || q2 < 3 || q1 >= 2 * q2 || q2 >= 2 * q1 ||
       (q1 - 1) % (2 * kN) || (q2 - 1) % (2 * kN))
@@ -429,6 +288,7 @@ void opir_ctx_destroy(opir_ctx *ctx) {
     if (e) (void)hipEventDestroy(e);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+  k2_ctx_free(ctx);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
 }
@@ -470,7 +330,8 @@ int opir_db_create(opir_ctx *ctx, size_t N0, size_t Nrest, size_t dims, size_t h
   *out = nullptr;
   if (N0 == 0 || Nrest == 0 || dims == 0 || h > 11) return set_err(ctx, OPIR_EINVAL, "invalid layout");
   const size_t cap = size_t{1} << h;
-  const size_t u = N0 + (size_t)ctx->G.l_ep * (dims - 1);
+  const size_t l_ep = ctx->k2 ? (size_t)ctx->G64.l_ep : (size_t)ctx->G.l_ep;
+  const size_t u = N0 + l_ep * (dims - 1);
   if (u > cap) return set_err(ctx, OPIR_EINVAL, "layout does not fit the expansion capacity");
   if (dims > 1 && (dims - 1 > 40 || Nrest > (size_t{1} << (dims - 1))))
     return set_err(ctx, OPIR_EINVAL, "more candidates than the selector tree addresses");
@@ -482,6 +343,15 @@ int opir_db_create(opir_ctx *ctx, size_t N0, size_t Nrest, size_t dims, size_t h
   db->num_pt = N0 * Nrest;
   db->u = u;
   db->exp_cts = u + 2;
+  if (ctx->k2) {
+    const int rc = k2_db_init(db);
+    if (rc) {
+      delete db;
+      return rc;
+    }
+    *out = db;
+    return OPIR_OK;
+  }
   const size_t bytes = db_bytes(db->geom);
   if (hipMalloc((void **)&db->d_db, bytes) != hipSuccess) {
     delete db;
@@ -499,6 +369,7 @@ int opir_db_create(opir_ctx *ctx, size_t N0, size_t Nrest, size_t dims, size_t h
 void opir_db_destroy(opir_db *db) {
   if (!db) return;
   (void)hipStreamSynchronize(db->ctx->stream);
+  k2_db_free(db);
   for (u32 *p : {db->exp_a, db->exp_b, db->qntt, db->sel, db->mid_a, db->mid_b})
     if (p) (void)hipFree(p);
   if (db->d_db) (void)hipFree(db->d_db);
@@ -511,6 +382,10 @@ void opir_db_destroy(opir_db *db) {
 int opir_db_load_plain(opir_db *db, const uint16_t *coeffs, size_t first_pt, size_t count) {
   if (!db || (!coeffs && count)) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;
+  if (ctx->k2) {
+    std::vector<uint64_t> w(coeffs, coeffs + count * ctx->N);
+    return k2_load_plain(db, w.data(), first_pt, count);
+  }
   if (first_pt > db->num_pt || count > db->num_pt - first_pt)
     return set_err(ctx, OPIR_EINVAL, "PirServer::load_data logical plaintext count exceeds its layout");
   const size_t chunk = 8192;
@@ -539,9 +414,22 @@ int opir_db_load_plain(opir_db *db, const uint16_t *coeffs, size_t first_pt, siz
   return rc;
 }
 
+int opir_db_load_plain_u64(opir_db *db, const uint64_t *coeffs, size_t first_pt, size_t count) {
+  if (!db || (!coeffs && count)) return OPIR_EINVAL;
+  opir_ctx *ctx = db->ctx;
+  if (ctx->k2) return k2_load_plain(db, coeffs, first_pt, count);
+  std::vector<uint16_t> w(count * kN);
+  for (size_t i = 0; i < w.size(); i++) {
+    if (coeffs[i] >= ctx->sch.plain_mod) return set_err(ctx, OPIR_EINVAL, "plaintext coefficient not reduced mod t");
+    w[i] = (uint16_t)coeffs[i];
+  }
+  return opir_db_load_plain(db, w.data(), first_pt, count);
+}
+
 int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi) {
   if (!db || !lo || !hi) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;
+  if (ctx->k2) return set_err(ctx, OPIR_EINVAL, "the composite (lo, hi) database layout exists only for K=1");
   const size_t n = (size_t)kN * db->num_pt;
   u32 *d_ref = nullptr;
   HIP_TRY(ctx, hipMalloc((void **)&d_ref, 2 * n * sizeof(u32)));
@@ -558,9 +446,27 @@ int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi) {
   return rc;
 }
 
+static void replace_client(opir_ctx *ctx, uint64_t id, const ClientKeys &k) {
+  auto it = ctx->clients.find(id);
+  if (it != ctx->clients.end()) {
+    (void)hipStreamSynchronize(ctx->stream);  // queued answers may still read the old keys
+    (void)hipFree(it->second.bv);
+    (void)hipFree(it->second.gsw);
+  }
+  ctx->clients[id] = k;
+}
+
 int opir_client_set_keys(opir_ctx *ctx, uint64_t id, const uint64_t *bv, size_t h, const uint64_t *gsw) {
   if (!ctx) return OPIR_EINVAL;
   if ((!bv && h) || !gsw) return set_err(ctx, OPIR_EINVAL, "PirServer session key bundle cannot be null");
+  if (h > 12) return set_err(ctx, OPIR_EINVAL, "BV key count out of range");
+  if (ctx->k2) {
+    ClientKeys k;
+    const int rc = k2_set_keys(ctx, bv, h, gsw, k);
+    if (rc) return rc;
+    replace_client(ctx, id, k);
+    return OPIR_OK;
+  }
   if (h > 11) return set_err(ctx, OPIR_EINVAL, "BV key count out of range");
   const size_t L = ctx->G.l_ks, lk = ctx->G.l_key;
   const u64 Q = ctx->S.Q, q1 = ctx->S.limb[0].q, q2 = ctx->S.limb[1].q;
@@ -601,13 +507,7 @@ int opir_client_set_keys(opir_ctx *ctx, uint64_t id, const uint64_t *bv, size_t 
     (void)hipFree(k.gsw);
     return set_err(ctx, OPIR_EHIP, "upload keys");
   }
-  auto it = ctx->clients.find(id);
-  if (it != ctx->clients.end()) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(it->second.bv);
-    (void)hipFree(it->second.gsw);
-  }
-  ctx->clients[id] = k;
+  replace_client(ctx, id, k);
   return OPIR_OK;
 }
 
@@ -624,47 +524,50 @@ int opir_client_remove(opir_ctx *ctx, uint64_t id) {
 
 int opir_db_workspace_bytes(const opir_db *db, size_t B, size_t *out) {
   if (!db || !out) return OPIR_EINVAL;
-  *out = ws_bytes(db, B);
+  *out = db->ctx->k2 ? k2_ws_bytes(db, B) : ws_bytes(db, B);
   return OPIR_OK;
 }
 
 int opir_db_reserve(opir_db *db, size_t B) {
   if (!db) return OPIR_EINVAL;
-  return B ? ensure_ws(db, B) : OPIR_OK;
+  if (!B) return OPIR_OK;
+  return db->ctx->k2 ? k2_ensure_ws(db, B) : ensure_ws(db, B);
 }
 
 int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out) {
   if (!ctx || !out) return OPIR_EINVAL;
-  *out = (h * ctx->G.l_ks + 2 * (size_t)ctx->G.l_key) * kCtWords * sizeof(u32);
+  *out = ctx->k2 ? (h * ctx->G64.l_ks + 2 * (size_t)ctx->G64.l_key) * k2::kCt * sizeof(u64)
+                 : (h * ctx->G.l_ks + 2 * (size_t)ctx->G.l_key) * kCtWords * sizeof(u32);
   return OPIR_OK;
 }
 
 int opir_answer_batch_device(opir_db *db, size_t B, const uint64_t *ids, const uint64_t *d_q, uint64_t *d_r) {
   if (!db || !ids || !d_q || !d_r) return OPIR_EINVAL;
   if (B == 0) return OPIR_OK;
-  return answer_device(db, B, ids, d_q, d_r);
+  return db->ctx->k2 ? k2_answer_device(db, B, ids, d_q, d_r) : answer_device(db, B, ids, d_q, d_r);
 }
 
 int opir_answer_batch(opir_db *db, size_t B, const uint64_t *ids, const uint64_t *queries, uint64_t *resps) {
   if (!db || !ids || !queries || !resps) return OPIR_EINVAL;
   if (B == 0) return OPIR_OK;
   opir_ctx *ctx = db->ctx;
-  int rc = validate_mod(ctx, queries, B * 2 * kN, ctx->S.Q, "query");
+  const size_t N = ctx->N, qvals = (ctx->k2 ? 4 : 2) * N, rvals = 2 * N;
+  int rc = ctx->k2 ? k2_validate_rns(ctx, queries, B * 2, "query") : validate_mod(ctx, queries, B * qvals, ctx->S.Q, "query");
   if (rc) return rc;
   for (size_t i = 0; i < B; i++)
     if (!ctx->clients.count(ids[i])) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(ids[i]));
-  rc = ensure_ws(db, B);
+  rc = ctx->k2 ? k2_ensure_ws(db, B) : ensure_ws(db, B);
   if (rc) return rc;
-  const size_t bytes = B * 2 * kN * sizeof(u64);
-  rc = ensure_pin(ctx, bytes);
+  const size_t qbytes = B * qvals * sizeof(u64), rbytes = B * rvals * sizeof(u64);
+  rc = ensure_pin(ctx, std::max(qbytes, rbytes));
   if (rc) return rc;
-  std::memcpy(ctx->pin, queries, bytes);
-  HIP_TRY(ctx, hipMemcpyAsync(db->q_u64, ctx->pin, bytes, hipMemcpyHostToDevice, ctx->stream));
-  rc = answer_device(db, B, ids, db->q_u64, db->r_u64);
+  std::memcpy(ctx->pin, queries, qbytes);
+  HIP_TRY(ctx, hipMemcpyAsync(db->q_u64, ctx->pin, qbytes, hipMemcpyHostToDevice, ctx->stream));
+  rc = ctx->k2 ? k2_answer_device(db, B, ids, db->q_u64, db->r_u64) : answer_device(db, B, ids, db->q_u64, db->r_u64);
   if (rc) return rc;
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->pin, db->r_u64, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->pin, db->r_u64, rbytes, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  std::memcpy(resps, ctx->pin, bytes);
+  std::memcpy(resps, ctx->pin, rbytes);
   return OPIR_OK;
 }
 
@@ -675,20 +578,11 @@ int opir_answer(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *resp)
 // ---------------------------------------------------------------------------
 // Stage entry points (host buffers)
 // ---------------------------------------------------------------------------
-namespace {
-struct DevBuf {
-  void *p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-};
-}  // namespace
-
-#define DEV_ALLOC(ctx, buf, bytes) HIP_TRY(ctx, hipMalloc(&(buf).p, (bytes)))
 
 static int ntt_entry(opir_ctx *ctx, uint64_t *data, size_t count, bool inverse) {
   if (!ctx || (!data && count)) return OPIR_EINVAL;
   if (!count) return OPIR_OK;
+  if (ctx->k2) return k2_ntt(ctx, data, count, inverse);
   int rc = validate_mod(ctx, data, count * kN, ctx->S.Q, "NTT input");
   if (rc) return rc;
   DevBuf d;
@@ -707,6 +601,7 @@ int opir_ntt_inv(opir_ctx *ctx, uint64_t *data, size_t count) { return ntt_entry
 int opir_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out) {
   if (!db || !query || !out) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;
+  if (ctx->k2) return k2_expand(db, id, query, out);
   int rc = validate_mod(ctx, query, 2 * kN, ctx->S.Q, "query");
   if (rc) return rc;
   if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
@@ -714,7 +609,7 @@ int opir_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out) 
   rc = ensure_ws(db, 1);
   if (rc) return rc;
   BatchPtrs P;
-  rc = upload_key_ptrs(ctx, &id, 1, P);
+  rc = upload_k1(ctx, &id, 1, P);
   if (rc) return rc;
   HIP_TRY(ctx, hipMemcpy(db->q_u64, query, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
   launch_split_u64(db->q_u64, db->exp_a, 2, ctx->S, ctx->stream);
@@ -732,6 +627,7 @@ int opir_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out) 
 int opir_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *sel) {
   if (!db || !expanded || !sel) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;
+  if (ctx->k2) return k2_complete(db, id, expanded, sel);
   const size_t nsel = db->dims - 1;
   if (!nsel) return OPIR_OK;
   if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
@@ -740,7 +636,7 @@ int opir_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *
   rc = ensure_ws(db, 1);
   if (rc) return rc;
   BatchPtrs P;
-  rc = upload_key_ptrs(ctx, &id, 1, P);
+  rc = upload_k1(ctx, &id, 1, P);
   if (rc) return rc;
   DevBuf e;
   DEV_ALLOC(ctx, e, db->u * 2 * kN * sizeof(u64));
@@ -762,6 +658,7 @@ int opir_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *
 int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
   if (!db || !query || !out) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;
+  if (ctx->k2) return k2_first_dim(db, query, out);
   int rc = validate_mod(ctx, query, db->N0 * 2 * kN, ctx->S.Q, "query");
   if (rc) return rc;
   rc = ensure_ws(db, 1);
@@ -785,6 +682,7 @@ int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
 int opir_other_dim(opir_db *db, const uint64_t *mid, const uint64_t *sel, uint64_t *out) {
   if (!db || !mid || !out || (db->dims > 1 && !sel)) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;
+  if (ctx->k2) return k2_other_dim(db, mid, sel, out);
   if (int e = check_answerable(db)) return e;
   const size_t nsel = db->dims - 1;
   const size_t sel_polys = nsel * 2 * ctx->G.l_ep * 2;
@@ -818,6 +716,7 @@ int opir_other_dim(opir_db *db, const uint64_t *mid, const uint64_t *sel, uint64
 
 int opir_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const uint64_t *ct, uint64_t *out) {
   if (!ctx || !gsw || !ct || !out) return OPIR_EINVAL;
+  if (ctx->k2) return k2_ext_prod(ctx, gsw, l, b, ct, out);
   if (l == 0 || l > 64 || b == 0 || b > 32) return set_err(ctx, OPIR_EINVAL, "invalid gadget");
   int rc = validate_mod(ctx, gsw, 2 * l * 2 * kN, ctx->S.Q, "RGSW");
   if (rc) return rc;
@@ -844,6 +743,7 @@ int opir_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const 
 
 int opir_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out) {
   if (!ctx || !ct || !out) return OPIR_EINVAL;
+  if (ctx->k2) return k2_mod_switch(ctx, ct, out);
   int rc = validate_mod(ctx, ct, 2 * kN, ctx->S.Q, "ciphertext");
   if (rc) return rc;
   DevBuf c64, c32, n32, o64;

@@ -1,0 +1,225 @@
+// Internal state of the C ABI (include/onionpir_gpu.h), shared by the
+// K = 1 composite path (opir_capi.hip) and the K = 2 path (opir_capi_k2.hip).
+#pragma once
+#include <algorithm>
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/onionpir_gpu.h"
+#include "opir_k2.h"
+#include "opir_kernels.h"
+
+using namespace opir;
+typedef unsigned __int128 u128;
+
+namespace capi {
+
+struct ClientKeys {
+  void *bv = nullptr;   // K=1: u32 [h][l_ks][2 (b,a)][2 limbs][N]; K=2: u64, same order
+  size_t h = 0;
+  void *gsw = nullptr;  // [2 l_key][2][2 limbs][N]
+};
+
+// Small ring of pinned staging slots for per-batch key-pointer tables.
+constexpr int kRing = 8;
+
+}  // namespace capi
+
+struct opir_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  opir_scheme sch{};
+  bool k2 = false;          // two-prime scheme (CONFIG_N4096_K2_MP)
+  size_t N = kN;            // ring degree
+  SchemeConst S{};          // K = 1 constants
+  GadgetCfg G{};
+  u32 *d_tables = nullptr;
+  k2::Scheme64 S64{};       // K = 2 constants
+  k2::Gadget64 G64{};
+  u64 *d_tables64 = nullptr;
+  std::unordered_map<uint64_t, capi::ClientKeys> clients;
+  std::string err;
+  bool profiling = false;
+  hipEvent_t ev[8] = {};
+  // pointer-table ring
+  const void **pin_ptrs[capi::kRing] = {};
+  const void **dev_ptrs[capi::kRing] = {};
+  hipEvent_t ring_ev[capi::kRing] = {};
+  size_t ring_cap = 0;
+  int ring_pos = 0;
+  // host staging for synchronous calls
+  void *pin = nullptr;
+  size_t pin_bytes = 0;
+};
+
+struct opir_db {
+  opir_ctx *ctx = nullptr;
+  size_t N0 = 0, Nrest = 0, dims = 0, h = 0, num_pt = 0, u = 0;
+  DbGeom geom;
+  int8_t *d_db = nullptr;  // K = 1: digit planes in MFMA fragment order (opir_scan.hip)
+  u64 *d_db64 = nullptr;   // K = 2: [limb][pt][N] NTT form
+  // workspace for up to ws_B queries (K = 1 u32 RNS cts / K = 2 u64 cts)
+  size_t ws_B = 0;
+  u32 *exp_a = nullptr, *exp_b = nullptr, *qntt = nullptr, *sel = nullptr, *mid_a = nullptr, *mid_b = nullptr;
+  int8_t *qfrag = nullptr;
+  u64 *e64_a = nullptr, *e64_b = nullptr, *q64 = nullptr, *sel64 = nullptr, *m64_a = nullptr, *m64_b = nullptr;
+  u64 *q_u64 = nullptr, *r_u64 = nullptr;
+  size_t exp_cts = 0;
+};
+
+namespace capi {
+
+extern thread_local std::string g_create_error;
+
+inline int set_err(opir_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                         \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return capi::set_err((ctx), OPIR_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCH_CHECK(ctx)                                                                          \
+  do {                                                                                             \
+    hipError_t e_ = hipGetLastError();                                                             \
+    if (e_ != hipSuccess) return capi::set_err((ctx), OPIR_EHIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+inline u64 mulmod_h(u64 a, u64 b, u64 m) { return (u64)((u128)a * b % m); }
+inline u64 powmod_h(u64 b, u64 e, u64 m) {
+  u64 r = 1 % m;
+  b %= m;
+  while (e) {
+    if (e & 1) r = mulmod_h(r, b, m);
+    b = mulmod_h(b, b, m);
+    e >>= 1;
+  }
+  return r;
+}
+inline bool inv_h(u64 a, u64 m, u64 &out) {
+  __int128 s0 = 1, s1 = 0, r0 = (__int128)(a % m), r1 = (__int128)m;
+  if (r0 == 0) return false;
+  while (r1) {
+    __int128 q = r0 / r1, t = r1;
+    r1 = r0 - q * r1; r0 = t;
+    t = s1; s1 = s0 - q * s1; s0 = t;
+  }
+  if (r0 != 1) return false;
+  out = (u64)(s0 < 0 ? s0 + (__int128)m : s0);
+  return true;
+}
+inline u32 shoup_h(u32 w, u32 q) { return (u32)(((u64)w << 32) / q); }
+inline u64 shoup64_h(u64 w, u64 q) { return (u64)(((u128)w << 64) / q); }
+inline size_t brv(size_t x, int logn) {
+  size_t y = 0;
+  for (int i = 0; i < logn; i++) { y = (y << 1) | (x & 1); x >>= 1; }
+  return y;
+}
+
+inline int ensure_pin(opir_ctx *ctx, size_t bytes) {
+  if (ctx->pin_bytes >= bytes) return OPIR_OK;
+  if (ctx->pin) (void)hipHostFree(ctx->pin);
+  ctx->pin = nullptr;
+  ctx->pin_bytes = 0;
+  HIP_TRY(ctx, hipHostMalloc(&ctx->pin, bytes, hipHostMallocDefault));
+  ctx->pin_bytes = bytes;
+  return OPIR_OK;
+}
+
+// Upload per-query key pointers for a batch through the pinned ring:
+// table[0..B) = BV keys, table[B..2B) = RGSW(s) keys.
+inline int upload_key_ptrs(opir_ctx *ctx, const uint64_t *ids, size_t B, const void *const **table) {
+  if (ctx->ring_cap < B) {
+    for (int i = 0; i < kRing; i++) {
+      if (ctx->ring_ev[i]) (void)hipEventSynchronize(ctx->ring_ev[i]);
+      if (ctx->pin_ptrs[i]) (void)hipHostFree(ctx->pin_ptrs[i]);
+      if (ctx->dev_ptrs[i]) (void)hipFree(ctx->dev_ptrs[i]);
+      ctx->pin_ptrs[i] = nullptr;
+      ctx->dev_ptrs[i] = nullptr;
+    }
+    size_t cap = 64;
+    while (cap < B) cap *= 2;
+    for (int i = 0; i < kRing; i++) {
+      HIP_TRY(ctx, hipHostMalloc((void **)&ctx->pin_ptrs[i], 2 * cap * sizeof(void *), hipHostMallocDefault));
+      HIP_TRY(ctx, hipMalloc((void **)&ctx->dev_ptrs[i], 2 * cap * sizeof(void *)));
+      if (!ctx->ring_ev[i]) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ring_ev[i], hipEventDisableTiming));
+    }
+    ctx->ring_cap = cap;
+  }
+  const int slot = ctx->ring_pos;
+  ctx->ring_pos = (ctx->ring_pos + 1) % kRing;
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ring_ev[slot]));
+  const void **hp = ctx->pin_ptrs[slot];
+  for (size_t i = 0; i < B; i++) {
+    auto it = ctx->clients.find(ids[i]);
+    if (it == ctx->clients.end())
+      return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(ids[i]));
+    hp[i] = it->second.bv;
+    hp[B + i] = it->second.gsw;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->dev_ptrs[slot], hp, 2 * B * sizeof(void *), hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipEventRecord(ctx->ring_ev[slot], ctx->stream));
+  *table = ctx->dev_ptrs[slot];
+  return OPIR_OK;
+}
+
+// Nodes of expansion level L (server.cpp:731-816): ceil(u * 2^L / 2^h), at most 2^L.
+inline size_t nodes_at_level(size_t u, size_t h, size_t L) {
+  const size_t cap = size_t{1} << h;
+  const size_t n = (u * (size_t{1} << L) + cap - 1) / cap;
+  const size_t full = size_t{1} << L;
+  return n < full ? n : full;
+}
+
+inline int validate_mod(opir_ctx *ctx, const uint64_t *v, size_t n, uint64_t mod, const char *what) {
+  for (size_t i = 0; i < n; i++)
+    if (v[i] >= mod) return set_err(ctx, OPIR_EINVAL, std::string(what) + " value not reduced mod Q");
+  return OPIR_OK;
+}
+
+// server.cpp:613-622: the selector tree must not be taller than the candidates.
+inline int check_answerable(opir_db *db) {
+  if (db->dims > 1 && db->Nrest < (size_t{1} << (db->dims - 2)))
+    return set_err(db->ctx, OPIR_EINVAL,
+                   "evaluate_other_dim: layout has fewer candidates than its selector tree; query-shape views "
+                   "cannot answer make_query");
+  return OPIR_OK;
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+#define DEV_ALLOC(ctx, buf, bytes) HIP_TRY(ctx, hipMalloc(&(buf).p, (bytes)))
+
+// K = 2 implementations behind the same entry points (opir_capi_k2.hip).
+int k2_validate_scheme(const opir_scheme *sch);  // host-only checks, before any device call
+int k2_ctx_init(opir_ctx *ctx, const opir_scheme *sch);
+int k2_validate_rns(opir_ctx *ctx, const uint64_t *v, size_t npoly, const char *what);
+void k2_ctx_free(opir_ctx *ctx);
+int k2_db_init(opir_db *db);
+size_t k2_ws_bytes(const opir_db *db, size_t B);
+int k2_ensure_ws(opir_db *db, size_t B);
+void k2_db_free(opir_db *db);
+int k2_load_plain(opir_db *db, const uint64_t *coeffs, size_t first_pt, size_t count);
+int k2_set_keys(opir_ctx *ctx, const uint64_t *bv, size_t h, const uint64_t *gsw, ClientKeys &k);
+int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u64 *d_r);
+int k2_ntt(opir_ctx *ctx, uint64_t *data, size_t count, bool inverse);
+int k2_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out);
+int k2_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *sel);
+int k2_first_dim(opir_db *db, const uint64_t *query, uint64_t *out);
+int k2_other_dim(opir_db *db, const uint64_t *mid, const uint64_t *sel, uint64_t *out);
+int k2_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const uint64_t *ct, uint64_t *out);
+int k2_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out);
+
+}  // namespace capi

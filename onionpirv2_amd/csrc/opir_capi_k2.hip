@@ -1,0 +1,428 @@
+// C ABI entry points for the two-prime scheme (CONFIG_N4096_K2_MP): the same
+// PirServer::make_query pipeline (src/server.cpp:895-947) over two ~60-bit
+// RNS limbs, with the kernels of opir_k2.hip. Ciphertexts cross the ABI in
+// the reference's layout: c0 = [limb0 N][limb1 N], then c1 (rlwe.h:17-31).
+#include "opir_capi_internal.h"
+
+namespace capi {
+
+namespace {
+
+constexpr size_t N2 = k2::kN;
+constexpr size_t kCt2 = k2::kCt;  // u64 values per ciphertext
+
+}  // namespace
+
+// Every polynomial of v ([npoly][2 limbs][N]) reduced mod its limb prime.
+int k2_validate_rns(opir_ctx *ctx, const uint64_t *v, size_t npoly, const char *what) {
+  const u64 q[2] = {ctx->S64.limb[0].q, ctx->S64.limb[1].q};
+  for (size_t p = 0; p < npoly; p++)
+    for (int m = 0; m < 2; m++) {
+      const uint64_t *x = v + (p * 2 + m) * N2;
+      for (size_t j = 0; j < N2; j++)
+        if (x[j] >= q[m]) return set_err(ctx, OPIR_EINVAL, std::string(what) + " value not reduced mod its RNS prime");
+    }
+  return OPIR_OK;
+}
+
+namespace {
+
+int validate_rns(opir_ctx *ctx, const uint64_t *v, size_t npoly, const char *what) {
+  return k2_validate_rns(ctx, v, npoly, what);
+}
+
+int upload_k2(opir_ctx *ctx, const uint64_t *ids, size_t B, k2::KeyPtrs64 &P) {
+  const void *const *t = nullptr;
+  const int rc = upload_key_ptrs(ctx, ids, B, &t);
+  if (rc) return rc;
+  P.bv = reinterpret_cast<const u64 *const *>(t);
+  P.gsw = reinterpret_cast<const u64 *const *>(t + B);
+  return OPIR_OK;
+}
+
+void rec(opir_ctx *ctx, int i) {
+  if (ctx->profiling) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+}
+
+// Expansion levels over e64_a / e64_b; returns the buffer holding the leaves.
+u64 *run_expand(opir_db *db, size_t B, const k2::KeyPtrs64 &P) {
+  opir_ctx *ctx = db->ctx;
+  const size_t stride = db->exp_cts * kCt2;
+  u64 *in = db->e64_a, *out = db->e64_b;
+  for (size_t L = 0; L < db->h; L++) {
+    k2::launch_k2_expand_level(in, out, stride, (int)nodes_at_level(db->u, db->h, L), (int)B, (int)L, P,
+                               (size_t)ctx->G64.l_ks * kCt2, ctx->G64, ctx->S64, ctx->stream);
+    std::swap(in, out);
+  }
+  return in;
+}
+
+// MUX tree (server.cpp:599-649) over NTT-form candidates in m64_a.
+u64 *run_other_dim(opir_db *db, size_t B, const u64 *sel) {
+  opir_ctx *ctx = db->ctx;
+  const size_t stride = db->Nrest * kCt2;
+  if (db->dims == 1) return db->m64_a;
+  const size_t hs = db->dims - 1;
+  const size_t sel_vals = 2 * (size_t)ctx->G64.l_ep * kCt2;
+  const size_t sel_stride = hs * sel_vals;
+  const size_t last = 2 * db->Nrest - (size_t{1} << hs);
+  const size_t offset = db->Nrest - last;
+  const size_t perfect = size_t{1} << (hs - 1);
+  u64 *in = db->m64_a, *out = db->m64_b;
+  k2::launch_k2_mux(in, out, stride, (int)perfect, (int)offset, 0, (int)B, sel, sel_stride, ctx->G64, ctx->S64,
+                    ctx->stream);
+  std::swap(in, out);
+  for (size_t a = 1; a < hs; a++) {
+    const size_t half = (size_t{1} << (hs - a)) >> 1;
+    k2::launch_k2_mux(in, out, stride, (int)half, -1, (int)half, (int)B, sel + a * sel_vals, sel_stride, ctx->G64,
+                      ctx->S64, ctx->stream);
+    std::swap(in, out);
+  }
+  return in;
+}
+
+}  // namespace
+
+int k2_validate_scheme(const opir_scheme *sch) {
+  const u64 q0 = sch->q1, q1 = sch->q2;
+  const u64 two_n = 2 * N2;
+  if (q0 < 3 || q1 < 3 || q0 >= (1ULL << 60) || q1 >= (1ULL << 60) || q0 == q1 || (q0 - 1) % two_n ||
+      (q1 - 1) % two_n || q0 >= 2 * q1 || q1 >= 2 * q0)
+    return set_err(nullptr, OPIR_EINVAL, "K=2 limb primes must be distinct, < 2^60, within a factor 2 and 1 mod 2N");
+  const u64 roots[2] = {sch->ntt_root % q0, sch->ntt_root2 % q1};
+  const u64 qs[2] = {q0, q1};
+  for (int m = 0; m < 2; m++)
+    if (powmod_h(roots[m], N2, qs[m]) != qs[m] - 1)
+      return set_err(nullptr, OPIR_EINVAL, "ntt_root / ntt_root2 is not a primitive 2N-th root of its prime");
+  // unsigned digits must stay below both primes; the signed BV digits' last
+  // shift b*(l_ks-1) must fit a 128-bit value with its sign
+  if (sch->l_ep == 0 || sch->l_key == 0 || sch->l_ks == 0 || sch->base_log2 == 0 || sch->base_log2 > 32 ||
+      sch->base_log2_key == 0 || sch->base_log2_key > 32 || sch->bv_base_log2 < 2 || sch->bv_base_log2 > 32 ||
+      (u64)sch->bv_base_log2 * (sch->l_ks - 1) > 126 || sch->l_ep > 64 || sch->l_key > 64)
+    return set_err(nullptr, OPIR_EINVAL, "invalid gadget parameters");
+  if (sch->small_q < 2 || sch->small_q >= q0) return set_err(nullptr, OPIR_EINVAL, "small_q must be below q0");
+  return OPIR_OK;
+}
+
+int k2_ctx_init(opir_ctx *ctx, const opir_scheme *sch) {
+  const u64 q0 = sch->q1, q1 = sch->q2;
+  const u64 roots[2] = {sch->ntt_root % q0, sch->ntt_root2 % q1};
+  const u64 qs[2] = {q0, q1};
+  std::vector<u64> tab(2 * 4 * N2);
+  for (int m = 0; m < 2; m++) {
+    const u64 q = qs[m], w = roots[m];
+    u64 wi = 0, ni = 0;
+    if (!inv_h(w, q, wi) || !inv_h(N2 % q, q, ni)) return set_err(nullptr, OPIR_EINVAL, "root or N not invertible");
+    u64 *psi = tab.data() + (size_t)m * 4 * N2, *psi_s = psi + N2, *ipsi = psi + 2 * N2, *ipsi_s = psi + 3 * N2;
+    u64 p = 1, pi = 1;
+    for (size_t i = 0; i < N2; i++) {
+      const size_t r = brv(i, 12);
+      psi[r] = p; psi_s[r] = shoup64_h(p, q);
+      ipsi[r] = pi; ipsi_s[r] = shoup64_h(pi, q);
+      p = mulmod_h(p, w, q);
+      pi = mulmod_h(pi, wi, q);
+    }
+    k2::Limb64 &L = ctx->S64.limb[m];
+    L.q = q; L.q2 = 2 * q;
+    L.r64 = (u64)(((u128)1 << 64) % q); L.r64s = shoup64_h(L.r64, q);
+    L.one_s = (u64)(((u128)1 << 64) / q);
+    L.ninv = ni; L.ninv_s = shoup64_h(ni, q);
+  }
+  HIP_TRY(nullptr, hipMalloc((void **)&ctx->d_tables64, tab.size() * sizeof(u64)));
+  HIP_TRY(nullptr, hipMemcpy(ctx->d_tables64, tab.data(), tab.size() * sizeof(u64), hipMemcpyHostToDevice));
+  for (int m = 0; m < 2; m++) {
+    k2::Limb64 &L = ctx->S64.limb[m];
+    const u64 *base = ctx->d_tables64 + (size_t)m * 4 * N2;
+    L.psi = base; L.psi_s = base + N2; L.ipsi = base + 2 * N2; L.ipsi_s = base + 3 * N2;
+  }
+  u64 q0inv = 0;
+  if (!inv_h(q0 % q1, q1, q0inv)) return set_err(nullptr, OPIR_EINVAL, "q0, q1 not coprime");
+  ctx->S64.q0inv = q0inv;
+  ctx->S64.q0inv_s = shoup64_h(q0inv, q1);
+  const u128 Q = (u128)q0 * q1;
+  ctx->S64.Q_lo = (u64)Q; ctx->S64.Q_hi = (u64)(Q >> 64);
+  ctx->S64.hQ_lo = (u64)(Q >> 1); ctx->S64.hQ_hi = (u64)(Q >> 65);
+  ctx->S64.small_q = sch->small_q;
+  ctx->G64 = k2::Gadget64{(int)sch->l_ep, (int)sch->base_log2, (int)sch->l_key, (int)sch->base_log2_key,
+                          (int)sch->l_ks, (int)sch->bv_base_log2};
+  ctx->k2 = true;
+  ctx->N = N2;
+  return OPIR_OK;
+}
+
+void k2_ctx_free(opir_ctx *ctx) {
+  if (ctx->d_tables64) (void)hipFree(ctx->d_tables64);
+  ctx->d_tables64 = nullptr;
+}
+
+int k2_db_init(opir_db *db) {
+  opir_ctx *ctx = db->ctx;
+  const size_t bytes = 2 * db->num_pt * N2 * sizeof(u64);
+  if (hipMalloc((void **)&db->d_db64, bytes) != hipSuccess)
+    return set_err(ctx, OPIR_ENOMEM, "hipMalloc database (" + std::to_string(bytes) + " bytes)");
+  HIP_TRY(ctx, hipMemsetAsync(db->d_db64, 0, bytes, ctx->stream));
+  return OPIR_OK;
+}
+
+size_t k2_ws_bytes(const opir_db *db, size_t B) {
+  const size_t ct = kCt2 * sizeof(u64);
+  const size_t nsel = db->dims - 1;
+  return 2 * B * db->exp_cts * ct + B * db->N0 * ct + B * (nsel ? nsel : 1) * 2 * db->ctx->G64.l_ep * ct +
+         2 * B * db->Nrest * ct + B * 4 * N2 * sizeof(u64) + B * 2 * N2 * sizeof(u64);
+}
+
+static void free_ws(opir_db *db) {
+  for (u64 **b : {&db->e64_a, &db->e64_b, &db->q64, &db->sel64, &db->m64_a, &db->m64_b, &db->q_u64, &db->r_u64}) {
+    if (*b) (void)hipFree(*b);
+    *b = nullptr;
+  }
+  db->ws_B = 0;
+}
+
+int k2_ensure_ws(opir_db *db, size_t B) {
+  opir_ctx *ctx = db->ctx;
+  if (db->ws_B >= B) return OPIR_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // queued work may still use the old buffers
+  free_ws(db);
+  const size_t ct = kCt2 * sizeof(u64), nsel = db->dims - 1;
+  HIP_TRY(ctx, hipMalloc((void **)&db->e64_a, B * db->exp_cts * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->e64_b, B * db->exp_cts * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->q64, B * db->N0 * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->sel64, B * (nsel ? nsel : 1) * 2 * ctx->G64.l_ep * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->m64_a, B * db->Nrest * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->m64_b, B * db->Nrest * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->q_u64, B * 4 * N2 * sizeof(u64)));
+  HIP_TRY(ctx, hipMalloc((void **)&db->r_u64, B * 2 * N2 * sizeof(u64)));
+  db->ws_B = B;
+  return OPIR_OK;
+}
+
+void k2_db_free(opir_db *db) {
+  free_ws(db);
+  if (db->d_db64) (void)hipFree(db->d_db64);
+  db->d_db64 = nullptr;
+}
+
+int k2_load_plain(opir_db *db, const uint64_t *coeffs, size_t first_pt, size_t count) {
+  opir_ctx *ctx = db->ctx;
+  if (first_pt > db->num_pt || count > db->num_pt - first_pt)
+    return set_err(ctx, OPIR_EINVAL, "PirServer::load_data logical plaintext count exceeds its layout");
+  const u64 t = ctx->sch.plain_mod;
+  for (size_t i = 0; i < count * N2; i++)
+    if (coeffs[i] >= t) return set_err(ctx, OPIR_EINVAL, "plaintext coefficient not reduced mod t");
+  const size_t chunk = 4096;
+  DevBuf d;
+  DEV_ALLOC(ctx, d, std::min(chunk, std::max<size_t>(count, 1)) * N2 * sizeof(u64));
+  for (size_t off = 0; off < count; off += chunk) {
+    const size_t n = std::min(chunk, count - off);
+    HIP_TRY(ctx, hipMemcpyAsync(d.p, coeffs + off * N2, n * N2 * sizeof(u64), hipMemcpyHostToDevice, ctx->stream));
+    k2::launch_k2_db_load((const u64 *)d.p, db->d_db64, db->num_pt, first_pt + off, n, ctx->S64, ctx->stream);
+    LAUNCH_CHECK(ctx);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return OPIR_OK;
+}
+
+int k2_set_keys(opir_ctx *ctx, const uint64_t *bv, size_t h, const uint64_t *gsw, ClientKeys &k) {
+  const size_t L = ctx->G64.l_ks, lk = ctx->G64.l_key;
+  const u64 q[2] = {ctx->S64.limb[0].q, ctx->S64.limb[1].q};
+  // bv in: [h][l_ks][a, b][limb][N]; device: [h][l_ks][b (-> c0), a (-> c1)][limb][N]
+  std::vector<u64> hbv(h * L * kCt2);
+  for (size_t r = 0; r < h * L; r++)
+    for (int ab = 0; ab < 2; ab++)
+      for (int m = 0; m < 2; m++) {
+        const uint64_t *src = bv + ((r * 2 + ab) * 2 + m) * N2;
+        u64 *dst = hbv.data() + r * kCt2 + ((size_t)(ab == 0 ? 1 : 0) * 2 + m) * N2;
+        for (size_t j = 0; j < N2; j++) {
+          if (src[j] >= q[m]) return set_err(ctx, OPIR_EINVAL, "BV key value not reduced mod its RNS prime");
+          dst[j] = src[j];
+        }
+      }
+  int rc = validate_rns(ctx, gsw, 2 * lk * 2, "RGSW key");
+  if (rc) return rc;
+  if (h) HIP_TRY(ctx, hipMalloc(&k.bv, hbv.size() * sizeof(u64)));
+  if (hipMalloc(&k.gsw, 2 * lk * kCt2 * sizeof(u64)) != hipSuccess) {
+    (void)hipFree(k.bv);
+    return set_err(ctx, OPIR_ENOMEM, "hipMalloc keys");
+  }
+  if ((h && hipMemcpy(k.bv, hbv.data(), hbv.size() * sizeof(u64), hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(k.gsw, gsw, 2 * lk * kCt2 * sizeof(u64), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(k.bv);
+    (void)hipFree(k.gsw);
+    return set_err(ctx, OPIR_EHIP, "upload keys");
+  }
+  k.h = h;
+  return OPIR_OK;
+}
+
+int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u64 *d_r) {
+  opir_ctx *ctx = db->ctx;
+  int rc = check_answerable(db);
+  if (rc) return rc;
+  rc = k2_ensure_ws(db, B);
+  if (rc) return rc;
+  k2::KeyPtrs64 P;
+  rc = upload_k2(ctx, ids, B, P);
+  if (rc) return rc;
+  for (size_t i = 0; i < B; i++)
+    if (ctx->clients[ids[i]].h < db->h)
+      return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
+  const size_t exp_stride = db->exp_cts * kCt2;
+  rec(ctx, 0);
+  // root ct of query i -> e64_a[i][0]
+  HIP_TRY(ctx, hipMemcpy2DAsync(db->e64_a, exp_stride * sizeof(u64), d_q, kCt2 * sizeof(u64), kCt2 * sizeof(u64), B,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  u64 *leaves = run_expand(db, B, P);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 1);
+  const size_t nsel = db->dims - 1;
+  const size_t sel_stride = nsel * 2 * (size_t)ctx->G64.l_ep * kCt2;
+  k2::launch_k2_complete(leaves + db->N0 * kCt2, db->sel64, exp_stride, sel_stride, (int)(nsel * ctx->G64.l_ep),
+                         (int)B, P, ctx->G64, ctx->S64, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 2);
+  k2::launch_k2_ct_ntt(leaves, db->q64, exp_stride, db->N0 * kCt2, (int)db->N0, (int)B, ctx->S64, ctx->stream);
+  rec(ctx, 5);
+  k2::launch_k2_scan(db->d_db64, db->q64, db->m64_a, db->N0 * kCt2, db->Nrest * kCt2, db->N0, db->Nrest, (int)B,
+                     ctx->S64, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 6);
+  rec(ctx, 3);
+  u64 *res = run_other_dim(db, B, db->sel64);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 4);
+  k2::launch_k2_final(res, d_r, db->Nrest * kCt2, (int)B, ctx->S64, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  rec(ctx, 7);
+  return OPIR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Stage entry points (host buffers)
+// ---------------------------------------------------------------------------
+int k2_ntt(opir_ctx *ctx, uint64_t *data, size_t count, bool inverse) {
+  int rc = validate_rns(ctx, data, count, "NTT input");
+  if (rc) return rc;
+  DevBuf d;
+  DEV_ALLOC(ctx, d, count * 2 * N2 * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(d.p, data, count * 2 * N2 * sizeof(u64), hipMemcpyHostToDevice));
+  k2::launch_k2_ntt((u64 *)d.p, count, inverse, ctx->S64, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(data, d.p, count * 2 * N2 * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int k2_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out) {
+  opir_ctx *ctx = db->ctx;
+  int rc = validate_rns(ctx, query, 2, "query");
+  if (rc) return rc;
+  if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  if (ctx->clients[id].h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
+  rc = k2_ensure_ws(db, 1);
+  if (rc) return rc;
+  k2::KeyPtrs64 P;
+  rc = upload_k2(ctx, &id, 1, P);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpy(db->e64_a, query, kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  u64 *leaves = run_expand(db, 1, P);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, leaves, db->u * kCt2 * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int k2_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *sel) {
+  opir_ctx *ctx = db->ctx;
+  const size_t nsel = db->dims - 1;
+  if (!nsel) return OPIR_OK;
+  if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  int rc = validate_rns(ctx, expanded, db->u * 2, "expanded");
+  if (rc) return rc;
+  rc = k2_ensure_ws(db, 1);
+  if (rc) return rc;
+  k2::KeyPtrs64 P;
+  rc = upload_k2(ctx, &id, 1, P);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpy(db->e64_a, expanded, db->u * kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  const size_t sel_vals = nsel * 2 * (size_t)ctx->G64.l_ep * kCt2;
+  k2::launch_k2_complete(db->e64_a + db->N0 * kCt2, db->sel64, 0, sel_vals, (int)(nsel * ctx->G64.l_ep), 1, P,
+                         ctx->G64, ctx->S64, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(sel, db->sel64, sel_vals * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int k2_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
+  opir_ctx *ctx = db->ctx;
+  int rc = validate_rns(ctx, query, db->N0 * 2, "query");
+  if (rc) return rc;
+  rc = k2_ensure_ws(db, 1);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpy(db->e64_a, query, db->N0 * kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  k2::launch_k2_ct_ntt(db->e64_a, db->q64, 0, 0, (int)db->N0, 1, ctx->S64, ctx->stream);
+  k2::launch_k2_scan(db->d_db64, db->q64, db->m64_a, 0, 0, db->N0, db->Nrest, 1, ctx->S64, ctx->stream);
+  k2::launch_k2_ntt(db->m64_a, db->Nrest * 2, true, ctx->S64, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, db->m64_a, db->Nrest * kCt2 * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int k2_other_dim(opir_db *db, const uint64_t *mid, const uint64_t *sel, uint64_t *out) {
+  opir_ctx *ctx = db->ctx;
+  if (int e = check_answerable(db)) return e;
+  const size_t nsel = db->dims - 1;
+  const size_t sel_vals = nsel * 2 * (size_t)ctx->G64.l_ep * kCt2;
+  int rc = validate_rns(ctx, mid, db->Nrest * 2, "candidates");
+  if (rc) return rc;
+  if (nsel) rc = validate_rns(ctx, sel, sel_vals / N2 / 2, "selectors");
+  if (rc) return rc;
+  rc = k2_ensure_ws(db, 1);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpy(db->m64_a, mid, db->Nrest * kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  k2::launch_k2_ntt(db->m64_a, db->Nrest * 2, false, ctx->S64, ctx->stream);
+  if (nsel) HIP_TRY(ctx, hipMemcpy(db->sel64, sel, sel_vals * sizeof(u64), hipMemcpyHostToDevice));
+  u64 *res = run_other_dim(db, 1, db->sel64);
+  k2::launch_k2_ntt(res, 2, true, ctx->S64, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, res, kCt2 * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int k2_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const uint64_t *ct, uint64_t *out) {
+  if (l == 0 || l > 64 || b == 0 || b > 32) return set_err(ctx, OPIR_EINVAL, "invalid gadget");
+  int rc = validate_rns(ctx, gsw, 2 * l * 2, "RGSW");
+  if (rc) return rc;
+  rc = validate_rns(ctx, ct, 2, "ciphertext");
+  if (rc) return rc;
+  DevBuf g, c, o;
+  DEV_ALLOC(ctx, g, 2 * l * kCt2 * sizeof(u64));
+  DEV_ALLOC(ctx, c, kCt2 * sizeof(u64));
+  DEV_ALLOC(ctx, o, kCt2 * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(g.p, gsw, 2 * l * kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(c.p, ct, kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  k2::launch_k2_ext_prod((const u64 *)g.p, (int)l, (int)b, (const u64 *)c.p, (u64 *)o.p, ctx->S64, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, kCt2 * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int k2_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out) {
+  int rc = validate_rns(ctx, ct, 2, "ciphertext");
+  if (rc) return rc;
+  DevBuf c, o;
+  DEV_ALLOC(ctx, c, kCt2 * sizeof(u64));
+  DEV_ALLOC(ctx, o, 2 * N2 * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(c.p, ct, kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  k2::launch_k2_mod_switch((const u64 *)c.p, (u64 *)o.p, ctx->S64, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, 2 * N2 * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+}  // namespace capi

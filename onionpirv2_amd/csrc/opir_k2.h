@@ -1,0 +1,82 @@
+// Launch interface of the two-prime (K = 2) answer path, CONFIG_N4096_K2_MP:
+// N = 4096, Q = q0 * q1 with two ~60-bit NTT primes, the whole pipeline in
+// two RNS limbs (src/server.cpp:205-262,363-527; src/gsw.cpp:22-85,184-250;
+// src/bv_keyswitch.cpp:373-481; src/server.cpp:1044-1075).
+//
+// Device ciphertext layout ("K2 ct"): [poly][limb][N] u64, 4*N values, each
+// limb in [0, q_limb); NTT-form polynomials in HEXL evaluation order, as the
+// reference stores them (c0 = [limb0 N][limb1 N], then c1).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace opir {
+namespace k2 {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+constexpr int kN = 4096;
+constexpr int kLB = 512;                 // threads per lane block (one limb)
+constexpr size_t kCt = 4 * (size_t)kN;   // u64 values per ciphertext
+
+struct Limb64 {
+  u64 q, q2;              // q, 2q
+  u64 r64, r64s;          // 2^64 mod q and its Shoup companion
+  u64 one_s;              // floor(2^64 / q)
+  u64 ninv, ninv_s;       // N^-1 mod q and Shoup companion
+  const u64 *psi, *psi_s;     // forward twiddles, bit-reversed powers
+  const u64 *ipsi, *ipsi_s;   // inverse twiddles
+};
+
+struct Scheme64 {
+  Limb64 limb[2];
+  u64 q0inv, q0inv_s;     // q0^-1 mod q1 (and Shoup companion mod q1)
+  u64 Q_lo, Q_hi;         // Q = q0 * q1
+  u64 hQ_lo, hQ_hi;       // Q >> 1
+  u64 small_q;
+};
+
+struct Gadget64 {
+  int l_ep, b_ep;         // data selectors: unsigned MP digits
+  int l_key, b_key;       // RGSW(s): unsigned MP digits
+  int l_ks, b_ks;         // BV keyswitch: signed MP digits
+};
+
+struct KeyPtrs64 {
+  const u64 *const *bv;   // per query: [h][l_ks][2 (b, a)][2 limbs][N]
+  const u64 *const *gsw;  // per query: [2 l_key][2 polys][2 limbs][N]
+};
+
+// NTT of `count` polynomials of one limb each: data[p] is limb (p % 2) of
+// polynomial p / 2 in [poly][limb][N] order. In place.
+void launch_k2_ntt(u64 *data, size_t count, bool inverse, const Scheme64 &S, hipStream_t st);
+// Expansion level: nodes [0, n) of in -> children 2r, 2r+1 of out (coefficient form).
+void launch_k2_expand_level(const u64 *in, u64 *out, size_t stride, int nnodes, int B, int level,
+                            const KeyPtrs64 &P, size_t key_level_values, const Gadget64 &G, const Scheme64 &S,
+                            hipStream_t st);
+// Selector completion: nrows expanded rows (coefficient form) -> sel [nsel][2 l_ep][ct] (NTT form).
+void launch_k2_complete(const u64 *rows, u64 *sel, size_t row_stride, size_t sel_stride, int nrows, int B,
+                        const KeyPtrs64 &P, const Gadget64 &G, const Scheme64 &S, hipStream_t st);
+// Forward NTT of ciphertexts: in (coef) -> out (NTT), count cts per query.
+void launch_k2_ct_ntt(const u64 *in, u64 *out, size_t in_stride, size_t out_stride, int count, int B,
+                      const Scheme64 &S, hipStream_t st);
+// First dimension: db [limb][pt][N] (NTT form), qntt [B][N0][ct] -> mid [B][Nrest][ct] (NTT form).
+void launch_k2_scan(const u64 *db, const u64 *qntt, u64 *mid, size_t q_stride, size_t mid_stride, size_t N0,
+                    size_t Nrest, int B, const Scheme64 &S, hipStream_t st);
+// MUX level on NTT-form candidates (same slot rules as the K = 1 path).
+void launch_k2_mux(const u64 *in, u64 *out, size_t stride, int nslots, int offset, int half, int B, const u64 *sel,
+                   size_t sel_stride, const Gadget64 &G, const Scheme64 &S, hipStream_t st);
+// INTT, drop q1 (round half up), centred rescale to small_q: res [B][ct] -> out [B][2][N].
+void launch_k2_final(const u64 *res, u64 *out, size_t res_stride, int B, const Scheme64 &S, hipStream_t st);
+// Plaintext coefficients (< t) [count][N] -> NTT per limb into db [limb][first + i][N].
+void launch_k2_db_load(const u64 *pts, u64 *db, size_t num_pt, size_t first, size_t count, const Scheme64 &S,
+                       hipStream_t st);
+// External product alone (test entry): gsw [2l][ct] NTT, ct coef -> out NTT.
+void launch_k2_ext_prod(const u64 *gsw, int l, int b, const u64 *ct, u64 *out, const Scheme64 &S, hipStream_t st);
+// Mod switch alone (test entry): coefficient-form ct -> [2][N] mod small_q.
+void launch_k2_mod_switch(const u64 *ct, u64 *out, const Scheme64 &S, hipStream_t st);
+
+}  // namespace k2
+}  // namespace opir

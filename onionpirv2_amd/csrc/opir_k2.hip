@@ -1,0 +1,672 @@
+// The two-prime answer path (CONFIG_N4096_K2_MP) on CDNA4.
+//
+// Arithmetic: each limb is a ~60-bit NTT prime, so values are u64 and every
+// product is a 64x64->128 product. Modular multiplication by a known
+// constant uses Shoup's method with a 64-bit companion (floor(w*2^64/q));
+// products of two variables are reduced from 128 bits as
+// hi*(2^64 mod q) + lo, both halves by Shoup. NTT butterflies are
+// Harvey-lazy in [0, 4q) (4q < 2^62).
+//
+// Thread layout: a 4096-point transform is a lane block of 512 threads with
+// 8 values each, four radix-8 rounds and three LDS exchanges; value c of
+// thread t sits at index g*8T + (t % T) + c*T with T = 512 >> 3r, g = t / T in
+// round r (r = 3: 8t + c). Tree nodes are 1024-thread workgroups, one lane
+// block per limb.
+//
+// Gadgets (src/gsw.cpp:22-85,184-250; src/bv_keyswitch.cpp:42-64,373-481):
+// the two limbs are recombined by CRT into X in [0, Q), Q ~ 2^120.
+//   data / key selectors: unsigned digits (X >> (b*i)) & (2^b - 1),
+//     stored at row poly*l + (l-1-i);
+//   BV keyswitch: signed digits in (-B/2, B/2] of X centred to (-Q/2, Q/2],
+//     digit i = ((floor((D + H_i) / B^i) + h) mod B) - h with h = B/2 - 1,
+//     H_i = h * (1 + B + ... + B^(i-1)) -- the closed form of the
+//     reference's carry loop, so every digit is computed independently.
+#include "opir_k2.h"
+
+namespace opir {
+namespace k2 {
+
+typedef unsigned __int128 u128;
+typedef __int128 i128;
+
+namespace {
+
+struct Barrier {
+  __device__ void operator()() const { __syncthreads(); }
+};
+
+constexpr int kLds = kN + kN / 32;  // padded u64 words of one exchange buffer
+
+__device__ __forceinline__ u64 csub(u64 x, u64 m) { return x >= m ? x - m : x; }
+__device__ __forceinline__ u64 addmod(u64 a, u64 b, u64 q) { return csub(a + b, q); }
+__device__ __forceinline__ u64 submod(u64 a, u64 b, u64 q) { return a >= b ? a - b : a + q - b; }
+
+// a * w mod q lazily in [0, 2q), any a < 2^64, w < q.
+__device__ __forceinline__ u64 shoup(u64 a, u64 w, u64 ws, u64 q) { return a * w - __umul64hi(a, ws) * q; }
+
+// (hi * 2^64 + lo) mod q, exact.
+__device__ __forceinline__ u64 reduce128(u64 hi, u64 lo, const Limb64 &L) {
+  const u64 a = shoup(hi, L.r64, L.r64s, L.q);
+  const u64 b = lo - __umul64hi(lo, L.one_s) * L.q;
+  return csub(csub(a + b, L.q2), L.q);
+}
+
+// a * b mod q for any a < 2^64, b < 2^64.
+__device__ __forceinline__ u64 mulmod(u64 a, u64 b, const Limb64 &L) { return reduce128(__umul64hi(a, b), a * b, L); }
+
+// CRT (r0 mod q0, r1 mod q1) -> X in [0, Q) (Garner, server.cpp:529-553 form).
+__device__ __forceinline__ u128 crt2(u64 r0, u64 r1, const Scheme64 &S) {
+  const u64 q1 = S.limb[1].q;
+  const u64 r0m = csub(r0, q1);  // r0 < q0 < 2 q1
+  const u64 s = csub(shoup(submod(r1, r0m, q1), S.q0inv, S.q0inv_s, q1), q1);
+  return (u128)S.limb[0].q * s + r0;
+}
+
+__device__ __forceinline__ u128 Q128(const Scheme64 &S) { return ((u128)S.Q_hi << 64) | S.Q_lo; }
+__device__ __forceinline__ u128 hQ128(const Scheme64 &S) { return ((u128)S.hQ_hi << 64) | S.hQ_lo; }
+
+// Signed residue r (|r| < 2^63) mod q.
+__device__ __forceinline__ u64 sres(int64_t r, u64 q) { return r >= 0 ? (u64)r : q - (u64)(-r); }
+
+// num / d for a quotient known to be < 2^bits (restoring division).
+__device__ __forceinline__ u64 div_small(u128 num, u64 d, int bits) {
+  u64 quo = 0;
+  for (int bit = bits - 1; bit >= 0; bit--) {
+    const u128 s = (u128)d << bit;
+    if (num >= s) {
+      num -= s;
+      quo |= (u64)1 << bit;
+    }
+  }
+  return quo;
+}
+
+// ---------------------------------------------------------------------------
+// 4096-point NTT on a 512-thread lane block
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 5); }
+__device__ __forceinline__ int r_index(int r, int t, int c) {
+  const int T = 512 >> (3 * r);
+  return (t / T) * 8 * T + (t % T) + c * T;
+}
+__device__ __forceinline__ int map0(int t, int c) { return t + 512 * c; }
+__device__ __forceinline__ int map3(int t, int c) { return 8 * t + c; }
+
+__device__ __forceinline__ void bf_fwd(u64 &X, u64 &Y, u64 w, u64 ws, const Limb64 &L) {
+  const u64 x = csub(X, L.q2);
+  const u64 t = shoup(Y, w, ws, L.q);
+  X = x + t;
+  Y = x - t + L.q2;
+}
+__device__ __forceinline__ void bf_inv(u64 &X, u64 &Y, u64 w, u64 ws, const Limb64 &L) {
+  const u64 x = X, y = Y;
+  X = csub(x + y, L.q2);
+  Y = shoup(x - y + L.q2, w, ws, L.q);
+}
+
+__device__ __forceinline__ void fwd8(u64 v[8], unsigned b, const Limb64 &L) {
+  {
+    const u64 w = L.psi[b], ws = L.psi_s[b];
+#pragma unroll
+    for (int c = 0; c < 4; c++) bf_fwd(v[c], v[c + 4], w, ws, L);
+  }
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const u64 w = L.psi[2 * b + h], ws = L.psi_s[2 * b + h];
+    bf_fwd(v[4 * h + 0], v[4 * h + 2], w, ws, L);
+    bf_fwd(v[4 * h + 1], v[4 * h + 3], w, ws, L);
+  }
+#pragma unroll
+  for (int h = 0; h < 4; h++) bf_fwd(v[2 * h], v[2 * h + 1], L.psi[4 * b + h], L.psi_s[4 * b + h], L);
+}
+
+__device__ __forceinline__ void inv8(u64 v[8], unsigned b, const Limb64 &L) {
+#pragma unroll
+  for (int h = 0; h < 4; h++) bf_inv(v[2 * h], v[2 * h + 1], L.ipsi[4 * b + h], L.ipsi_s[4 * b + h], L);
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const u64 w = L.ipsi[2 * b + h], ws = L.ipsi_s[2 * b + h];
+    bf_inv(v[4 * h + 0], v[4 * h + 2], w, ws, L);
+    bf_inv(v[4 * h + 1], v[4 * h + 3], w, ws, L);
+  }
+  {
+    const u64 w = L.ipsi[b], ws = L.ipsi_s[b];
+#pragma unroll
+    for (int c = 0; c < 4; c++) bf_inv(v[c], v[c + 4], w, ws, L);
+  }
+}
+
+// LDS exchange between rounds. t is passed by reference and made opaque
+// afterwards, so the next round's twiddle addresses (functions of t) cannot be
+// computed and loaded before the exchange: hoisting all rounds' twiddles
+// exceeds the 128-register budget of a 1024-thread workgroup.
+__device__ __forceinline__ void exch(u64 v[8], u64 *lds, int &t, int from, int to) {
+#pragma unroll
+  for (int c = 0; c < 8; c++) lds[pidx(r_index(from, t, c))] = v[c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = lds[pidx(r_index(to, t, c))];
+  __syncthreads();
+  asm volatile("" : "+v"(t));
+}
+
+// map0 [0, q) in -> map3 out, [0, q) when reduce, else [0, 4q).
+__device__ __forceinline__ void ntt_fwd(u64 v[8], u64 *lds, int t, const Limb64 &L, bool reduce = true) {
+  asm volatile("" : "+v"(t));
+  fwd8(v, 1, L);
+  exch(v, lds, t, 0, 1);
+  fwd8(v, 8 + t / 64, L);
+  exch(v, lds, t, 1, 2);
+  fwd8(v, 64 + t / 8, L);
+  exch(v, lds, t, 2, 3);
+  fwd8(v, 512 + t, L);
+  if (reduce) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = csub(csub(v[c], L.q2), L.q);
+  }
+}
+
+// map3 [0, q) in -> map0 out in [0, q).
+__device__ __forceinline__ void ntt_inv(u64 v[8], u64 *lds, int t, const Limb64 &L) {
+  asm volatile("" : "+v"(t));
+  inv8(v, 512 + t, L);
+  exch(v, lds, t, 3, 2);
+  inv8(v, 64 + t / 8, L);
+  exch(v, lds, t, 2, 1);
+  inv8(v, 8 + t / 64, L);
+  exch(v, lds, t, 1, 0);
+  inv8(v, 1, L);
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = csub(shoup(v[c], L.ninv, L.ninv_s, L.q), L.q);
+}
+
+__device__ __forceinline__ void load0(u64 v[8], const u64 *p, int t) {
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = p[map0(t, c)];
+}
+__device__ __forceinline__ void store0(u64 *p, const u64 v[8], int t) {
+#pragma unroll
+  for (int c = 0; c < 8; c++) p[map0(t, c)] = v[c];
+}
+__device__ __forceinline__ void load3(u64 v[8], const u64 *p, int t) {
+  const ulonglong2 *q = reinterpret_cast<const ulonglong2 *>(p + 8 * t);
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const ulonglong2 x = q[c];
+    v[2 * c] = x.x;
+    v[2 * c + 1] = x.y;
+  }
+}
+__device__ __forceinline__ void store3(u64 *p, const u64 v[8], int t) {
+  ulonglong2 *q = reinterpret_cast<ulonglong2 *>(p + 8 * t);
+#pragma unroll
+  for (int c = 0; c < 4; c++) q[c] = make_ulonglong2(v[2 * c], v[2 * c + 1]);
+}
+
+// acc[c] += v[c] * key[8t + c] mod q, two key values at a time.
+__device__ __forceinline__ void mac_row(u64 (&acc)[8], const u64 v[8], const u64 *key, int t, const Limb64 &L) {
+  const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(key + 8 * t);
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const ulonglong2 k = kp[c];
+    acc[2 * c] = addmod(acc[2 * c], mulmod(v[2 * c], k.x, L), L.q);
+    acc[2 * c + 1] = addmod(acc[2 * c + 1], mulmod(v[2 * c + 1], k.y, L), L.q);
+  }
+}
+
+// One polynomial's external-product contribution (gsw.cpp:119-182 with the
+// MP decomposition): X[c] are the recombined coefficients (map0) of poly p;
+// digits i < l of base 2^b go through the forward NTT of this lane block's
+// limb and are multiplied into acc0/acc1 with rows p*l + (l-1-i) of g
+// ([rows][2 outputs][2 limbs][N], NTT form).
+__device__ __forceinline__ void ep_poly(const u128 *xs, int p, int l, int b, const u64 *__restrict__ g, int limb,
+                                        int t_in, const Limb64 &L, u64 *lds, u64 (&acc0)[8], u64 (&acc1)[8]) {
+  const u64 mask = ((u64)1 << b) - 1;
+#pragma unroll 1
+  for (int i = 0; i < l; i++) {
+    // opaque copy of t: keeps per-thread addresses inside the loop (register pressure)
+    int t = t_in;
+    asm volatile("" : "+v"(t));
+    u64 v[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = (u64)(xs[map0(t, c)] >> (b * i)) & mask;  // < 2^b < q
+    ntt_fwd(v, lds, t, L, false);
+    const u64 *row = g + (size_t)(p * l + (l - 1 - i)) * kCt;
+    mac_row(acc0, v, row + (size_t)(0 * 2 + limb) * kN, t, L);
+    mac_row(acc1, v, row + (size_t)(1 * 2 + limb) * kN, t, L);
+  }
+}
+
+// Stage this lane block's limb of a coefficient-form polynomial (map0) and
+// recombine both limbs of every coefficient into xs[j] (LDS).
+__device__ __forceinline__ void stage_compose(const u64 v[8], u64 (*stage)[kLds], int limb, int t, const Scheme64 &S,
+                                              u128 *xs) {
+  store0(stage[limb], v, t);
+  __syncthreads();
+  for (int j = threadIdx.x; j < kN; j += 1024) xs[j] = crt2(stage[0][j], stage[1][j], S);
+  __syncthreads();
+}
+
+// Centred rescale of a mod-q0 value to small_q (utils.cpp:445-462).
+__device__ __forceinline__ u64 rescale_q0(u64 a, const Scheme64 &S) {
+  const u64 q0 = S.limb[0].q, h = q0 / 2;
+  const bool neg = a >= h;  // v >= inp/2 -> v - inp
+  const u64 mag = neg ? q0 - a : a;
+  const u64 quo = div_small((u128)mag * S.small_q + h, q0, 64);
+  const u64 r = quo % S.small_q;
+  return neg ? (r ? S.small_q - r : 0) : r;
+}
+
+// Drop q1 with round half up (server.cpp:1044-1075), then rescale.
+__device__ __forceinline__ u64 final_coeff(u64 r0, u64 r1, const Scheme64 &S) {
+  const u64 q0 = S.limb[0].q, q1 = S.limb[1].q;
+  const u128 x = crt2(r0, r1, S);
+  u64 o = div_small(x + (q1 >> 1), q1, 64);
+  if (o >= q0) o -= q0;
+  return rescale_q0(o, S);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k2_ntt(u64 *data, int inverse, Scheme64 S) {
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  u64 *p = data + ((size_t)blockIdx.x * 2 + limb) * kN;
+  u64 v[8];
+  if (!inverse) {
+    load0(v, p, t);
+    ntt_fwd(v, lds[limb], t, L);
+    store3(p, v, t);
+  } else {
+    load3(v, p, t);
+    ntt_inv(v, lds[limb], t, L);
+    store0(p, v, t);
+  }
+}
+
+struct ExpandArgs64 {
+  const u64 *in;
+  u64 *out;
+  size_t stride;
+  const u64 *const *bv;
+  size_t key_level_values;
+  int level, kinv, l_ks, b_ks;
+};
+
+// Expansion level (server.cpp:786-810, bv_keyswitch.cpp:373-481).
+__global__ __launch_bounds__(1024) void k2_expand_level(ExpandArgs64 A, Scheme64 S) {
+  __shared__ u128 xs[kN];  // recombined coefficients of the polynomial being decomposed
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  const int node = blockIdx.x, qi = blockIdx.y;
+  const u64 *c = A.in + qi * A.stride + (size_t)node * kCt;
+  const u64 *c1 = c + (size_t)(2 + limb) * kN, *c0 = c + (size_t)limb * kN;
+  // sigma_k(c1) (coefficient form): out[j] = in[i0] or -in[i0 - N], i0 = j*kinv mod 2N
+  u64 v[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const int j = map0(t, cc);
+    const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+    const u64 x = c1[i0 < kN ? i0 : i0 - kN];
+    v[cc] = (i0 < kN || !x) ? x : L.q - x;
+  }
+  // centred CRT value D of sigma(c1), as a two's-complement 128-bit integer
+  stage_compose(v, lds, limb, t, S, xs);
+  {
+    const u128 Q = Q128(S), hQ = hQ128(S);
+    for (int j = threadIdx.x; j < kN; j += 1024)
+      if (xs[j] > hQ) xs[j] -= Q;  // wraps to the two's complement of X - Q
+    __syncthreads();
+  }
+  u64 accb[8], acca[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
+  const int b = A.b_ks;
+  const i128 h = ((i128)1 << (b - 1)) - 1;
+  const u64 *key = A.bv[qi] + (size_t)A.level * A.key_level_values;
+  i128 H = 0;
+  const int t_in = t;
+  for (int i = 0; i < A.l_ks; i++) {
+    int t = t_in;
+    asm volatile("" : "+v"(t));
+    u64 d[8];
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      const i128 di = ((i128)xs[map0(t, cc)] + H) >> (b * i);
+      const int64_t r = (int64_t)(((u64)(di + h)) & (((u64)1 << b) - 1)) - (int64_t)h;
+      d[cc] = sres(r, L.q);
+    }
+    H += h << (b * i);
+    ntt_fwd(d, lds[limb], t, L, false);
+    const u64 *row = key + (size_t)i * kCt;  // [b (-> c0), a (-> c1)][limb][N]
+    mac_row(accb, d, row + (size_t)(0 * 2 + limb) * kN, t, L);
+    mac_row(acca, d, row + (size_t)(1 * 2 + limb) * kN, t, L);
+  }
+  ntt_inv(accb, lds[limb], t, L);  // delta_b, coefficient form (map0)
+  ntt_inv(acca, lds[limb], t, L);  // delta_a
+  // c' = (sigma(c0) + delta_b, delta_a); left = c + c', right = (c - c') * X^-k
+  u64 *left = A.out + qi * A.stride + (size_t)(2 * node) * kCt;
+  u64 *right = left + kCt;
+  const int k = 1 << A.level;
+  for (int p = 0; p < 2; p++) {
+    const u64 *cp = p ? c1 : c0;
+    u64 cv[8], dv[8];
+    load0(cv, cp, t);
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      u64 w = p ? acca[cc] : accb[cc];
+      if (p == 0) {
+        const int j = map0(t, cc);
+        const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+        const u64 x = c0[i0 < kN ? i0 : i0 - kN];
+        w = addmod(w, (i0 < kN || !x) ? x : L.q - x, L.q);
+      }
+      left[(size_t)(p * 2 + limb) * kN + map0(t, cc)] = addmod(cv[cc], w, L.q);
+      dv[cc] = submod(cv[cc], w, L.q);
+    }
+    store0(lds[limb], dv, t);
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      const int j = map0(t, cc), s = j + k;
+      const u64 x = lds[limb][s < kN ? s : s - kN];
+      right[(size_t)(p * 2 + limb) * kN + j] = (s < kN || !x) ? x : L.q - x;
+    }
+    __syncthreads();
+  }
+}
+
+struct CompleteArgs64 {
+  const u64 *rows;
+  u64 *sel;
+  size_t row_stride, sel_stride;
+  const u64 *const *gsw;
+  int l_ep, l_key, b_key;
+};
+
+// Selector completion (server.cpp:823-855, gsw.cpp:317-357).
+__global__ __launch_bounds__(1024) void k2_complete(CompleteArgs64 A, Scheme64 S) {
+  __shared__ u128 xs[kN];  // recombined coefficients of the polynomial being decomposed
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  const int g = blockIdx.x, qi = blockIdx.y;
+  const int a = g / A.l_ep, r = g % A.l_ep;
+  const u64 *row = A.rows + qi * A.row_stride + (size_t)g * kCt;
+  u64 *selb = A.sel + qi * A.sel_stride + (size_t)a * (2 * A.l_ep) * kCt;
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+#pragma unroll 1
+  for (int p = 0; p < 2; p++) {
+    u64 v[8];
+    load0(v, row + (size_t)(p * 2 + limb) * kN, t);
+    stage_compose(v, lds, limb, t, S, xs);
+    ntt_fwd(v, lds[limb], t, L);
+    store3(selb + (size_t)r * kCt + (size_t)(p * 2 + limb) * kN, v, t);
+    ep_poly(xs, p, A.l_key, A.b_key, A.gsw[qi], limb, t, L, lds[limb], acc0, acc1);
+    __syncthreads();  // xs is rewritten for the next polynomial
+  }
+  u64 *bot = selb + (size_t)(A.l_ep + r) * kCt;
+  store3(bot + (size_t)(0 * 2 + limb) * kN, acc0, t);
+  store3(bot + (size_t)(1 * 2 + limb) * kN, acc1, t);
+}
+
+__global__ __launch_bounds__(1024) void k2_ct_ntt(const u64 *in, u64 *out, size_t in_stride, size_t out_stride,
+                                                  Scheme64 S) {
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  const int ct = blockIdx.x >> 1, p = blockIdx.x & 1, qi = blockIdx.y;
+  u64 v[8];
+  load0(v, in + qi * in_stride + (size_t)ct * kCt + (size_t)(p * 2 + limb) * kN, t);
+  ntt_fwd(v, lds[limb], t, L);
+  store3(out + qi * out_stride + (size_t)ct * kCt + (size_t)(p * 2 + limb) * kN, v, t);
+}
+
+// First dimension (matrix.cpp:24-151, server.cpp:363-527): one thread per
+// (limb, row, level) and column tile of kCols query columns (c = 2*query +
+// poly); the DB value is read once per tile and multiplied into each
+// column's 128-bit sum, folded into a reduced residue every 64 terms
+// (64 products < 2^126).
+constexpr int kCols = 8;
+
+__global__ __launch_bounds__(256) void k2_scan(const u64 *__restrict__ db, const u64 *__restrict__ qntt,
+                                               u64 *__restrict__ mid, size_t q_stride, size_t mid_stride, int N0,
+                                               int Nrest, int ncols, int nct, Scheme64 S) {
+  const unsigned W = gridDim.x, p = blockIdx.x;
+  const unsigned Lg = (W % 8 == 0) ? (p % 8) * (W / 8) + p / 8 : p;  // same-XCD neighbours share DB rows
+  const int ct = Lg % nct;
+  unsigned rest = Lg / nct;
+  const int lvb = rest % (kN / 256);
+  rest /= (kN / 256);
+  const int row = rest % Nrest;
+  const int limb = rest / Nrest;
+  const Limb64 &L = S.limb[limb];
+  const int lv = lvb * 256 + threadIdx.x;
+  const int c0 = ct * kCols, cn = min(kCols, ncols - c0);
+  const size_t num_pt = (size_t)N0 * Nrest;
+  const u64 *a = db + ((size_t)limb * num_pt + (size_t)row * N0) * kN + lv;
+  u64 lo[kCols], hi[kCols], red[kCols];
+#pragma unroll
+  for (int j = 0; j < kCols; j++) lo[j] = hi[j] = red[j] = 0;
+  for (int k0 = 0; k0 < N0; k0 += 64) {
+    const int k1 = min(N0, k0 + 64);
+    for (int k = k0; k < k1; k++) {
+      const u64 x = a[(size_t)k * kN];
+#pragma unroll
+      for (int j = 0; j < kCols; j++) {
+        if (j < cn) {
+          const int c = c0 + j;
+          const u64 y = qntt[(c >> 1) * q_stride + (size_t)k * kCt + (size_t)((c & 1) * 2 + limb) * kN + lv];
+          const u64 pl = x * y, ph = __umul64hi(x, y);
+          lo[j] += pl;
+          hi[j] += ph + (lo[j] < pl);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kCols; j++) {
+      red[j] = addmod(red[j], reduce128(hi[j], lo[j], L), L.q);
+      lo[j] = hi[j] = 0;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kCols; j++)
+    if (j < cn) {
+      const int c = c0 + j;
+      mid[(c >> 1) * mid_stride + (size_t)row * kCt + (size_t)((c & 1) * 2 + limb) * kN + lv] = red[j];
+    }
+}
+
+struct MuxArgs64 {
+  const u64 *in;
+  u64 *out;
+  size_t stride;
+  const u64 *sel;
+  size_t sel_stride;
+  int offset, half, l, b;
+};
+
+// MUX (server.cpp:599-722) on NTT-form candidates: res = x + EP(sel, INTT(y - x)).
+__global__ __launch_bounds__(1024) void k2_mux(MuxArgs64 A, Scheme64 S) {
+  __shared__ u128 xs[kN];  // recombined coefficients of the polynomial being decomposed
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  const int s = blockIdx.x, qi = blockIdx.y;
+  const u64 *in = A.in + qi * A.stride;
+  u64 *dst = A.out + qi * A.stride + (size_t)s * kCt;
+  int xi, yi;
+  if (A.offset >= 0) {
+    if (s < A.offset) {
+      const u64 *src = in + (size_t)s * kCt;
+      for (int e = threadIdx.x; e < (int)kCt; e += 1024) dst[e] = src[e];
+      return;
+    }
+    xi = A.offset + 2 * (s - A.offset);
+    yi = xi + 1;
+  } else {
+    xi = s;
+    yi = s + A.half;
+  }
+  const u64 *x = in + (size_t)xi * kCt, *y = in + (size_t)yi * kCt;
+  const u64 *g = A.sel + qi * A.sel_stride;
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+#pragma unroll 1
+  for (int p = 0; p < 2; p++) {
+    u64 v[8], w[8];
+    load3(v, y + (size_t)(p * 2 + limb) * kN, t);
+    load3(w, x + (size_t)(p * 2 + limb) * kN, t);
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = submod(v[c], w[c], L.q);
+    ntt_inv(v, lds[limb], t, L);
+    stage_compose(v, lds, limb, t, S, xs);
+    ep_poly(xs, p, A.l, A.b, g, limb, t, L, lds[limb], acc0, acc1);
+    __syncthreads();  // xs is rewritten for the next polynomial
+  }
+  u64 x0[8], x1[8];
+  load3(x0, x + (size_t)(0 * 2 + limb) * kN, t);
+  load3(x1, x + (size_t)(1 * 2 + limb) * kN, t);
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    x0[c] = addmod(x0[c], acc0[c], L.q);
+    x1[c] = addmod(x1[c], acc1[c], L.q);
+  }
+  store3(dst + (size_t)(0 * 2 + limb) * kN, x0, t);
+  store3(dst + (size_t)(1 * 2 + limb) * kN, x1, t);
+}
+
+// INTT (when ntt_in), then the K = 2 mod switch per coefficient. grid (2 polys, B).
+__global__ __launch_bounds__(1024) void k2_final(const u64 *res, u64 *out, size_t res_stride, int ntt_in,
+                                                 Scheme64 S) {
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  const int p = blockIdx.x, qi = blockIdx.y;
+  const u64 *src = res + qi * res_stride + (size_t)(p * 2 + limb) * kN;
+  u64 v[8];
+  if (ntt_in) {
+    load3(v, src, t);
+    ntt_inv(v, lds[limb], t, L);
+  } else {
+    load0(v, src, t);
+  }
+  __syncthreads();
+  store0(lds[limb], v, t);
+  __syncthreads();
+  u64 *o = out + ((size_t)qi * 2 + p) * kN;
+  for (int j = threadIdx.x; j < kN; j += 1024) o[j] = final_coeff(lds[0][j], lds[1][j], S);
+}
+
+__global__ __launch_bounds__(1024) void k2_db_load(const u64 *pts, u64 *db, size_t num_pt, size_t first,
+                                                   Scheme64 S) {
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  const size_t i = blockIdx.x;
+  u64 v[8];
+  load0(v, pts + i * kN, t);  // < t < q: already reduced
+  ntt_fwd(v, lds[limb], t, L);
+  store3(db + ((size_t)limb * num_pt + first + i) * kN, v, t);
+}
+
+__global__ __launch_bounds__(1024) void k2_ext_prod(const u64 *gsw, int l, int b, const u64 *ct, u64 *out,
+                                                    Scheme64 S) {
+  __shared__ u128 xs[kN];  // recombined coefficients of the polynomial being decomposed
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+#pragma unroll 1
+  for (int p = 0; p < 2; p++) {
+    u64 v[8];
+    load0(v, ct + (size_t)(p * 2 + limb) * kN, t);
+    stage_compose(v, lds, limb, t, S, xs);
+    ep_poly(xs, p, l, b, gsw, limb, t, L, lds[limb], acc0, acc1);
+    __syncthreads();  // xs is rewritten for the next polynomial
+  }
+  store3(out + (size_t)(0 * 2 + limb) * kN, acc0, t);
+  store3(out + (size_t)(1 * 2 + limb) * kN, acc1, t);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+void launch_k2_ntt(u64 *data, size_t count, bool inverse, const Scheme64 &S, hipStream_t st) {
+  if (count) k2_ntt<<<(unsigned)count, 1024, 0, st>>>(data, inverse ? 1 : 0, S);
+}
+
+void launch_k2_expand_level(const u64 *in, u64 *out, size_t stride, int nnodes, int B, int level,
+                            const KeyPtrs64 &P, size_t key_level_values, const Gadget64 &G, const Scheme64 &S,
+                            hipStream_t st) {
+  if (nnodes <= 0 || B <= 0) return;
+  ExpandArgs64 A;
+  A.in = in; A.out = out; A.stride = stride; A.bv = P.bv; A.key_level_values = key_level_values;
+  A.level = level; A.l_ks = G.l_ks; A.b_ks = G.b_ks;
+  const unsigned gk = (unsigned)(kN >> level) + 1;
+  unsigned inv = 1;  // gk^-1 mod 2N = gk^(N-1): the unit group mod 2N = 2^13 has order N
+  for (int e = 0; e < kN - 1; e++) inv = (inv * gk) & (2 * kN - 1);
+  A.kinv = (int)inv;
+  k2_expand_level<<<dim3(nnodes, B), 1024, 0, st>>>(A, S);
+}
+
+void launch_k2_complete(const u64 *rows, u64 *sel, size_t row_stride, size_t sel_stride, int nrows, int B,
+                        const KeyPtrs64 &P, const Gadget64 &G, const Scheme64 &S, hipStream_t st) {
+  if (nrows <= 0 || B <= 0) return;
+  CompleteArgs64 A;
+  A.rows = rows; A.sel = sel; A.row_stride = row_stride; A.sel_stride = sel_stride; A.gsw = P.gsw;
+  A.l_ep = G.l_ep; A.l_key = G.l_key; A.b_key = G.b_key;
+  k2_complete<<<dim3(nrows, B), 1024, 0, st>>>(A, S);
+}
+
+void launch_k2_ct_ntt(const u64 *in, u64 *out, size_t in_stride, size_t out_stride, int count, int B,
+                      const Scheme64 &S, hipStream_t st) {
+  if (count <= 0 || B <= 0) return;
+  k2_ct_ntt<<<dim3(2 * count, B), 1024, 0, st>>>(in, out, in_stride, out_stride, S);
+}
+
+void launch_k2_scan(const u64 *db, const u64 *qntt, u64 *mid, size_t q_stride, size_t mid_stride, size_t N0,
+                    size_t Nrest, int B, const Scheme64 &S, hipStream_t st) {
+  const int ncols = 2 * B, nct = (ncols + kCols - 1) / kCols;
+  const size_t W = (size_t)nct * (kN / 256) * Nrest * 2;
+  k2_scan<<<(unsigned)W, 256, 0, st>>>(db, qntt, mid, q_stride, mid_stride, (int)N0, (int)Nrest, ncols, nct, S);
+}
+
+void launch_k2_mux(const u64 *in, u64 *out, size_t stride, int nslots, int offset, int half, int B, const u64 *sel,
+                   size_t sel_stride, const Gadget64 &G, const Scheme64 &S, hipStream_t st) {
+  if (nslots <= 0 || B <= 0) return;
+  MuxArgs64 A;
+  A.in = in; A.out = out; A.stride = stride; A.sel = sel; A.sel_stride = sel_stride;
+  A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
+  k2_mux<<<dim3(nslots, B), 1024, 0, st>>>(A, S);
+}
+
+void launch_k2_final(const u64 *res, u64 *out, size_t res_stride, int B, const Scheme64 &S, hipStream_t st) {
+  if (B > 0) k2_final<<<dim3(2, B), 1024, 0, st>>>(res, out, res_stride, 1, S);
+}
+
+void launch_k2_mod_switch(const u64 *ct, u64 *out, const Scheme64 &S, hipStream_t st) {
+  k2_final<<<dim3(2, 1), 1024, 0, st>>>(ct, out, 0, 0, S);
+}
+
+void launch_k2_db_load(const u64 *pts, u64 *db, size_t num_pt, size_t first, size_t count, const Scheme64 &S,
+                       hipStream_t st) {
+  if (count) k2_db_load<<<(unsigned)count, 1024, 0, st>>>(pts, db, num_pt, first, S);
+}
+
+void launch_k2_ext_prod(const u64 *gsw, int l, int b, const u64 *ct, u64 *out, const Scheme64 &S, hipStream_t st) {
+  k2_ext_prod<<<1, 1024, 0, st>>>(gsw, l, b, ct, out, S);
+}
+
+}  // namespace k2
+}  // namespace opir

@@ -20,7 +20,7 @@ OPIR_ENOMEM = -5
 # Every symbol include/onionpir_gpu.h declares.
 EXPORTED = [
     "opir_ctx_create", "opir_ctx_destroy", "opir_last_error", "opir_ctx_set_stream", "opir_ctx_synchronize",
-    "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt",
+    "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt", "opir_db_load_plain_u64",
     "opir_client_set_keys", "opir_client_remove",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes",
@@ -50,6 +50,7 @@ class OpirScheme(ctypes.Structure):
         ("l_ep", ctypes.c_uint32), ("base_log2", ctypes.c_uint32),
         ("l_key", ctypes.c_uint32), ("base_log2_key", ctypes.c_uint32),
         ("l_ks", ctypes.c_uint32), ("bv_base_log2", ctypes.c_uint32),
+        ("ntt_root2", ctypes.c_uint64),
     ]
 
 
@@ -76,6 +77,7 @@ def load_library():
         "opir_db_destroy": ([P], None),
         "opir_db_load_plain": ([P, P, sz, sz], ctypes.c_int),
         "opir_db_load_ntt": ([P, P, P], ctypes.c_int),
+        "opir_db_load_plain_u64": ([P, P, sz, sz], ctypes.c_int),
         "opir_client_set_keys": ([P, ctypes.c_uint64, P, sz, P], ctypes.c_int),
         "opir_client_remove": ([P, ctypes.c_uint64], ctypes.c_int),
         "opir_answer": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
@@ -123,6 +125,8 @@ class GpuContext:
         s = OpirScheme(**scheme)
         self.scheme = dict(scheme)
         self.N = scheme["poly_degree"]
+        self.K = 1 if scheme.get("composite", 1) else 2  # RNS limbs per polynomial at the boundary
+        self.KN = self.K * self.N
         h = ctypes.c_void_p()
         rc = self.lib.opir_ctx_create(ctypes.byref(s), device, ctypes.byref(h))
         if rc != OPIR_OK:
@@ -177,18 +181,18 @@ class GpuContext:
 
     def ntt_fwd(self, polys):
         a = _u64(polys).copy()
-        self.check(self.lib.opir_ntt_fwd(self.h, _ptr(a), a.size // self.N))
+        self.check(self.lib.opir_ntt_fwd(self.h, _ptr(a), a.size // self.KN))
         return a
 
     def ntt_inv(self, polys):
         a = _u64(polys).copy()
-        self.check(self.lib.opir_ntt_inv(self.h, _ptr(a), a.size // self.N))
+        self.check(self.lib.opir_ntt_inv(self.h, _ptr(a), a.size // self.KN))
         return a
 
     def ext_prod(self, gsw, l, base_log2, ct):
         g = _u64(gsw)
         c = _u64(ct)
-        out = np.zeros((2, self.N), dtype=np.uint64)
+        out = np.zeros((2, self.KN), dtype=np.uint64)
         self.check(self.lib.opir_ext_prod(self.h, _ptr(g), l, base_log2, _ptr(c), _ptr(out)))
         return out
 
@@ -206,6 +210,7 @@ class GpuDatabase:
         self.ctx = ctx
         self.lib = ctx.lib
         self.N = ctx.N
+        self.KN = ctx.KN
         self.N0, self.Nrest, self.dims, self.h = fst_dim_sz, other_dim_sz, num_dims, expansion_height
         self.u = fst_dim_sz + ctx.scheme["l_ep"] * (num_dims - 1)
         h = ctypes.c_void_p()
@@ -225,8 +230,12 @@ class GpuDatabase:
             pass
 
     def load_plain(self, pts, first_pt=0):
-        a = np.ascontiguousarray(pts, dtype=np.uint16)
-        self.ctx.check(self.lib.opir_db_load_plain(self.h, _ptr(a), first_pt, a.size // self.N))
+        if self.ctx.scheme["plain_mod"] <= 1 << 16:
+            a = np.ascontiguousarray(pts, dtype=np.uint16)
+            self.ctx.check(self.lib.opir_db_load_plain(self.h, _ptr(a), first_pt, a.size // self.N))
+        else:
+            a = _u64(pts)
+            self.ctx.check(self.lib.opir_db_load_plain_u64(self.h, _ptr(a), first_pt, a.size // self.N))
 
     def load_ntt(self, lo, hi):
         lo = np.ascontiguousarray(lo, dtype=np.uint32)
@@ -254,27 +263,27 @@ class GpuDatabase:
 
     def expand(self, client_id, query):
         q = _u64(query)
-        out = np.zeros((self.u, 2, self.N), dtype=np.uint64)
+        out = np.zeros((self.u, 2, self.KN), dtype=np.uint64)
         self.ctx.check(self.lib.opir_expand(self.h, client_id, _ptr(q), _ptr(out)))
         return out
 
     def complete(self, client_id, expanded):
         e = _u64(expanded)
         ns = self.dims - 1
-        out = np.zeros((ns, 2 * self.ctx.scheme["l_ep"], 2 * self.N), dtype=np.uint64)
+        out = np.zeros((ns, 2 * self.ctx.scheme["l_ep"], 2 * self.KN), dtype=np.uint64)
         self.ctx.check(self.lib.opir_complete(self.h, client_id, _ptr(e), _ptr(out)))
         return out
 
     def first_dim(self, query_cts):
         q = _u64(query_cts)
-        out = np.zeros((self.Nrest, 2, self.N), dtype=np.uint64)
+        out = np.zeros((self.Nrest, 2, self.KN), dtype=np.uint64)
         self.ctx.check(self.lib.opir_first_dim(self.h, _ptr(q), _ptr(out)))
         return out
 
     def other_dim(self, mid, sel):
         m = _u64(mid)
         s = _u64(sel) if sel is not None else np.zeros(1, dtype=np.uint64)
-        out = np.zeros((2, self.N), dtype=np.uint64)
+        out = np.zeros((2, self.KN), dtype=np.uint64)
         self.ctx.check(self.lib.opir_other_dim(self.h, _ptr(m), _ptr(s), _ptr(out)))
         return out
 

@@ -33,3 +33,30 @@ def test_ctx_create_rejects_bad_scheme_without_gpu():
     assert lib.opir_ctx_create(ctypes.byref(s), 0, ctypes.byref(h)) == gpu.OPIR_EINVAL
     assert b"composite" in lib.opir_last_error(None)
     assert lib.opir_answer(None, 0, None, None) == gpu.OPIR_EINVAL
+
+
+def test_k2_scheme_validation_without_gpu(O):
+    """The two-prime scheme of CONFIG_N4096_K2_MP passes host-side validation
+    (it then needs a device); malformed variants are rejected first."""
+    from onionpirv2_amd import gpu
+    lib = gpu.load_library()
+    p = O.PirParams("CONFIG_N4096_K2_MP")
+    q0, q1 = p.rns_mods
+    good = dict(poly_degree=p.N, composite=0, q1=q0, q2=q1, ntt_root=O.ntt_root(p.N, q0),
+                ntt_root2=O.ntt_root(p.N, q1), plain_mod=p.plain_mod, small_q=p.small_q, l_ep=p.l_ep,
+                base_log2=p.base_log2, l_key=p.l_key, base_log2_key=p.base_log2_key, l_ks=p.l_ks,
+                bv_base_log2=p.bv_base_log2)
+
+    def create(**over):
+        s = gpu.OpirScheme(**{**good, **over})
+        h = ctypes.c_void_p()
+        rc = lib.opir_ctx_create(ctypes.byref(s), 0, ctypes.byref(h))
+        if rc == gpu.OPIR_OK:
+            lib.opir_ctx_destroy(h)
+        return rc
+
+    assert create() in (gpu.OPIR_OK, gpu.OPIR_EHIP)       # valid: fails only for want of a device
+    assert create(ntt_root2=2) == gpu.OPIR_EINVAL          # not a primitive 2N-th root
+    assert create(bv_base_log2=20) == gpu.OPIR_EINVAL      # last signed digit shift beyond 126 bits
+    assert create(small_q=q0 + 1) == gpu.OPIR_EINVAL
+    assert create(poly_degree=2048) == gpu.OPIR_EINVAL

@@ -1,0 +1,182 @@
+"""GPU parity of the two-prime path (CONFIG_N4096_K2_MP: N = 4096, two ~60-bit
+RNS limbs) against the CPU oracle, bit-exact, through the C ABI.
+
+Layouts (with_layout, K = 2 gadgets l_ep = 5):
+  (43, 5)        N0=16,  dims=3, Nrest=3   ragged deepest level (offset 1)
+  (700, 8)       N0=128, dims=4, Nrest=6   ragged with offset 2
+  (1024, 8)      N0=128, dims=4, Nrest=8   perfect tree
+  (43, 5, False) N0=27,  dims=2, Nrest=2   non-power-of-two first dimension
+  (1, 0)         singleton: no expansion, no selectors
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = [(43, 5, True), (700, 8, True), (1024, 8, True), (43, 5, False), (1, 0, True)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def k2p(O):
+    return O.PirParams("CONFIG_N4096_K2_MP")
+
+
+def scheme_k2(O, p):
+    q0, q1 = p.rns_mods
+    return dict(poly_degree=p.N, composite=0, q1=q0, q2=q1, ntt_root=O.ntt_root(p.N, q0), ntt_root2=O.ntt_root(p.N, q1),
+                plain_mod=p.plain_mod, small_q=p.small_q, l_ep=p.l_ep, base_log2=p.base_log2, l_key=p.l_key,
+                base_log2_key=p.base_log2_key, l_ks=p.l_ks, bv_base_log2=p.bv_base_log2)
+
+
+@pytest.fixture(scope="module")
+def ctx2(O, k2p):
+    from onionpirv2_amd.gpu import GpuContext
+    say("[gpu] creating K=2 context")
+    ctx = GpuContext(scheme_k2(O, k2p))
+    yield ctx
+    ctx.close()
+
+
+def rand_rns(rng, p, shape):
+    """Random polynomials [..., 2 limbs, N], each limb reduced mod its prime."""
+    out = np.empty(shape + (2, p.N), dtype=np.uint64)
+    for m, q in enumerate(p.rns_mods):
+        out[..., m, :] = rng.integers(0, q, size=shape + (p.N,), dtype=np.uint64)
+    return out
+
+
+class Setup:
+    def __init__(self, O, base, layout, seed, ctx, cid=11):
+        from onionpirv2_amd.gpu import GpuDatabase
+        self.p = p = base.with_layout(*layout)
+        self.cid = cid
+        self.client = O.PirClient(p, cid, seed)
+        self.bv = self.client.create_bv_galois_keys()
+        self.gsw = self.client.generate_gsw_from_key()
+        self.server = O.PirServer(p)
+        logical = p.target_num_pt
+        self.indices = sorted({0, logical - 1, (seed * 7919) % logical})
+        self.server.load_data(logical, "random", seed, self.indices)
+        self.server.set_client_keys(cid, self.bv, self.gsw)
+        self.db = GpuDatabase(ctx, p.fst_dim_sz, p.other_dim_sz, p.num_dims, p.expansion_height)
+        draws = O.mt19937_64_draws(seed, logical * p.N)
+        self.db.load_plain((draws % np.uint64(p.plain_mod)).reshape(logical, p.N))
+        ctx.set_client_keys(cid, self.bv, self.gsw)
+
+
+@pytest.mark.parametrize("nq", [1, 3])
+def test_k2_ntt_matches_oracle(O, k2p, ctx2, nq):
+    say(f"[gpu] K=2 ntt nq={nq}")
+    rng = np.random.default_rng(50 + nq)
+    polys = rand_rns(rng, k2p, (nq,))
+    exp = polys.copy()
+    for i in range(nq):
+        for m, q in enumerate(k2p.rns_mods):
+            exp[i, m] = O.ntt_fwd(polys[i, m], k2p.N, q)
+    got = ctx2.ntt_fwd(polys.reshape(nq, -1)).reshape(exp.shape)
+    assert np.array_equal(got, exp)
+    assert np.array_equal(ctx2.ntt_inv(got.reshape(nq, -1)).reshape(exp.shape), polys)
+
+
+def test_k2_ntt_kat_product(k2p, ctx2):
+    """tests/test_hexl_ntt.cpp:41-59 on both limbs: (1+2x+3x^2)(4+5x)."""
+    N = k2p.N
+    a = np.zeros((2, N), np.uint64); a[:, :3] = [1, 2, 3]
+    b = np.zeros((2, N), np.uint64); b[:, :2] = [4, 5]
+    fa = ctx2.ntt_fwd(a.reshape(1, -1)).reshape(2, N)
+    fb = ctx2.ntt_fwd(b.reshape(1, -1)).reshape(2, N)
+    prod = np.array([[(int(x) * int(y)) % q for x, y in zip(fa[m], fb[m])] for m, q in enumerate(k2p.rns_mods)],
+                    dtype=np.uint64)
+    c = ctx2.ntt_inv(prod.reshape(1, -1)).reshape(2, N)
+    for m in range(2):
+        assert list(c[m, :5]) == [4, 13, 22, 15, 0]
+
+
+@pytest.mark.parametrize("which", ["key", "data"])
+def test_k2_external_product_matches_oracle(O, k2p, ctx2, which):
+    say(f"[gpu] K=2 external product ({which})")
+    p = k2p
+    l, b = (p.l_key, p.base_log2_key) if which == "key" else (p.l_ep, p.base_log2)
+    sk, _ = O.gen_sk_and_encrypt(p, np.zeros(p.N, np.uint64), 5)
+    one = np.zeros(p.N, np.uint64); one[0] = 1
+    gsw = O.plain_to_gsw(p, one, sk, l, b, 9)
+    ct = rand_rns(np.random.default_rng(7), p, (2,)).reshape(2, -1)
+    exp = O.external_product(p, gsw, ct, l, b)
+    assert np.array_equal(ctx2.ext_prod(gsw, l, b, ct), exp)
+
+
+def test_k2_mod_switch_matches_oracle(O, k2p, ctx2):
+    say("[gpu] K=2 mod switch")
+    p = k2p
+    ct = rand_rns(np.random.default_rng(3), p, (2,))
+    q0, q1 = p.rns_mods
+    ct[0, 0, :4] = [0, 1, q0 // 2, q0 - 1]
+    ct[0, 1, :4] = [0, 1, q1 // 2, q1 - 1]
+    ct = ct.reshape(2, -1)
+    srv = O.PirServer(p.with_layout(1, 0))
+    assert np.array_equal(ctx2.mod_switch(ct), srv.mod_switch(ct))
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: "x".join(map(str, l)))
+def test_k2_stages_match_oracle(O, k2p, ctx2, layout):
+    say(f"[gpu] K=2 stages layout={layout}")
+    s = Setup(O, k2p, layout, seed=3000 + layout[0], ctx=ctx2)
+    p = s.p
+    query = s.client.fast_generate_query(p, s.indices[-1])
+    say("  expand")
+    exp_o = s.server.fast_expand_qry(s.cid, query)
+    exp_g = s.db.expand(s.cid, query)
+    bad = [j for j in range(len(exp_o)) if not np.array_equal(exp_g[j], exp_o[j])]
+    assert not bad, f"leaves differing from the oracle: {bad}"
+    sel_o = None
+    if p.num_dims > 1:
+        say("  complete")
+        sel_o = s.server.complete_selectors(s.cid, exp_o)
+        assert np.array_equal(s.db.complete(s.cid, exp_o), sel_o)
+    say("  first dim")
+    fq = exp_o[: p.fst_dim_sz]
+    mid_o = s.server.evaluate_first_dim(fq)
+    assert np.array_equal(s.db.first_dim(fq), mid_o)
+    if p.num_dims > 1:
+        say("  other dim")
+        res_o = s.server.evaluate_other_dim(mid_o, sel_o)
+        assert np.array_equal(s.db.other_dim(mid_o, sel_o), res_o)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: "x".join(map(str, l)))
+def test_k2_answer_bit_exact_and_decrypts(O, k2p, ctx2, layout):
+    say(f"[gpu] K=2 answer layout={layout}")
+    s = Setup(O, k2p, layout, seed=4000 + layout[0], ctx=ctx2)
+    p = s.p
+    for idx in s.indices:
+        query = s.client.fast_generate_query(p, idx)
+        resp_o = s.server.make_query(s.cid, query)
+        resp_g = s.db.answer(s.cid, query)
+        assert np.array_equal(resp_g, resp_o), f"index {idx}"
+        wire = s.server.save_resp_to_stream(resp_g)
+        assert len(wire) == (2 * p.N * 50 + 7) // 8
+        pt, budget = s.client.decrypt_mod_q(s.client.load_resp_from_stream(wire))
+        assert np.array_equal(pt, s.server.direct_get_original_plaintext(idx)) and budget > 0
+
+
+def test_k2_batch_equals_single(O, k2p, ctx2):
+    say("[gpu] K=2 batch vs single")
+    s = Setup(O, k2p, (1024, 8, True), seed=88, ctx=ctx2, cid=31)
+    other = O.PirClient(s.p, 32, 89)
+    ctx2.set_client_keys(32, other.create_bv_galois_keys(), other.generate_gsw_from_key())
+    qs, ids = [], []
+    for j, idx in enumerate([5, 1023, 600, 5]):
+        c = s.client if j % 2 == 0 else other
+        qs.append(c.fast_generate_query(s.p, idx))
+        ids.append(c.client_id)
+    batch = s.db.answer_batch(ids, np.stack(qs))
+    for j in range(len(qs)):
+        assert np.array_equal(batch[j], s.db.answer(ids[j], qs[j]))
+    assert np.array_equal(batch[0], s.server.make_query(31, qs[0]))

@@ -1,0 +1,160 @@
+"""Pins for the oracle's two-prime path (CONFIG_N4096_K2_MP), CPU only.
+
+The reference's own tests are compiled for one config at a time and hold no
+fixtures for this one, so each piece is pinned by what the mathematics fixes:
+closed-form NTT evaluation, CRT recomposition of the multi-precision gadget
+digits, RGSW selection, BV keyswitch = automorphism, the mod-switch round trip
+and end-to-end decryption (tests/test_pir.cpp's contract).
+
+Parity unpinned, as for K = 1: the NTT evaluation order (HEXL's bit-reversed
+convention) and the choice of minimal primitive root; both only matter for
+NTT-form data crossing the API.
+"""
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def p2(O):
+    return O.PirParams("CONFIG_N4096_K2_MP")
+
+
+def brv(x, bits):
+    return int(format(x, f"0{bits}b")[::-1], 2)
+
+
+def crt(r0, r1, q0, q1):
+    return (r0 + q0 * (((r1 - r0) * pow(q0, -1, q1)) % q1)) % (q0 * q1)
+
+
+def test_k2_params(O, p2):
+    """database_constants.h:111-124; pir.cpp:14-98."""
+    q0, q1 = p2.rns_mods
+    assert p2.N == 4096 and q0 != q1
+    for q in (q0, q1):
+        assert O.is_prime(q) and q.bit_length() == 60 and (q - 1) % (2 * p2.N) == 0
+    assert (p2.l_ep, p2.l_key, p2.l_ks) == (5, 8, 8)
+    assert p2.small_q.bit_length() == 50 and O.is_prime(p2.small_q)
+    assert p2.plain_mod.bit_length() == 40
+    assert p2.ct_mod_width == 120
+    assert p2.l_ep * p2.base_log2 >= 120 and p2.l_key * p2.base_log2_key >= 120
+    assert p2.q0_inv_mod_q1 == pow(q0, -1, q1)
+
+
+def test_k2_ntt_brute_force(O, p2):
+    N = p2.N
+    rng = np.random.default_rng(1)
+    for q in p2.rns_mods:
+        psi = O.ntt_root(N, q)
+        assert pow(psi, N, q) == q - 1
+        a = rng.integers(0, q, size=N, dtype=np.uint64)
+        got = O.ntt_fwd(a, N, q)
+        ai = [int(x) for x in a]
+        for j in [0, 1, 1000, N - 1]:
+            e = (2 * brv(j, 12) + 1)
+            w = pow(psi, e, q)
+            acc, wp = 0, 1
+            for x in ai:
+                acc = (acc + x * wp) % q
+                wp = wp * w % q
+            assert int(got[j]) == acc
+        assert np.array_equal(O.ntt_inv(got, N, q), a)
+
+
+def test_k2_ntt_kat_product(O, p2):
+    N = p2.N
+    for q in p2.rns_mods:
+        a = np.zeros(N, np.uint64); a[:3] = [1, 2, 3]
+        b = np.zeros(N, np.uint64); b[:2] = [4, 5]
+        fa, fb = O.ntt_fwd(a, N, q), O.ntt_fwd(b, N, q)
+        prod = np.array([(int(x) * int(y)) % q for x, y in zip(fa, fb)], dtype=np.uint64)
+        assert list(O.ntt_inv(prod, N, q)[:5]) == [4, 13, 22, 15, 0]
+
+
+@pytest.mark.parametrize("which", ["data", "key"])
+def test_k2_mp_digits_recompose(O, p2, which):
+    """gsw.cpp:22-85,184-250: unsigned base-2^b digits of the CRT value, row
+    poly*l + (l-1-i) holds digit i, the same small integer in both limbs."""
+    l, b = (p2.l_ep, p2.base_log2) if which == "data" else (p2.l_key, p2.base_log2_key)
+    q0, q1 = p2.rns_mods
+    N = p2.N
+    rng = np.random.default_rng(9)
+    ct = np.empty((2, 2 * N), np.uint64)
+    for m, q in enumerate((q0, q1)):
+        ct[:, m * N:(m + 1) * N] = rng.integers(0, q, size=(2, N), dtype=np.uint64)
+    ct[0, 0], ct[0, N] = q0 - 1, q1 - 1            # X = Q - 1
+    rows = O.decompose_rows(p2, ct, l, b)
+    for poly in range(2):
+        for k in [0, 1, 2049, N - 1]:
+            X = crt(int(ct[poly, k]), int(ct[poly, N + k]), q0, q1)
+            digits = [int(rows[poly * l + (l - 1 - i)][k]) for i in range(l)]
+            assert all(d < (1 << b) for d in digits)
+            assert [int(rows[poly * l + (l - 1 - i)][N + k]) for i in range(l)] == digits
+            assert sum(d << (b * i) for i, d in enumerate(digits)) == X
+
+
+def test_k2_external_product_selects(O, p2):
+    N, t = p2.N, p2.plain_mod
+    msg = np.zeros(N, np.uint64); msg[:3] = [t // 2 + 1, 12345, t - 1]
+    sk, ct = O.gen_sk_and_encrypt(p2, msg, 11)
+    for m, expect in [(1, msg), (0, np.zeros(N, np.uint64))]:
+        pt = np.zeros(N, np.uint64); pt[0] = m
+        g = O.plain_to_gsw(p2, pt, sk, p2.l_ep, p2.base_log2, 12 + m)
+        out = O.external_product(p2, g, ct, p2.l_ep, p2.base_log2)  # NTT form
+        # decrypt_rns is the RNS decryption (rlwe.cpp); decrypt_and_budget is single-modulus only
+        assert np.array_equal(O.decrypt_rns(p2, sk, out, True), expect)
+
+
+def test_k2_bv_keyswitch_is_automorphism(O, p2):
+    """bv_keyswitch.cpp:373-481 (signed MP digits, tie (-B/2, B/2])."""
+    N, t = p2.N, p2.plain_mod
+    msg = np.zeros(N, np.uint64)
+    msg[[0, 1, 2, 10, 4000]] = [1, 2, 3, t - 5, 77]
+    sk, ct = O.gen_sk_and_encrypt(p2, msg, 21)
+    for k in (N + 1, N // 4 + 1):
+        key = O.gen_bv_ks_key(p2, sk, k, 22)
+        out = O.bv_apply(p2, ct, k, key)
+        assert np.array_equal(O.decrypt_rns(p2, sk, out, False), O.automorphism_coeff(msg, k, t))
+
+
+def test_k2_mod_switch_round_trip(O, p2):
+    p = p2.with_layout(43, 5)
+    c = O.PirClient(p, 1, 31)
+    msg = np.zeros(p.N, np.uint64)
+    msg[:10] = (np.arange(10, dtype=np.uint64) * np.uint64(987654321)) % np.uint64(p.plain_mod)
+    ct = c.encrypt_bfv(msg, 32)
+    ms = O.PirServer(p).mod_switch(ct)
+    assert ms.shape == (2, p.N) and int(ms.max()) < p.small_q
+    pt, budget = c.decrypt_mod_q(ms)
+    assert np.array_equal(pt, msg) and budget > 0
+
+
+def test_k2_level_mat_mat_u64(O, p2):
+    """matrix.cpp:24-151 with 60-bit moduli: exact u128 sums, seeded inputs."""
+    q = p2.rns_mods[0]
+    m, n, lv = 3, 512, 2
+    r = O.mt19937_64_draws(0xC0FFEE, m * n * lv + n * 2 * lv)
+    A = r[: m * n * lv] % np.uint64(q)
+    B = r[m * n * lv:] % np.uint64(q)
+    got = O.level_mat_mat_u64(A, B, m, n, lv, q)
+    for level in range(lv):
+        for i in range(m):
+            for c in range(2):
+                want = sum(int(A[level * m * n + i * n + k]) * int(B[level * n * 2 + 2 * k + c]) for k in range(n)) % q
+                assert int(got[level * m * 2 + 2 * i + c]) == want
+
+
+@pytest.mark.parametrize("layout", [(43, 5, True), (700, 8, True)])
+def test_k2_end_to_end_decrypts(O, p2, layout):
+    p = p2.with_layout(*layout)
+    c = O.PirClient(p, 9, 5678 + layout[0])
+    srv = O.PirServer(p)
+    idx = p.target_num_pt - 1
+    srv.load_data(p.target_num_pt, "random", 17, [idx, 0])
+    srv.set_client_keys(9, c.create_bv_galois_keys(), c.generate_gsw_from_key())
+    for i in [idx, 0]:
+        resp = srv.make_query(9, c.fast_generate_query(p, i))
+        wire = srv.save_resp_to_stream(resp)
+        assert len(wire) == (2 * p.N * 50 + 7) // 8
+        pt, budget = c.decrypt_mod_q(c.load_resp_from_stream(wire))
+        assert np.array_equal(pt, srv.direct_get_original_plaintext(i)) and budget >= 1
