@@ -48,6 +48,11 @@ def parse():
     ap.add_argument("--serve-clients", type=int, default=512)
     ap.add_argument("--serve-budget-gib", type=float, default=48.0,
                     help="HBM budget for client keys and batch workspaces of the serving loop")
+    ap.add_argument("--no-k2", action="store_true", help="skip the CONFIG_N4096_K2_MP leg")
+    ap.add_argument("--k2-target", type=int, default=2**21, help="n=4096 layout: target plaintexts")
+    ap.add_argument("--k2-height", type=int, default=10)
+    ap.add_argument("--k2-batch", type=int, default=32)
+    ap.add_argument("--k2-steps", type=int, default=5)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="run a plain device copy of this many GiB first (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against known bytes)")
@@ -151,6 +156,104 @@ def serving(args, H, params, server, nkeyed, log):
         loop.stop()
     return {"databases": ["256x1024 (4 GiB NTT form)", "256x256 (1 GiB NTT form)"], "clients": nc,
             "hbm_budget_gib": args.serve_budget_gib, "batch_limits": limits, "rates": rows}
+
+
+def k2_leg(args, H, G, torch, log):
+    """CONFIG_N4096_K2_MP (two ~60-bit RNS limbs, N = 4096) on the layout
+    with_layout({2^21, 10}) = 512 x 4096 plaintexts, 128 GiB in NTT form,
+    generated in HBM from a seed. Plain single-query path and one fused batch."""
+    import ctypes
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    p = H.PirParams("CONFIG_N4096_K2_MP").with_layout(args.k2_target, args.k2_height, True)
+    N, B = p.N, args.k2_batch
+    log(f"[k2] layout N0={p.fst_dim_sz} Nrest={p.other_dim_sz} dims={p.num_dims} num_pt={p.num_pt} "
+        f"({p.num_pt * N * 2 * 8 / 2**30:.0f} GiB NTT form)")
+    t0 = time.time()
+    server = H.PirServer(p)
+    idxs = distinct_indices(max(B, 2), p.num_pt, args.seed + 3)
+    server.gen_data_device(idxs[:2], args.seed + 3)
+    log(f"[k2] database generated on the device in {time.time() - t0:.1f}s")
+
+    def mk(j):
+        c = H.PirClient(p, 7000 + j, args.seed + 31 * j + 1)
+        bv, gsw = c.create_bv_galois_keys(), c.generate_gsw_from_key()
+        return c, bv, gsw, c.fast_generate_query(p, idxs[j])
+
+    t0 = time.time()
+    with ThreadPoolExecutor(8) as ex:
+        cl = list(ex.map(mk, range(B)))
+    for j, (c, bv, gsw, _) in enumerate(cl):
+        server.set_client_bv_galois_key(7000 + j, bv)
+        server.set_client_gsw_key(7000 + j, gsw)
+    queries = np.stack([x[3] for x in cl])
+    log(f"[k2] {B} clients keyed, queries built in {time.time() - t0:.1f}s")
+    for j in range(2):
+        r = server.make_query(7000 + j, queries[j].copy())
+        pt, budget = cl[j][0].decrypt_mod_q(r)
+        assert np.array_equal(pt, server.direct_get_original_plaintext(idxs[j])), "wrong n=4096 PIR answer"
+        log(f"[k2] query {j} decrypts to plaintext {idxs[j]}, noise budget {budget}")
+    db_h, ctx_h = server.db_handle(), server.ctx_handle()
+    stream = torch.cuda.current_stream()
+    assert G.ctx_call_raw("opir_ctx_set_stream", ctx_h, ctypes.c_void_p(stream.cuda_stream)) == 0
+    dev = torch.device("cuda:0")
+    d_q = torch.from_numpy(queries.view(np.int64)).to(dev)
+    d_r = torch.empty((B, 2, N), dtype=torch.int64, device=dev)
+    ids = np.arange(7000, 7000 + B, dtype=np.uint64)
+
+    def answer(b, off=0):
+        rc = G.answer_batch_device_raw(db_h, ids[off:off + b], d_q[off:off + b].data_ptr(), d_r[off:off + b].data_ptr())
+        if rc:
+            raise RuntimeError(G.last_error_raw(ctx_h))
+
+    def timed(b, steps):
+        answer(b)
+        torch.cuda.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        evs[0].record()
+        for i in range(steps):
+            answer(b, (i * b) % B if b < B else 0)
+            evs[i + 1].record()
+        torch.cuda.synchronize()
+        per = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+        return sum(per) / steps, per
+
+    ms_plain, per = timed(1, args.k2_steps)
+    G.ctx_call_raw("opir_ctx_set_profiling", ctx_h, 1)
+    answer(1)
+    out = (ctypes.c_float * 6)()
+    assert G.ctx_call_raw("opir_last_stage_ms", ctx_h, out) == 0
+    G.ctx_call_raw("opir_ctx_set_profiling", ctx_h, 0)
+    names = ["Expand", "Convert", "First dim", "Other dim", "Modulus switching"]
+    stages = {k: round(v, 3) for k, v in zip(names, out[:5])}
+    scan_ms = out[5]
+    scan_bytes = p.num_pt * N * 2 * 8 + p.fst_dim_sz * 2 * 2 * N * 8 + p.other_dim_sz * 2 * 2 * N * 8
+    gib = p.target_num_pt * p.pt_size / 2**30
+    log(f"[k2] plain {ms_plain:.2f} ms/query (p50 {pct(per, 50):.2f} p99 {pct(per, 99):.2f}), stages {stages}, "
+        f"scan {scan_ms:.2f} ms -> {scan_bytes / scan_ms / 1e6:.0f} GB/s")
+    res = {"workload": "OnionPIRv2 make_query, CONFIG_N4096_K2_MP, with_layout({2097152,10}) = 512 x 4096 "
+                       "plaintexts (39 GiB logical, 128 GiB NTT form), synthetic seeded data generated in HBM",
+           "fst_dim_sz": p.fst_dim_sz, "other_dim_sz": p.other_dim_sz, "num_dims": p.num_dims,
+           "db_gib_per_s_plain": round(gib / (ms_plain * 1e-3), 2), "ms_per_query_plain": round(ms_plain, 3),
+           "queries_per_s_plain": round(1e3 / ms_plain, 2),
+           "latency_ms_plain": {"p50": round(pct(per, 50), 3), "p99": round(pct(per, 99), 3)},
+           "stage_ms_plain": stages,
+           "scan": {"kernel": "k2_scan (VALU u64 x u64 -> u128)", "kernel_ms": round(scan_ms, 3),
+                    "algorithmic_bytes": scan_bytes, "achieved_gbs": round(scan_bytes / scan_ms / 1e6, 1),
+                    "frac_of_hbm_peak": round(scan_bytes / scan_ms / 1e6 / HBM_PEAK_GBS, 4)}}
+    if B > 1:
+        ms_b, per_b = timed(B, 2)
+        res["batched"] = {"batch": B, "ms_per_batch": round(ms_b, 2), "queries_per_s": round(B * 1e3 / ms_b, 2),
+                          "speedup_vs_plain_qps": round(B * ms_plain / ms_b, 3)}
+        log(f"[k2] batch {B}: {ms_b:.1f} ms -> {B * 1e3 / ms_b:.1f} q/s")
+        answer(B)
+        torch.cuda.synchronize()
+        rb = d_r.cpu().numpy().view(np.uint64)
+        for j in (0, B - 1):
+            assert np.array_equal(rb[j], server.make_query(7000 + j, queries[j].copy())), "n=4096 batch != plain"
+    del d_q, d_r, server
+    torch.cuda.synchronize()
+    return res
 
 
 def cpu_baseline(args, log):
@@ -353,6 +456,8 @@ def main():
     if not args.no_serve:
         torch.cuda.synchronize()
         result["serving"] = serving(args, H, params, server, nclients, log)
+    if not args.no_k2:
+        result["n4096_k2"] = k2_leg(args, H, G, torch, log)
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(args, log)

@@ -93,6 +93,12 @@ int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi);
 /* Same as opir_db_load_plain with 64-bit coefficients (< t), for plaintext
  * moduli beyond 16 bits (CONFIG_N4096_K2_MP: t ~ 2^39). */
 int opir_db_load_plain_u64(opir_db *db, const uint64_t *coeffs, size_t first_pt, size_t count);
+/* Replaces PirServer::gen_data (src/server.cpp:53-66) for large databases:
+ * fills every plaintext on the device, coefficient c of plaintext i =
+ * splitmix64(seed + i*N + c) mod t (splitmix64 as in Vigna's reference
+ * code), so no plaintext crosses PCIe. The host recomputes any recorded
+ * plaintext with the same formula. */
+int opir_db_gen_random(opir_db *db, uint64_t seed);
 
 /* Replaces PirServer::set_client_bv_galois_key + set_client_gsw_key /
  * set_client_session_keys (src/server.cpp:857-875). bv: [h][l_ks][2: a, b][N]

@@ -426,6 +426,30 @@ int opir_db_load_plain_u64(opir_db *db, const uint64_t *coeffs, size_t first_pt,
   return opir_db_load_plain(db, w.data(), first_pt, count);
 }
 
+int opir_db_gen_random(opir_db *db, uint64_t seed) {
+  if (!db) return OPIR_EINVAL;
+  opir_ctx *ctx = db->ctx;
+  const u64 t = ctx->sch.plain_mod;
+  const size_t N = ctx->N, chunk = ctx->k2 ? 4096 : 8192;
+  DevBuf pts, tmp;
+  DEV_ALLOC(ctx, pts, chunk * N * (ctx->k2 ? sizeof(u64) : sizeof(uint16_t)));
+  if (!ctx->k2) DEV_ALLOC(ctx, tmp, 2 * chunk * kN * sizeof(u32));
+  for (size_t off = 0; off < db->num_pt; off += chunk) {
+    const size_t n = std::min(chunk, db->num_pt - off);
+    if (ctx->k2) {
+      launch_gen_plain_u64((u64 *)pts.p, off, n, (int)N, seed, t, ctx->stream);
+      k2::launch_k2_db_load((const u64 *)pts.p, db->d_db64, db->num_pt, off, n, ctx->S64, ctx->stream);
+    } else {
+      launch_gen_plain_u16((uint16_t *)pts.p, off, n, (int)N, seed, t, ctx->stream);
+      launch_db_ntt_plain((const uint16_t *)pts.p, (u32 *)tmp.p, n, ctx->S, ctx->stream);
+      launch_db_scatter((const u32 *)tmp.p, db->d_db, off, n, db->geom, ctx->stream);
+    }
+    LAUNCH_CHECK(ctx);
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return OPIR_OK;
+}
+
 int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi) {
   if (!db || !lo || !hi) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;

@@ -76,6 +76,11 @@ size_t db_bytes(const DbGeom &g);
 QfragGeom make_qfrag_geom(size_t N0, int B);
 size_t qfrag_bytes(const QfragGeom &q);
 
+// Seeded synthetic plaintexts [count][n]: coefficient c of plaintext i is
+// splitmix64(seed + i*n + c) mod t.
+void launch_gen_plain_u16(uint16_t *out, size_t first_pt, size_t count, int n, u64 seed, u64 t, hipStream_t st);
+void launch_gen_plain_u64(u64 *out, size_t first_pt, size_t count, int n, u64 seed, u64 t, hipStream_t st);
+
 // DB preprocessing (NTT of plaintexts, then scatter into the scan layout).
 void launch_db_ntt_plain(const uint16_t *pts, u32 *tmp, size_t count, const SchemeConst &S, hipStream_t st);
 void launch_db_scatter(const u32 *tmp, int8_t *db, size_t first_pt, size_t count, const DbGeom &g, hipStream_t st);

@@ -147,6 +147,32 @@ __global__ __launch_bounds__(512) void k_db_ntt_plain(const uint16_t *__restrict
   store_map3(tmp + ((size_t)sb * gridDim.x + blockIdx.x) * kN, v, t);
 }
 
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Seeded synthetic plaintexts, generated where they are stored: coefficient
+// c of plaintext i is splitmix64(seed + i*N + c) mod t (the host recomputes
+// recorded plaintexts with the same formula).
+__device__ __forceinline__ u64 splitmix64_d(u64 x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+
+template <typename T>
+__global__ void k_gen_plain(T *__restrict__ out, u64 first_pt, size_t count, int n, u64 seed, u64 t) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count * (size_t)n) return;
+  out[i] = (T)(splitmix64_d(seed + first_pt * (u64)n + i) % t);
+}
+
+void launch_gen_plain_u16(uint16_t *out, size_t first_pt, size_t count, int n, u64 seed, u64 t, hipStream_t st) {
+  if (count) k_gen_plain<uint16_t><<<cdiv(count * n, 256), 256, 0, st>>>(out, first_pt, count, n, seed, t);
+}
+void launch_gen_plain_u64(u64 *out, size_t first_pt, size_t count, int n, u64 seed, u64 t, hipStream_t st) {
+  if (count) k_gen_plain<u64><<<cdiv(count * n, 256), 256, 0, st>>>(out, first_pt, count, n, seed, t);
+}
+
 // Forward NTT of both polys of `count` cts per query (evaluate_first_dim's
 // query transform, server.cpp:331-337). grid (count*2, B).
 __global__ __launch_bounds__(512) void k_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride,
@@ -249,7 +275,6 @@ __global__ __launch_bounds__(512) void k_ext_prod(const u32 *gsw, int l, int b, 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 void launch_split_u64(const u64 *in, u32 *out, size_t npoly, const SchemeConst &S, hipStream_t st) {
   k_split_u64<<<cdiv(npoly * kN, 256), 256, 0, st>>>(in, out, npoly, 0, 0, S);

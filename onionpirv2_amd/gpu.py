@@ -20,7 +20,7 @@ OPIR_ENOMEM = -5
 # Every symbol include/onionpir_gpu.h declares.
 EXPORTED = [
     "opir_ctx_create", "opir_ctx_destroy", "opir_last_error", "opir_ctx_set_stream", "opir_ctx_synchronize",
-    "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt", "opir_db_load_plain_u64",
+    "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt", "opir_db_load_plain_u64", "opir_db_gen_random",
     "opir_client_set_keys", "opir_client_remove",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes",
@@ -78,6 +78,7 @@ def load_library():
         "opir_db_load_plain": ([P, P, sz, sz], ctypes.c_int),
         "opir_db_load_ntt": ([P, P, P], ctypes.c_int),
         "opir_db_load_plain_u64": ([P, P, sz, sz], ctypes.c_int),
+        "opir_db_gen_random": ([P, ctypes.c_uint64], ctypes.c_int),
         "opir_client_set_keys": ([P, ctypes.c_uint64, P, sz, P], ctypes.c_int),
         "opir_client_remove": ([P, ctypes.c_uint64], ctypes.c_int),
         "opir_answer": ([P, ctypes.c_uint64, P, P], ctypes.c_int),

@@ -172,6 +172,8 @@ PYBIND11_MODULE(_opir_host, m) {
       .def(py::init<const PirParams &, std::shared_ptr<GpuScheme>>(), py::arg("params"), py::arg("scheme"))
       .def("gen_data", &PirServer::gen_data, py::arg("record_indices") = std::vector<size_t>{}, py::arg("seed") = 0,
            py::call_guard<py::gil_scoped_release>())
+      .def("gen_data_device", &PirServer::gen_data_device, py::arg("record_indices") = std::vector<size_t>{},
+           py::arg("seed") = 0, py::call_guard<py::gil_scoped_release>())
       .def("load_data",
            [](PirServer &s, size_t logical, py::function src, std::vector<size_t> rec) {
              const size_t N = s.get_params().get_poly_degree();
@@ -225,10 +227,11 @@ PYBIND11_MODULE(_opir_host, m) {
                py::gil_scoped_release nogil;
                r = s.make_queries(ids, v);
              }
-             Arr a({(py::ssize_t)n, (py::ssize_t)2, (py::ssize_t)N});
+             const size_t rn = n ? r[0].c0.size() : 0;  // responses hold one limb mod small_q
+             Arr a({(py::ssize_t)n, (py::ssize_t)2, (py::ssize_t)rn});
              for (size_t i = 0; i < n; i++) {
-               std::memcpy(a.mutable_data() + i * 2 * N, r[i].c0.data(), N * 8);
-               std::memcpy(a.mutable_data() + i * 2 * N + N, r[i].c1.data(), N * 8);
+               std::memcpy(a.mutable_data() + i * 2 * rn, r[i].c0.data(), rn * 8);
+               std::memcpy(a.mutable_data() + i * 2 * rn + rn, r[i].c1.data(), rn * 8);
              }
              return a;
            })
@@ -309,9 +312,12 @@ PYBIND11_MODULE(_opir_host, m) {
                run = run_poisson(l, pool, lambda, duration, seed);
              }
              const size_t n = run.results.size();
+             size_t RN = 0;  // response values per polynomial (one limb mod small_q)
+             for (const auto &x : run.results)
+               if (!x.resp.empty()) { RN = x.resp.size() / 2; break; }
              py::array_t<int64_t> idx((py::ssize_t)n), batch((py::ssize_t)n);
              py::array_t<double> lat((py::ssize_t)n), wait((py::ssize_t)n);
-             Arr resp({(py::ssize_t)n, (py::ssize_t)2, (py::ssize_t)N});
+             Arr resp({(py::ssize_t)n, (py::ssize_t)2, (py::ssize_t)RN});
              size_t errors = 0;
              for (size_t i = 0; i < n; i++) {
                const ServeResult &x = run.results[i];
@@ -319,10 +325,10 @@ PYBIND11_MODULE(_opir_host, m) {
                batch.mutable_data()[i] = (int64_t)x.batch;
                lat.mutable_data()[i] = (x.t_done - x.t_submit) * 1e3;
                wait.mutable_data()[i] = (x.t_start - x.t_submit) * 1e3;
-               if (x.resp.size() == 2 * N)
-                 std::memcpy(resp.mutable_data() + i * 2 * N, x.resp.data(), 2 * N * 8);
+               if (RN && x.resp.size() == 2 * RN)
+                 std::memcpy(resp.mutable_data() + i * 2 * RN, x.resp.data(), 2 * RN * 8);
                else {
-                 std::memset(resp.mutable_data() + i * 2 * N, 0, 2 * N * 8);
+                 if (RN) std::memset(resp.mutable_data() + i * 2 * RN, 0, 2 * RN * 8);
                  errors++;
                }
              }

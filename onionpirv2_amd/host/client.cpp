@@ -1,8 +1,10 @@
 // PirClient on the CPU: secret key, BV Galois keys, RGSW(s), packed query,
 // response decoding and small-q decryption.
 // Follows src/client.cpp:14-430, src/rlwe.cpp:174-276, src/gsw.cpp:359-422,
-// src/bv_keyswitch.cpp:166-276, src/utils.cpp:407-443. Single logical
-// modulus schemes (K = 1, composite or prime).
+// src/bv_keyswitch.cpp:166-276, src/utils.cpp:407-443, for one logical
+// modulus (K = 1: a prime, or the composite q1*q2 of CONFIG_N2048_K1_COMP)
+// and for K = 2 RNS limbs (CONFIG_N4096_K2_MP). Polynomials are limb-major,
+// K*N values, as in the reference.
 #include <bit>
 #include <cmath>
 #include <cstring>
@@ -35,54 +37,67 @@ void uniform_poly(u64 *out, size_t N, u64 q, std::mt19937_64 &rng) {
   }
 }
 
-// rlwe.cpp:196-215 / bv_keyswitch.cpp:166-185: rounded Gaussian mod q
-void gaussian_poly(u64 *out, size_t N, u64 q, double sigma, std::mt19937_64 &rng) {
+// rlwe.cpp:196-215 / bv_keyswitch.cpp:166-185: one rounded Gaussian per
+// coefficient, the same integer in every limb
+void gaussian_shared(u64 *out, size_t N, const std::vector<u64> &qs, double sigma, std::mt19937_64 &rng) {
   std::normal_distribution<double> dist(0.0, sigma);
-  for (size_t i = 0; i < N; i++) {
-    const int64_t e = std::llround(dist(rng));
-    const u64 mag = (u64)(e < 0 ? -e : e) % q;
-    out[i] = (e < 0 && mag) ? q - mag : mag;
-  }
+  std::vector<int64_t> e(N);
+  for (size_t i = 0; i < N; i++) e[i] = std::llround(dist(rng));
+  for (size_t k = 0; k < qs.size(); k++)
+    for (size_t i = 0; i < N; i++) {
+      const u64 mag = (u64)(e[i] < 0 ? -e[i] : e[i]) % qs[k];
+      out[k * N + i] = (e[i] < 0 && mag) ? qs[k] - mag : mag;
+    }
 }
 
-std::shared_ptr<RingMod> ring_for(const PirParams &p) {
-  if (p.K() != 1) throw std::invalid_argument("host PirClient supports single-modulus schemes");
-  const u64 Q = p.get_rns_mods()[0];
-  if (p.q1()) return std::make_shared<RingMod>(p.get_poly_degree(), Q, p.ntt_root(), p.q1(), p.q2());
-  return std::make_shared<RingMod>(p.get_poly_degree(), Q, utils::minimal_primitive_root(2 * p.get_poly_degree(), Q));
+std::vector<std::shared_ptr<RingMod>> rings_for(const PirParams &p) {
+  const size_t N = p.get_poly_degree();
+  std::vector<std::shared_ptr<RingMod>> r;
+  if (p.q1()) {  // composite Q = q1*q2: one ring over Q, run as two CRT limbs
+    r.push_back(std::make_shared<RingMod>(N, p.get_rns_mods()[0], p.ntt_root(), p.q1(), p.q2()));
+    return r;
+  }
+  for (u64 q : p.get_rns_mods()) r.push_back(std::make_shared<RingMod>(N, q, utils::minimal_primitive_root(2 * N, q)));
+  return r;
 }
 
 }  // namespace
 
 PirClient::PirClient(const PirParams &params, size_t client_id, u64 seed)
-    : client_id_(client_id), params_(params), rng_(seed), seed_(seed), ring_(ring_for(params)) {
-  // rlwe.cpp:177-193,219-228: ternary secret {0, 1, -1} from uniform_int(0, 2), NTT form
-  const size_t N = params.get_poly_degree();
-  const u64 Q = params.get_rns_mods()[0];
+    : client_id_(client_id), params_(params), rng_(seed), seed_(seed), qs_(params.get_rns_mods()),
+      rings_(rings_for(params)) {
+  // rlwe.cpp:177-193,219-228: ternary secret {0, 1, -1} from uniform_int(0, 2),
+  // the same integers in every limb, NTT form per limb
+  const size_t N = params.get_poly_degree(), K = qs_.size();
   std::uniform_int_distribution<int> dist(0, 2);
-  sk_.data.resize(N);
-  for (size_t i = 0; i < N; i++) {
-    const int v = dist(rng_);
-    sk_.data[i] = v == 0 ? 0 : (v == 1 ? 1 : Q - 1);
+  std::vector<int> tern(N);
+  for (size_t i = 0; i < N; i++) tern[i] = dist(rng_);
+  sk_.data.resize(K * N);
+  for (size_t k = 0; k < K; k++) {
+    for (size_t i = 0; i < N; i++) sk_.data[k * N + i] = tern[i] == 0 ? 0 : (tern[i] == 1 ? 1 : qs_[k] - 1);
+    rings_[k]->ntt_fwd(sk_.data.data() + k * N);
   }
-  ring_->ntt_fwd(sk_.data.data());
 }
 
 void PirClient::encrypt_zero(RlweCt &ct) {
-  // rlwe.cpp:230-276 (K = 1): c1 = a, c0 = -(a*s + e), coefficient form
-  const size_t N = params_.get_poly_degree();
-  const u64 Q = params_.get_rns_mods()[0];
-  ct.c0.assign(N, 0);
-  ct.c1.assign(N, 0);
-  uniform_poly(ct.c1.data(), N, Q, rng_);
-  std::vector<u64> e(N), a(ct.c1);
-  gaussian_poly(e.data(), N, Q, params_.get_noise_std_dev(), rng_);
-  ring_->ntt_fwd(a.data());
-  ring_->mul(a.data(), sk_.data.data(), ct.c0.data());
-  ring_->ntt_inv(ct.c0.data());
-  for (size_t i = 0; i < N; i++) {
-    const u64 s = (ct.c0[i] + e[i]) % Q;
-    ct.c0[i] = s ? Q - s : 0;
+  // rlwe.cpp:230-276: c1 = a (uniform per limb), c0 = -(a*s + e), coefficient form
+  const size_t N = params_.get_poly_degree(), K = qs_.size();
+  ct.c0.assign(K * N, 0);
+  ct.c1.assign(K * N, 0);
+  for (size_t k = 0; k < K; k++) uniform_poly(ct.c1.data() + k * N, N, qs_[k], rng_);
+  std::vector<u64> e(K * N), a(N);
+  gaussian_shared(e.data(), N, qs_, params_.get_noise_std_dev(), rng_);
+  for (size_t k = 0; k < K; k++) {
+    const u64 q = qs_[k];
+    u64 *c0 = ct.c0.data() + k * N;
+    std::memcpy(a.data(), ct.c1.data() + k * N, N * 8);
+    rings_[k]->ntt_fwd(a.data());
+    rings_[k]->mul(a.data(), sk_.data.data() + k * N, c0);
+    rings_[k]->ntt_inv(c0);
+    for (size_t i = 0; i < N; i++) {
+      const u64 s = (c0[i] + e[k * N + i]) % q;
+      c0[i] = s ? q - s : 0;
+    }
   }
   ct.ntt_form = false;
 }
@@ -110,11 +125,12 @@ std::vector<size_t> PirClient::get_query_indices(const PirParams &qp, size_t pt_
 RlweCt PirClient::fast_generate_query(size_t pt_idx) { return fast_generate_query(params_, pt_idx); }
 
 RlweCt PirClient::fast_generate_query(const PirParams &qp, size_t pt_idx) {
-  // client.cpp:107-166 (K = 1)
+  // client.cpp:107-166: Delta * (2^h)^-1 (mod t rounding) at BitRev(column)
   if (!params_.scheme_compatible(qp))
     throw std::invalid_argument("PirClient query layout is not scheme-compatible with its session");
   if (pt_idx >= qp.get_num_pt()) throw std::out_of_range("PIR plaintext index exceeds the runtime layout");
-  const u64 Q = qp.get_rns_mods()[0], t = qp.get_plain_mod();
+  const size_t N = qp.get_poly_degree(), K = qs_.size();
+  const u64 t = qp.get_plain_mod();
   const auto idx = get_query_indices(qp, pt_idx);
   const size_t h = qp.get_expan_height();
   u64 inv = 0;
@@ -122,8 +138,22 @@ RlweCt PirClient::fast_generate_query(const PirParams &qp, size_t pt_idx) {
   const size_t slot = utils::bit_reverse(idx[0], h);
   RlweCt q;
   encrypt_zero(q);
-  const u64 scaled = (u64)(((u128)Q * inv + (t >> 1)) / t) % Q;  // round(Q * inv / t)
-  q.c0[slot] = (q.c0[slot] + scaled) % Q;
+  if (K == 1) {
+    const u64 Q = qs_[0];
+    const u64 scaled = (u64)(((u128)Q * inv + (t >> 1)) / t) % Q;  // round(Q * inv / t)
+    q.c0[slot] = (q.c0[slot] + scaled) % Q;
+  } else {
+    // Q = q0*q1 < 2^128: Delta*inv + round((Q - Delta*t) * inv / t), reduced per limb
+    const u128 Q = (u128)qs_[0] * qs_[1];
+    const u128 Delta = Q / t;
+    const u64 r = (u64)(Q - Delta * t);
+    const u64 rr = (u64)(((u128)r * inv + (t >> 1)) / t);
+    const u128 sm = Delta * inv + rr;
+    for (size_t k = 0; k < K; k++) {
+      const size_t j = k * N + slot;
+      q.c0[j] = (u64)(((u128)q.c0[j] + (u64)(sm % qs_[k])) % qs_[k]);
+    }
+  }
   add_gsw_to_query(qp, q, idx);
   return q;
 }
@@ -133,16 +163,19 @@ void PirClient::add_gsw_to_query(const PirParams &qp, RlweCt &query, const std::
   if (!params_.scheme_compatible(qp))
     throw std::invalid_argument("PirClient query layout is not scheme-compatible with its session");
   if (idx.size() == 1) return;
-  const u64 Q = qp.get_rns_mods()[0];
+  const size_t N = qp.get_poly_degree(), K = qs_.size();
   const size_t h = qp.get_expan_height(), l = qp.get_l();
-  u64 inv = 0;
-  utils::try_invert_uint_mod(size_t{1} << h, Q, inv);
-  const auto gad = utils::gsw_gadget(l, qp.get_base_log2(), qp.get_rns_mods());
+  std::vector<u64> inv(K);
+  for (size_t m = 0; m < K; m++) utils::try_invert_uint_mod(size_t{1} << h, qs_[m], inv[m]);
+  const auto gad = utils::gsw_gadget(l, qp.get_base_log2(), qs_);
   for (size_t i = 1; i < idx.size(); i++) {
     if (idx[i] != 1) continue;
     for (size_t k = 0; k < l; k++) {
       const size_t slot = utils::bit_reverse(qp.get_fst_dim_sz() + (i - 1) * l + k, h);
-      query.c0[slot] = (query.c0[slot] + mmul(gad[0][k], inv, Q)) % Q;
+      for (size_t m = 0; m < K; m++) {
+        u64 &v = query.c0[m * N + slot];
+        v = (v + mmul(gad[m][k], inv[m], qs_[m])) % qs_[m];
+      }
     }
   }
 }
@@ -151,38 +184,47 @@ bvks::BvGaloisKeys PirClient::create_bv_galois_keys() {
   // bv_keyswitch.cpp:191-276; key RNG seeded from the client seed (the
   // reference draws that seed from random_device).
   const size_t N = params_.get_poly_degree(), L = params_.get_l_ks(), b = params_.get_bv_base_log2();
-  const u64 Q = params_.get_rns_mods()[0];
+  const size_t K = qs_.size();
   std::mt19937_64 rng(splitmix64(seed_));
   bvks::BvGaloisKeys keys;
-  std::vector<u64> s_coef(sk_.data), sigma_s(N), e(N);
-  ring_->ntt_inv(s_coef.data());
+  std::vector<u64> s_coef(sk_.data), sigma_s(K * N), e(K * N);
+  for (size_t k = 0; k < K; k++) rings_[k]->ntt_inv(s_coef.data() + k * N);
   for (size_t lvl = 0; lvl < params_.get_expan_height(); lvl++) {
     const uint32_t g = (uint32_t)((N >> lvl) + 1);
-    // sigma_g(s) in coefficient form, then NTT (utils.cpp:58-82)
+    // sigma_g(s) in coefficient form per limb, then NTT (utils.cpp:58-82)
     std::fill(sigma_s.begin(), sigma_s.end(), 0);
-    for (size_t i = 0; i < N; i++) {
-      const size_t dest = (i * g) % (2 * N);
-      if (dest < N) sigma_s[dest] = s_coef[i];
-      else sigma_s[dest - N] = s_coef[i] ? Q - s_coef[i] : 0;
+    for (size_t k = 0; k < K; k++) {
+      const u64 q = qs_[k];
+      const u64 *sc = s_coef.data() + k * N;
+      u64 *ss = sigma_s.data() + k * N;
+      for (size_t i = 0; i < N; i++) {
+        const size_t dest = (i * g) % (2 * N);
+        if (dest < N) ss[dest] = sc[i];
+        else ss[dest - N] = sc[i] ? q - sc[i] : 0;
+      }
+      rings_[k]->ntt_fwd(ss);
     }
-    ring_->ntt_fwd(sigma_s.data());
     bvks::BvKeySwitchKey key;
     key.galois_k = g;
     key.cts.resize(L);
     for (size_t r = 0; r < L; r++) {
-      u64 Br = 1;
-      for (size_t k = 0; k < r * b; k++) Br = (u64)(((u128)Br << 1) % Q);
       auto &row = key.cts[r];
-      row.a.assign(N, 0);
-      row.b.assign(N, 0);
-      uniform_poly(row.a.data(), N, Q, rng);
-      gaussian_poly(e.data(), N, Q, params_.get_noise_std_dev(), rng);
-      ring_->ntt_fwd(row.a.data());
-      ring_->ntt_fwd(e.data());
-      for (size_t j = 0; j < N; j++) {
-        // b = sigma(s) * B^r - a*s + e, NTT form
-        const u64 msg = mmul(sigma_s[j], Br, Q), as = mmul(row.a[j], sk_.data[j], Q);
-        row.b[j] = ((msg + Q - as) % Q + e[j]) % Q;
+      row.a.assign(K * N, 0);
+      row.b.assign(K * N, 0);
+      for (size_t k = 0; k < K; k++) uniform_poly(row.a.data() + k * N, N, qs_[k], rng);
+      gaussian_shared(e.data(), N, qs_, params_.get_noise_std_dev(), rng);
+      for (size_t k = 0; k < K; k++) {
+        const u64 q = qs_[k];
+        u64 Br = 1 % q;
+        for (size_t z = 0; z < r * b; z++) Br = (u64)(((u128)Br << 1) % q);
+        u64 *a = row.a.data() + k * N, *bb = row.b.data() + k * N, *ek = e.data() + k * N;
+        rings_[k]->ntt_fwd(a);
+        rings_[k]->ntt_fwd(ek);
+        for (size_t j = 0; j < N; j++) {
+          // b = sigma(s) * B^r - a*s + e, NTT form
+          const u64 msg = mmul(sigma_s[k * N + j], Br, q), as = mmul(a[j], sk_.data[k * N + j], q);
+          bb[j] = ((msg + q - as) % q + ek[j]) % q;
+        }
       }
     }
     keys.keys.push_back(std::move(key));
@@ -191,24 +233,34 @@ bvks::BvGaloisKeys PirClient::create_bv_galois_keys() {
 }
 
 GSWCt PirClient::generate_gsw_from_key() {
-  // client.cpp:25-38 + gsw.cpp:359-422 (K = 1): RGSW(s) with the key gadget
-  const size_t N = params_.get_poly_degree(), l = params_.get_l_key();
-  const u64 Q = params_.get_rns_mods()[0];
-  std::vector<u64> s_coef(sk_.data);
-  ring_->ntt_inv(s_coef.data());
-  const auto gad = utils::gsw_gadget(l, params_.get_base_log2_key(), params_.get_rns_mods());
-  GSWCt out(2 * l, std::vector<u64>(2 * N));
+  // client.cpp:25-38 + gsw.cpp:359-422: RGSW(s) with the key gadget; the
+  // secret's coefficients (0, 1, -1) are carried into every limb.
+  const size_t N = params_.get_poly_degree(), l = params_.get_l_key(), K = qs_.size();
+  std::vector<u64> s_coef(sk_.data.begin(), sk_.data.begin() + N);
+  rings_[0]->ntt_inv(s_coef.data());
+  const u64 q0 = qs_[0];
+  const auto gad = utils::gsw_gadget(l, params_.get_base_log2_key(), qs_);
+  GSWCt out(2 * l, std::vector<u64>(2 * K * N));
   RlweCt ct;
   for (size_t half = 0; half < 2; half++)
     for (size_t r = 0; r < l; r++) {
       encrypt_zero(ct);
-      u64 *tgt = ct.data(half);
-      for (size_t j = 0; j < N; j++) tgt[j] = (tgt[j] + mmul(s_coef[j] % Q, gad[0][r], Q)) % Q;
-      ring_->ntt_fwd(ct.c0.data());
-      ring_->ntt_fwd(ct.c1.data());
+      for (size_t k = 0; k < K; k++) {
+        const u64 qk = qs_[k];
+        u64 *tgt = ct.data(half) + k * N;
+        for (size_t j = 0; j < N; j++) {
+          const u64 s = s_coef[j];
+          const u64 canon = (K == 1 || s <= q0 / 2) ? s % qk : (q0 - s >= qk ? (qk - (q0 - s) % qk) % qk : qk - (q0 - s));
+          tgt[j] = (tgt[j] + mmul(canon, gad[k][r], qk)) % qk;
+        }
+      }
+      for (size_t k = 0; k < K; k++) {
+        rings_[k]->ntt_fwd(ct.c0.data() + k * N);
+        rings_[k]->ntt_fwd(ct.c1.data() + k * N);
+      }
       auto &row = out[half * l + r];
-      std::memcpy(row.data(), ct.c0.data(), N * 8);
-      std::memcpy(row.data() + N, ct.c1.data(), N * 8);
+      std::memcpy(row.data(), ct.c0.data(), K * N * 8);
+      std::memcpy(row.data() + K * N, ct.c1.data(), K * N * 8);
     }
   return out;
 }
@@ -249,8 +301,8 @@ RlwePt PirClient::decrypt_mod_q(const RlweCt &ct, int *budget) const {
   const size_t N = params_.get_poly_degree();
   const u64 q = params_.get_small_q(), t = params_.get_plain_mod();
   RingMod small(N, q, utils::minimal_primitive_root(2 * N, q));
-  std::vector<u64> s(sk_.data), c1(N), ph(N);
-  ring_->ntt_inv(s.data());
+  std::vector<u64> s(sk_.data.begin(), sk_.data.begin() + N), c1(N), ph(N);
+  rings_[0]->ntt_inv(s.data());
   for (size_t i = 0; i < N; i++) s[i] = s[i] > 1 ? q - 1 : s[i];
   small.ntt_fwd(s.data());
   for (size_t i = 0; i < N; i++) c1[i] = ct.c1[i] % q;

@@ -195,8 +195,9 @@ class PirClient {
   PirParams params_;
   std::mt19937_64 rng_;
   u64 seed_;
-  std::shared_ptr<RingMod> ring_;
-  RlweSk sk_;
+  std::vector<u64> qs_;                           // RNS limb moduli (one logical Q for K = 1)
+  std::vector<std::shared_ptr<RingMod>> rings_;   // one negacyclic NTT ring per limb
+  RlweSk sk_;                                     // K*N values, NTT form per limb
 };
 
 // Owns one GPU context (scheme tables, device keys, stream); shareable by
@@ -226,6 +227,9 @@ class PirServer {
   PirServer &operator=(const PirServer &) = delete;
 
   void gen_data(const std::vector<size_t> &record_indices = {}, u64 seed = 0);
+  // gen_data with the plaintexts generated on the GPU (no host copy of the
+  // database): coefficient c of plaintext i = splitmix64(seed + i*N + c) mod t.
+  void gen_data_device(const std::vector<size_t> &record_indices = {}, u64 seed = 0);
   void load_data(size_t logical_num_pt, const PlaintextSource &source,
                  const std::vector<size_t> &record_indices = {});
   RlweCt make_query(size_t client_id, RlweCt &query);

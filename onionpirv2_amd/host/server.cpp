@@ -13,13 +13,23 @@
 namespace onionpir {
 
 GpuScheme::GpuScheme(const PirParams &p, int device) : params_(p) {
-  if (!p.q1()) throw std::invalid_argument("the GPU path needs the composite scheme (CONFIG_N2048_K1_COMP)");
   opir_scheme s{};
   s.poly_degree = (uint32_t)p.get_poly_degree();
-  s.composite = 1;
-  s.q1 = p.q1();
-  s.q2 = p.q2();
-  s.ntt_root = p.ntt_root();
+  if (p.q1()) {  // CONFIG_N2048_K1_COMP: one logical limb mod q1*q2
+    s.composite = 1;
+    s.q1 = p.q1();
+    s.q2 = p.q2();
+    s.ntt_root = p.ntt_root();
+  } else if (p.K() == 2) {  // CONFIG_N4096_K2_MP: two RNS limbs
+    const size_t N = p.get_poly_degree();
+    s.composite = 0;
+    s.q1 = p.get_rns_mods()[0];
+    s.q2 = p.get_rns_mods()[1];
+    s.ntt_root = utils::minimal_primitive_root(2 * N, s.q1);
+    s.ntt_root2 = utils::minimal_primitive_root(2 * N, s.q2);
+  } else {
+    throw std::invalid_argument("the GPU path runs CONFIG_N2048_K1_COMP and CONFIG_N4096_K2_MP");
+  }
   s.plain_mod = p.get_plain_mod();
   s.small_q = p.get_small_q();
   s.l_ep = (uint32_t)p.get_l();
@@ -47,18 +57,22 @@ void GpuScheme::check(int rc) const {
 }
 
 void GpuScheme::set_client_keys(size_t id, const bvks::BvGaloisKeys &bv, const GSWCt &gsw) {
-  const size_t N = params_.get_poly_degree(), L = params_.get_l_ks();
-  std::vector<uint64_t> fb(bv.keys.size() * L * 2 * N), fg(gsw.size() * 2 * N);
+  const size_t N = params_.get_poly_degree(), L = params_.get_l_ks(), KN = N * params_.K();
+  std::vector<uint64_t> fb(bv.keys.size() * L * 2 * KN), fg(gsw.size() * 2 * KN);
   for (size_t i = 0; i < bv.keys.size(); i++) {
     const auto &k = bv.get((uint32_t)((N >> i) + 1));
     if (k.cts.size() != L) throw std::invalid_argument("BV key row count mismatch");
     for (size_t r = 0; r < L; r++) {
-      std::memcpy(&fb[((i * L + r) * 2 + 0) * N], k.cts[r].a.data(), N * 8);
-      std::memcpy(&fb[((i * L + r) * 2 + 1) * N], k.cts[r].b.data(), N * 8);
+      if (k.cts[r].a.size() != KN || k.cts[r].b.size() != KN) throw std::invalid_argument("BV key row size mismatch");
+      std::memcpy(&fb[((i * L + r) * 2 + 0) * KN], k.cts[r].a.data(), KN * 8);
+      std::memcpy(&fb[((i * L + r) * 2 + 1) * KN], k.cts[r].b.data(), KN * 8);
     }
   }
   if (gsw.size() != 2 * params_.get_l_key()) throw std::invalid_argument("RGSW key row count mismatch");
-  for (size_t r = 0; r < gsw.size(); r++) std::memcpy(&fg[r * 2 * N], gsw[r].data(), 2 * N * 8);
+  for (size_t r = 0; r < gsw.size(); r++) {
+    if (gsw[r].size() != 2 * KN) throw std::invalid_argument("RGSW key row size mismatch");
+    std::memcpy(&fg[r * 2 * KN], gsw[r].data(), 2 * KN * 8);
+  }
   check(opir_client_set_keys(ctx_, id, fb.data(), bv.keys.size(), fg.data()));
 }
 
@@ -98,22 +112,22 @@ void PirServer::load_data(size_t logical, const PlaintextSource &source, const s
   const std::unordered_set<size_t> rs(record.begin(), record.end());
   const u64 t = params_.get_plain_mod();
   const size_t chunk = 16384;
-  std::vector<uint16_t> buf(chunk * N);
+  std::vector<uint64_t> buf(chunk * N);
   RlwePt pt;
   for (size_t base = 0; base < num_pt; base += chunk) {
     const size_t n = std::min(chunk, num_pt - base);
     for (size_t j = 0; j < n; j++) {
       const size_t id = base + j;
-      uint16_t *dst = &buf[j * N];
+      uint64_t *dst = &buf[j * N];
       if (id < logical) {
         source(id, pt);
         if (pt.data.size() != N) throw std::invalid_argument("Plaintext source returned the wrong coefficient count");
         for (size_t c = 0; c < N; c++) {
           if (pt.data[c] >= t) throw std::invalid_argument("Plaintext source coefficient exceeds the plaintext modulus");
-          dst[c] = (uint16_t)pt.data[c];
+          dst[c] = pt.data[c];
         }
       } else {
-        std::memset(dst, 0, N * sizeof(uint16_t));
+        std::memset(dst, 0, N * sizeof(uint64_t));
       }
       if (rs.count(id)) {
         RlwePt rec;
@@ -121,7 +135,30 @@ void PirServer::load_data(size_t logical, const PlaintextSource &source, const s
         recorded_[id] = std::move(rec);
       }
     }
-    scheme_->check(opir_db_load_plain(db_, buf.data(), base, n));
+    scheme_->check(opir_db_load_plain_u64(db_, buf.data(), base, n));
+  }
+}
+
+static u64 splitmix64_h(u64 x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+
+void PirServer::gen_data_device(const std::vector<size_t> &record, u64 seed) {
+  // server.cpp:53-66 with the plaintexts generated in HBM (opir_db_gen_random)
+  const size_t num_pt = params_.get_num_pt(), N = params_.get_poly_degree();
+  const u64 t = params_.get_plain_mod();
+  for (size_t r : record)
+    if (r >= num_pt) throw std::invalid_argument("PirServer::gen_data record index exceeds its layout");
+  scheme_->check(opir_db_gen_random(db_, seed));
+  recorded_.clear();
+  for (size_t r : record) {
+    RlwePt rec;
+    rec.data.resize(N);
+    for (size_t c = 0; c < N; c++) rec.data[c] = splitmix64_h(seed + r * N + c) % t;
+    recorded_[r] = std::move(rec);
   }
 }
 
@@ -190,7 +227,7 @@ static void require_keys(const std::map<size_t, bvks::BvGaloisKeys> &bv, const s
 RlweCt PirServer::make_query(size_t id, RlweCt &query) {  // server.cpp:895-947
   require_keys(bv_keys_, gsw_keys_, sessions_, id);
   const size_t N = params_.get_poly_degree();
-  auto q = flatten(query, N);
+  auto q = flatten(query, N * params_.K());
   std::vector<uint64_t> r(2 * N);
   scheme_->check(opir_answer(db_, id, q.data(), r.data()));
   return unflatten(r.data(), N, false);
@@ -198,12 +235,12 @@ RlweCt PirServer::make_query(size_t id, RlweCt &query) {  // server.cpp:895-947
 
 std::vector<RlweCt> PirServer::make_queries(const std::vector<size_t> &ids, const std::vector<RlweCt> &queries) {
   if (ids.size() != queries.size()) throw std::invalid_argument("one client id per query");
-  const size_t N = params_.get_poly_degree(), B = ids.size();
-  std::vector<uint64_t> q(B * 2 * N), r(B * 2 * N), id64(ids.begin(), ids.end());
+  const size_t N = params_.get_poly_degree(), B = ids.size(), KN = N * params_.K();
+  std::vector<uint64_t> q(B * 2 * KN), r(B * 2 * N), id64(ids.begin(), ids.end());
   for (size_t i = 0; i < B; i++) {
     require_keys(bv_keys_, gsw_keys_, sessions_, ids[i]);
-    auto f = flatten(queries[i], N);
-    std::memcpy(&q[i * 2 * N], f.data(), 2 * N * 8);
+    auto f = flatten(queries[i], KN);
+    std::memcpy(&q[i * 2 * KN], f.data(), 2 * KN * 8);
   }
   scheme_->check(opir_answer_batch(db_, B, id64.data(), q.data(), r.data()));
   std::vector<RlweCt> out;
@@ -213,7 +250,7 @@ std::vector<RlweCt> PirServer::make_queries(const std::vector<size_t> &ids, cons
 
 std::vector<RlweCt> PirServer::expand_query(size_t id, RlweCt &query) const {  // server.cpp:818-821
   require_keys(bv_keys_, gsw_keys_, sessions_, id);
-  const size_t N = params_.get_poly_degree();
+  const size_t N = params_.get_poly_degree() * params_.K();  // values per polynomial
   const size_t u = params_.get_fst_dim_sz() + params_.get_l() * (params_.get_num_dims() - 1);
   auto q = flatten(query, N);
   std::vector<uint64_t> out(u * 2 * N);
@@ -224,7 +261,7 @@ std::vector<RlweCt> PirServer::expand_query(size_t id, RlweCt &query) const {  /
 }
 
 std::vector<GSWCt> PirServer::complete_selectors(size_t id, const std::vector<RlweCt> &expanded) {
-  const size_t N = params_.get_poly_degree(), l = params_.get_l(), ns = params_.get_num_dims() - 1;
+  const size_t N = params_.get_poly_degree() * params_.K(), l = params_.get_l(), ns = params_.get_num_dims() - 1;
   std::vector<GSWCt> sel(ns);
   if (!ns) return sel;
   const size_t u = params_.get_fst_dim_sz() + l * ns;
@@ -245,7 +282,7 @@ std::vector<GSWCt> PirServer::complete_selectors(size_t id, const std::vector<Rl
 }
 
 std::vector<RlweCt> PirServer::evaluate_first_dim(std::vector<RlweCt> &sv) {
-  const size_t N = params_.get_poly_degree(), N0 = params_.get_fst_dim_sz(), Nr = params_.get_other_dim_sz();
+  const size_t N = params_.get_poly_degree() * params_.K(), N0 = params_.get_fst_dim_sz(), Nr = params_.get_other_dim_sz();
   if (sv.size() < N0) throw std::invalid_argument("evaluate_first_dim expects fst_dim_sz ciphertexts");
   std::vector<uint64_t> q(N0 * 2 * N), out(Nr * 2 * N);
   for (size_t i = 0; i < N0; i++) {
@@ -259,7 +296,7 @@ std::vector<RlweCt> PirServer::evaluate_first_dim(std::vector<RlweCt> &sv) {
 }
 
 RlweCt PirServer::evaluate_other_dim(std::vector<RlweCt> &mid, std::vector<GSWCt> &sel) {
-  const size_t N = params_.get_poly_degree(), Nr = params_.get_other_dim_sz(), l = params_.get_l();
+  const size_t N = params_.get_poly_degree() * params_.K(), Nr = params_.get_other_dim_sz(), l = params_.get_l();
   if (mid.size() < Nr) throw std::invalid_argument("evaluate_other_dim expects other_dim_sz candidates");
   std::vector<uint64_t> m(Nr * 2 * N), s(sel.size() * 2 * l * 2 * N + 1), out(2 * N);
   for (size_t i = 0; i < Nr; i++) {

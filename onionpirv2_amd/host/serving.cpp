@@ -75,8 +75,8 @@ double ServingLoop::now() const {
 
 uint64_t ServingLoop::submit(size_t db, size_t client_id, std::vector<u64> query) {
   if (db >= lanes_.size()) throw std::out_of_range("unknown database index");
-  const size_t N = lanes_[db]->srv->get_params().get_poly_degree();
-  if (query.size() != 2 * N) throw std::invalid_argument("query must hold [c0 N][c1 N]");
+  const PirParams &p = lanes_[db]->srv->get_params();
+  if (query.size() != 2 * p.get_poly_degree() * p.K()) throw std::invalid_argument("query must hold [c0 K*N][c1 K*N]");
   std::lock_guard<std::mutex> g(mu_);
   if (stop_) throw std::runtime_error("serving loop is stopped");
   const uint64_t t = next_ticket_++;
@@ -112,7 +112,7 @@ void ServingLoop::stop() {
 void ServingLoop::worker(size_t d) {
   DbLane &lane = *lanes_[d];
   PirServer &srv = *lane.srv;
-  const size_t N = srv.get_params().get_poly_degree();
+  const size_t N = srv.get_params().get_poly_degree(), QN = N * srv.get_params().K();  // response / query values per poly
   const size_t limit = limits_[d];
   std::vector<uint64_t> ids, q, r;
   for (;;) {
@@ -129,11 +129,11 @@ void ServingLoop::worker(size_t d) {
     }
     const size_t B = batch.size();
     ids.resize(B);
-    q.resize(B * 2 * N);
+    q.resize(B * 2 * QN);
     r.resize(B * 2 * N);
     for (size_t i = 0; i < B; i++) {
       ids[i] = batch[i].client_id;
-      std::memcpy(&q[i * 2 * N], batch[i].query.data(), 2 * N * sizeof(u64));
+      std::memcpy(&q[i * 2 * QN], batch[i].query.data(), 2 * QN * sizeof(u64));
     }
     const double ts = now();
     int rc;

@@ -56,7 +56,7 @@ class ServingLoop {
   ServingLoop(const ServingLoop &) = delete;
   ServingLoop &operator=(const ServingLoop &) = delete;
 
-  // Thread-safe. query: [c0 N][c1 N] coefficient form mod Q.
+  // Thread-safe. query: [c0 K*N][c1 K*N] coefficient form (K RNS limbs).
   uint64_t submit(size_t db, size_t client_id, std::vector<u64> query);
   // Moves completed results into out; returns how many.
   size_t poll(std::vector<ServeResult> &out);
