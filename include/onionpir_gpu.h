@@ -137,6 +137,15 @@ int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out);
 int opir_ctx_set_profiling(opir_ctx *ctx, int on);
 int opir_last_stage_ms(opir_ctx *ctx, float *out6);
 
+/* First-dimension scan tile shape (no counterpart in the reference, whose
+ * scan has one CPU loop nest): -1 (the default) picks the shape per batch
+ * size; 0 .. count-1 forces one, for tuning and for tests that check every
+ * shape against the same answers. A forced shape that cannot take a batch
+ * size makes the answer call fail with OPIR_EINVAL. Results are identical
+ * for every shape. */
+int opir_ctx_set_scan_variant(opir_ctx *ctx, int variant);
+int opir_scan_variant_count(const opir_ctx *ctx, int *out);
+
 /* ---- Stage entry points (host buffers), one per reference stage ---- */
 
 /* utils::ntt_fwd / ntt_inv over Q (src/utils.cpp:84-90), in place on

@@ -130,7 +130,9 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   launch_ct_ntt(leaves, db->qntt, exp_stride, db->N0 * kCtWords, (int)db->N0, (int)B, ctx->S, ctx->stream);
   launch_qfrag(db->qntt, db->qfrag, db->N0 * kCtWords, (int)B, db->geom, ctx->stream);
   rec(ctx, 5);
-  launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream);
+  if (launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream,
+                  ctx->scan_variant))
+    return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
   LAUNCH_CHECK(ctx);
   rec(ctx, 6);
   rec(ctx, 3);
@@ -312,6 +314,20 @@ int opir_ctx_set_profiling(opir_ctx *ctx, int on) {
   return OPIR_OK;
 }
 
+int opir_ctx_set_scan_variant(opir_ctx *ctx, int variant) {
+  if (!ctx) return OPIR_EINVAL;
+  if (variant < -1 || variant >= scan_variant_count(ctx->k2 ? 8 : 4))
+    return set_err(ctx, OPIR_EINVAL, "scan variant out of range");
+  ctx->scan_variant = variant;
+  return OPIR_OK;
+}
+
+int opir_scan_variant_count(const opir_ctx *ctx, int *out) {
+  if (!ctx || !out) return OPIR_EINVAL;
+  *out = scan_variant_count(ctx->k2 ? 8 : 4);
+  return OPIR_OK;
+}
+
 int opir_last_stage_ms(opir_ctx *ctx, float *o) {
   if (!ctx || !o) return OPIR_EINVAL;
   if (!ctx->profiling) return set_err(ctx, OPIR_EINVAL, "profiling is off");
@@ -433,12 +449,12 @@ int opir_db_gen_random(opir_db *db, uint64_t seed) {
   const size_t N = ctx->N, chunk = ctx->k2 ? 4096 : 8192;
   DevBuf pts, tmp;
   DEV_ALLOC(ctx, pts, chunk * N * (ctx->k2 ? sizeof(u64) : sizeof(uint16_t)));
-  if (!ctx->k2) DEV_ALLOC(ctx, tmp, 2 * chunk * kN * sizeof(u32));
+  DEV_ALLOC(ctx, tmp, 2 * chunk * N * (ctx->k2 ? sizeof(u64) : sizeof(u32)));
   for (size_t off = 0; off < db->num_pt; off += chunk) {
     const size_t n = std::min(chunk, db->num_pt - off);
     if (ctx->k2) {
       launch_gen_plain_u64((u64 *)pts.p, off, n, (int)N, seed, t, ctx->stream);
-      k2::launch_k2_db_load((const u64 *)pts.p, db->d_db64, db->num_pt, off, n, ctx->S64, ctx->stream);
+      if (int rc = k2_ingest(db, (const u64 *)pts.p, (u64 *)tmp.p, off, n)) return rc;
     } else {
       launch_gen_plain_u16((uint16_t *)pts.p, off, n, (int)N, seed, t, ctx->stream);
       launch_db_ntt_plain((const uint16_t *)pts.p, (u32 *)tmp.p, n, ctx->S, ctx->stream);
@@ -693,7 +709,8 @@ int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
   launch_split_u64((u64 *)e.p, db->exp_a, db->N0 * 2, ctx->S, ctx->stream);
   launch_ct_ntt(db->exp_a, db->qntt, 0, 0, (int)db->N0, 1, ctx->S, ctx->stream);
   launch_qfrag(db->qntt, db->qfrag, 0, 1, db->geom, ctx->stream);
-  launch_scan(db->d_db, db->qfrag, db->mid_a, 0, 1, db->geom, ctx->S, ctx->stream);
+  if (launch_scan(db->d_db, db->qfrag, db->mid_a, 0, 1, db->geom, ctx->S, ctx->stream, ctx->scan_variant))
+    return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
   DevBuf o;
   DEV_ALLOC(ctx, o, db->Nrest * 2 * kN * sizeof(u64));
   launch_ct_to_coef_u64(db->mid_a, (u64 *)o.p, db->Nrest, ctx->S, ctx->stream);

@@ -44,6 +44,7 @@ struct opir_ctx {
   std::unordered_map<uint64_t, capi::ClientKeys> clients;
   std::string err;
   bool profiling = false;
+  int scan_variant = -1;    // first-dimension scan tile shape, -1 = default per batch size
   hipEvent_t ev[8] = {};
   // pointer-table ring
   const void **pin_ptrs[capi::kRing] = {};
@@ -60,8 +61,7 @@ struct opir_db {
   opir_ctx *ctx = nullptr;
   size_t N0 = 0, Nrest = 0, dims = 0, h = 0, num_pt = 0, u = 0;
   DbGeom geom;
-  int8_t *d_db = nullptr;  // K = 1: digit planes in MFMA fragment order (opir_scan.hip)
-  u64 *d_db64 = nullptr;   // K = 2: [limb][pt][N] NTT form
+  int8_t *d_db = nullptr;  // digit planes in MFMA fragment order (opir_scan.hip), P = 4 (K = 1) or 8 (K = 2)
   // workspace for up to ws_B queries (K = 1 u32 RNS cts / K = 2 u64 cts)
   size_t ws_B = 0;
   u32 *exp_a = nullptr, *exp_b = nullptr, *qntt = nullptr, *sel = nullptr, *mid_a = nullptr, *mid_b = nullptr;
@@ -211,6 +211,7 @@ int k2_db_init(opir_db *db);
 size_t k2_ws_bytes(const opir_db *db, size_t B);
 int k2_ensure_ws(opir_db *db, size_t B);
 void k2_db_free(opir_db *db);
+int k2_ingest(opir_db *db, const u64 *d_pts, u64 *d_tmp, size_t first_pt, size_t n);
 int k2_load_plain(opir_db *db, const uint64_t *coeffs, size_t first_pt, size_t count);
 int k2_set_keys(opir_ctx *ctx, const uint64_t *bv, size_t h, const uint64_t *gsw, ClientKeys &k);
 int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u64 *d_r);

@@ -157,18 +157,21 @@ void k2_ctx_free(opir_ctx *ctx) {
 
 int k2_db_init(opir_db *db) {
   opir_ctx *ctx = db->ctx;
-  const size_t bytes = 2 * db->num_pt * N2 * sizeof(u64);
-  if (hipMalloc((void **)&db->d_db64, bytes) != hipSuccess)
+  // 8 balanced int8 digit planes per 60-bit residue, MFMA fragment order
+  db->geom = make_db_geom(db->N0, db->Nrest, 8, (int)N2);
+  const size_t bytes = db_bytes(db->geom);
+  if (hipMalloc((void **)&db->d_db, bytes) != hipSuccess)
     return set_err(ctx, OPIR_ENOMEM, "hipMalloc database (" + std::to_string(bytes) + " bytes)");
-  HIP_TRY(ctx, hipMemsetAsync(db->d_db64, 0, bytes, ctx->stream));
+  HIP_TRY(ctx, hipMemsetAsync(db->d_db, 0, bytes, ctx->stream));
   return OPIR_OK;
 }
 
 size_t k2_ws_bytes(const opir_db *db, size_t B) {
   const size_t ct = kCt2 * sizeof(u64);
   const size_t nsel = db->dims - 1;
-  return 2 * B * db->exp_cts * ct + B * db->N0 * ct + B * (nsel ? nsel : 1) * 2 * db->ctx->G64.l_ep * ct +
-         2 * B * db->Nrest * ct + B * 4 * N2 * sizeof(u64) + B * 2 * N2 * sizeof(u64);
+  return qfrag_bytes(make_qfrag_geom(db->N0, (int)B, 8, (int)N2)) + 2 * B * db->exp_cts * ct + B * db->N0 * ct +
+         B * (nsel ? nsel : 1) * 2 * db->ctx->G64.l_ep * ct + 2 * B * db->Nrest * ct + B * 4 * N2 * sizeof(u64) +
+         B * 2 * N2 * sizeof(u64);
 }
 
 static void free_ws(opir_db *db) {
@@ -176,6 +179,8 @@ static void free_ws(opir_db *db) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
   }
+  if (db->qfrag) (void)hipFree(db->qfrag);
+  db->qfrag = nullptr;
   db->ws_B = 0;
 }
 
@@ -193,14 +198,26 @@ int k2_ensure_ws(opir_db *db, size_t B) {
   HIP_TRY(ctx, hipMalloc((void **)&db->m64_b, B * db->Nrest * ct));
   HIP_TRY(ctx, hipMalloc((void **)&db->q_u64, B * 4 * N2 * sizeof(u64)));
   HIP_TRY(ctx, hipMalloc((void **)&db->r_u64, B * 2 * N2 * sizeof(u64)));
+  HIP_TRY(ctx, hipMalloc((void **)&db->qfrag, qfrag_bytes(make_qfrag_geom(db->N0, (int)B, 8, (int)N2))));
   db->ws_B = B;
   return OPIR_OK;
 }
 
 void k2_db_free(opir_db *db) {
+  if (!db->ctx->k2) return;
   free_ws(db);
-  if (db->d_db64) (void)hipFree(db->d_db64);
-  db->d_db64 = nullptr;
+  if (db->d_db) (void)hipFree(db->d_db);
+  db->d_db = nullptr;
+}
+
+// NTT `n` plaintexts (u64 coefficients < t, device) per limb into tmp and
+// scatter them into the digit-plane database at first_pt.
+int k2_ingest(opir_db *db, const u64 *d_pts, u64 *d_tmp, size_t first_pt, size_t n) {
+  opir_ctx *ctx = db->ctx;
+  k2::launch_k2_db_load(d_pts, d_tmp, n, 0, n, ctx->S64, ctx->stream);
+  launch_db_scatter64(d_tmp, db->d_db, first_pt, n, db->geom, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  return OPIR_OK;
 }
 
 int k2_load_plain(opir_db *db, const uint64_t *coeffs, size_t first_pt, size_t count) {
@@ -210,14 +227,14 @@ int k2_load_plain(opir_db *db, const uint64_t *coeffs, size_t first_pt, size_t c
   const u64 t = ctx->sch.plain_mod;
   for (size_t i = 0; i < count * N2; i++)
     if (coeffs[i] >= t) return set_err(ctx, OPIR_EINVAL, "plaintext coefficient not reduced mod t");
-  const size_t chunk = 4096;
-  DevBuf d;
-  DEV_ALLOC(ctx, d, std::min(chunk, std::max<size_t>(count, 1)) * N2 * sizeof(u64));
+  const size_t chunk = 4096, cn = std::min(chunk, std::max<size_t>(count, 1));
+  DevBuf d, tmp;
+  DEV_ALLOC(ctx, d, cn * N2 * sizeof(u64));
+  DEV_ALLOC(ctx, tmp, 2 * cn * N2 * sizeof(u64));
   for (size_t off = 0; off < count; off += chunk) {
     const size_t n = std::min(chunk, count - off);
     HIP_TRY(ctx, hipMemcpyAsync(d.p, coeffs + off * N2, n * N2 * sizeof(u64), hipMemcpyHostToDevice, ctx->stream));
-    k2::launch_k2_db_load((const u64 *)d.p, db->d_db64, db->num_pt, first_pt + off, n, ctx->S64, ctx->stream);
-    LAUNCH_CHECK(ctx);
+    if (int rc = k2_ingest(db, (const u64 *)d.p, (u64 *)tmp.p, first_pt + off, n)) return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   }
   return OPIR_OK;
@@ -282,9 +299,11 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
   LAUNCH_CHECK(ctx);
   rec(ctx, 2);
   k2::launch_k2_ct_ntt(leaves, db->q64, exp_stride, db->N0 * kCt2, (int)db->N0, (int)B, ctx->S64, ctx->stream);
+  launch_qfrag64(db->q64, db->qfrag, db->N0 * kCt2, (int)B, db->geom, ctx->stream);
   rec(ctx, 5);
-  k2::launch_k2_scan(db->d_db64, db->q64, db->m64_a, db->N0 * kCt2, db->Nrest * kCt2, db->N0, db->Nrest, (int)B,
-                     ctx->S64, ctx->stream);
+  if (launch_scan64(db->d_db, db->qfrag, db->m64_a, db->Nrest * kCt2, (int)B, db->geom, ctx->S64, ctx->stream,
+                    ctx->scan_variant))
+    return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
   LAUNCH_CHECK(ctx);
   rec(ctx, 6);
   rec(ctx, 3);
@@ -362,7 +381,9 @@ int k2_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
   if (rc) return rc;
   HIP_TRY(ctx, hipMemcpy(db->e64_a, query, db->N0 * kCt2 * sizeof(u64), hipMemcpyHostToDevice));
   k2::launch_k2_ct_ntt(db->e64_a, db->q64, 0, 0, (int)db->N0, 1, ctx->S64, ctx->stream);
-  k2::launch_k2_scan(db->d_db64, db->q64, db->m64_a, 0, 0, db->N0, db->Nrest, 1, ctx->S64, ctx->stream);
+  launch_qfrag64(db->q64, db->qfrag, 0, 1, db->geom, ctx->stream);
+  if (launch_scan64(db->d_db, db->qfrag, db->m64_a, 0, 1, db->geom, ctx->S64, ctx->stream, ctx->scan_variant))
+    return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
   k2::launch_k2_ntt(db->m64_a, db->Nrest * 2, true, ctx->S64, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

@@ -62,15 +62,12 @@ void launch_k2_complete(const u64 *rows, u64 *sel, size_t row_stride, size_t sel
 // Forward NTT of ciphertexts: in (coef) -> out (NTT), count cts per query.
 void launch_k2_ct_ntt(const u64 *in, u64 *out, size_t in_stride, size_t out_stride, int count, int B,
                       const Scheme64 &S, hipStream_t st);
-// First dimension: db [limb][pt][N] (NTT form), qntt [B][N0][ct] -> mid [B][Nrest][ct] (NTT form).
-void launch_k2_scan(const u64 *db, const u64 *qntt, u64 *mid, size_t q_stride, size_t mid_stride, size_t N0,
-                    size_t Nrest, int B, const Scheme64 &S, hipStream_t st);
 // MUX level on NTT-form candidates (same slot rules as the K = 1 path).
 void launch_k2_mux(const u64 *in, u64 *out, size_t stride, int nslots, int offset, int half, int B, const u64 *sel,
                    size_t sel_stride, const Gadget64 &G, const Scheme64 &S, hipStream_t st);
 // INTT, drop q1 (round half up), centred rescale to small_q: res [B][ct] -> out [B][2][N].
 void launch_k2_final(const u64 *res, u64 *out, size_t res_stride, int B, const Scheme64 &S, hipStream_t st);
-// Plaintext coefficients (< t) [count][N] -> NTT per limb into db [limb][first + i][N].
+// Plaintext coefficients (< t) [count][N] -> NTT per limb into out [limb][first + i][N] (of num_pt).
 void launch_k2_db_load(const u64 *pts, u64 *db, size_t num_pt, size_t first, size_t count, const Scheme64 &S,
                        hipStream_t st);
 // External product alone (test entry): gsw [2l][ct] NTT, ct coef -> out NTT.

@@ -429,61 +429,6 @@ __global__ __launch_bounds__(1024) void k2_ct_ntt(const u64 *in, u64 *out, size_
   store3(out + qi * out_stride + (size_t)ct * kCt + (size_t)(p * 2 + limb) * kN, v, t);
 }
 
-// First dimension (matrix.cpp:24-151, server.cpp:363-527): one thread per
-// (limb, row, level) and column tile of kCols query columns (c = 2*query +
-// poly); the DB value is read once per tile and multiplied into each
-// column's 128-bit sum, folded into a reduced residue every 64 terms
-// (64 products < 2^126).
-constexpr int kCols = 8;
-
-__global__ __launch_bounds__(256) void k2_scan(const u64 *__restrict__ db, const u64 *__restrict__ qntt,
-                                               u64 *__restrict__ mid, size_t q_stride, size_t mid_stride, int N0,
-                                               int Nrest, int ncols, int nct, Scheme64 S) {
-  const unsigned W = gridDim.x, p = blockIdx.x;
-  const unsigned Lg = (W % 8 == 0) ? (p % 8) * (W / 8) + p / 8 : p;  // same-XCD neighbours share DB rows
-  const int ct = Lg % nct;
-  unsigned rest = Lg / nct;
-  const int lvb = rest % (kN / 256);
-  rest /= (kN / 256);
-  const int row = rest % Nrest;
-  const int limb = rest / Nrest;
-  const Limb64 &L = S.limb[limb];
-  const int lv = lvb * 256 + threadIdx.x;
-  const int c0 = ct * kCols, cn = min(kCols, ncols - c0);
-  const size_t num_pt = (size_t)N0 * Nrest;
-  const u64 *a = db + ((size_t)limb * num_pt + (size_t)row * N0) * kN + lv;
-  u64 lo[kCols], hi[kCols], red[kCols];
-#pragma unroll
-  for (int j = 0; j < kCols; j++) lo[j] = hi[j] = red[j] = 0;
-  for (int k0 = 0; k0 < N0; k0 += 64) {
-    const int k1 = min(N0, k0 + 64);
-    for (int k = k0; k < k1; k++) {
-      const u64 x = a[(size_t)k * kN];
-#pragma unroll
-      for (int j = 0; j < kCols; j++) {
-        if (j < cn) {
-          const int c = c0 + j;
-          const u64 y = qntt[(c >> 1) * q_stride + (size_t)k * kCt + (size_t)((c & 1) * 2 + limb) * kN + lv];
-          const u64 pl = x * y, ph = __umul64hi(x, y);
-          lo[j] += pl;
-          hi[j] += ph + (lo[j] < pl);
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kCols; j++) {
-      red[j] = addmod(red[j], reduce128(hi[j], lo[j], L), L.q);
-      lo[j] = hi[j] = 0;
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < kCols; j++)
-    if (j < cn) {
-      const int c = c0 + j;
-      mid[(c >> 1) * mid_stride + (size_t)row * kCt + (size_t)((c & 1) * 2 + limb) * kN + lv] = red[j];
-    }
-}
-
 struct MuxArgs64 {
   const u64 *in;
   u64 *out;
@@ -633,13 +578,6 @@ void launch_k2_ct_ntt(const u64 *in, u64 *out, size_t in_stride, size_t out_stri
                       const Scheme64 &S, hipStream_t st) {
   if (count <= 0 || B <= 0) return;
   k2_ct_ntt<<<dim3(2 * count, B), 1024, 0, st>>>(in, out, in_stride, out_stride, S);
-}
-
-void launch_k2_scan(const u64 *db, const u64 *qntt, u64 *mid, size_t q_stride, size_t mid_stride, size_t N0,
-                    size_t Nrest, int B, const Scheme64 &S, hipStream_t st) {
-  const int ncols = 2 * B, nct = (ncols + kCols - 1) / kCols;
-  const size_t W = (size_t)nct * (kN / 256) * Nrest * 2;
-  k2_scan<<<(unsigned)W, 256, 0, st>>>(db, qntt, mid, q_stride, mid_stride, (int)N0, (int)Nrest, ncols, nct, S);
 }
 
 void launch_k2_mux(const u64 *in, u64 *out, size_t stride, int nslots, int offset, int half, int B, const u64 *sel,

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "opir_device.h"
+#include "opir_k2.h"
 
 namespace opir {
 
@@ -63,17 +64,20 @@ void launch_ntt_rns(const u32 *in, u32 *out, size_t npoly_rns, bool inverse, con
                     hipStream_t st);
 
 // Database geometry: pt = row * N0 + k; rows padded to 16 (nrt tiles), k to
-// 64 (nkb blocks). Device layout: balanced int8 digit planes in MFMA
-// A-fragment order, see opir_scan.hip.
+// 64 (nkb blocks); nlv NTT levels per limb; P balanced int8 digit planes per
+// residue (4 for 29-bit limbs, 8 for 60-bit limbs). Device layout: digit
+// planes in MFMA A-fragment order, see opir_scan.hip.
 struct DbGeom {
   size_t N0 = 0, Nrest = 0, nrt = 0, nkb = 0;
+  int P = 4, nlv = kN;
 };
 struct QfragGeom {
   int N0 = 0, nkb = 0, nct = 0, ncm = 0;  // column tiles of 16, stored columns per tile
+  int P = 4, nlv = kN;
 };
-DbGeom make_db_geom(size_t N0, size_t Nrest);
+DbGeom make_db_geom(size_t N0, size_t Nrest, int P = 4, int nlv = kN);
 size_t db_bytes(const DbGeom &g);
-QfragGeom make_qfrag_geom(size_t N0, int B);
+QfragGeom make_qfrag_geom(size_t N0, int B, int P = 4, int nlv = kN);
 size_t qfrag_bytes(const QfragGeom &q);
 
 // Seeded synthetic plaintexts [count][n]: coefficient c of plaintext i is
@@ -98,10 +102,20 @@ void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel
 // First-dim scan in two launches: launch_qfrag turns the selectors qntt
 // [B][N0][2][2][N] into MFMA B fragments (qfrag_bytes(make_qfrag_geom(N0, B))
 // bytes), launch_scan multiplies the DB by them -> mid [B][Nrest][2][2][N]
-// (NTT form).
+// (NTT form). variant: index into the scan's tile-shape table (-1 = the
+// default for B); returns -1 if that shape cannot take B queries.
 void launch_qfrag(const u32 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st);
-void launch_scan(const int8_t *db, const int8_t *qfrag, u32 *mid, size_t mid_stride, int B, const DbGeom &g,
-                 const SchemeConst &S, hipStream_t st);
+int launch_scan(const int8_t *db, const int8_t *qfrag, u32 *mid, size_t mid_stride, int B, const DbGeom &g,
+                const SchemeConst &S, hipStream_t st, int variant);
+// Number of scan tile shapes for P digit planes (4: K = 1, 8: K = 2).
+int scan_variant_count(int P);
+// The same for the two-prime scheme (P = 8 planes, u64 residues):
+// tmp [limb][count][nlv] u64 -> DB; qntt [B][N0][2][2][nlv] u64 -> qfrag;
+// mid [B][Nrest][2][2][nlv] u64.
+void launch_db_scatter64(const u64 *tmp, int8_t *db, size_t first_pt, size_t count, const DbGeom &g, hipStream_t st);
+void launch_qfrag64(const u64 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st);
+int launch_scan64(const int8_t *db, const int8_t *qfrag, u64 *mid, size_t mid_stride, int B, const DbGeom &g,
+                  const k2::Scheme64 &S, hipStream_t st, int variant);
 // MUX level. ragged: slot s < offset copies in[s]; s >= offset muxes in[off+2j], in[off+2j+1].
 // perfect (offset < 0): slot s muxes in[s], in[s+half].
 void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,

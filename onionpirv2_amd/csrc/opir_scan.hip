@@ -1,17 +1,19 @@
 // First-dimension DB scan on the matrix cores (src/server.cpp:313-361,
-// src/matrix.cpp:232-350):
+// src/matrix.cpp:24-151,232-350):
 //   C[limb][l][i][c] = sum_k A[limb][l][i][k] * Bq[limb][l][k][c] mod q_limb
 // for every NTT level l, candidate row i < Nrest, first-dim column k < N0 and
 // query column c = 2*query + poly.
 //
-// Exactness with int8 MFMA: every residue a < 2^30 is written in balanced
-// base-256 digits a = d0 + 2^8 d1 + 2^16 d2 + 2^24 d3 with d0..d2 in
-// [-128, 128) and d3 in [0, 64]. Then
+// Exactness with int8 MFMA: every residue a < 2^(8P-2) is written in P
+// balanced base-256 digits a = sum_s 256^s d_s, d_0..d_{P-2} in [-128, 128)
+// and a small non-negative top digit. Then
 //   sum_k a_k b_k = sum_{s,t} 256^(s+t) sum_k d_{s,k} e_{t,k},
-// where each inner sum is one int8 GEMM with int32 accumulation. The 16
-// plane pairs accumulate into 7 int32 tiles G_u (u = s+t); |G_u| <= 4 * N0p
-// * 2^14 < 2^31 for N0p <= 2^14. The epilogue forms sum_u (256^u mod q) G_u
-// in int64 (|.| < 2^58) and reduces it mod q. No rounding anywhere.
+// where each inner sum is one int8 GEMM with int32 accumulation. The P^2
+// plane pairs accumulate into 2P-1 int32 tiles G_u (u = s+t); at most P pairs
+// share a u, so |G_u| <= P * N0 * 2^14 < 2^31 for P * N0 <= 2^17. No
+// rounding anywhere. P = 4 covers the 29-bit limbs of CONFIG_N2048_K1_COMP,
+// P = 8 the 60-bit limbs of CONFIG_N4096_K2_MP; both store 1 byte per digit,
+// i.e. exactly the bytes of the u32 / u64 residues.
 //
 // MFMA v_mfma_i32_16x16x64_i8 operand layout (gfx950), lane l, byte j:
 //   A (16 x 64): row = l % 16, k = 16 * (l / 16) + j
@@ -21,51 +23,69 @@
 // Database layout ("fragment-native"): for each (limb, level, row tile rt of
 // 16 rows, k block kb of 64, digit plane s) one 1 KiB block holding the A
 // fragment exactly as the 64 lanes load it (16 bytes per lane), so every
-// wave load is one contiguous 1 KiB line run.
+// wave load is one contiguous 1 KiB run.
+//
+// Two workgroup shapes, both 4 waves = 4 row tiles, one limb, a run of 16
+// levels, results of LV consecutive levels staged in LDS and written as
+// level-contiguous runs of the output ciphertexts:
+//  - direct (HBM-bound, small batches): one column tile, each wave loads its
+//    own A and B fragments, no barrier in the main loop;
+//  - LDS-shared (MFMA-bound, large batches): CB column tiles; per level and k
+//    block the tiles' B fragments are staged once in LDS (double-buffered)
+//    and shared by the 4 waves, each of which multiplies its A fragments
+//    (prefetched one k block ahead) into CB x (2P-1) accumulator tiles.
+// The shape is picked per batch size from a table (launch_scan).
+#include "opir_k2.h"
 #include "opir_kernels.h"
 
 namespace opir {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned __int128 u128;
 
-__device__ __forceinline__ void balanced_digits(u32 a, int (&d)[4]) {
-  int x = (int)a;  // a < 2^30
+namespace {
+
+template <int P>
+__device__ __forceinline__ void balanced_digits(u64 a, int (&d)[P]) {
+  int64_t x = (int64_t)a;  // a < 2^(8P-2)
 #pragma unroll
-  for (int s = 0; s < 3; s++) {
-    const int r = ((x + 128) & 255) - 128;
+  for (int s = 0; s < P - 1; s++) {
+    const int r = (int)(((x + 128) & 255) - 128);
     d[s] = r;
     x = (x - r) >> 8;
   }
-  d[3] = x;
+  d[P - 1] = (int)x;
 }
 
 __device__ __forceinline__ size_t db_byte_offset(const DbGeom &g, int limb, int lv, size_t row, size_t k, int s) {
   const size_t rt = row >> 4, lr = row & 15, kb = k >> 6, grp = (k & 63) >> 4, j = k & 15;
-  const size_t blk = ((((size_t)limb * kN + lv) * g.nrt + rt) * g.nkb + kb) * 4 + s;
+  const size_t blk = ((((size_t)limb * g.nlv + lv) * g.nrt + rt) * g.nkb + kb) * g.P + s;
   return blk * 1024 + (grp * 16 + lr) * 16 + j;
 }
 
+template <int P>
 __device__ __forceinline__ void store_digits(int8_t *db, const DbGeom &g, int limb, int lv, size_t row, size_t k,
-                                             u32 a) {
-  int d[4];
-  balanced_digits(a, d);
+                                             u64 a) {
+  int d[P];
+  balanced_digits<P>(a, d);
 #pragma unroll
-  for (int s = 0; s < 4; s++) db[db_byte_offset(g, limb, lv, row, k, s)] = (int8_t)d[s];
+  for (int s = 0; s < P; s++) db[db_byte_offset(g, limb, lv, row, k, s)] = (int8_t)d[s];
 }
 
-// tmp [limb][count][N] (pt-major, NTT form) -> fragment-native DB.
+// tmp [limb][count][nlv] (pt-major, NTT form) -> fragment-native DB.
 // Tile: 64 pts x 32 levels staged through LDS for coalesced reads.
-__global__ __launch_bounds__(256) void k_db_scatter(const u32 *__restrict__ tmp, int8_t *__restrict__ db,
+template <int P, typename V>
+__global__ __launch_bounds__(256) void k_db_scatter(const V *__restrict__ tmp, int8_t *__restrict__ db,
                                                     size_t first_pt, size_t count, DbGeom g) {
-  __shared__ u32 tile[32][65];
+  __shared__ V tile[32][65];
   const int limb = blockIdx.z;
   const size_t pt0 = (size_t)blockIdx.x * 64;
   const int lv0 = blockIdx.y * 32;
-  const u32 *src = tmp + (size_t)limb * count * kN;
+  const V *src = tmp + (size_t)limb * count * g.nlv;
   for (int e = threadIdx.x; e < 64 * 32; e += 256) {
     const int pl = e / 32, ll = e % 32;
     const size_t pt = pt0 + pl;
-    tile[ll][pl] = pt < count ? src[pt * kN + lv0 + ll] : 0;
+    tile[ll][pl] = pt < count ? src[pt * g.nlv + lv0 + ll] : 0;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < 64 * 32; e += 256) {
@@ -73,7 +93,7 @@ __global__ __launch_bounds__(256) void k_db_scatter(const u32 *__restrict__ tmp,
     const size_t pt = pt0 + pl;
     if (pt >= count) continue;
     const size_t gpt = first_pt + pt;
-    store_digits(db, g, limb, lv0 + ll, gpt / g.N0, gpt % g.N0, tile[ll][pl]);
+    store_digits<P>(db, g, limb, lv0 + ll, gpt / g.N0, gpt % g.N0, (u64)tile[ll][pl]);
   }
 }
 
@@ -81,184 +101,544 @@ __global__ __launch_bounds__(256) void k_db_scatter(const u32 *__restrict__ tmp,
 __global__ void k_db_from_ref(const u32 *__restrict__ ref, int8_t *__restrict__ db, DbGeom g) {
   const size_t num_pt = g.N0 * g.Nrest;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 2 * (size_t)kN * num_pt) return;
-  const size_t pt = i % num_pt, lv = (i / num_pt) % kN, limb = i / (num_pt * kN);
-  store_digits(db, g, (int)limb, (int)lv, pt / g.N0, pt % g.N0, ref[i]);
+  if (i >= 2 * (size_t)g.nlv * num_pt) return;
+  const size_t pt = i % num_pt, lv = (i / num_pt) % g.nlv, limb = i / (num_pt * g.nlv);
+  store_digits<4>(db, g, (int)limb, (int)lv, pt / g.N0, pt % g.N0, ref[i]);
 }
 
-// Query columns -> B fragments. qntt [B][N0][2 polys][2 limbs][N] (NTT form)
-// -> qfrag [limb][level][ct][kb][t][grp][n < ncm][16 bytes], ct = column
+// Query columns -> B fragments. qntt [B][N0][2 polys][2 limbs][nlv] (NTT form)
+// -> qfrag [limb][level][ct][kb][plane][grp][n < ncm][16 bytes], ct = column
 // tile of 16 query columns (c = 2*query + poly), ncm = min(16, 2B).
-// Workgroup: one (query, poly, limb, 16-k block, 64-level block).
-__global__ __launch_bounds__(256) void k_qfrag(const u32 *__restrict__ qntt, int8_t *__restrict__ qfrag,
+// Workgroup: one (query, poly, limb, 16-k block, 64-level block); thread
+// (level ll, plane group) packs 16 k values' digits of its planes.
+template <int P, typename V>
+__global__ __launch_bounds__(256) void k_qfrag(const V *__restrict__ qntt, int8_t *__restrict__ qfrag,
                                                size_t q_stride, QfragGeom qg) {
-  __shared__ u32 tile[16][65];
+  __shared__ V tile[16][65];
   const int col = blockIdx.x;  // 2*query + poly
   const int qi = col >> 1, p = col & 1;
   const int k0 = blockIdx.y * 16;
   const int lv0 = (blockIdx.z >> 1) * 64, limb = blockIdx.z & 1;
-  const u32 *src = qntt + (size_t)qi * q_stride;
+  const V *src = qntt + (size_t)qi * q_stride;
   for (int e = threadIdx.x; e < 16 * 64; e += 256) {
     const int kk = e / 64, ll = e % 64;
     const int k = k0 + kk;
-    tile[kk][ll] = k < qg.N0 ? src[(((size_t)k * 2 + p) * 2 + limb) * kN + lv0 + ll] : 0;
+    tile[kk][ll] = k < qg.N0 ? src[(((size_t)k * 2 + p) * 2 + limb) * qg.nlv + lv0 + ll] : 0;
   }
   __syncthreads();
-  const int ll = threadIdx.x & 63, t = threadIdx.x >> 6;
+  const int ll = threadIdx.x & 63, tq = threadIdx.x >> 6;  // tq: planes tq, tq+4, ...
   const int lv = lv0 + ll;
-  u32 w[4];
-#pragma unroll
-  for (int q4 = 0; q4 < 4; q4++) {
-    u32 packed = 0;
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      int d[4];
-      balanced_digits(tile[q4 * 4 + b][ll], d);
-      packed |= ((u32)(uint8_t)(int8_t)d[t]) << (8 * b);
-    }
-    w[q4] = packed;
-  }
   const int ct = col >> 4, n = col & 15, kb = k0 >> 6, grp = (k0 & 63) >> 4;
-  const size_t off = ((((size_t)limb * kN + lv) * qg.nct + ct) * qg.nkb + kb) * (size_t)(4 * 4 * qg.ncm * 16) +
-                     ((size_t)(t * 4 + grp) * qg.ncm + n) * 16;
-  *reinterpret_cast<uint4 *>(qfrag + off) = make_uint4(w[0], w[1], w[2], w[3]);
+  const size_t base = ((((size_t)limb * qg.nlv + lv) * qg.nct + ct) * qg.nkb + kb) * (size_t)(P * 4 * qg.ncm * 16);
+#pragma unroll
+  for (int pp = 0; pp < P / 4; pp++) {
+    const int s = tq + 4 * pp;
+    u32 w[4];
+#pragma unroll
+    for (int q4 = 0; q4 < 4; q4++) {
+      u32 packed = 0;
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        int d[P];
+        balanced_digits<P>((u64)tile[q4 * 4 + b][ll], d);
+        packed |= ((u32)(uint8_t)(int8_t)d[s]) << (8 * b);
+      }
+      w[q4] = packed;
+    }
+    *reinterpret_cast<uint4 *>(qfrag + base + ((size_t)(s * 4 + grp) * qg.ncm + n) * 16) =
+        make_uint4(w[0], w[1], w[2], w[3]);
+  }
 }
 
-// The scan. Workgroup = 4 waves = 4 row tiles (64 rows) x LV levels x one
-// column tile, one limb. blockIdx.x is remapped so that consecutive logical
-// blocks run on the same XCD: the column tiles of one DB tile, then the row
-// blocks of one level block (which share the query fragments).
-constexpr int kScanLV = 16;
-
-__global__ __launch_bounds__(256) void k_scan_mfma(const int8_t *__restrict__ db, const int8_t *__restrict__ qfrag,
-                                                   u32 *__restrict__ mid, size_t mid_stride, DbGeom g, QfragGeom qg,
-                                                   int nrb, int B, SchemeConst S) {
-  const unsigned W = gridDim.x, p = blockIdx.x;
-  const unsigned L = (p % 8) * (W / 8) + p / 8;  // XCD-aware: consecutive L on one XCD
-  // column tiles innermost: the WGs reading the same DB tile run together on
-  // one XCD, so the tile comes from HBM once and from L2 for the others
-  const int ct = L % qg.nct;
-  unsigned rest = L / qg.nct;
-  const int rb = rest % nrb;
-  rest /= nrb;
-  const int lb = rest % (kN / kScanLV);
-  const int limb = rest / (kN / kScanLV);
-  const LimbConst &Lc = S.limb[limb];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rt = rb * 4 + wave;
-  const int n = lane & 15, grp = lane >> 4;
-  const int ncols_here = min(16, 2 * B - ct * 16);
-  const bool has_col = n < ncols_here;
-  // 256^u mod q and an offset C = q * ceil(2^59 / q) making the int64 sum positive
-  u32 wu[7];
-  {
+// ---------------------------------------------------------------------------
+// Epilogues: sum_u (256^u mod q) G_u  ->  residue mod q
+// ---------------------------------------------------------------------------
+struct EpiK1 {
+  static constexpr int P = 4;
+  typedef u32 Res;
+  u32 wu[2 * P - 1];
+  u64 pos_off;  // q * ceil(2^59 / q): makes the int64 sum positive (|sum| < 2^56)
+  const LimbConst *L;
+  __device__ void init(const SchemeConst &S, int limb) {
+    L = &S.limb[limb];
     u64 x = 1;
 #pragma unroll
-    for (int u = 0; u < 7; u++) {
+    for (int u = 0; u < 2 * P - 1; u++) {
       wu[u] = (u32)x;
-      x = (x << 8) % Lc.q;
+      x = (x << 8) % L->q;
     }
+    pos_off = ((((u64)1 << 59) / L->q) + 1) * L->q;
   }
-  const u64 pos_off = ((((u64)1 << 59) / Lc.q) + 1) * Lc.q;
-  const size_t qtile = (size_t)(4 * 4 * qg.ncm * 16);
-  // per wave: [level][row 16][column 16] results, written out as level runs
-  __shared__ u32 stage_lds[4][kScanLV][16][17];
+  __device__ Res combine(const int (&g)[2 * P - 1]) const {
+    int64_t v = 0;
+#pragma unroll
+    for (int u = 0; u < 2 * P - 1; u++) v += (int64_t)wu[u] * (int64_t)g[u];
+    return reduce_u64((u64)(v + (int64_t)pos_off), *L);
+  }
+};
+
+struct EpiK2 {
+  static constexpr int P = 8;
+  typedef u64 Res;
+  u32 wlo[2 * P - 1], whi[2 * P - 1];  // 256^u mod q split in 32-bit halves
+  u64 off_lo, off_hi;                  // q * ceil(2^92 / q)
+  const k2::Limb64 *L;
+  __device__ void init(const k2::Scheme64 &S, int limb) {
+    L = &S.limb[limb];
+    const u64 q = L->q;
+    u64 x = 1;
+#pragma unroll
+    for (int u = 0; u < 2 * P - 1; u++) {
+      wlo[u] = (u32)x;
+      whi[u] = (u32)(x >> 32);
+      x = (u64)(((u128)x << 8) % q);
+    }
+    const u128 off = ((((u128)1 << 92) / q) + 1) * (u128)q;
+    off_lo = (u64)off;
+    off_hi = (u64)(off >> 64);
+  }
+  __device__ Res combine(const int (&g)[2 * P - 1]) const {
+    // |S_lo| < 15 * 2^32 * 2^26 < 2^63, |S_hi| < 15 * 2^28 * 2^26; total < 2^91
+    int64_t slo = 0, shi = 0;
+#pragma unroll
+    for (int u = 0; u < 2 * P - 1; u++) {
+      slo += (int64_t)wlo[u] * (int64_t)g[u];
+      shi += (int64_t)whi[u] * (int64_t)g[u];
+    }
+    const __int128 tot = (__int128)slo + ((__int128)shi << 32) + (__int128)(((u128)off_hi << 64) | off_lo);
+    const u128 t = (u128)tot;
+    const u64 hi = (u64)(t >> 64), lo = (u64)t;
+    // (hi * 2^64 + lo) mod q, both halves by Shoup
+    const u64 a = hi * L->r64 - __umul64hi(hi, L->r64s) * L->q;
+    const u64 b = lo - __umul64hi(lo, L->one_s) * L->q;
+    u64 r = a + b;
+    r = r >= L->q2 ? r - L->q2 : r;
+    return r >= L->q ? r - L->q : r;
+  }
+};
+
+template <typename Epi>
+struct ScanArgs {
+  const int8_t *db;
+  const int8_t *qfrag;
+  typename Epi::Res *mid;
+  size_t mid_stride;  // values between queries
+  size_t ct_vals;     // values per output ciphertext (4 * nlv)
+  DbGeom g;
+  QfragGeom qg;
+  int nrb, ncb, B;
+};
+
+constexpr int kLevelsPerWG = 16;
+
+// NT: nontemporal A loads, for grids where each DB tile has one reader
+// (one column block); with several, the tile's later readers find it in L2.
+template <int CB, int LV, bool NT, typename Epi, typename SchemeT>
+__global__ __launch_bounds__(256) void k_scan(ScanArgs<Epi> A, SchemeT S) {
+  constexpr int P = Epi::P, NU = 2 * P - 1;
+  constexpr int BCHUNK = CB * P * 1024;  // bytes of one k block's B fragments
+  constexpr int BV = BCHUNK / 16 / 256;  // uint4 per thread to stage them
+  typedef typename Epi::Res Res;
+  __shared__ __align__(16) int8_t bsh[2][BCHUNK];
+  __shared__ Res stage[4][LV][16][CB * 16 + 1];
+  const DbGeom &g = A.g;
+  const QfragGeom &qg = A.qg;
+  const unsigned W = gridDim.x, pblk = blockIdx.x;
+  const unsigned Lg = (W % 8 == 0) ? (pblk % 8) * (W / 8) + pblk / 8 : pblk;  // same-XCD neighbours
+  // column blocks innermost: the workgroups sharing a DB tile run together
+  const int cbk = Lg % A.ncb;
+  unsigned rest = Lg / A.ncb;
+  const int rb = rest % A.nrb;
+  rest /= A.nrb;
+  const int nlvb = g.nlv / kLevelsPerWG;
+  const int lvb = rest % nlvb;
+  const int limb = rest / nlvb;
+  Epi epi;
+  epi.init(S, limb);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rt = rb * 4 + wave;
+  const bool row_ok = rt < (int)g.nrt;
+  const int ct0 = cbk * CB;
+  const size_t qtile = (size_t)(P * 4 * qg.ncm * 16);
+
+  // cooperative load of one k block's B fragments (CB tiles, all 16 columns;
+  // columns a partial last tile does not store are zero)
+  auto load_b = [&](int lv, int kb, uint4 (&r)[BV]) {
+#pragma unroll
+    for (int i = 0; i < BV; i++) {
+      const int e = threadIdx.x + 256 * i;  // 16-byte unit
+      const int cb = e / (P * 64), rem = e % (P * 64);
+      const int s = rem / 64, grp = (rem / 16) % 4, n = rem % 16;
+      const int ct = ct0 + cb;
+      r[i] = make_uint4(0, 0, 0, 0);
+      if (ct < qg.nct && n < qg.ncm)
+        r[i] = *reinterpret_cast<const uint4 *>(A.qfrag +
+                                                ((((size_t)limb * g.nlv + lv) * qg.nct + ct) * qg.nkb + kb) * qtile +
+                                                ((size_t)(s * 4 + grp) * qg.ncm + n) * 16);
+    }
+  };
+  auto store_b = [&](int buf, const uint4 (&r)[BV]) {
+#pragma unroll
+    for (int i = 0; i < BV; i++) reinterpret_cast<uint4 *>(bsh[buf])[threadIdx.x + 256 * i] = r[i];
+  };
+  auto load_a = [&](int lv, int kb, v4i (&a)[P]) {
+    const int8_t *p =
+        A.db + ((((size_t)limb * g.nlv + lv) * g.nrt + rt) * g.nkb + kb) * (size_t)(P * 1024) + lane * 16;
+#pragma unroll
+    for (int s = 0; s < P; s++) {
+      const v4i *q = reinterpret_cast<const v4i *>(p + s * 1024);
+      a[s] = row_ok ? (NT ? __builtin_nontemporal_load(q) : *q) : (v4i){0, 0, 0, 0};
+    }
+  };
+
+  for (int lsub = 0; lsub < kLevelsPerWG; lsub += LV) {
 #pragma unroll 1
-  for (int lvl = 0; lvl < kScanLV; lvl++) {
-    const int lv = lb * kScanLV + lvl;
-    v4i acc[7];
+    for (int li = 0; li < LV; li++) {
+      const int lv = lvb * kLevelsPerWG + lsub + li;
+      v4i acc[CB][NU];
 #pragma unroll
-    for (int u = 0; u < 7; u++) acc[u] = (v4i){0, 0, 0, 0};
-    const int8_t *abase = db + ((((size_t)limb * kN + lv) * g.nrt + rt) * g.nkb) * 4096 + lane * 16;
-    const int8_t *bbase = qfrag + (((size_t)limb * kN + lv) * qg.nct + ct) * qg.nkb * qtile;
-    if (rt < (int)g.nrt) {
+      for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+        for (int u = 0; u < NU; u++) acc[cb][u] = (v4i){0, 0, 0, 0};
+      uint4 breg[BV];
+      v4i a_cur[P], a_nxt[P];
+      load_b(lv, 0, breg);
+      load_a(lv, 0, a_cur);
+      __syncthreads();  // the previous level's readers of bsh[0] are done
+      store_b(0, breg);
+#pragma unroll 1
       for (int kb = 0; kb < (int)g.nkb; kb++) {
-        v4i a[4], b[4];
+        __syncthreads();  // B(kb) is visible to all waves
+        const bool more = kb + 1 < (int)g.nkb;
+        if (more) {
+          load_b(lv, kb + 1, breg);
+          load_a(lv, kb + 1, a_nxt);
+        }
+        const int8_t *bb = bsh[kb & 1];
 #pragma unroll
-        for (int s = 0; s < 4; s++) a[s] = *reinterpret_cast<const v4i *>(abase + (size_t)kb * 4096 + s * 1024);
+        for (int cb = 0; cb < CB; cb++) {
+          v4i bf[P];
 #pragma unroll
-        for (int t = 0; t < 4; t++)
-          b[t] = has_col ? *reinterpret_cast<const v4i *>(bbase + kb * qtile + ((size_t)(t * 4 + grp) * qg.ncm + n) * 16)
-                         : (v4i){0, 0, 0, 0};
+          for (int t = 0; t < P; t++) bf[t] = *reinterpret_cast<const v4i *>(bb + ((cb * P + t) * 64 + lane) * 16);
 #pragma unroll
-        for (int s = 0; s < 4; s++)
+          for (int s = 0; s < P; s++)
 #pragma unroll
-          for (int t = 0; t < 4; t++) acc[s + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[s], b[t], acc[s + t], 0, 0, 0);
+            for (int t = 0; t < P; t++)
+              acc[cb][s + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a_cur[s], bf[t], acc[cb][s + t], 0, 0, 0);
+        }
+        if (more) {
+          // the other buffer: its last readers (k block kb-1) passed this k block's barrier
+          store_b((kb + 1) & 1, breg);
+#pragma unroll
+          for (int s = 0; s < P; s++) a_cur[s] = a_nxt[s];
+        }
       }
+      // epilogue: D row = 4 * (lane / 16) + r, column = lane % 16
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int gs[NU];
+#pragma unroll
+          for (int u = 0; u < NU; u++) gs[u] = acc[cb][u][r];
+          stage[wave][li][(lane >> 4) * 4 + r][cb * 16 + (lane & 15)] = epi.combine(gs);
+        }
     }
+    __syncthreads();
+    // (row, column) pairs of this wave's row tile: LV consecutive levels each
+    const int ncols = min(CB * 16, 2 * A.B - ct0 * 16);
+    for (int e = lane; e < 16 * ncols; e += 64) {
+      const int rr = e / ncols, nn = e % ncols;
+      const size_t row = (size_t)rt * 16 + rr;
+      if (row >= g.Nrest) continue;
+      const int c = ct0 * 16 + nn, qi = c >> 1, pp = c & 1;
+      Res *dst = A.mid + qi * A.mid_stride + row * A.ct_vals + (size_t)(pp * 2 + limb) * g.nlv +
+                 lvb * kLevelsPerWG + lsub;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      int64_t v = 0;
-#pragma unroll
-      for (int u = 0; u < 7; u++) v += (int64_t)wu[u] * (int64_t)acc[u][r];
-      stage_lds[wave][lvl][grp * 4 + r][n] = reduce_u64((u64)(v + (int64_t)pos_off), Lc);
+      for (int li = 0; li < LV; li++) dst[li] = stage[wave][li][rr][nn];
     }
-  }
-  __syncthreads();
-  // (row, column) pairs of this wave's tile, LV consecutive levels each
-  for (int e = lane; e < 16 * ncols_here; e += 64) {
-    const int rr = e / ncols_here, nn = e % ncols_here;
-    const size_t row = (size_t)rt * 16 + rr;
-    if (row >= g.Nrest) continue;
-    const int c = ct * 16 + nn, qi = c >> 1, pp = c & 1;
-    uint4 *dst = reinterpret_cast<uint4 *>(mid + qi * mid_stride + row * kCtWords + (size_t)(pp * 2 + limb) * kN +
-                                           lb * kScanLV);
-#pragma unroll
-    for (int v = 0; v < kScanLV / 4; v++)
-      dst[v] = make_uint4(stage_lds[wave][4 * v][rr][nn], stage_lds[wave][4 * v + 1][rr][nn],
-                          stage_lds[wave][4 * v + 2][rr][nn], stage_lds[wave][4 * v + 3][rr][nn]);
   }
 }
+
+// Direct variant: one column tile per workgroup, no LDS for B and no barrier
+// in the main loop. Each wave streams its own row tile: per level it issues
+// the A fragments of KC k blocks at once (nontemporal when the tile has no
+// other reader: the database then passes through once and should not
+// displace the query fragments in L2), with the tile's B fragments (small,
+// L2-resident), then runs their MFMAs. Lanes
+// past the stored query columns load column 0 and multiply by zero, so every
+// load is unconditional and the compiler's in-order load counter lets the
+// MFMAs of the first k block start before the last load has landed.
+// SC: result columns staged per row (>= stored columns of the tile: 2 for
+// one query). For the HBM-bound small batches.
+template <int KC, int LV, int SC, bool NT, typename Epi, typename SchemeT>
+__global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S) {
+  constexpr int P = Epi::P, NU = 2 * P - 1;
+  typedef typename Epi::Res Res;
+  __shared__ Res stage[4][LV][16][SC + 1];
+  const DbGeom &g = A.g;
+  const QfragGeom &qg = A.qg;
+  const unsigned W = gridDim.x, pblk = blockIdx.x;
+  const unsigned Lg = (W % 8 == 0) ? (pblk % 8) * (W / 8) + pblk / 8 : pblk;
+  const int ct = Lg % A.ncb;
+  unsigned rest = Lg / A.ncb;
+  const int rb = rest % A.nrb;
+  rest /= A.nrb;
+  const int nlvb = g.nlv / kLevelsPerWG;
+  const int lvb = rest % nlvb;
+  const int limb = rest / nlvb;
+  Epi epi;
+  epi.init(S, limb);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rt = rb * 4 + wave;
+  // a row tile past the last one re-reads the last tile; its rows are not stored
+  const size_t rta = rt < (int)g.nrt ? (size_t)rt : g.nrt - 1;
+  const int n = lane & 15, grp = lane >> 4;
+  const bool has_col = n < qg.ncm;
+  int bmask = has_col ? -1 : 0;
+  asm volatile("" : "+v"(bmask));  // opaque: keeps the B loads unconditional
+  const size_t qtile = (size_t)(P * 4 * qg.ncm * 16);
+  const int nkb = (int)g.nkb;
+  const int lv0 = lvb * kLevelsPerWG;
+  const int8_t *abase = A.db + (((size_t)limb * g.nlv + lv0) * g.nrt + rta) * g.nkb * (size_t)(P * 1024) + lane * 16;
+  const size_t astride = (size_t)g.nrt * g.nkb * (P * 1024);  // bytes between levels
+  const int8_t *bbase = A.qfrag + (((size_t)limb * g.nlv + lv0) * qg.nct + ct) * qg.nkb * qtile +
+                        ((size_t)grp * qg.ncm + (has_col ? n : 0)) * 16;
+  const size_t bstride = (size_t)qg.nct * qg.nkb * qtile;
+
+  for (int lsub = 0; lsub < kLevelsPerWG; lsub += LV) {
+#pragma unroll 1
+    for (int li = 0; li < LV; li++) {
+      const int8_t *pa = abase + (size_t)(lsub + li) * astride;
+      const int8_t *pb = bbase + (size_t)(lsub + li) * bstride;
+      v4i acc[NU];
+#pragma unroll
+      for (int u = 0; u < NU; u++) acc[u] = (v4i){0, 0, 0, 0};
+#pragma unroll 1
+      for (int kc = 0; kc < nkb; kc += KC) {
+        v4i a[KC][P], b[KC][P];
+#pragma unroll
+        for (int j = 0; j < KC; j++)
+#pragma unroll
+          for (int s = 0; s < P; s++)
+          {
+            const v4i *q = reinterpret_cast<const v4i *>(pa + (size_t)(kc + j) * (P * 1024) + s * 1024);
+            a[j][s] = NT ? __builtin_nontemporal_load(q) : *q;
+          }
+#pragma unroll
+        for (int j = 0; j < KC; j++)
+#pragma unroll
+          for (int t = 0; t < P; t++)
+            b[j][t] = *reinterpret_cast<const v4i *>(pb + (size_t)(kc + j) * qtile + (size_t)t * 4 * qg.ncm * 16);
+#pragma unroll
+        for (int j = 0; j < KC; j++)
+#pragma unroll
+          for (int s = 0; s < P; s++)
+#pragma unroll
+            for (int t = 0; t < P; t++)
+              acc[s + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[j][s], b[j][t] & bmask, acc[s + t], 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int gs[NU];
+#pragma unroll
+        for (int u = 0; u < NU; u++) gs[u] = acc[u][r];
+        const Res v = epi.combine(gs);
+        if (n < SC) stage[wave][li][grp * 4 + r][n] = v;
+      }
+    }
+    __syncthreads();
+    const int ncols = min(16, 2 * A.B - ct * 16);
+    for (int e = lane; e < 16 * ncols; e += 64) {
+      const int rr = e / ncols, nn = e % ncols;
+      const size_t row = (size_t)rt * 16 + rr;
+      if (row >= g.Nrest) continue;
+      const int c = ct * 16 + nn, qi = c >> 1, pp = c & 1;
+      Res *dst = A.mid + qi * A.mid_stride + row * A.ct_vals + (size_t)(pp * 2 + limb) * g.nlv + lv0 + lsub;
+#pragma unroll
+      for (int li = 0; li < LV; li++) dst[li] = stage[wave][li][rr][nn];
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
 
 static inline unsigned cdivu(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
-DbGeom make_db_geom(size_t N0, size_t Nrest) {
+DbGeom make_db_geom(size_t N0, size_t Nrest, int P, int nlv) {
   DbGeom g;
   g.N0 = N0;
   g.Nrest = Nrest;
   g.nrt = (Nrest + 15) / 16;
   g.nkb = (N0 + 63) / 64;
+  g.P = P;
+  g.nlv = nlv;
   return g;
 }
 
-size_t db_bytes(const DbGeom &g) { return 2 * (size_t)kN * g.nrt * g.nkb * 4096; }
+size_t db_bytes(const DbGeom &g) { return 2 * (size_t)g.nlv * g.nrt * g.nkb * (size_t)g.P * 1024; }
 
-QfragGeom make_qfrag_geom(size_t N0, int B) {
+QfragGeom make_qfrag_geom(size_t N0, int B, int P, int nlv) {
   QfragGeom q;
   q.N0 = (int)N0;
   q.nkb = (int)((N0 + 63) / 64);
   q.nct = (2 * B + 15) / 16;
   q.ncm = 2 * B < 16 ? 2 * B : 16;
+  q.P = P;
+  q.nlv = nlv;
   return q;
 }
 
-size_t qfrag_bytes(const QfragGeom &q) { return 2 * (size_t)kN * q.nct * q.nkb * (size_t)(4 * 4 * q.ncm * 16); }
+size_t qfrag_bytes(const QfragGeom &q) { return 2 * (size_t)q.nlv * q.nct * q.nkb * (size_t)(q.P * 4 * q.ncm * 16); }
 
 void launch_db_scatter(const u32 *tmp, int8_t *db, size_t first_pt, size_t count, const DbGeom &g, hipStream_t st) {
   if (!count) return;
-  dim3 grid(cdivu(count, 64), kN / 32, 2);
-  k_db_scatter<<<grid, 256, 0, st>>>(tmp, db, first_pt, count, g);
+  dim3 grid(cdivu(count, 64), g.nlv / 32, 2);
+  k_db_scatter<4, u32><<<grid, 256, 0, st>>>(tmp, db, first_pt, count, g);
+}
+
+void launch_db_scatter64(const u64 *tmp, int8_t *db, size_t first_pt, size_t count, const DbGeom &g,
+                         hipStream_t st) {
+  if (!count) return;
+  dim3 grid(cdivu(count, 64), g.nlv / 32, 2);
+  k_db_scatter<8, u64><<<grid, 256, 0, st>>>(tmp, db, first_pt, count, g);
 }
 
 void launch_db_from_ref(const u32 *ref, int8_t *db, const DbGeom &g, hipStream_t st) {
-  const size_t total = 2 * (size_t)kN * g.N0 * g.Nrest;
+  const size_t total = 2 * (size_t)g.nlv * g.N0 * g.Nrest;
   k_db_from_ref<<<cdivu(total, 256), 256, 0, st>>>(ref, db, g);
 }
 
 void launch_qfrag(const u32 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st) {
-  const QfragGeom qg = make_qfrag_geom(g.N0, B);
-  // k_qfrag writes every stored column and every k block (zeros for k >= N0);
-  // the scan never reads the unstored columns of a partial last tile.
-  dim3 pg(2 * B, cdivu(qg.nkb * 64, 16), (kN / 64) * 2);
-  k_qfrag<<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg);
+  const QfragGeom qg = make_qfrag_geom(g.N0, B, 4, g.nlv);
+  // every stored column and every k block is written (zeros for k >= N0)
+  dim3 pg(2 * B, cdivu(qg.nkb * 64, 16), (g.nlv / 64) * 2);
+  k_qfrag<4, u32><<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg);
 }
 
-void launch_scan(const int8_t *db, const int8_t *qfrag, u32 *mid, size_t mid_stride, int B, const DbGeom &g,
-                 const SchemeConst &S, hipStream_t st) {
-  const QfragGeom qg = make_qfrag_geom(g.N0, B);
-  const int nrb = (int)cdivu(g.nrt, 4);
-  const size_t W = (size_t)2 * (kN / kScanLV) * qg.nct * nrb;
-  k_scan_mfma<<<(unsigned)W, 256, 0, st>>>(db, qfrag, mid, mid_stride, g, qg, nrb, B, S);
+void launch_qfrag64(const u64 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st) {
+  const QfragGeom qg = make_qfrag_geom(g.N0, B, 8, g.nlv);
+  dim3 pg(2 * B, cdivu(qg.nkb * 64, 16), (g.nlv / 64) * 2);
+  k_qfrag<8, u64><<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg);
+}
+
+// Scan variants: (kind, column tiles per workgroup CB, k blocks per load
+// batch KC, levels staged per flush LV, staged columns SC). A direct variant
+// takes one column tile per workgroup, needs nkb % KC == 0 and stages
+// min(16, 2B) <= SC columns.
+struct ScanShape {
+  bool direct;
+  int CB, KC, LV, SC;
+};
+
+template <typename Epi>
+static ScanArgs<Epi> scan_args(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride,
+                               int B, const DbGeom &g, int CB) {
+  ScanArgs<Epi> A;
+  A.db = db; A.qfrag = qfrag; A.mid = mid; A.mid_stride = mid_stride; A.ct_vals = 4 * (size_t)g.nlv;
+  A.g = g;
+  A.qg = make_qfrag_geom(g.N0, B, Epi::P, g.nlv);
+  A.nrb = (int)cdivu(g.nrt, 4);
+  A.ncb = (int)cdivu(A.qg.nct, CB);
+  A.B = B;
+  return A;
+}
+
+template <typename Epi>
+static unsigned scan_grid(const ScanArgs<Epi> &A) {
+  return (unsigned)((size_t)2 * (A.g.nlv / kLevelsPerWG) * A.nrb * A.ncb);
+}
+
+template <int CB, int LV, typename Epi, typename SchemeT>
+static void run_lds(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
+                    const DbGeom &g, const SchemeT &S, hipStream_t st) {
+  const ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, CB);
+  if (A.ncb == 1)
+    k_scan<CB, LV, true, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+  else
+    k_scan<CB, LV, false, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+}
+
+template <int KC, int LV, int SC, typename Epi, typename SchemeT>
+static void run_direct(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
+                       const DbGeom &g, const SchemeT &S, hipStream_t st) {
+  const ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, 1);
+  if (A.ncb == 1)
+    k_scan_direct<KC, LV, SC, true, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+  else
+    k_scan_direct<KC, LV, SC, false, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+}
+
+template <typename Epi, typename SchemeT>
+using ScanFn = void (*)(const int8_t *, const int8_t *, typename Epi::Res *, size_t, int, const DbGeom &,
+                        const SchemeT &, hipStream_t);
+
+template <typename Epi, typename SchemeT>
+struct ScanEntry {
+  ScanShape shape;
+  ScanFn<Epi, SchemeT> fn;
+};
+
+typedef k2::Scheme64 S64;
+static const ScanEntry<EpiK1, SchemeConst> kScanK1[] = {
+    {{true, 1, 4, 16, 2}, run_direct<4, 16, 2, EpiK1, SchemeConst>},
+    {{true, 1, 4, 16, 16}, run_direct<4, 16, 16, EpiK1, SchemeConst>},
+    {{true, 1, 2, 16, 16}, run_direct<2, 16, 16, EpiK1, SchemeConst>},
+    {{true, 1, 1, 16, 16}, run_direct<1, 16, 16, EpiK1, SchemeConst>},
+    {{false, 1, 1, 4, 16}, run_lds<1, 4, EpiK1, SchemeConst>},
+    {{false, 2, 1, 4, 32}, run_lds<2, 4, EpiK1, SchemeConst>},
+    {{false, 2, 1, 2, 32}, run_lds<2, 2, EpiK1, SchemeConst>},
+    {{false, 4, 1, 2, 64}, run_lds<4, 2, EpiK1, SchemeConst>},
+};
+
+static const ScanEntry<EpiK2, S64> kScanK2[] = {
+    {{true, 1, 2, 16, 2}, run_direct<2, 16, 2, EpiK2, S64>},
+    {{true, 1, 1, 16, 2}, run_direct<1, 16, 2, EpiK2, S64>},
+    {{true, 1, 2, 8, 16}, run_direct<2, 8, 16, EpiK2, S64>},
+    {{true, 1, 1, 8, 16}, run_direct<1, 8, 16, EpiK2, S64>},
+    {{false, 1, 1, 2, 16}, run_lds<1, 2, EpiK2, S64>},
+    {{false, 2, 1, 2, 32}, run_lds<2, 2, EpiK2, S64>},
+    {{false, 2, 1, 1, 32}, run_lds<2, 1, EpiK2, S64>},
+};
+
+int scan_variant_count(int P) { return P == 4 ? (int)std::size(kScanK1) : (int)std::size(kScanK2); }
+
+static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
+  if (!s.direct) return true;
+  return g.nkb % s.KC == 0 && (2 * B < 16 ? 2 * B : 16) <= s.SC;
+}
+
+// Default shapes per batch size, in order of preference (measured on MI355X
+// at the 256 x 1024 and 512 x 4096 layouts): direct streaming while the scan
+// is HBM-bound, B fragments shared through LDS once it is MFMA-bound.
+static const int kPrefK1[3][4] = {{3, 0, 4, -1}, {3, 2, 4, -1}, {5, 4, -1, -1}};
+static const int kPrefK2[3][4] = {{1, 3, 4, -1}, {3, 4, -1, -1}, {4, 5, -1, -1}};
+
+template <typename Epi, typename SchemeT, size_t NV>
+static int dispatch(const ScanEntry<Epi, SchemeT> (&tab)[NV], const int (&pref)[3][4], int variant,
+                    const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
+                    const DbGeom &g, const SchemeT &S, hipStream_t st) {
+  if (variant < 0) {
+    const int cls = 2 * B <= 2 ? 0 : (2 * B <= 16 ? 1 : 2);
+    for (int i = 0; i < 4 && variant < 0; i++) {
+      const int v = pref[cls][i];
+      if (v >= 0 && applicable(tab[v].shape, B, g)) variant = v;
+    }
+  }
+  if (variant < 0 || variant >= (int)NV || !applicable(tab[variant].shape, B, g)) return -1;
+  tab[variant].fn(db, qfrag, mid, mid_stride, B, g, S, st);
+  return 0;
+}
+
+int launch_scan(const int8_t *db, const int8_t *qfrag, u32 *mid, size_t mid_stride, int B, const DbGeom &g,
+                const SchemeConst &S, hipStream_t st, int variant) {
+  return dispatch(kScanK1, kPrefK1, variant, db, qfrag, mid, mid_stride, B, g, S, st);
+}
+
+int launch_scan64(const int8_t *db, const int8_t *qfrag, u64 *mid, size_t mid_stride, int B, const DbGeom &g,
+                  const k2::Scheme64 &S, hipStream_t st, int variant) {
+  return dispatch(kScanK2, kPrefK2, variant, db, qfrag, mid, mid_stride, B, g, S, st);
 }
 
 }  // namespace opir

@@ -24,7 +24,7 @@ EXPORTED = [
     "opir_client_set_keys", "opir_client_remove",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes",
-    "opir_ctx_set_profiling", "opir_last_stage_ms",
+    "opir_ctx_set_profiling", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_scan_variant_count",
     "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
     "opir_ext_prod", "opir_mod_switch",
 ]
@@ -89,6 +89,8 @@ def load_library():
         "opir_client_key_bytes": ([P, sz, ctypes.POINTER(sz)], ctypes.c_int),
         "opir_ctx_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
         "opir_last_stage_ms": ([P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "opir_ctx_set_scan_variant": ([P, ctypes.c_int], ctypes.c_int),
+        "opir_scan_variant_count": ([P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "opir_ntt_fwd": ([P, P, sz], ctypes.c_int),
         "opir_ntt_inv": ([P, P, sz], ctypes.c_int),
         "opir_expand": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
@@ -165,6 +167,15 @@ class GpuContext:
 
     def set_profiling(self, on=True):
         self.check(self.lib.opir_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def set_scan_variant(self, variant=-1):
+        """Force a first-dimension scan tile shape (-1: default per batch size)."""
+        self.check(self.lib.opir_ctx_set_scan_variant(self.h, int(variant)))
+
+    def scan_variant_count(self):
+        out = ctypes.c_int(0)
+        self.check(self.lib.opir_scan_variant_count(self.h, ctypes.byref(out)))
+        return out.value
 
     def last_stage_ms(self):
         out = (ctypes.c_float * 6)()

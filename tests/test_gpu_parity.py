@@ -188,3 +188,46 @@ def test_errors_map_to_reference_exceptions(comp_params, gpu_ctx):
         db.answer(987654, bad)
     with pytest.raises(InvalidArgument):
         db.load_plain(np.zeros((p.num_pt + 1, p.N), np.uint16))
+
+
+def _each_scan_shape(ctx, run):
+    """run() under every first-dimension scan tile shape; shapes that cannot
+    take the batch raise InvalidArgument. Returns {shape: result}."""
+    from onionpirv2_amd.gpu import InvalidArgument
+    out = {}
+    try:
+        for v in range(ctx.scan_variant_count()):
+            ctx.set_scan_variant(v)
+            try:
+                out[v] = run()
+            except InvalidArgument:
+                continue
+    finally:
+        ctx.set_scan_variant(-1)
+    return out
+
+
+@pytest.mark.parametrize("layout", LAYOUTS[:4], ids=lambda l: "x".join(map(str, l)))
+def test_every_scan_shape_matches_oracle(O, comp_params, gpu_ctx, layout):
+    """Every tile shape of the first-dimension scan gives the oracle's
+    evaluate_first_dim output (one query), and the same 9-query batch answers
+    (18 query columns: one full and one partial column tile)."""
+    say(f"[gpu] scan shapes layout={layout}")
+    s = Setup(O, comp_params, layout, seed=3000 + layout[0], gpu_ctx=gpu_ctx)
+    p = s.p
+    query = s.client.fast_generate_query(p, s.indices[-1])
+    fq = s.server.fast_expand_qry(s.cid, query)[: p.fst_dim_sz]
+    mid_o = s.server.evaluate_first_dim(fq)
+    mids = _each_scan_shape(gpu_ctx, lambda: s.db.first_dim(fq))
+    assert len(mids) >= 5
+    for v, m in mids.items():
+        assert np.array_equal(m, mid_o), f"scan shape {v}"
+    idxs = [(s.indices[-1] + 13 * j) % p.target_num_pt for j in range(9)]
+    qs = np.stack([s.client.fast_generate_query(p, i) for i in idxs])
+    ids = [s.cid] * len(idxs)
+    want = s.db.answer_batch(ids, qs)
+    assert np.array_equal(want[0], s.server.make_query(s.cid, qs[0]))
+    got = _each_scan_shape(gpu_ctx, lambda: s.db.answer_batch(ids, qs))
+    assert len(got) >= 4
+    for v, b in got.items():
+        assert np.array_equal(b, want), f"scan shape {v}"

@@ -180,3 +180,29 @@ def test_k2_batch_equals_single(O, k2p, ctx2):
     for j in range(len(qs)):
         assert np.array_equal(batch[j], s.db.answer(ids[j], qs[j]))
     assert np.array_equal(batch[0], s.server.make_query(31, qs[0]))
+
+
+@pytest.mark.parametrize("layout", LAYOUTS[:4], ids=lambda l: "x".join(map(str, l)))
+def test_k2_every_scan_shape_matches_oracle(O, k2p, ctx2, layout):
+    """Every tile shape of the two-prime scan gives the oracle's first
+    dimension (one query) and the same 9-query batch answers."""
+    from test_gpu_parity import _each_scan_shape
+    say(f"[gpu] K=2 scan shapes layout={layout}")
+    s = Setup(O, k2p, layout, seed=4000 + layout[0], ctx=ctx2, cid=41)
+    p = s.p
+    query = s.client.fast_generate_query(p, s.indices[-1])
+    fq = s.server.fast_expand_qry(s.cid, query)[: p.fst_dim_sz]
+    mid_o = s.server.evaluate_first_dim(fq)
+    mids = _each_scan_shape(ctx2, lambda: s.db.first_dim(fq))
+    assert len(mids) >= 5
+    for v, m in mids.items():
+        assert np.array_equal(m, mid_o), f"scan shape {v}"
+    idxs = [(s.indices[-1] + 13 * j) % p.target_num_pt for j in range(9)]
+    qs = np.stack([s.client.fast_generate_query(p, i) for i in idxs])
+    ids = [s.cid] * len(idxs)
+    want = s.db.answer_batch(ids, qs)
+    assert np.array_equal(want[0], s.server.make_query(s.cid, qs[0]))
+    got = _each_scan_shape(ctx2, lambda: s.db.answer_batch(ids, qs))
+    assert len(got) >= 4
+    for v, b in got.items():
+        assert np.array_equal(b, want), f"scan shape {v}"
