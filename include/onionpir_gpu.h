@@ -133,6 +133,8 @@ int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out);
 /* Per-stage device time of the last answer call under the reference's stage
  * names (src/includes/logging.h:122-134): [0] Expand, [1] Convert,
  * [2] First dim, [3] Other dim, [4] Modulus switching, [5] scan kernel only.
+ * Convert (selector completion) runs on a second stream concurrently with
+ * First dim; both are timed from the end of Expand.
  * Requires opir_ctx_set_profiling(ctx, 1) before the answer call. */
 int opir_ctx_set_profiling(opir_ctx *ctx, int on);
 int opir_last_stage_ms(opir_ctx *ctx, float *out6);

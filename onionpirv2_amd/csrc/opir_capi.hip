@@ -24,14 +24,15 @@ size_t ws_bytes(const opir_db *db, size_t B) {
   const size_t ct = kCtWords * sizeof(u32);
   const size_t nsel = db->dims - 1;
   return qfrag_bytes(make_qfrag_geom(db->N0, (int)B)) + 2 * B * db->exp_cts * ct + B * db->N0 * ct +
-         B * (nsel ? nsel : 1) * 2 * db->ctx->G.l_ep * ct + 2 * B * db->Nrest * ct + 2 * B * 2 * kN * sizeof(u64);
+         B * (nsel ? nsel : 1) * 2 * db->ctx->G.l_ep * ct + 2 * B * db->Nrest * ct + 2 * B * 2 * kN * sizeof(u64) +
+         kMuxSplitMaxBlocks * ct;
 }
 
 int ensure_ws(opir_db *db, size_t B) {
   opir_ctx *ctx = db->ctx;
   if (db->ws_B >= B) return OPIR_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // queued work may still use the old buffers
-  u32 **bufs[] = {&db->exp_a, &db->exp_b, &db->qntt, &db->sel, &db->mid_a, &db->mid_b};
+  u32 **bufs[] = {&db->exp_a, &db->exp_b, &db->qntt, &db->sel, &db->mid_a, &db->mid_b, &db->part};
   for (auto b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -51,14 +52,15 @@ int ensure_ws(opir_db *db, size_t B) {
   HIP_TRY(ctx, hipMalloc((void **)&db->sel, B * (nsel ? nsel : 1) * 2 * ctx->G.l_ep * ct));
   HIP_TRY(ctx, hipMalloc((void **)&db->mid_a, B * db->Nrest * ct));
   HIP_TRY(ctx, hipMalloc((void **)&db->mid_b, B * db->Nrest * ct));
+  HIP_TRY(ctx, hipMalloc((void **)&db->part, kMuxSplitMaxBlocks * ct));
   HIP_TRY(ctx, hipMalloc((void **)&db->q_u64, B * 2 * kN * sizeof(u64)));
   HIP_TRY(ctx, hipMalloc((void **)&db->r_u64, B * 2 * kN * sizeof(u64)));
   db->ws_B = B;
   return OPIR_OK;
 }
 
-void rec(opir_ctx *ctx, int i) {
-  if (ctx->profiling) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+void rec(opir_ctx *ctx, int i, bool aux = false) {
+  if (ctx->profiling) (void)hipEventRecord(ctx->ev[i], aux ? ctx->aux_stream : ctx->stream);
 }
 
 // Expansion (server.cpp:731-816) for B queries whose root cts are already in
@@ -69,7 +71,8 @@ u32 *run_expand(opir_db *db, size_t B, const BatchPtrs &P) {
   u32 *in = db->exp_a, *out = db->exp_b;
   for (size_t L = 0; L < db->h; L++) {
     const size_t n = nodes_at_level(db->u, db->h, L);
-    launch_expand_level(in, out, stride, stride, (int)n, (int)B, (int)L, P, ctx->G, ctx->S, ctx->stream);
+    launch_expand_level(in, out, stride, stride, (int)n, (int)B, (int)L, P, ctx->G, ctx->S, ctx->stream,
+                        db->part);
     std::swap(in, out);
   }
   return in;
@@ -88,13 +91,20 @@ u32 *run_other_dim(opir_db *db, size_t B, const u32 *sel) {
   const size_t offset = db->Nrest - last;
   const size_t perfect = size_t{1} << (hs - 1);
   u32 *in = db->mid_a, *out = db->mid_b;
-  launch_mux(in, out, stride, (int)perfect, (int)offset, 0, (int)B, sel, sel_stride, ctx->G, ctx->S, ctx->stream);
-  std::swap(in, out);
+  // levels with few nodes spread each node over several workgroups
+  auto level = [&](int nslots, int off, int half, const u32 *s) {
+    const int Gs = mux_split_groups(nslots, off, (int)B, ctx->G);
+    if (Gs)
+      launch_mux_split(in, out, stride, nslots, off, half, (int)B, s, sel_stride, db->part, Gs, ctx->G, ctx->S,
+                       ctx->stream);
+    else
+      launch_mux(in, out, stride, nslots, off, half, (int)B, s, sel_stride, ctx->G, ctx->S, ctx->stream);
+    std::swap(in, out);
+  };
+  level((int)perfect, (int)offset, 0, sel);
   for (size_t a = 1; a < hs; a++) {
     const size_t half = (size_t{1} << (hs - a)) >> 1;
-    launch_mux(in, out, stride, (int)half, -1, (int)half, (int)B, sel + a * sel_words, sel_stride, ctx->G, ctx->S,
-               ctx->stream);
-    std::swap(in, out);
+    level((int)half, -1, (int)half, sel + a * sel_words);
   }
   return in;
 }
@@ -121,12 +131,14 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   u32 *leaves = run_expand(db, B, P);
   LAUNCH_CHECK(ctx);
   rec(ctx, 1);
+  // selector completion on the auxiliary stream, concurrent with the scan
+  fork_aux(ctx);
   const size_t nsel = db->dims - 1;
   const size_t sel_stride = nsel * 2 * (size_t)ctx->G.l_ep * kCtWords;
   launch_complete(leaves + db->N0 * kCtWords, db->sel, exp_stride, sel_stride, (int)(nsel * ctx->G.l_ep), (int)B, P,
-                  ctx->G, ctx->S, ctx->stream);
+                  ctx->G, ctx->S, ctx->aux_stream);
   LAUNCH_CHECK(ctx);
-  rec(ctx, 2);
+  rec(ctx, 2, true);
   launch_ct_ntt(leaves, db->qntt, exp_stride, db->N0 * kCtWords, (int)db->N0, (int)B, ctx->S, ctx->stream);
   launch_qfrag(db->qntt, db->qfrag, db->N0 * kCtWords, (int)B, db->geom, ctx->stream);
   rec(ctx, 5);
@@ -136,6 +148,7 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   LAUNCH_CHECK(ctx);
   rec(ctx, 6);
   rec(ctx, 3);
+  join_aux(ctx);
   u32 *res = run_other_dim(db, B, db->sel);
   LAUNCH_CHECK(ctx);
   rec(ctx, 4);
@@ -166,8 +179,12 @@ static int ctx_create_k2(const opir_scheme *sch, int device, opir_ctx **out) {
   if (hipSetDevice(device) != hipSuccess) return fail(OPIR_EHIP, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) return fail(OPIR_EHIP, "stream create");
   ctx->stream = ctx->own_stream;
+  if (hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking) != hipSuccess) return fail(OPIR_EHIP, "stream create");
   for (auto &e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail(OPIR_EHIP, "event create");
+  if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail(OPIR_EHIP, "event create");
   const int rc = k2_ctx_init(ctx, sch);
   if (rc) {
     const std::string m = g_create_error;
@@ -217,8 +234,12 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   if (hipSetDevice(device) != hipSuccess) return fail("hipSetDevice failed");
   if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) return fail("stream create");
   ctx->stream = ctx->own_stream;
+  if (hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking) != hipSuccess) return fail("stream create");
   for (auto &e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail("event create");
+  if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail("event create");
   // twiddle tables per limb
   std::vector<u32> tab(2 * 4 * kN);
   const u64 Q = q1 * q2;
@@ -277,6 +298,7 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
 void opir_ctx_destroy(opir_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
   for (auto &kv : ctx->clients) {
     (void)hipFree(kv.second.bv);
     (void)hipFree(kv.second.gsw);
@@ -291,6 +313,9 @@ void opir_ctx_destroy(opir_ctx *ctx) {
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->d_tables) (void)hipFree(ctx->d_tables);
   k2_ctx_free(ctx);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
 }
@@ -333,8 +358,8 @@ int opir_last_stage_ms(opir_ctx *ctx, float *o) {
   if (!ctx->profiling) return set_err(ctx, OPIR_EINVAL, "profiling is off");
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   HIP_TRY(ctx, hipEventElapsedTime(&o[0], ctx->ev[0], ctx->ev[1]));  // Expand
-  HIP_TRY(ctx, hipEventElapsedTime(&o[1], ctx->ev[1], ctx->ev[2]));  // Convert
-  HIP_TRY(ctx, hipEventElapsedTime(&o[2], ctx->ev[2], ctx->ev[3]));  // First dim (query NTT + scan)
+  HIP_TRY(ctx, hipEventElapsedTime(&o[1], ctx->ev[1], ctx->ev[2]));  // Convert (auxiliary stream)
+  HIP_TRY(ctx, hipEventElapsedTime(&o[2], ctx->ev[1], ctx->ev[3]));  // First dim (query NTT + scan), concurrent with Convert
   HIP_TRY(ctx, hipEventElapsedTime(&o[3], ctx->ev[3], ctx->ev[4]));  // Other dim
   HIP_TRY(ctx, hipEventElapsedTime(&o[4], ctx->ev[4], ctx->ev[7]));  // Modulus switching
   HIP_TRY(ctx, hipEventElapsedTime(&o[5], ctx->ev[5], ctx->ev[6]));  // scan kernel
@@ -386,7 +411,7 @@ void opir_db_destroy(opir_db *db) {
   if (!db) return;
   (void)hipStreamSynchronize(db->ctx->stream);
   k2_db_free(db);
-  for (u32 *p : {db->exp_a, db->exp_b, db->qntt, db->sel, db->mid_a, db->mid_b})
+  for (u32 *p : {db->exp_a, db->exp_b, db->qntt, db->sel, db->mid_a, db->mid_b, db->part})
     if (p) (void)hipFree(p);
   if (db->d_db) (void)hipFree(db->d_db);
   if (db->qfrag) (void)hipFree(db->qfrag);

@@ -46,6 +46,10 @@ struct opir_ctx {
   bool profiling = false;
   int scan_variant = -1;    // first-dimension scan tile shape, -1 = default per batch size
   hipEvent_t ev[8] = {};
+  // auxiliary stream (default priority): selector completion runs on it
+  // concurrently with the first-dimension scan
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // pointer-table ring
   const void **pin_ptrs[capi::kRing] = {};
   const void **dev_ptrs[capi::kRing] = {};
@@ -65,6 +69,7 @@ struct opir_db {
   // workspace for up to ws_B queries (K = 1 u32 RNS cts / K = 2 u64 cts)
   size_t ws_B = 0;
   u32 *exp_a = nullptr, *exp_b = nullptr, *qntt = nullptr, *sel = nullptr, *mid_a = nullptr, *mid_b = nullptr;
+  u32 *part = nullptr;  // partial external products of split tree levels (kMuxSplitMaxBlocks cts)
   int8_t *qfrag = nullptr;
   u64 *e64_a = nullptr, *e64_b = nullptr, *q64 = nullptr, *sel64 = nullptr, *m64_a = nullptr, *m64_b = nullptr;
   u64 *q_u64 = nullptr, *r_u64 = nullptr;
@@ -92,6 +97,17 @@ inline int set_err(opir_ctx *ctx, int code, const std::string &msg) {
     hipError_t e_ = hipGetLastError();                                                             \
     if (e_ != hipSuccess) return capi::set_err((ctx), OPIR_EHIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
   } while (0)
+
+// Fork: work queued on ctx->aux_stream after this runs after everything
+// queued on ctx->stream so far. Join: ctx->stream waits for the aux work.
+inline void fork_aux(opir_ctx *ctx) {
+  (void)hipEventRecord(ctx->ev_fork, ctx->stream);
+  (void)hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0);
+}
+inline void join_aux(opir_ctx *ctx) {
+  (void)hipEventRecord(ctx->ev_join, ctx->aux_stream);
+  (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+}
 
 inline u64 mulmod_h(u64 a, u64 b, u64 m) { return (u64)((u128)a * b % m); }
 inline u64 powmod_h(u64 b, u64 e, u64 m) {

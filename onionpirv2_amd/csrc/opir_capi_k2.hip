@@ -40,8 +40,8 @@ int upload_k2(opir_ctx *ctx, const uint64_t *ids, size_t B, k2::KeyPtrs64 &P) {
   return OPIR_OK;
 }
 
-void rec(opir_ctx *ctx, int i) {
-  if (ctx->profiling) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+void rec(opir_ctx *ctx, int i, bool aux = false) {
+  if (ctx->profiling) (void)hipEventRecord(ctx->ev[i], aux ? ctx->aux_stream : ctx->stream);
 }
 
 // Expansion levels over e64_a / e64_b; returns the buffer holding the leaves.
@@ -292,12 +292,14 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
   u64 *leaves = run_expand(db, B, P);
   LAUNCH_CHECK(ctx);
   rec(ctx, 1);
+  // selector completion on the auxiliary stream, concurrent with the scan
+  fork_aux(ctx);
   const size_t nsel = db->dims - 1;
   const size_t sel_stride = nsel * 2 * (size_t)ctx->G64.l_ep * kCt2;
   k2::launch_k2_complete(leaves + db->N0 * kCt2, db->sel64, exp_stride, sel_stride, (int)(nsel * ctx->G64.l_ep),
-                         (int)B, P, ctx->G64, ctx->S64, ctx->stream);
+                         (int)B, P, ctx->G64, ctx->S64, ctx->aux_stream);
   LAUNCH_CHECK(ctx);
-  rec(ctx, 2);
+  rec(ctx, 2, true);
   k2::launch_k2_ct_ntt(leaves, db->q64, exp_stride, db->N0 * kCt2, (int)db->N0, (int)B, ctx->S64, ctx->stream);
   launch_qfrag64(db->q64, db->qfrag, db->N0 * kCt2, (int)B, db->geom, ctx->stream);
   rec(ctx, 5);
@@ -307,6 +309,7 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
   LAUNCH_CHECK(ctx);
   rec(ctx, 6);
   rec(ctx, 3);
+  join_aux(ctx);
   u64 *res = run_other_dim(db, B, db->sel64);
   LAUNCH_CHECK(ctx);
   rec(ctx, 4);

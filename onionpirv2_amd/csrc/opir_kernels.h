@@ -90,9 +90,15 @@ void launch_db_ntt_plain(const uint16_t *pts, u32 *tmp, size_t count, const Sche
 void launch_db_scatter(const u32 *tmp, int8_t *db, size_t first_pt, size_t count, const DbGeom &g, hipStream_t st);
 void launch_db_from_ref(const u32 *ref, int8_t *db, const DbGeom &g, hipStream_t st);
 
+// Ciphertext blocks of the scratch for split (narrow) tree levels.
+constexpr size_t kMuxSplitMaxBlocks = 1024;
+
 // Expansion level: nodes [0, n) of input buffer -> children 2r, 2r+1 of output buffer.
+// part (kMuxSplitMaxBlocks cts, or null): scratch for spreading a narrow
+// level's digits over several workgroups.
 void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int nnodes, int B,
-                         int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st);
+                         int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st,
+                         u32 *part);
 // Forward NTT of ciphertexts (both polys): in coef RNS cts -> out NTT RNS cts.
 void launch_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int count, int B,
                    const SchemeConst &S, hipStream_t st);
@@ -120,6 +126,14 @@ int launch_scan64(const int8_t *db, const int8_t *qfrag, u64 *mid, size_t mid_st
 // perfect (offset < 0): slot s muxes in[s], in[s+half].
 void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
                 const u32 *sel, size_t sel_stride, const GadgetCfg &G, const SchemeConst &S, hipStream_t st);
+// The same level with each node's digits spread over 2G workgroups
+// (partials in part, kMuxSplitMaxBlocks ciphertexts), then combined; for
+// levels with few nodes. mux_split_groups returns G, or 0 when the level is
+// too wide for the split (then launch_mux).
+int mux_split_groups(int nslots, int offset, int B, const GadgetCfg &G);
+void launch_mux_split(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
+                      const u32 *sel, size_t sel_stride, u32 *part, int Gs, const GadgetCfg &G, const SchemeConst &S,
+                      hipStream_t st);
 // INTT + CRT + centred rescale to small_q: res [B][ct] NTT form -> out [B][2][N] u64.
 void launch_final(const u32 *res, u64 *out, size_t res_stride, int B, const SchemeConst &S, hipStream_t st);
 // NTT-form RNS ct -> coefficient-form u64 mod Q ct (test entry points).

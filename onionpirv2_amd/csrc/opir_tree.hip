@@ -24,6 +24,14 @@ struct Barrier {
   __device__ void operator()() const { __syncthreads(); }
 };
 
+// Digit groups of a split level: one digit per workgroup, at most
+// kSplitMaxG groups (then ceil(l / kSplitMaxG) digits each).
+constexpr int kSplitMaxG = 8;
+inline int split_groups(int l) {
+  const int dg = (l + kSplitMaxG - 1) / kSplitMaxG;
+  return (l + dg - 1) / dg;
+}
+
 // Digit i of the balanced base-2^b expansion of D, as a residue mod q:
 // sext_b(floor((D + H_i) / 2^(b*i))), H_i = 2^(b-1) * sum_{j<i} 2^(b*j).
 __device__ __forceinline__ i64 digit_offset(int i, int b) {
@@ -174,6 +182,119 @@ __global__ __launch_bounds__(1024) void k_expand_level(ExpandArgs A, SchemeConst
 }
 
 // ---------------------------------------------------------------------------
+// Split expansion level for levels with few nodes: the node's l_ks digits are
+// spread over G workgroups of two lane blocks (both limbs), each writing its
+// partial (delta_b, delta_a) sums (NTT form, reduced mod q) to part block
+// qi * (nnodes * G) + node * G + g; k_expand_fin adds them up, runs the INTTs
+// and forms both children as k_expand_level does.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void k_expand_part(ExpandArgs A, u32 *__restrict__ part, int G, SchemeConst S) {
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
+  __shared__ u32 c1[2][kN];   // c1 per limb
+  __shared__ u32 sig[2][kN];  // sigma(c1) per limb
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[limb];
+  const int node = blockIdx.x / G, g = blockIdx.x % G, qi = blockIdx.y;
+  const u32 *c = A.in + qi * A.in_stride + (size_t)node * kCtWords;
+  {
+    u32 cv[8];
+    load_map0(cv, c + (1 * 2 + limb) * kN, t);
+    store_map0(c1[limb], cv, t);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const int j = map0(t, cc);
+    const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+    const u32 x = c1[limb][i0 < kN ? i0 : i0 - kN];
+    sig[limb][j] = (i0 < kN || !x) ? x : L.q - x;
+  }
+  __syncthreads();
+  i64 D[8];
+  compose_centre8(sig[0], sig[1], t, S, D);
+  u64 accb[8], acca[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
+  const int dg = (A.l_ks + G - 1) / G, i0 = g * dg, i1 = min(A.l_ks, i0 + dg);
+  const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
+  digit_mac(D, i0, i1, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[limb], accb, acca);
+  u32 ob[8], oa[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    ob[cc] = reduce_u64(accb[cc], L);
+    oa[cc] = reduce_u64(acca[cc], L);
+  }
+  u32 *dst = part + ((size_t)qi * gridDim.x + blockIdx.x) * kCtWords;
+  store_map3(dst + (0 * 2 + limb) * kN, ob, t);
+  store_map3(dst + (1 * 2 + limb) * kN, oa, t);
+}
+
+// Lane block (limb, h): delta_h = INTT(sum of the G partials), then child
+// polynomials h as in k_expand_level.
+__global__ __launch_bounds__(1024) void k_expand_fin(ExpandArgs A, const u32 *__restrict__ part, int G, int nnodes,
+                                                     SchemeConst S) {
+  __shared__ __align__(16) u32 nlds[4][kLdsPoly];
+  __shared__ u32 sh[2][2][kN];
+  const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
+  const int limb = lb & 1, h = lb >> 1;
+  const LimbConst &L = S.limb[limb];
+  const int node = blockIdx.x, qi = blockIdx.y;
+  const u32 *c = A.in + qi * A.in_stride + (size_t)node * kCtWords;
+  const u32 *pp = part + ((size_t)qi * nnodes * G + (size_t)node * G) * kCtWords + (h * 2 + limb) * kN;
+  u32 keep[8];
+  {
+    uint4 w[kSplitMaxG][2];  // all G partials in flight at once
+#pragma unroll
+    for (int j = 0; j < kSplitMaxG; j++)
+      if (j < G) {
+        w[j][0] = reinterpret_cast<const uint4 *>(pp + (size_t)j * kCtWords)[2 * t];
+        w[j][1] = reinterpret_cast<const uint4 *>(pp + (size_t)j * kCtWords)[2 * t + 1];
+      }
+    u64 acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < kSplitMaxG; j++)
+      if (j < G) {
+        acc[0] += w[j][0].x; acc[1] += w[j][0].y; acc[2] += w[j][0].z; acc[3] += w[j][0].w;
+        acc[4] += w[j][1].x; acc[5] += w[j][1].y; acc[6] += w[j][1].z; acc[7] += w[j][1].w;
+      }
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) keep[cc] = reduce_u64(acc[cc], L);
+  }
+  ntt_inv_2048(keep, nlds[lb], t, L, Barrier());  // delta_b (h=0) or delta_a (h=1), coefficient form
+  u32 *left = A.out + qi * A.out_stride + (size_t)(2 * node) * kCtWords;
+  u32 *right = left + kCtWords;
+  const u32 *ch = c + (h * 2 + limb) * kN;
+  u32 cv[8];
+  load_map0(cv, ch, t);
+  if (h == 0) {  // c'_0 = sigma(c0) + delta_b
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      const int j = map0(t, cc);
+      const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+      const u32 x = ch[i0 < kN ? i0 : i0 - kN];
+      keep[cc] = add_mod(keep[cc], (i0 < kN || !x) ? x : L.q - x, L.q);
+    }
+  }
+  u32 l_out[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    l_out[cc] = add_mod(cv[cc], keep[cc], L.q);
+    cv[cc] = sub_mod(cv[cc], keep[cc], L.q);
+  }
+  store_map0(left + (h * 2 + limb) * kN, l_out, t);
+  // right = (c - c') * X^-k: out[j] = d[j+k] if j+k < N else -d[j+k-N]
+  store_map0(sh[h][limb], cv, t);
+  __syncthreads();
+  const int k = 1 << A.level;
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const int j = map0(t, cc), s = j + k;
+    const u32 x = sh[h][limb][s < kN ? s : s - kN];
+    right[(h * 2 + limb) * kN + j] = (s < kN || !x) ? x : L.q - x;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Selector completion (server.cpp:823-855, gsw.cpp:317-357): top row r =
 // NTT(row); bottom row = external_product(RGSW(s), row), key gadget.
 // Lane block (limb, p): poly p of the row, all l_key digits of it.
@@ -300,10 +421,100 @@ __global__ __launch_bounds__(1024) void k_mux(MuxArgs A, SchemeConst S) {
 }
 
 // ---------------------------------------------------------------------------
+// Split MUX for levels with few nodes (latency-bound: k_mux keeps one CU per
+// node busy with a chain of six transforms per lane block). The node's work
+// is spread over 2G workgroups of two lane blocks (both limbs of poly p):
+// each runs INTT(y - x) of poly p and the digits [g*dg, (g+1)*dg) of it, and
+// writes its partial external product (both outputs, both limbs, NTT form,
+// reduced mod q) to part; k_mux_fin adds x and the 2G partials.
+// part block of (query qi, muxed slot m, poly p, group g):
+//   qi * (nmux * 2G) + (m * 2 + p) * G + g, one ciphertext each.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mux_operands(const MuxArgs &A, int m, int &xi, int &yi) {
+  if (A.offset >= 0) {  // ragged: muxed slot offset + m pairs in[offset + 2m], in[offset + 2m + 1]
+    xi = A.offset + 2 * m;
+    yi = xi + 1;
+  } else {
+    xi = m;
+    yi = m + A.half;
+  }
+}
+
+__global__ __launch_bounds__(512) void k_mux_part(MuxArgs A, u32 *__restrict__ part, int G, SchemeConst S) {
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
+  __shared__ u32 sh[2][kN];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[limb];
+  const int qi = blockIdx.y;
+  const int g = blockIdx.x % G, p = (blockIdx.x / G) & 1, m = blockIdx.x / (2 * G);
+  int xi, yi;
+  mux_operands(A, m, xi, yi);
+  const u32 *in = A.in + qi * A.stride;
+  const u32 *x = in + (size_t)xi * kCtWords, *y = in + (size_t)yi * kCtWords;
+  u32 v[8], w[8];
+  load_map3(v, y + (p * 2 + limb) * kN, t);
+  load_map3(w, x + (p * 2 + limb) * kN, t);
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = sub_mod(v[c], w[c], L.q);
+  ntt_inv_2048(v, nlds[limb], t, L, Barrier());
+  store_map0(sh[limb], v, t);
+  __syncthreads();
+  i64 D[8];
+  compose_centre8(sh[0], sh[1], t, S, D);
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  const int dg = (A.l + G - 1) / G, i0 = g * dg, i1 = min(A.l, i0 + dg);
+  digit_mac(D, i0, i1, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[limb], acc0, acc1);
+  u32 o0[8], o1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    o0[c] = reduce_u64(acc0[c], L);
+    o1[c] = reduce_u64(acc1[c], L);
+  }
+  u32 *dst = part + ((size_t)qi * gridDim.x + blockIdx.x) * kCtWords;
+  store_map3(dst + (0 * 2 + limb) * kN, o0, t);
+  store_map3(dst + (1 * 2 + limb) * kN, o1, t);
+}
+
+// out[s] = in[s] for ragged slots s < offset, else x + sum of the node's 2G
+// partials. Two workgroups per slot, one 16-byte word per thread, all 2G + 1
+// loads in flight at once (G <= kSplitMaxG).
+__global__ __launch_bounds__(1024) void k_mux_fin(MuxArgs A, const u32 *__restrict__ part, int G, int nmux,
+                                                  SchemeConst S) {
+  const int s = blockIdx.x >> 1, qi = blockIdx.y;
+  const int e = (blockIdx.x & 1) * 1024 + threadIdx.x;  // 16-byte word of the ciphertext, < kCtWords / 4
+  const u32 *in = A.in + qi * A.stride;
+  uint4 *dst = reinterpret_cast<uint4 *>(A.out + qi * A.stride + (size_t)s * kCtWords);
+  if (A.offset >= 0 && s < A.offset) {
+    dst[e] = reinterpret_cast<const uint4 *>(in + (size_t)s * kCtWords)[e];
+    return;
+  }
+  const int m = A.offset >= 0 ? s - A.offset : s;
+  int xi, yi;
+  mux_operands(A, m, xi, yi);
+  const uint4 a = reinterpret_cast<const uint4 *>(in + (size_t)xi * kCtWords)[e];
+  const uint4 *pp = reinterpret_cast<const uint4 *>(part + ((size_t)qi * nmux * 2 * G + (size_t)m * 2 * G) * kCtWords);
+  uint4 b[2 * kSplitMaxG];
+#pragma unroll
+  for (int j = 0; j < 2 * kSplitMaxG; j++)
+    if (j < 2 * G) b[j] = pp[(size_t)j * (kCtWords / 4) + e];
+  u64 s0 = a.x, s1 = a.y, s2 = a.z, s3 = a.w;
+#pragma unroll
+  for (int j = 0; j < 2 * kSplitMaxG; j++)
+    if (j < 2 * G) {
+      s0 += b[j].x; s1 += b[j].y; s2 += b[j].z; s3 += b[j].w;
+    }
+  const LimbConst &L = S.limb[((4 * e) / kN) & 1];  // polynomial index = output * 2 + limb
+  dst[e] = make_uint4(reduce_u64(s0, L), reduce_u64(s1, L), reduce_u64(s2, L), reduce_u64(s3, L));
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int nnodes, int B,
-                         int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
+                         int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st,
+                         u32 *part) {
   ExpandArgs A;
   A.in = in; A.out = out; A.in_stride = in_stride; A.out_stride = out_stride;
   A.bv_keys = P.bv_keys;
@@ -314,6 +525,12 @@ void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_s
   for (int e = 0; e < kN - 1; e++) inv = (inv * gk) & (2 * kN - 1);
   A.kinv = (int)inv;
   A.l_ks = G.l_ks; A.b_ks = G.b_ks;
+  const int Gs = split_groups(G.l_ks);
+  if (part && (size_t)nnodes * Gs * B <= kMuxSplitMaxBlocks) {
+    k_expand_part<<<dim3(nnodes * Gs, B), 512, 0, st>>>(A, part, Gs, S);
+    k_expand_fin<<<dim3(nnodes, B), 1024, 0, st>>>(A, part, Gs, nnodes, S);
+    return;
+  }
   dim3 grid(nnodes, B);
   k_expand_level<<<grid, 1024, 0, st>>>(A, S);
 }
@@ -336,6 +553,24 @@ void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, 
   A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
   dim3 grid(nslots, B);
   k_mux<<<grid, 1024, 0, st>>>(A, S);
+}
+
+int mux_split_groups(int nslots, int offset, int B, const GadgetCfg &G) {
+  const int nmux = offset >= 0 ? nslots - offset : nslots;
+  if (nmux <= 0) return 0;
+  const int Gs = split_groups(G.l_ep);
+  return (size_t)nmux * 2 * Gs * B <= kMuxSplitMaxBlocks ? Gs : 0;
+}
+
+void launch_mux_split(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
+                      const u32 *sel, size_t sel_stride, u32 *part, int Gs, const GadgetCfg &G, const SchemeConst &S,
+                      hipStream_t st) {
+  MuxArgs A;
+  A.in = in; A.out = out; A.stride = stride; A.sel = sel; A.sel_stride = sel_stride;
+  A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
+  const int nmux = offset >= 0 ? nslots - offset : nslots;
+  k_mux_part<<<dim3(nmux * 2 * Gs, B), 512, 0, st>>>(A, part, Gs, S);
+  k_mux_fin<<<dim3(2 * nslots, B), 1024, 0, st>>>(A, part, Gs, nmux, S);
 }
 
 }  // namespace opir
