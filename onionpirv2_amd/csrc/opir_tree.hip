@@ -27,6 +27,7 @@ struct Barrier {
 // Digit groups of a split level: one digit per workgroup, at most
 // kSplitMaxG groups (then ceil(l / kSplitMaxG) digits each).
 constexpr int kSplitMaxG = 8;
+constexpr size_t kSplitTargetBlocks = 512;
 inline int split_groups(int l) {
   const int dg = (l + kSplitMaxG - 1) / kSplitMaxG;
   return (l + dg - 1) / dg;
@@ -56,6 +57,9 @@ __device__ __forceinline__ void compose_centre8(const u32 *lo, const u32 *hi, in
 // Digits [i0, i1) of one polynomial: NTT each digit (this block's limb) and
 // accumulate digit x key row into two u64 accumulators per value.
 // row(i) = row_base + (rev ? l-1-i : i); key rows are [2 outputs][2 limbs][N].
+// kKeyFirst: issue the key row's loads before the digit's NTT (16 more live
+// registers), so that their latency hides behind it.
+template <bool kKeyFirst = false>
 __device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int l, int b, bool rev, int row_base,
                                           const u32 *__restrict__ key, int limb, int t_in, const LimbConst &L,
                                           u32 *nttlds, u64 (&acc0)[8], u64 (&acc1)[8]) {
@@ -67,15 +71,22 @@ __device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int
     int t = t_in;
     asm volatile("" : "+v"(t));
     const i64 H = digit_offset(i, b);
+    // key row: [2 outputs][2 limbs][N] words; byte offsets fit in 32 bits
+    const unsigned row = (unsigned)(row_base + (rev ? l - 1 - i : i));
+    const unsigned off = row * (unsigned)(kCtWords * 4) + (unsigned)limb * (kN * 4) + ((unsigned)t << 5);
+    uint4 a0, a1, b0, b1;
+    if (kKeyFirst) {
+      a0 = ld_off<uint4>(key, off), a1 = ld_off<uint4>(key, off + 16);
+      b0 = ld_off<uint4>(key, off + 2 * kN * 4), b1 = ld_off<uint4>(key, off + 2 * kN * 4 + 16);
+    }
     u32 v[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) v[c] = digit_residue(D[c], H, b * i, b, L.q);
     ntt_fwd_2048<false>(v, nttlds, t, L, bar);  // [0, 4q): at most 10 products of < 2^60 per sum
-    // key row: [2 outputs][2 limbs][N] words; byte offsets fit in 32 bits
-    const unsigned row = (unsigned)(row_base + (rev ? l - 1 - i : i));
-    const unsigned off = row * (unsigned)(kCtWords * 4) + (unsigned)limb * (kN * 4) + ((unsigned)t << 5);
-    const uint4 a0 = ld_off<uint4>(key, off), a1 = ld_off<uint4>(key, off + 16);
-    const uint4 b0 = ld_off<uint4>(key, off + 2 * kN * 4), b1 = ld_off<uint4>(key, off + 2 * kN * 4 + 16);
+    if (!kKeyFirst) {
+      a0 = ld_off<uint4>(key, off), a1 = ld_off<uint4>(key, off + 16);
+      b0 = ld_off<uint4>(key, off + 2 * kN * 4), b1 = ld_off<uint4>(key, off + 2 * kN * 4 + 16);
+    }
     const u32 k0[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
     const u32 k1[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
@@ -130,7 +141,7 @@ __global__ __launch_bounds__(1024) void k_expand_level(ExpandArgs A, SchemeConst
   for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
   const int half = (A.l_ks + 1) / 2;
   const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
-  digit_mac(D, h * half, h ? A.l_ks : half, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[lb], accb, acca);
+  digit_mac<true>(D, h * half, h ? A.l_ks : half, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[lb], accb, acca);
   // exchange partial sums: block h keeps output h (0: delta_b, 1: delta_a)
   u32 keep[8], give[8];
 #pragma unroll
@@ -217,7 +228,7 @@ __global__ __launch_bounds__(512) void k_expand_part(ExpandArgs A, u32 *__restri
   for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
   const int dg = (A.l_ks + G - 1) / G, i0 = g * dg, i1 = min(A.l_ks, i0 + dg);
   const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
-  digit_mac(D, i0, i1, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[limb], accb, acca);
+  digit_mac<true>(D, i0, i1, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[limb], accb, acca);
   u32 ob[8], oa[8];
 #pragma unroll
   for (int cc = 0; cc < 8; cc++) {
@@ -229,16 +240,16 @@ __global__ __launch_bounds__(512) void k_expand_part(ExpandArgs A, u32 *__restri
   store_map3(dst + (1 * 2 + limb) * kN, oa, t);
 }
 
-// Lane block (limb, h): delta_h = INTT(sum of the G partials), then child
-// polynomials h as in k_expand_level.
-__global__ __launch_bounds__(1024) void k_expand_fin(ExpandArgs A, const u32 *__restrict__ part, int G, int nnodes,
-                                                     SchemeConst S) {
-  __shared__ __align__(16) u32 nlds[4][kLdsPoly];
-  __shared__ u32 sh[2][2][kN];
-  const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
-  const int limb = lb & 1, h = lb >> 1;
+// One workgroup per (node, h, limb): delta_h = INTT(sum of the G partials),
+// then child polynomials h of this limb as in k_expand_level.
+__global__ __launch_bounds__(256) void k_expand_fin(ExpandArgs A, const u32 *__restrict__ part, int G, int nnodes,
+                                                    SchemeConst S) {
+  __shared__ __align__(16) u32 nlds[kLdsPoly];
+  __shared__ u32 sh[kN];
+  const int t = threadIdx.x;
+  const int limb = blockIdx.x & 1, h = (blockIdx.x >> 1) & 1;
   const LimbConst &L = S.limb[limb];
-  const int node = blockIdx.x, qi = blockIdx.y;
+  const int node = blockIdx.x >> 2, qi = blockIdx.y;
   const u32 *c = A.in + qi * A.in_stride + (size_t)node * kCtWords;
   const u32 *pp = part + ((size_t)qi * nnodes * G + (size_t)node * G) * kCtWords + (h * 2 + limb) * kN;
   u32 keep[8];
@@ -260,20 +271,25 @@ __global__ __launch_bounds__(1024) void k_expand_fin(ExpandArgs A, const u32 *__
 #pragma unroll
     for (int cc = 0; cc < 8; cc++) keep[cc] = reduce_u64(acc[cc], L);
   }
-  ntt_inv_2048(keep, nlds[lb], t, L, Barrier());  // delta_b (h=0) or delta_a (h=1), coefficient form
-  u32 *left = A.out + qi * A.out_stride + (size_t)(2 * node) * kCtWords;
-  u32 *right = left + kCtWords;
+  // c (poly h of this limb) and, for h = 0, sigma(c0), loaded ahead of the INTT
   const u32 *ch = c + (h * 2 + limb) * kN;
-  u32 cv[8];
+  u32 cv[8], sg[8];
   load_map0(cv, ch, t);
-  if (h == 0) {  // c'_0 = sigma(c0) + delta_b
+  if (h == 0) {
 #pragma unroll
     for (int cc = 0; cc < 8; cc++) {
       const int j = map0(t, cc);
       const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
       const u32 x = ch[i0 < kN ? i0 : i0 - kN];
-      keep[cc] = add_mod(keep[cc], (i0 < kN || !x) ? x : L.q - x, L.q);
+      sg[cc] = (i0 < kN || !x) ? x : L.q - x;
     }
+  }
+  ntt_inv_2048(keep, nlds, t, L, Barrier());  // delta_b (h=0) or delta_a (h=1), coefficient form
+  u32 *left = A.out + qi * A.out_stride + (size_t)(2 * node) * kCtWords;
+  u32 *right = left + kCtWords;
+  if (h == 0) {  // c'_0 = sigma(c0) + delta_b
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) keep[cc] = add_mod(keep[cc], sg[cc], L.q);
   }
   u32 l_out[8];
 #pragma unroll
@@ -283,13 +299,13 @@ __global__ __launch_bounds__(1024) void k_expand_fin(ExpandArgs A, const u32 *__
   }
   store_map0(left + (h * 2 + limb) * kN, l_out, t);
   // right = (c - c') * X^-k: out[j] = d[j+k] if j+k < N else -d[j+k-N]
-  store_map0(sh[h][limb], cv, t);
+  store_map0(sh, cv, t);
   __syncthreads();
   const int k = 1 << A.level;
 #pragma unroll
   for (int cc = 0; cc < 8; cc++) {
     const int j = map0(t, cc), s = j + k;
-    const u32 x = sh[h][limb][s < kN ? s : s - kN];
+    const u32 x = sh[s < kN ? s : s - kN];
     right[(h * 2 + limb) * kN + j] = (s < kN || !x) ? x : L.q - x;
   }
 }
@@ -319,7 +335,7 @@ __global__ __launch_bounds__(1024) void k_complete(CompleteArgs A, SchemeConst S
   u64 acc0[8], acc1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
-  digit_mac(D, 0, A.l_key, A.l_key, A.b_key, true, p * A.l_key, A.gsw_keys[qi], limb, t, L, nlds[lb], acc0, acc1);
+  digit_mac<true>(D, 0, A.l_key, A.l_key, A.b_key, true, p * A.l_key, A.gsw_keys[qi], limb, t, L, nlds[lb], acc0, acc1);
   u32 o0[8], o1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -390,7 +406,7 @@ __global__ __launch_bounds__(1024) void k_mux(MuxArgs A, SchemeConst S) {
   u64 acc0[8], acc1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
-  digit_mac(D, 0, A.l, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[lb], acc0, acc1);
+  digit_mac<true>(D, 0, A.l, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[lb], acc0, acc1);
   u32 o0[8], o1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -465,7 +481,8 @@ __global__ __launch_bounds__(512) void k_mux_part(MuxArgs A, u32 *__restrict__ p
 #pragma unroll
   for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
   const int dg = (A.l + G - 1) / G, i0 = g * dg, i1 = min(A.l, i0 + dg);
-  digit_mac(D, i0, i1, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[limb], acc0, acc1);
+  digit_mac<true>(D, i0, i1, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[limb], acc0,
+                  acc1);
   u32 o0[8], o1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -478,12 +495,12 @@ __global__ __launch_bounds__(512) void k_mux_part(MuxArgs A, u32 *__restrict__ p
 }
 
 // out[s] = in[s] for ragged slots s < offset, else x + sum of the node's 2G
-// partials. Two workgroups per slot, one 16-byte word per thread, all 2G + 1
-// loads in flight at once (G <= kSplitMaxG).
-__global__ __launch_bounds__(1024) void k_mux_fin(MuxArgs A, const u32 *__restrict__ part, int G, int nmux,
-                                                  SchemeConst S) {
-  const int s = blockIdx.x >> 1, qi = blockIdx.y;
-  const int e = (blockIdx.x & 1) * 1024 + threadIdx.x;  // 16-byte word of the ciphertext, < kCtWords / 4
+// partials. Eight workgroups per slot, one 16-byte word per thread, all
+// 2G + 1 loads in flight at once (G <= kSplitMaxG).
+__global__ __launch_bounds__(256) void k_mux_fin(MuxArgs A, const u32 *__restrict__ part, int G, int nmux,
+                                                 SchemeConst S) {
+  const int s = blockIdx.x >> 3, qi = blockIdx.y;
+  const int e = (blockIdx.x & 7) * 256 + threadIdx.x;  // 16-byte word of the ciphertext, < kCtWords / 4
   const u32 *in = A.in + qi * A.stride;
   uint4 *dst = reinterpret_cast<uint4 *>(A.out + qi * A.stride + (size_t)s * kCtWords);
   if (A.offset >= 0 && s < A.offset) {
@@ -528,7 +545,7 @@ void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_s
   const int Gs = split_groups(G.l_ks);
   if (part && (size_t)nnodes * Gs * B <= kMuxSplitMaxBlocks) {
     k_expand_part<<<dim3(nnodes * Gs, B), 512, 0, st>>>(A, part, Gs, S);
-    k_expand_fin<<<dim3(nnodes, B), 1024, 0, st>>>(A, part, Gs, nnodes, S);
+    k_expand_fin<<<dim3(nnodes * 4, B), 256, 0, st>>>(A, part, Gs, nnodes, S);
     return;
   }
   dim3 grid(nnodes, B);
@@ -558,8 +575,14 @@ void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, 
 int mux_split_groups(int nslots, int offset, int B, const GadgetCfg &G) {
   const int nmux = offset >= 0 ? nslots - offset : nslots;
   if (nmux <= 0) return 0;
-  const int Gs = split_groups(G.l_ep);
-  return (size_t)nmux * 2 * Gs * B <= kMuxSplitMaxBlocks ? Gs : 0;
+  // about kSplitTargetBlocks workgroups: one digit each for narrow levels,
+  // several digits per workgroup for wider ones, no split for the widest
+  const size_t pairs = (size_t)nmux * 2 * B;
+  if (pairs > kSplitTargetBlocks) return 0;
+  int Gs = (int)std::min<size_t>(split_groups(G.l_ep), kSplitTargetBlocks / pairs);
+  const int dg = (G.l_ep + Gs - 1) / Gs;
+  Gs = (G.l_ep + dg - 1) / dg;  // no empty groups
+  return pairs * Gs <= kMuxSplitMaxBlocks ? Gs : 0;
 }
 
 void launch_mux_split(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
@@ -570,7 +593,7 @@ void launch_mux_split(const u32 *in, u32 *out, size_t stride, int nslots, int of
   A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
   const int nmux = offset >= 0 ? nslots - offset : nslots;
   k_mux_part<<<dim3(nmux * 2 * Gs, B), 512, 0, st>>>(A, part, Gs, S);
-  k_mux_fin<<<dim3(2 * nslots, B), 1024, 0, st>>>(A, part, Gs, nmux, S);
+  k_mux_fin<<<dim3(8 * nslots, B), 256, 0, st>>>(A, part, Gs, nmux, S);
 }
 
 }  // namespace opir
