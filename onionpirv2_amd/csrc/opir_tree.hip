@@ -436,6 +436,67 @@ __global__ __launch_bounds__(1024) void k_mux(MuxArgs A, SchemeConst S) {
   }
 }
 
+__device__ __forceinline__ void mux_operands(const MuxArgs &A, int m, int &xi, int &yi) {
+  if (A.offset >= 0) {  // ragged: muxed slot offset + m pairs in[offset + 2m], in[offset + 2m + 1]
+    xi = A.offset + 2 * m;
+    yi = xi + 1;
+  } else {
+    xi = m;
+    yi = m + A.half;
+  }
+}
+
+// The same node in one 512-thread workgroup (lane block = limb) that handles
+// poly p = 0 and then p = 1, accumulating both into one sum: half the
+// threads per node, no cross-pair exchange, so two or more nodes share a CU
+// and their barrier waits overlap. For wide (throughput-bound) levels.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_mux_serial(MuxArgs A, SchemeConst S) {
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
+  __shared__ u32 sh[2][kN];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[limb];
+  const int s = blockIdx.x, qi = blockIdx.y;
+  const u32 *in = A.in + qi * A.stride;
+  u32 *dst = A.out + qi * A.stride + (size_t)s * kCtWords;
+  if (A.offset >= 0 && s < A.offset) {  // unpaired candidate of the ragged level stays in place
+    const u32 *src = in + (size_t)s * kCtWords;
+    for (int e = threadIdx.x; e < (int)kCtWords / 4; e += 512)
+      reinterpret_cast<uint4 *>(dst)[e] = reinterpret_cast<const uint4 *>(src)[e];
+    return;
+  }
+  int xi, yi;
+  mux_operands(A, A.offset >= 0 ? s - A.offset : s, xi, yi);
+  const u32 *x = in + (size_t)xi * kCtWords, *y = in + (size_t)yi * kCtWords;
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    u32 v[8], w[8];
+    load_map3(v, y + (p * 2 + limb) * kN, t);
+    load_map3(w, x + (p * 2 + limb) * kN, t);
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = sub_mod(v[c], w[c], L.q);
+    ntt_inv_2048(v, nlds[limb], t, L, Barrier());  // its first barrier also orders sh reuse
+    store_map0(sh[limb], v, t);
+    __syncthreads();
+    i64 D[8];
+    compose_centre8(sh[0], sh[1], t, S, D);
+    digit_mac<true>(D, 0, A.l, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[limb], acc0,
+                    acc1);
+  }
+  u32 x0[8], x1[8];
+  load_map3(x0, x + (0 * 2 + limb) * kN, t);
+  load_map3(x1, x + (1 * 2 + limb) * kN, t);
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    x0[c] = add_mod(x0[c], reduce_u64(acc0[c], L), L.q);
+    x1[c] = add_mod(x1[c], reduce_u64(acc1[c], L), L.q);
+  }
+  store_map3(dst + (0 * 2 + limb) * kN, x0, t);
+  store_map3(dst + (1 * 2 + limb) * kN, x1, t);
+}
+
 // ---------------------------------------------------------------------------
 // Split MUX for levels with few nodes (latency-bound: k_mux keeps one CU per
 // node busy with a chain of six transforms per lane block). The node's work
@@ -446,15 +507,6 @@ __global__ __launch_bounds__(1024) void k_mux(MuxArgs A, SchemeConst S) {
 // part block of (query qi, muxed slot m, poly p, group g):
 //   qi * (nmux * 2G) + (m * 2 + p) * G + g, one ciphertext each.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void mux_operands(const MuxArgs &A, int m, int &xi, int &yi) {
-  if (A.offset >= 0) {  // ragged: muxed slot offset + m pairs in[offset + 2m], in[offset + 2m + 1]
-    xi = A.offset + 2 * m;
-    yi = xi + 1;
-  } else {
-    xi = m;
-    yi = m + A.half;
-  }
-}
 
 __global__ __launch_bounds__(512) void k_mux_part(MuxArgs A, u32 *__restrict__ part, int G, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
@@ -569,7 +621,7 @@ void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, 
   A.in = in; A.out = out; A.stride = stride; A.sel = sel; A.sel_stride = sel_stride;
   A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
   dim3 grid(nslots, B);
-  k_mux<<<grid, 1024, 0, st>>>(A, S);
+  k_mux_serial<<<grid, 512, 0, st>>>(A, S);
 }
 
 int mux_split_groups(int nslots, int offset, int B, const GadgetCfg &G) {
