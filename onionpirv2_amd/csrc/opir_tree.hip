@@ -192,6 +192,81 @@ __global__ __launch_bounds__(1024) void k_expand_level(ExpandArgs A, SchemeConst
   }
 }
 
+// The same level with one 512-thread workgroup per node (lane block = limb):
+// all l_ks digits, then delta_b and delta_a in turn. Two nodes share a CU.
+// For wide levels.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_expand_serial(ExpandArgs A,
+                                                                                               SchemeConst S) {
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
+  __shared__ u32 cs[2][kN];   // c1 per limb, later shift staging
+  __shared__ u32 sig[2][kN];  // sigma(c1) per limb
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[limb];
+  const int node = blockIdx.x, qi = blockIdx.y;
+  const u32 *c = A.in + qi * A.in_stride + (size_t)node * kCtWords;
+  {
+    u32 cv[8];
+    load_map0(cv, c + (1 * 2 + limb) * kN, t);
+    store_map0(cs[limb], cv, t);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const int j = map0(t, cc);
+    const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+    const u32 x = cs[limb][i0 < kN ? i0 : i0 - kN];
+    sig[limb][j] = (i0 < kN || !x) ? x : L.q - x;
+  }
+  __syncthreads();
+  u64 accb[8], acca[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
+  {
+    i64 D[8];
+    compose_centre8(sig[0], sig[1], t, S, D);
+    const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
+    digit_mac<true>(D, 0, A.l_ks, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[limb], accb, acca);
+  }
+  u32 *left = A.out + qi * A.out_stride + (size_t)(2 * node) * kCtWords;
+  u32 *right = left + kCtWords;
+  const int k = 1 << A.level;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    u32 keep[8];
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) keep[cc] = reduce_u64(h ? acca[cc] : accb[cc], L);
+    ntt_inv_2048(keep, nlds[limb], t, L, Barrier());  // delta_h, coefficient form
+    const u32 *ch = c + (h * 2 + limb) * kN;
+    u32 cv[8];
+    load_map0(cv, ch, t);
+    if (h == 0) {  // c'_0 = sigma(c0) + delta_b
+#pragma unroll
+      for (int cc = 0; cc < 8; cc++) {
+        const int j = map0(t, cc);
+        const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+        const u32 x = ch[i0 < kN ? i0 : i0 - kN];
+        keep[cc] = add_mod(keep[cc], (i0 < kN || !x) ? x : L.q - x, L.q);
+      }
+    }
+    u32 l_out[8];
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      l_out[cc] = add_mod(cv[cc], keep[cc], L.q);
+      cv[cc] = sub_mod(cv[cc], keep[cc], L.q);
+    }
+    store_map0(left + (h * 2 + limb) * kN, l_out, t);
+    // right = (c - c') * X^-k; the INTT's barriers ordered the previous reads of cs
+    store_map0(cs[limb], cv, t);
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      const int j = map0(t, cc), s = j + k;
+      const u32 x = cs[limb][s < kN ? s : s - kN];
+      right[(h * 2 + limb) * kN + j] = (s < kN || !x) ? x : L.q - x;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Split expansion level for levels with few nodes: the node's l_ks digits are
 // spread over G workgroups of two lane blocks (both limbs), each writing its
@@ -361,6 +436,52 @@ __global__ __launch_bounds__(1024) void k_complete(CompleteArgs A, SchemeConst S
     store_map3(bot + (0 * 2 + limb) * kN, o0, t);
     store_map3(bot + (1 * 2 + limb) * kN, o1, t);
   }
+}
+
+// The same row with one 512-thread workgroup (lane block = limb) that handles
+// poly p = 0 and then p = 1. The sum is reduced between the two halves: each
+// half adds l_key products of < 2^60.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_complete_serial(CompleteArgs A,
+                                                                                                 SchemeConst S) {
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
+  __shared__ u32 sh[2][kN];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[limb];
+  const int g = blockIdx.x, qi = blockIdx.y;
+  const int a = g / A.l_ep, r = g % A.l_ep;
+  const u32 *row = A.leaves + qi * A.leaf_stride + (size_t)g * kCtWords;
+  u32 *selb = A.sel + qi * A.sel_stride + (size_t)a * (2 * A.l_ep) * kCtWords;
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    u32 v[8];
+    load_map0(v, row + (p * 2 + limb) * kN, t);
+    store_map0(sh[limb], v, t);  // the previous half's digit transforms ordered its reads of sh
+    ntt_fwd_2048(v, nlds[limb], t, L, Barrier());  // its barriers also publish sh
+    store_map3(selb + (size_t)r * kCtWords + (p * 2 + limb) * kN, v, t);
+    i64 D[8];
+    compose_centre8(sh[0], sh[1], t, S, D);
+    if (p == 1) {
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        acc0[c] = reduce_u64(acc0[c], L);
+        acc1[c] = reduce_u64(acc1[c], L);
+      }
+    }
+    digit_mac<true>(D, 0, A.l_key, A.l_key, A.b_key, true, p * A.l_key, A.gsw_keys[qi], limb, t, L, nlds[limb], acc0,
+                    acc1);
+  }
+  u32 o0[8], o1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    o0[c] = reduce_u64(acc0[c], L);
+    o1[c] = reduce_u64(acc1[c], L);
+  }
+  u32 *bot = selb + (size_t)(A.l_ep + r) * kCtWords;
+  store_map3(bot + (0 * 2 + limb) * kN, o0, t);
+  store_map3(bot + (1 * 2 + limb) * kN, o1, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -601,7 +722,7 @@ void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_s
     return;
   }
   dim3 grid(nnodes, B);
-  k_expand_level<<<grid, 1024, 0, st>>>(A, S);
+  k_expand_serial<<<grid, 512, 0, st>>>(A, S);
 }
 
 void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
@@ -611,7 +732,10 @@ void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel
   A.leaves = leaves; A.sel = sel; A.leaf_stride = leaf_stride; A.sel_stride = sel_stride;
   A.gsw_keys = P.gsw_keys; A.l_ep = G.l_ep; A.l_key = G.l_key; A.b_key = G.b_key;
   dim3 grid(nrows, B);
-  k_complete<<<grid, 1024, 0, st>>>(A, S);
+  if ((size_t)nrows * B >= 256)  // enough rows to fill the device twice over
+    k_complete_serial<<<grid, 512, 0, st>>>(A, S);
+  else
+    k_complete<<<grid, 1024, 0, st>>>(A, S);
 }
 
 void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
