@@ -27,7 +27,11 @@ constexpr int kN = 2048;        // ring degree of the composite scheme
 constexpr int kLogN = 11;
 constexpr int kLB = 256;        // threads per lane block
 constexpr int kPer = 8;         // values per thread
-constexpr int kLdsPoly = kN + (kN / 32) * 4;  // padded LDS words per polynomial
+constexpr int kLdsOne = kN + (kN / 32) * 4;  // padded LDS words of one polynomial
+// NTT scratch per lane block: three exchange buffers used in rotation, so an
+// exchange needs one barrier (write, barrier, read): a buffer is rewritten
+// only three exchanges later, after two more barriers.
+constexpr int kLdsPoly = 3 * kLdsOne;
 
 // Constants of one 29-bit RNS limb.
 struct LimbConst {
@@ -227,12 +231,14 @@ template <int R> __device__ __forceinline__ void lds_get(const u32 *lds, int t, 
   }
 }
 
+// One exchange through buffer `buf` of the lane block's rotation. Every
+// transform uses buffers 0, 1, 2 in this order, so consecutive transforms on
+// the same scratch keep the rotation.
 template <int FROM, int TO, typename Bar>
-__device__ __forceinline__ void lds_exchange(u32 v[8], u32 *lds, int t, Bar bar) {
-  lds_put<FROM>(lds, t, v);
+__device__ __forceinline__ void lds_exchange(u32 v[8], u32 *lds, int buf, int t, Bar bar) {
+  lds_put<FROM>(lds + buf * kLdsOne, t, v);
   bar();
-  lds_get<TO>(lds, t, v);
-  bar();
+  lds_get<TO>(lds + buf * kLdsOne, t, v);
 }
 
 // kReduce = false leaves outputs lazily in [0, 4q) (< 2^31), for callers that
@@ -240,11 +246,11 @@ __device__ __forceinline__ void lds_exchange(u32 v[8], u32 *lds, int t, Bar bar)
 template <bool kReduce = true, typename Bar>
 __device__ __forceinline__ void ntt_fwd_2048(u32 v[8], u32 *lds, int t, const LimbConst &L, Bar bar) {
   fwd_round8(v, 1, 0, L);                       // stages 0-2 (map0, g = 0)
-  lds_exchange<0, 1>(v, lds, t, bar);
+  lds_exchange<0, 1>(v, lds, 0, t, bar);
   fwd_round8(v, 8, t / 32, L);                  // stages 3-5
-  lds_exchange<1, 2>(v, lds, t, bar);
+  lds_exchange<1, 2>(v, lds, 1, t, bar);
   fwd_round8(v, 64, t / 4, L);                  // stages 6-8
-  lds_exchange<2, 3>(v, lds, t, bar);
+  lds_exchange<2, 3>(v, lds, 2, t, bar);
   // stages 9-10 on 8 consecutive values: twiddles w[512 + 2t + h], w[1024 + 4t + h]
   {
     const unsigned u = (unsigned)t;
@@ -282,11 +288,11 @@ __device__ __forceinline__ void ntt_inv_2048(u32 v[8], u32 *lds, int t, const Li
     bf_inv(v[4], v[6], w2.y, s2.y, L);
     bf_inv(v[5], v[7], w2.y, s2.y, L);
   }
-  lds_exchange<3, 2>(v, lds, t, bar);
+  lds_exchange<3, 2>(v, lds, 0, t, bar);
   inv_round8(v, 64, t / 4, L);
-  lds_exchange<2, 1>(v, lds, t, bar);
+  lds_exchange<2, 1>(v, lds, 1, t, bar);
   inv_round8(v, 8, t / 32, L);
-  lds_exchange<1, 0>(v, lds, t, bar);
+  lds_exchange<1, 0>(v, lds, 2, t, bar);
   inv_round8(v, 1, 0, L);
 #pragma unroll
   for (int c = 0; c < 8; c++) v[c] = csub(mul_shoup(v[c], L.ninv, L.ninv_s, L.q), L.q);

@@ -103,103 +103,15 @@ __device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int
 // Expansion level (server.cpp:786-810 with bv_keyswitch.cpp:296-367):
 //   c' = (sigma(c0) + sum d_i*b_i, sum d_i*a_i), d_i digits of sigma(c1)
 //   out[2r] = c + c',  out[2r+1] = (c - c') * X^-k
-// Lane block (limb, h): digits [h*ceil(L/2), ...) of limb `limb`; afterwards
-// block h finishes output polynomial h (h = 0: c0 side, h = 1: c1 side).
+// Wide levels: one 512-thread workgroup per node (lane block = limb), all
+// l_ks digits, then delta_b and delta_a in turn; two nodes share a CU.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_expand_level(ExpandArgs A, SchemeConst S) {
-  __shared__ __align__(16) u32 nlds[4][kLdsPoly];
-  __shared__ u32 sh[2][2][kN];  // [poly][limb]: c, later partial sums, later shift staging
-  __shared__ u32 sig[2][kN];    // sigma(c1) per limb
-  const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // lb is wave-uniform
-  const int limb = lb & 1, h = lb >> 1;
-  const LimbConst &L = S.limb[limb];
-  const int node = blockIdx.x, qi = blockIdx.y;
-  const u32 *c = A.in + qi * A.in_stride + (size_t)node * kCtWords;
-  {
-    u32 cv[8];  // this block's polynomial h of the input, limb `limb`
-    load_map0(cv, c + (h * 2 + limb) * kN, t);
-    store_map0(sh[h][limb], cv, t);
-  }
-  __syncthreads();
-  // sigma_k gather: out[j] = in[i0] or -in[i0 - N], i0 = j * kinv mod 2N
-  // (only sigma(c1) feeds the digits; sigma(c0) is re-gathered at the end so
-  // that neither stays in registers across the transforms)
-  if (h == 1) {
-#pragma unroll
-    for (int cc = 0; cc < 8; cc++) {
-      const int j = map0(t, cc);
-      const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
-      const u32 x = sh[1][limb][i0 < kN ? i0 : i0 - kN];
-      sig[limb][j] = (i0 < kN || !x) ? x : L.q - x;
-    }
-  }
-  __syncthreads();
-  i64 D[8];
-  compose_centre8(sig[0], sig[1], t, S, D);
-  u64 accb[8], acca[8];
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
-  const int half = (A.l_ks + 1) / 2;
-  const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
-  digit_mac<true>(D, h * half, h ? A.l_ks : half, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[lb], accb, acca);
-  // exchange partial sums: block h keeps output h (0: delta_b, 1: delta_a)
-  u32 keep[8], give[8];
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) {
-    const u32 b = reduce_u64(accb[cc], L), a = reduce_u64(acca[cc], L);
-    keep[cc] = h ? a : b;
-    give[cc] = h ? b : a;
-  }
-  __syncthreads();  // all reads of sh (gather) are long done; reuse it
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) sh[1 - h][limb][map3(t, cc)] = give[cc];
-  __syncthreads();
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) keep[cc] = add_mod(keep[cc], sh[h][limb][map3(t, cc)], L.q);
-  ntt_inv_2048(keep, nlds[lb], t, L, Barrier());  // delta_b (h=0) or delta_a (h=1), coefficient form
-  // c'_h = sigma(c0) + delta_b (h=0) or delta_a (h=1)
-  u32 *left = A.out + qi * A.out_stride + (size_t)(2 * node) * kCtWords;
-  u32 *right = left + kCtWords;
-  const u32 *ch = c + (h * 2 + limb) * kN;
-  u32 cv[8];
-  load_map0(cv, ch, t);
-  if (h == 0) {  // c'_0 = sigma(c0) + delta_b
-#pragma unroll
-    for (int cc = 0; cc < 8; cc++) {
-      const int j = map0(t, cc);
-      const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
-      const u32 x = ch[i0 < kN ? i0 : i0 - kN];
-      keep[cc] = add_mod(keep[cc], (i0 < kN || !x) ? x : L.q - x, L.q);
-    }
-  }
-  u32 l_out[8];
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) {
-    l_out[cc] = add_mod(cv[cc], keep[cc], L.q);
-    cv[cc] = sub_mod(cv[cc], keep[cc], L.q);
-  }
-  store_map0(left + (h * 2 + limb) * kN, l_out, t);
-  // right = (c - c') * X^-k: out[j] = d[j+k] if j+k < N else -d[j+k-N]
-  __syncthreads();
-  store_map0(sh[h][limb], cv, t);
-  __syncthreads();
-  const int k = 1 << A.level;
-#pragma unroll
-  for (int cc = 0; cc < 8; cc++) {
-    const int j = map0(t, cc), s = j + k;
-    const u32 x = sh[h][limb][s < kN ? s : s - kN];
-    right[(h * 2 + limb) * kN + j] = (s < kN || !x) ? x : L.q - x;
-  }
-}
-
-// The same level with one 512-thread workgroup per node (lane block = limb):
-// all l_ks digits, then delta_b and delta_a in turn. Two nodes share a CU.
-// For wide levels.
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_expand_serial(ExpandArgs A,
                                                                                                SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
-  __shared__ u32 cs[2][kN];   // c1 per limb, later shift staging
-  __shared__ u32 sig[2][kN];  // sigma(c1) per limb
+  __shared__ u32 cs[2][kN];  // c1 per limb, later shift staging
+  // sigma(c1) per limb, in lane block 0's NTT scratch (free until the digits)
+  u32(*sig)[kN] = reinterpret_cast<u32(*)[kN]>(&nlds[0][0]);
   const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
   const LimbConst &L = S.limb[limb];
   const int node = blockIdx.x, qi = blockIdx.y;
@@ -224,6 +136,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
   {
     i64 D[8];
     compose_centre8(sig[0], sig[1], t, S, D);
+    __syncthreads();  // sig is read; the digit transforms reuse its LDS
     const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
     digit_mac<true>(D, 0, A.l_ks, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[limb], accb, acca);
   }
@@ -276,8 +189,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(512) void k_expand_part(ExpandArgs A, u32 *__restrict__ part, int G, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
-  __shared__ u32 c1[2][kN];   // c1 per limb
-  __shared__ u32 sig[2][kN];  // sigma(c1) per limb
+  __shared__ u32 c1[2][kN];  // c1 per limb
+  // sigma(c1) per limb, in lane block 0's NTT scratch (free until the digits)
+  u32(*sig)[kN] = reinterpret_cast<u32(*)[kN]>(&nlds[0][0]);
   const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
   const LimbConst &L = S.limb[limb];
   const int node = blockIdx.x / G, g = blockIdx.x % G, qi = blockIdx.y;
@@ -298,6 +212,7 @@ __global__ __launch_bounds__(512) void k_expand_part(ExpandArgs A, u32 *__restri
   __syncthreads();
   i64 D[8];
   compose_centre8(sig[0], sig[1], t, S, D);
+  __syncthreads();  // sig is read; the digit transforms reuse its LDS
   u64 accb[8], acca[8];
 #pragma unroll
   for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
@@ -488,74 +403,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 // MUX level (server.cpp:599-722) on NTT-form candidates:
 //   res = x + EP(sel, INTT(y - x))   (EP output in NTT form)
 // equal to the reference's coefficient-form x + INTT(EP(sel, y - x)).
-// Lane block (limb, p): INTT of poly p of y - x, then the l digits of poly p.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_mux(MuxArgs A, SchemeConst S) {
-  __shared__ __align__(16) u32 nlds[4][kLdsPoly];
-  __shared__ u32 sh[2][2][kN];
-  const int lb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // lb is wave-uniform
-  const int limb = lb & 1, p = lb >> 1;
-  const LimbConst &L = S.limb[limb];
-  const int s = blockIdx.x, qi = blockIdx.y;
-  const u32 *in = A.in + qi * A.stride;
-  u32 *dst = A.out + qi * A.stride + (size_t)s * kCtWords;
-  int xi, yi;
-  if (A.offset >= 0) {
-    if (s < A.offset) {  // unpaired candidate of the ragged level stays in place
-      const u32 *src = in + (size_t)s * kCtWords;
-      for (int e = threadIdx.x; e < (int)kCtWords / 4; e += 1024)
-        reinterpret_cast<uint4 *>(dst)[e] = reinterpret_cast<const uint4 *>(src)[e];
-      return;
-    }
-    xi = A.offset + 2 * (s - A.offset);
-    yi = xi + 1;
-  } else {
-    xi = s;
-    yi = s + A.half;
-  }
-  const u32 *x = in + (size_t)xi * kCtWords, *y = in + (size_t)yi * kCtWords;
-  u32 v[8], w[8];
-  load_map3(v, y + (p * 2 + limb) * kN, t);
-  load_map3(w, x + (p * 2 + limb) * kN, t);
-#pragma unroll
-  for (int c = 0; c < 8; c++) v[c] = sub_mod(v[c], w[c], L.q);
-  ntt_inv_2048(v, nlds[lb], t, L, Barrier());
-  store_map0(sh[p][limb], v, t);
-  __syncthreads();
-  i64 D[8];
-  compose_centre8(sh[p][0], sh[p][1], t, S, D);
-  u64 acc0[8], acc1[8];
-#pragma unroll
-  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
-  digit_mac<true>(D, 0, A.l, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[lb], acc0, acc1);
-  u32 o0[8], o1[8];
-#pragma unroll
-  for (int c = 0; c < 8; c++) {
-    o0[c] = reduce_u64(acc0[c], L);
-    o1[c] = reduce_u64(acc1[c], L);
-  }
-  __syncthreads();
-  if (p == 1) {
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-      sh[0][limb][map3(t, c)] = o0[c];
-      sh[1][limb][map3(t, c)] = o1[c];
-    }
-  }
-  __syncthreads();
-  if (p == 0) {
-    u32 x0[8], x1[8];
-    load_map3(x0, x + (0 * 2 + limb) * kN, t);
-    load_map3(x1, x + (1 * 2 + limb) * kN, t);
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-      x0[c] = add_mod(x0[c], add_mod(o0[c], sh[0][limb][map3(t, c)], L.q), L.q);
-      x1[c] = add_mod(x1[c], add_mod(o1[c], sh[1][limb][map3(t, c)], L.q), L.q);
-    }
-    store_map3(dst + (0 * 2 + limb) * kN, x0, t);
-    store_map3(dst + (1 * 2 + limb) * kN, x1, t);
-  }
-}
 
 __device__ __forceinline__ void mux_operands(const MuxArgs &A, int m, int &xi, int &yi) {
   if (A.offset >= 0) {  // ragged: muxed slot offset + m pairs in[offset + 2m], in[offset + 2m + 1]
@@ -567,10 +415,10 @@ __device__ __forceinline__ void mux_operands(const MuxArgs &A, int m, int &xi, i
   }
 }
 
-// The same node in one 512-thread workgroup (lane block = limb) that handles
-// poly p = 0 and then p = 1, accumulating both into one sum: half the
-// threads per node, no cross-pair exchange, so two or more nodes share a CU
-// and their barrier waits overlap. For wide (throughput-bound) levels.
+// Wide levels: one 512-thread workgroup per node (lane block = limb) that
+// handles poly p = 0 and then p = 1 (INTT of poly p of y - x, then its l
+// digits), accumulating both into one sum; two nodes share a CU and their
+// barrier waits overlap.
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_mux_serial(MuxArgs A, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 sh[2][kN];
