@@ -85,10 +85,10 @@ def pct(xs, p):
 
 
 def committed_traffic(args):
-    """HBM bytes per k_scan_mfma launch from the committed rocprofv3 --pmc
+    """HBM bytes per first-dimension scan launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy), when the
     workload is the one it was measured on."""
-    path = os.path.join(ROOT, "profiles", "k_scan_mfma_traffic_256x1024.md")
+    path = os.path.join(ROOT, "profiles", "k_scan_traffic_256x1024.md")
     if (args.target, args.height) != (262144, 9) or not os.path.exists(path):
         return None, None
     for line in open(path):
