@@ -25,6 +25,11 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# Each database context has two streams (answer path + selector completion);
+# with HIP's default of 4 hardware queues, the streams of the serving leg's
+# three databases would share queues and serialise behind each other. Set
+# before the HIP runtime starts; an explicit setting in the environment wins.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 HBM_PEAK_GBS = 8000.0
 
@@ -43,16 +48,21 @@ def parse():
                     help="batch sizes of the batch sweep (comma list; empty to skip)")
     ap.add_argument("--seed", type=int, default=20251142)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
-    ap.add_argument("--serve-lambdas", default="50,200,1000,2000", help="Poisson arrival rates (1/s)")
+    ap.add_argument("--serve-lambdas", default="10,50,200,1000,2000", help="Poisson arrival rates (1/s)")
+    ap.add_argument("--serve-k2-entries", type=int, default=2,
+                    help="pool entries for the n=4096 database in the serving mix (of 128 + this many)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per rate")
     ap.add_argument("--serve-clients", type=int, default=512)
-    ap.add_argument("--serve-budget-gib", type=float, default=48.0,
+    ap.add_argument("--serve-budget-gib", type=float, default=96.0,
                     help="HBM budget for client keys and batch workspaces of the serving loop")
     ap.add_argument("--no-k2", action="store_true", help="skip the CONFIG_N4096_K2_MP leg")
     ap.add_argument("--k2-target", type=int, default=2**21, help="n=4096 layout: target plaintexts")
     ap.add_argument("--k2-height", type=int, default=10)
     ap.add_argument("--k2-batch", type=int, default=32)
     ap.add_argument("--k2-steps", type=int, default=5)
+    ap.add_argument("--scan-shape", default="autotune",
+                    help="first-dimension scan tile shape: autotune (timed per batch-size class), default "
+                         "(measured table) or a shape index")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="run a plain device copy of this many GiB first (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against known bytes)")
@@ -76,6 +86,18 @@ def distinct_indices(n, num_pt, seed):
     return out
 
 
+def scan_shape_arg(args):
+    return {"autotune": -2, "default": -1}.get(args.scan_shape, None) if not args.scan_shape.isdigit() \
+        else int(args.scan_shape)
+
+
+def tuned_shape(G, db_h, b):
+    import ctypes
+    out = ctypes.c_int(-1)
+    G.ctx_call_raw("opir_db_tuned_scan_variant", db_h, ctypes.c_size_t(b), ctypes.byref(out))
+    return out.value
+
+
 def pct(xs, p):
     xs = sorted(xs)
     k = (len(xs) - 1) * p / 100.0
@@ -97,11 +119,12 @@ def committed_traffic(args):
     return None, None
 
 
-def serving(args, H, params, server, nkeyed, log):
-    """Continuous batching over two resident databases: the benchmark DB
-    (256 x 1024) and a 256 x 256 one (1 GiB NTT form), many clients with
-    their own keys on the device, Poisson arrivals swept over rates. Every
-    response is checked against the same query answered alone."""
+def serving(args, H, params, server, log, k2=None):
+    """Continuous batching over resident databases: the benchmark DB
+    (256 x 1024, n=2048), a 256 x 256 one (1 GiB NTT form) and, unless
+    --no-k2, the n=4096 two-prime DB (512 x 4096, 128 GiB NTT form); many
+    clients with their own keys on the device, Poisson arrivals swept over
+    rates. Every response is checked against the same query answered alone."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
     p2 = H.PirParams().with_layout(65536, 9, True)
@@ -131,44 +154,59 @@ def serving(args, H, params, server, nkeyed, log):
 
     with ThreadPoolExecutor(8) as ex:
         qs = list(ex.map(mq, entries))
-    alone = [servers[d].make_query(5000 + j, q.copy()) for (d, j), q in zip(entries, qs)]
-    log(f"[serve] {nc} clients keyed on 2 databases, {len(entries)}-query pool answered alone, "
-        f"{time.time() - t0:.1f}s")
-    loop = H.ServingLoop(servers, max_batch=64, hbm_budget=int(args.serve_budget_gib * 2**30),
-                         keyed_clients=nkeyed + nc)
+    ids = [5000 + j for _, j in entries]
+    names = ["256x1024 n=2048 (4 GiB NTT form)", "256x256 n=2048 (1 GiB NTT form)"]
+    if k2 is not None and args.serve_k2_entries > 0:
+        servers.append(k2["server"])
+        names.append("512x4096 n=4096 K=2 (128 GiB NTT form)")
+        for j in range(min(args.serve_k2_entries, len(k2["queries"]))):
+            entries.append((2, j))
+            ids.append(7000 + j)
+            qs.append(k2["queries"][j])
+    alone = [servers[d].make_query(cid, q.copy()) for (d, _), cid, q in zip(entries, ids, qs)]
+    log(f"[serve] {nc} clients keyed on the n=2048 databases, {len(entries)}-query pool over {len(servers)} "
+        f"databases answered alone, {time.time() - t0:.1f}s")
+    loop = H.ServingLoop(servers, max_batch=64, hbm_budget=int(args.serve_budget_gib * 2**30), keyed_clients=0)
     rows = []
     try:
         for lam in [float(x) for x in args.serve_lambdas.split(",") if x]:
-            run = loop.run_poisson([d for d, _ in entries], [5000 + j for _, j in entries], np.stack(qs), lam,
-                                   args.serve_seconds, args.seed + int(lam))
+            # queries of different schemes differ in length: the loop takes a list
+            run = loop.run_poisson([d for d, _ in entries], ids, qs, lam, args.serve_seconds, args.seed + int(lam))
             n = len(run["pool_index"])
             exact = sum(np.array_equal(run["resp"][i], alone[run["pool_index"][i]]) for i in range(n))
             lat = run["latency_ms"].tolist()
+            per_db = {}
+            for d in range(len(servers)):
+                ld = [lat[i] for i in range(n) if entries[run["pool_index"][i]][0] == d]
+                if ld:
+                    per_db[names[d]] = {"requests": len(ld), "p50_ms": round(pct(ld, 50), 3),
+                                        "p99_ms": round(pct(ld, 99), 3)}
             rows.append({"lambda_qps": lam, "requests": n, "achieved_qps": round(run["achieved_qps"], 1),
                          "p50_ms": round(pct(lat, 50), 3), "p99_ms": round(pct(lat, 99), 3),
                          "mean_batch": round(float(run["batch"].mean()), 2), "bit_exact_vs_alone": f"{exact}/{n}",
-                         "errors": run["errors"]})
+                         "errors": run["errors"], "per_database": per_db})
             log(f"[serve] lambda {lam:.0f}/s: {n} requests, {run['achieved_qps']:.0f} q/s, p50 {pct(lat, 50):.2f} ms "
-                f"p99 {pct(lat, 99):.2f} ms, mean batch {run['batch'].mean():.1f}, exact {exact}/{n}")
+                f"p99 {pct(lat, 99):.2f} ms, mean batch {run['batch'].mean():.1f}, exact {exact}/{n}, "
+                f"per db {[(k.split()[0], v['requests'], v['p50_ms'], v['p99_ms']) for k, v in per_db.items()]}")
             assert exact == n and run["errors"] == 0, "served response differs from the query answered alone"
         limits = list(loop.batch_limits)
     finally:
         loop.stop()
-    return {"databases": ["256x1024 (4 GiB NTT form)", "256x256 (1 GiB NTT form)"], "clients": nc,
+    return {"databases": names, "clients_n2048": nc,
+            "clients_n4096": min(args.serve_k2_entries, len(k2["queries"])) if k2 is not None else 0,
             "hbm_budget_gib": args.serve_budget_gib, "batch_limits": limits, "rates": rows}
 
 
-def k2_leg(args, H, G, torch, log):
-    """CONFIG_N4096_K2_MP (two ~60-bit RNS limbs, N = 4096) on the layout
-    with_layout({2^21, 10}) = 512 x 4096 plaintexts, 128 GiB in NTT form,
-    generated in HBM from a seed. Plain single-query path and one fused batch."""
-    import ctypes
+def k2_setup(args, H, log):
+    """The n=4096 two-prime database (with_layout({2^21, 10}) = 512 x 4096
+    plaintexts, 128 GiB in NTT form, generated in HBM from a seed) and
+    k2_batch keyed clients with one query each."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
     p = H.PirParams("CONFIG_N4096_K2_MP").with_layout(args.k2_target, args.k2_height, True)
-    N, B = p.N, args.k2_batch
+    B = args.k2_batch
     log(f"[k2] layout N0={p.fst_dim_sz} Nrest={p.other_dim_sz} dims={p.num_dims} num_pt={p.num_pt} "
-        f"({p.num_pt * N * 2 * 8 / 2**30:.0f} GiB NTT form)")
+        f"({p.num_pt * p.N * 2 * 8 / 2**30:.0f} GiB NTT form)")
     t0 = time.time()
     server = H.PirServer(p)
     idxs = distinct_indices(max(B, 2), p.num_pt, args.seed + 3)
@@ -182,12 +220,24 @@ def k2_leg(args, H, G, torch, log):
 
     t0 = time.time()
     with ThreadPoolExecutor(8) as ex:
-        cl = list(ex.map(mk, range(B)))
+        cl = list(ex.map(mk, range(max(B, 2))))
     for j, (c, bv, gsw, _) in enumerate(cl):
         server.set_client_bv_galois_key(7000 + j, bv)
         server.set_client_gsw_key(7000 + j, gsw)
     queries = np.stack([x[3] for x in cl])
-    log(f"[k2] {B} clients keyed, queries built in {time.time() - t0:.1f}s")
+    log(f"[k2] {len(cl)} clients keyed, queries built in {time.time() - t0:.1f}s")
+    return {"params": p, "server": server, "clients": [x[0] for x in cl], "queries": queries, "idxs": idxs}
+
+
+def k2_leg(args, H, G, torch, log, k2):
+    """CONFIG_N4096_K2_MP (two ~60-bit RNS limbs, N = 4096) on the layout
+    with_layout({2^21, 10}) = 512 x 4096 plaintexts, 128 GiB in NTT form:
+    plain single-query path and one fused batch."""
+    import ctypes
+    import numpy as np
+    p, server, queries, idxs = k2["params"], k2["server"], k2["queries"], k2["idxs"]
+    cl = [(c,) for c in k2["clients"]]
+    N, B = p.N, args.k2_batch
     for j in range(2):
         r = server.make_query(7000 + j, queries[j].copy())
         pt, budget = cl[j][0].decrypt_mod_q(r)
@@ -196,6 +246,7 @@ def k2_leg(args, H, G, torch, log):
     db_h, ctx_h = server.db_handle(), server.ctx_handle()
     stream = torch.cuda.current_stream()
     assert G.ctx_call_raw("opir_ctx_set_stream", ctx_h, ctypes.c_void_p(stream.cuda_stream)) == 0
+    assert G.ctx_call_raw("opir_ctx_set_scan_variant", ctx_h, scan_shape_arg(args)) == 0
     dev = torch.device("cuda:0")
     d_q = torch.from_numpy(queries.view(np.int64)).to(dev)
     d_r = torch.empty((B, 2, N), dtype=torch.int64, device=dev)
@@ -251,7 +302,7 @@ def k2_leg(args, H, G, torch, log):
         rb = d_r.cpu().numpy().view(np.uint64)
         for j in (0, B - 1):
             assert np.array_equal(rb[j], server.make_query(7000 + j, queries[j].copy())), "n=4096 batch != plain"
-    del d_q, d_r, server
+    del d_q, d_r
     torch.cuda.synchronize()
     return res
 
@@ -341,6 +392,7 @@ def main():
     torch.cuda.set_stream(stream)
     assert stream.cuda_stream, "expected a non-NULL HIP stream"
     assert G.ctx_call_raw("opir_ctx_set_stream", ctx_h, __import__("ctypes").c_void_p(stream.cuda_stream)) == 0
+    assert G.ctx_call_raw("opir_ctx_set_scan_variant", ctx_h, scan_shape_arg(args)) == 0
     if args.copy_probe_gib > 0:
         n = int(args.copy_probe_gib * 2**30) // 4
         src = torch.ones(n, dtype=torch.int32, device=dev)
@@ -424,7 +476,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
                      "kernel": "k_scan_direct / k_scan (first-dimension scan, int8 MFMA over 4 digit planes)", "kernel_ms": round(scan, 4),
-                     "algorithmic_bytes": scan_bytes},
+                     "algorithmic_bytes": scan_bytes, "scan_shape": tuned_shape(G, db_h, 1)},
+        "scan_shape_mode": args.scan_shape,
     }
     if not args.no_batch:
         total_ms, per_b, wall = timed(B, max(3, args.steps // 4), 1)
@@ -434,7 +487,8 @@ def main():
                              "queries_per_s": round(B * 1e3 / ms_b, 2),
                              "db_gib_per_s": round(B * gib_db / (ms_b * 1e-3), 3),
                              "speedup_vs_plain_qps": round((B * 1e3 / ms_b) / (1e3 / ms_plain), 3),
-                             "latency_ms": {"p50": round(pct(per_b, 50), 3), "p99": round(pct(per_b, 99), 3)}}
+                             "latency_ms": {"p50": round(pct(per_b, 50), 3), "p99": round(pct(per_b, 99), 3)},
+                             "scan_shape": tuned_shape(G, db_h, B)}
         log(f"[batch] B={B}: {ms_b:.2f} ms/batch -> {B * 1e3 / ms_b:.1f} q/s")
         # batched answers must equal the plain ones
         answer(B)
@@ -449,15 +503,18 @@ def main():
             total_ms, per_s, _ = timed(b, steps_s, 1)
             ms_s = total_ms / steps_s
             rows.append({"batch": b, "ms_per_batch": round(ms_s, 3), "queries_per_s": round(b * 1e3 / ms_s, 2),
-                         "p50_ms": round(pct(per_s, 50), 3), "p99_ms": round(pct(per_s, 99), 3), "steps": steps_s})
+                         "p50_ms": round(pct(per_s, 50), 3), "p99_ms": round(pct(per_s, 99), 3), "steps": steps_s,
+                         "scan_shape": tuned_shape(G, db_h, b)})
             log(f"[sweep] B={b}: {ms_s:.2f} ms/batch -> {b * 1e3 / ms_s:.1f} q/s")
         if rows:
             result["batch_sweep"] = rows
+    k2 = k2_setup(args, H, log) if not args.no_k2 else None
     if not args.no_serve:
         torch.cuda.synchronize()
-        result["serving"] = serving(args, H, params, server, nclients, log)
-    if not args.no_k2:
-        result["n4096_k2"] = k2_leg(args, H, G, torch, log)
+        result["serving"] = serving(args, H, params, server, log, k2)
+    if k2 is not None:
+        result["n4096_k2"] = k2_leg(args, H, G, torch, log, k2)
+        del k2
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(args, log)

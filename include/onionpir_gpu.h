@@ -129,6 +129,8 @@ int opir_answer_batch_device(opir_db *db, size_t B, const uint64_t *client_ids, 
 int opir_db_workspace_bytes(const opir_db *db, size_t B, size_t *out);
 int opir_db_reserve(opir_db *db, size_t B);
 int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out);
+/* Device bytes of all client keys resident on ctx now. */
+int opir_ctx_resident_key_bytes(const opir_ctx *ctx, size_t *out);
 
 /* Per-stage device time of the last answer call under the reference's stage
  * names (src/includes/logging.h:122-134): [0] Expand, [1] Convert,
@@ -140,13 +142,20 @@ int opir_ctx_set_profiling(opir_ctx *ctx, int on);
 int opir_last_stage_ms(opir_ctx *ctx, float *out6);
 
 /* First-dimension scan tile shape (no counterpart in the reference, whose
- * scan has one CPU loop nest): -1 (the default) picks the shape per batch
- * size; 0 .. count-1 forces one, for tuning and for tests that check every
- * shape against the same answers. A forced shape that cannot take a batch
- * size makes the answer call fail with OPIR_EINVAL. Results are identical
- * for every shape. */
+ * scan has one CPU loop nest): OPIR_SCAN_DEFAULT (-1, the default) picks the
+ * shape per batch size from a table measured on MI355X; OPIR_SCAN_AUTOTUNE
+ * (-2) times every applicable shape on the live operands the first time a
+ * database sees a batch-size class (powers of two of 2B) and keeps the
+ * fastest; 0 .. count-1 forces one, for tuning and for tests that check
+ * every shape against the same answers. A forced shape that cannot take a
+ * batch size makes the answer call fail with OPIR_EINVAL. Results are
+ * identical for every shape. */
+#define OPIR_SCAN_DEFAULT (-1)
+#define OPIR_SCAN_AUTOTUNE (-2)
 int opir_ctx_set_scan_variant(opir_ctx *ctx, int variant);
 int opir_scan_variant_count(const opir_ctx *ctx, int *out);
+/* The shape autotuning picked for B queries on db (-1 if not tuned yet). */
+int opir_db_tuned_scan_variant(const opir_db *db, size_t B, int *out);
 
 /* ---- Stage entry points (host buffers), one per reference stage ---- */
 

@@ -142,8 +142,10 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   launch_ct_ntt(leaves, db->qntt, exp_stride, db->N0 * kCtWords, (int)db->N0, (int)B, ctx->S, ctx->stream);
   launch_qfrag(db->qntt, db->qfrag, db->N0 * kCtWords, (int)B, db->geom, ctx->stream);
   rec(ctx, 5);
-  if (launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream,
-                  ctx->scan_variant))
+  const int sv = pick_scan_variant(db, B, 4, [&](int v) {
+    launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream, v);
+  });
+  if (launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream, sv))
     return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
   LAUNCH_CHECK(ctx);
   rec(ctx, 6);
@@ -341,9 +343,15 @@ int opir_ctx_set_profiling(opir_ctx *ctx, int on) {
 
 int opir_ctx_set_scan_variant(opir_ctx *ctx, int variant) {
   if (!ctx) return OPIR_EINVAL;
-  if (variant < -1 || variant >= scan_variant_count(ctx->k2 ? 8 : 4))
+  if (variant < -2 || variant >= scan_variant_count(ctx->k2 ? 8 : 4))
     return set_err(ctx, OPIR_EINVAL, "scan variant out of range");
   ctx->scan_variant = variant;
+  return OPIR_OK;
+}
+
+int opir_db_tuned_scan_variant(const opir_db *db, size_t B, int *out) {
+  if (!db || !out || B == 0) return OPIR_EINVAL;
+  *out = db->tuned_scan[scan_class(B)];
   return OPIR_OK;
 }
 
@@ -606,6 +614,18 @@ int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out) {
   return OPIR_OK;
 }
 
+int opir_ctx_resident_key_bytes(const opir_ctx *ctx, size_t *out) {
+  if (!ctx || !out) return OPIR_EINVAL;
+  size_t tot = 0;
+  for (const auto &kv : ctx->clients) {
+    size_t kb = 0;
+    opir_client_key_bytes(ctx, kv.second.h, &kb);
+    tot += kb;
+  }
+  *out = tot;
+  return OPIR_OK;
+}
+
 int opir_answer_batch_device(opir_db *db, size_t B, const uint64_t *ids, const uint64_t *d_q, uint64_t *d_r) {
   if (!db || !ids || !d_q || !d_r) return OPIR_EINVAL;
   if (B == 0) return OPIR_OK;
@@ -734,7 +754,8 @@ int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
   launch_split_u64((u64 *)e.p, db->exp_a, db->N0 * 2, ctx->S, ctx->stream);
   launch_ct_ntt(db->exp_a, db->qntt, 0, 0, (int)db->N0, 1, ctx->S, ctx->stream);
   launch_qfrag(db->qntt, db->qfrag, 0, 1, db->geom, ctx->stream);
-  if (launch_scan(db->d_db, db->qfrag, db->mid_a, 0, 1, db->geom, ctx->S, ctx->stream, ctx->scan_variant))
+  if (launch_scan(db->d_db, db->qfrag, db->mid_a, 0, 1, db->geom, ctx->S, ctx->stream,
+                  ctx->scan_variant == -2 ? -1 : ctx->scan_variant))
     return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
   DevBuf o;
   DEV_ALLOC(ctx, o, db->Nrest * 2 * kN * sizeof(u64));

@@ -44,7 +44,7 @@ struct opir_ctx {
   std::unordered_map<uint64_t, capi::ClientKeys> clients;
   std::string err;
   bool profiling = false;
-  int scan_variant = -1;    // first-dimension scan tile shape, -1 = default per batch size
+  int scan_variant = -1;    // first-dimension scan tile shape: -1 table default, -2 autotune, >= 0 forced
   hipEvent_t ev[8] = {};
   // auxiliary stream (default priority): selector completion runs on it
   // concurrently with the first-dimension scan
@@ -74,7 +74,55 @@ struct opir_db {
   u64 *e64_a = nullptr, *e64_b = nullptr, *q64 = nullptr, *sel64 = nullptr, *m64_a = nullptr, *m64_b = nullptr;
   u64 *q_u64 = nullptr, *r_u64 = nullptr;
   size_t exp_cts = 0;
+  // autotuned scan shape per batch-size class (ceil(log2(2B))), -1 = not tuned
+  int tuned_scan[16] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
 };
+
+namespace capi {
+inline int scan_class(size_t B) {
+  int c = 0;
+  while (c < 15 && (size_t{1} << c) < 2 * B) c++;
+  return c;
+}
+
+// The scan shape for B queries. In autotune mode (-2) the first call per
+// batch-size class times every applicable shape on the live operands (two
+// runs each, HIP events on the context stream; every shape computes the same
+// bytes) and keeps the fastest. launch(v) enqueues the scan with shape v.
+template <typename Launch>
+int pick_scan_variant(opir_db *db, size_t B, int P, Launch launch) {
+  opir_ctx *ctx = db->ctx;
+  if (ctx->scan_variant != -2) return ctx->scan_variant;
+  int &slot = db->tuned_scan[scan_class(B)];
+  if (slot >= 0 && scan_variant_ok(P, slot, (int)B, db->geom)) return slot;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -1;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return -1;
+  }
+  float best = 1e30f;
+  int pick = -1;
+  for (int v = 0; v < scan_variant_count(P); v++) {
+    if (!scan_variant_ok(P, v, (int)B, db->geom)) continue;
+    launch(v);  // warm-up
+    (void)hipEventRecord(e0, ctx->stream);
+    launch(v);
+    launch(v);
+    (void)hipEventRecord(e1, ctx->stream);
+    float ms = 0;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+    if (ms < best) {
+      best = ms;
+      pick = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  slot = pick;
+  return pick;
+}
+}  // namespace capi
 
 namespace capi {
 

@@ -303,8 +303,10 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
   k2::launch_k2_ct_ntt(leaves, db->q64, exp_stride, db->N0 * kCt2, (int)db->N0, (int)B, ctx->S64, ctx->stream);
   launch_qfrag64(db->q64, db->qfrag, db->N0 * kCt2, (int)B, db->geom, ctx->stream);
   rec(ctx, 5);
-  if (launch_scan64(db->d_db, db->qfrag, db->m64_a, db->Nrest * kCt2, (int)B, db->geom, ctx->S64, ctx->stream,
-                    ctx->scan_variant))
+  const int sv = pick_scan_variant(db, B, 8, [&](int v) {
+    launch_scan64(db->d_db, db->qfrag, db->m64_a, db->Nrest * kCt2, (int)B, db->geom, ctx->S64, ctx->stream, v);
+  });
+  if (launch_scan64(db->d_db, db->qfrag, db->m64_a, db->Nrest * kCt2, (int)B, db->geom, ctx->S64, ctx->stream, sv))
     return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
   LAUNCH_CHECK(ctx);
   rec(ctx, 6);
@@ -385,7 +387,8 @@ int k2_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
   HIP_TRY(ctx, hipMemcpy(db->e64_a, query, db->N0 * kCt2 * sizeof(u64), hipMemcpyHostToDevice));
   k2::launch_k2_ct_ntt(db->e64_a, db->q64, 0, 0, (int)db->N0, 1, ctx->S64, ctx->stream);
   launch_qfrag64(db->q64, db->qfrag, 0, 1, db->geom, ctx->stream);
-  if (launch_scan64(db->d_db, db->qfrag, db->m64_a, 0, 1, db->geom, ctx->S64, ctx->stream, ctx->scan_variant))
+  if (launch_scan64(db->d_db, db->qfrag, db->m64_a, 0, 1, db->geom, ctx->S64, ctx->stream,
+                    ctx->scan_variant == -2 ? -1 : ctx->scan_variant))
     return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
   k2::launch_k2_ntt(db->m64_a, db->Nrest * 2, true, ctx->S64, ctx->stream);
   LAUNCH_CHECK(ctx);

@@ -113,8 +113,10 @@ void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel
 void launch_qfrag(const u32 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st);
 int launch_scan(const int8_t *db, const int8_t *qfrag, u32 *mid, size_t mid_stride, int B, const DbGeom &g,
                 const SchemeConst &S, hipStream_t st, int variant);
-// Number of scan tile shapes for P digit planes (4: K = 1, 8: K = 2).
+// Number of scan tile shapes for P digit planes (4: K = 1, 8: K = 2), and
+// whether shape `variant` can take B queries on geometry g.
 int scan_variant_count(int P);
+bool scan_variant_ok(int P, int variant, int B, const DbGeom &g);
 // The same for the two-prime scheme (P = 8 planes, u64 residues):
 // tmp [limb][count][nlv] u64 -> DB; qntt [B][N0][2][2][nlv] u64 -> qfrag;
 // mid [B][Nrest][2][2][nlv] u64.

@@ -631,6 +631,11 @@ static int dispatch(const ScanEntry<Epi, SchemeT> (&tab)[NV], const int (&pref)[
   return 0;
 }
 
+bool scan_variant_ok(int P, int variant, int B, const DbGeom &g) {
+  if (variant < 0 || variant >= scan_variant_count(P)) return false;
+  return applicable(P == 4 ? kScanK1[variant].shape : kScanK2[variant].shape, B, g);
+}
+
 int launch_scan(const int8_t *db, const int8_t *qfrag, u32 *mid, size_t mid_stride, int B, const DbGeom &g,
                 const SchemeConst &S, hipStream_t st, int variant) {
   return dispatch(kScanK1, kPrefK1, variant, db, qfrag, mid, mid_stride, B, g, S, st);

@@ -16,6 +16,8 @@ OPIR_ERANGE = -2
 OPIR_ERUNTIME = -3
 OPIR_EHIP = -4
 OPIR_ENOMEM = -5
+SCAN_DEFAULT = -1
+SCAN_AUTOTUNE = -2
 
 # Every symbol include/onionpir_gpu.h declares.
 EXPORTED = [
@@ -23,8 +25,8 @@ EXPORTED = [
     "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt", "opir_db_load_plain_u64", "opir_db_gen_random",
     "opir_client_set_keys", "opir_client_remove",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
-    "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes",
-    "opir_ctx_set_profiling", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_scan_variant_count",
+    "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
+    "opir_ctx_set_profiling", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
     "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
     "opir_ext_prod", "opir_mod_switch",
 ]
@@ -87,10 +89,12 @@ def load_library():
         "opir_db_workspace_bytes": ([P, sz, ctypes.POINTER(sz)], ctypes.c_int),
         "opir_db_reserve": ([P, sz], ctypes.c_int),
         "opir_client_key_bytes": ([P, sz, ctypes.POINTER(sz)], ctypes.c_int),
+        "opir_ctx_resident_key_bytes": ([P, ctypes.POINTER(sz)], ctypes.c_int),
         "opir_ctx_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
         "opir_last_stage_ms": ([P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "opir_ctx_set_scan_variant": ([P, ctypes.c_int], ctypes.c_int),
         "opir_scan_variant_count": ([P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "opir_db_tuned_scan_variant": ([P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "opir_ntt_fwd": ([P, P, sz], ctypes.c_int),
         "opir_ntt_inv": ([P, P, sz], ctypes.c_int),
         "opir_expand": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
@@ -169,7 +173,9 @@ class GpuContext:
         self.check(self.lib.opir_ctx_set_profiling(self.h, 1 if on else 0))
 
     def set_scan_variant(self, variant=-1):
-        """Force a first-dimension scan tile shape (-1: default per batch size)."""
+        """First-dimension scan tile shape: SCAN_DEFAULT (-1, table per batch
+        size), SCAN_AUTOTUNE (-2, timed on first use per batch-size class), or
+        a forced shape index."""
         self.check(self.lib.opir_ctx_set_scan_variant(self.h, int(variant)))
 
     def scan_variant_count(self):
@@ -272,6 +278,12 @@ class GpuDatabase:
         self.ctx.check(self.lib.opir_answer_batch_device(self.h, len(ids), _ptr(ids),
                                                          ctypes.c_void_p(d_queries_ptr),
                                                          ctypes.c_void_p(d_resps_ptr)))
+
+    def tuned_scan_variant(self, batch):
+        """Scan shape the autotuner picked for this batch size (-1: not tuned)."""
+        out = ctypes.c_int(-1)
+        self.ctx.check(self.lib.opir_db_tuned_scan_variant(self.h, batch, ctypes.byref(out)))
+        return out.value
 
     def expand(self, client_id, query):
         q = _u64(query)

@@ -298,26 +298,34 @@ PYBIND11_MODULE(_opir_host, m) {
       .def("stop", &ServingLoop::stop, py::call_guard<py::gil_scoped_release>())
       .def_property_readonly("batch_limits", &ServingLoop::batch_limits)
       .def("run_poisson",
-           [](ServingLoop &l, std::vector<size_t> dbs, std::vector<size_t> cids, Arr qs, double lambda, double duration,
-              u64 seed) {
-             if (qs.ndim() != 3 || qs.shape(1) != 2 || (size_t)qs.shape(0) != dbs.size() || dbs.size() != cids.size())
-               throw std::invalid_argument("pool: dbs, client ids and queries [P, 2, N] must match");
-             const size_t P = qs.shape(0), N = qs.shape(2);
-             std::vector<PoolEntry> pool(P);
-             for (size_t i = 0; i < P; i++)
-               pool[i] = PoolEntry{dbs[i], cids[i], std::vector<u64>(qs.data() + i * 2 * N, qs.data() + (i + 1) * 2 * N)};
+           // qs: one query [2, K*N] per pool entry (a list, or a [P, 2, K*N] array when all
+           // databases share a scheme). resp comes back as a list of [2, N] arrays.
+           [](ServingLoop &l, std::vector<size_t> dbs, std::vector<size_t> cids, py::object qs, double lambda,
+              double duration, u64 seed) {
+             std::vector<Arr> qv;
+             if (py::isinstance<py::array>(qs) && Arr::ensure(qs).ndim() == 3) {
+               Arr a = Arr::ensure(qs);
+               for (py::ssize_t i = 0; i < a.shape(0); i++) qv.push_back(Arr::ensure(qs[py::int_(i)]));
+             } else {
+               for (auto h : qs) qv.push_back(Arr::ensure(h));
+             }
+             if (qv.size() != dbs.size() || dbs.size() != cids.size())
+               throw std::invalid_argument("pool: dbs, client ids and queries must have one entry each");
+             std::vector<PoolEntry> pool(qv.size());
+             for (size_t i = 0; i < qv.size(); i++) {
+               Arr q = Arr::ensure(qv[i]);
+               if (!q || q.ndim() != 2 || q.shape(0) != 2) throw std::invalid_argument("each query must be [2, K*N]");
+               pool[i] = PoolEntry{dbs[i], cids[i], std::vector<u64>(q.data(), q.data() + q.size())};
+             }
              LoadRun run;
              {
                py::gil_scoped_release nogil;
                run = run_poisson(l, pool, lambda, duration, seed);
              }
              const size_t n = run.results.size();
-             size_t RN = 0;  // response values per polynomial (one limb mod small_q)
-             for (const auto &x : run.results)
-               if (!x.resp.empty()) { RN = x.resp.size() / 2; break; }
              py::array_t<int64_t> idx((py::ssize_t)n), batch((py::ssize_t)n);
              py::array_t<double> lat((py::ssize_t)n), wait((py::ssize_t)n);
-             Arr resp({(py::ssize_t)n, (py::ssize_t)2, (py::ssize_t)RN});
+             py::list resp;
              size_t errors = 0;
              for (size_t i = 0; i < n; i++) {
                const ServeResult &x = run.results[i];
@@ -325,12 +333,11 @@ PYBIND11_MODULE(_opir_host, m) {
                batch.mutable_data()[i] = (int64_t)x.batch;
                lat.mutable_data()[i] = (x.t_done - x.t_submit) * 1e3;
                wait.mutable_data()[i] = (x.t_start - x.t_submit) * 1e3;
-               if (RN && x.resp.size() == 2 * RN)
-                 std::memcpy(resp.mutable_data() + i * 2 * RN, x.resp.data(), 2 * RN * 8);
-               else {
-                 if (RN) std::memset(resp.mutable_data() + i * 2 * RN, 0, 2 * RN * 8);
-                 errors++;
-               }
+               const size_t RN = x.resp.size() / 2;  // response values per polynomial
+               Arr a({(py::ssize_t)2, (py::ssize_t)RN});
+               if (RN) std::memcpy(a.mutable_data(), x.resp.data(), 2 * RN * 8);
+               if (x.resp.empty() || !x.error.empty()) errors++;
+               resp.append(a);
              }
              py::dict d;
              d["offered_qps"] = run.offered_qps;

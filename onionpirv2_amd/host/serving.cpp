@@ -45,9 +45,15 @@ ServingLoop::ServingLoop(std::vector<PirServer *> dbs, const ServeConfig &cfg) {
     if (!by_ctx.count(ctx)) {
       ctx_mus_.push_back(std::make_unique<std::mutex>());
       by_ctx[ctx] = ctx_mus_.back().get();
-      size_t kb = 0;
-      s->gpu_scheme()->check(opir_client_key_bytes(ctx, s->get_params().get_expan_height(), &kb));
-      fixed += cfg.keyed_clients * kb;
+      if (cfg.keyed_clients) {  // planned key count at this context's key size
+        size_t kb = 0;
+        s->gpu_scheme()->check(opir_client_key_bytes(ctx, s->get_params().get_expan_height(), &kb));
+        fixed += cfg.keyed_clients * kb;
+      } else {  // the keys resident now
+        size_t kb = 0;
+        s->gpu_scheme()->check(opir_ctx_resident_key_bytes(ctx, &kb));
+        fixed += kb;
+      }
     }
     auto lane = std::make_unique<DbLane>();
     lane->srv = s;

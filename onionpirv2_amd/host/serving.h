@@ -30,7 +30,7 @@ namespace onionpir {
 struct ServeConfig {
   size_t max_batch = 64;      // upper bound per database and batch
   size_t hbm_budget = 0;      // bytes for keys + workspaces; 0 = bounded by max_batch only
-  size_t keyed_clients = 0;   // clients whose keys are resident, counted against the budget
+  size_t keyed_clients = 0;   // clients per context counted against the budget; 0 = the keys resident at start
 };
 
 struct ServeResult {

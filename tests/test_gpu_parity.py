@@ -231,3 +231,29 @@ def test_every_scan_shape_matches_oracle(O, comp_params, gpu_ctx, layout):
     assert len(got) >= 4
     for v, b in got.items():
         assert np.array_equal(b, want), f"scan shape {v}"
+
+
+def test_autotuned_scan_matches_default(O, comp_params, gpu_ctx):
+    """Autotune mode times the applicable shapes once per batch-size class,
+    keeps one, and answers exactly as the default shape does."""
+    from onionpirv2_amd.gpu import SCAN_AUTOTUNE, SCAN_DEFAULT
+    say("[gpu] autotuned scan")
+    s = Setup(O, comp_params, (4096, 9, True), seed=5150, gpu_ctx=gpu_ctx, cid=51)
+    p = s.p
+    idxs = [(17 * j + 3) % p.target_num_pt for j in range(9)]
+    qs = np.stack([s.client.fast_generate_query(p, i) for i in idxs])
+    ids = [s.cid] * len(idxs)
+    want1 = s.db.answer(s.cid, qs[0])
+    want9 = s.db.answer_batch(ids, qs)
+    assert s.db.tuned_scan_variant(1) == -1
+    try:
+        gpu_ctx.set_scan_variant(SCAN_AUTOTUNE)
+        for _ in range(2):
+            assert np.array_equal(s.db.answer(s.cid, qs[0]), want1)
+            assert np.array_equal(s.db.answer_batch(ids, qs), want9)
+    finally:
+        gpu_ctx.set_scan_variant(SCAN_DEFAULT)
+    n = gpu_ctx.scan_variant_count()
+    assert 0 <= s.db.tuned_scan_variant(1) < n
+    assert 0 <= s.db.tuned_scan_variant(9) < n
+    assert np.array_equal(want1, s.server.make_query(s.cid, qs[0]))

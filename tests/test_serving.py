@@ -110,3 +110,40 @@ def test_budget_too_small_is_rejected(served):
     H, servers, _, _, _ = served
     with pytest.raises(ValueError):
         H.ServingLoop(servers, max_batch=8, hbm_budget=1 << 20, keyed_clients=4)
+
+
+@pytest.mark.gpu
+def test_mixed_schemes_served_equal_alone(served):
+    """An n=4096 two-prime database served beside the n=2048 ones: queries of
+    both schemes in one Poisson mix, every response equal to the same query
+    answered alone; admission counts the keys resident on each context."""
+    H, servers, clients, pool, alone = served
+    print("[gpu] serving mixed schemes", flush=True)
+    p2 = H.PirParams("CONFIG_N4096_K2_MP").with_layout(1024, 8, True)
+    s2 = H.PirServer(p2)
+    s2.gen_data_device([3, 1000], 4242)
+    c2 = H.PirClient(p2, 800, 31)
+    s2.set_client_bv_galois_key(800, c2.create_bv_galois_keys())
+    s2.set_client_gsw_key(800, c2.generate_gsw_from_key())
+    q2 = [c2.fast_generate_query(p2, i) for i in (3, 1000)]
+    a2 = [s2.make_query(800, q.copy()) for q in q2]
+    loop = H.ServingLoop(servers + [s2], max_batch=8, hbm_budget=24 << 30)
+    try:
+        assert all(1 <= b <= 8 for b in loop.batch_limits)
+        dbs = [d for d, _, _, _ in pool] + [2, 2]
+        cids = [c for _, c, _, _ in pool] + [800, 800]
+        qs = [q for _, _, _, q in pool] + q2
+        run = loop.run_poisson(dbs, cids, qs, 600.0, 0.5, 777)
+    finally:
+        loop.stop()
+    want = alone + a2
+    n = len(run["pool_index"])
+    assert n > 100 and run["errors"] == 0
+    seen_k2 = 0
+    for i in range(n):
+        k = run["pool_index"][i]
+        assert np.array_equal(run["resp"][i], want[k])
+        seen_k2 += k >= len(pool)
+    assert seen_k2 > 0
+    pt, budget = c2.decrypt_mod_q(a2[1])
+    assert np.array_equal(pt, s2.direct_get_original_plaintext(1000)) and budget > 0
