@@ -371,7 +371,11 @@ __global__ __launch_bounds__(256) void k_scan(ScanArgs<Epi> A, SchemeT S) {
 // MFMAs of the first k block start before the last load has landed.
 // SC: result columns staged per row (>= stored columns of the tile: 2 for
 // one query). For the HBM-bound small batches.
-template <int KC, int LV, int SC, bool NT, typename Epi, typename SchemeT>
+// XPF: load the next batch of k blocks right after the current batch's MFMAs
+// (across level boundaries, so a level's epilogue overlaps the next level's
+// loads; the registers stay live through the epilogue). Otherwise each batch
+// is loaded at the top of its own iteration.
+template <int KC, int LV, int SC, bool NT, bool XPF, typename Epi, typename SchemeT>
 __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S) {
   constexpr int P = Epi::P, NU = 2 * P - 1;
   typedef typename Epi::Res Res;
@@ -406,30 +410,35 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
                         ((size_t)grp * qg.ncm + (has_col ? n : 0)) * 16;
   const size_t bstride = (size_t)qg.nct * qg.nkb * qtile;
 
+  // one register buffer for a batch of KC k blocks
+  const int nch = nkb / KC;
+  v4i a[KC][P], b[KC][P];
+  auto load = [&](int lrel, int ch) {
+    const int8_t *pa = abase + (size_t)lrel * astride + (size_t)ch * KC * (P * 1024);
+    const int8_t *pb = bbase + (size_t)lrel * bstride + (size_t)ch * KC * qtile;
+#pragma unroll
+    for (int j = 0; j < KC; j++)
+#pragma unroll
+      for (int s = 0; s < P; s++) {
+        const v4i *q = reinterpret_cast<const v4i *>(pa + (size_t)j * (P * 1024) + s * 1024);
+        a[j][s] = NT ? __builtin_nontemporal_load(q) : *q;
+      }
+#pragma unroll
+    for (int j = 0; j < KC; j++)
+#pragma unroll
+      for (int t = 0; t < P; t++)
+        b[j][t] = *reinterpret_cast<const v4i *>(pb + (size_t)j * qtile + (size_t)t * 4 * qg.ncm * 16);
+  };
+  if (XPF) load(0, 0);
   for (int lsub = 0; lsub < kLevelsPerWG; lsub += LV) {
 #pragma unroll 1
     for (int li = 0; li < LV; li++) {
-      const int8_t *pa = abase + (size_t)(lsub + li) * astride;
-      const int8_t *pb = bbase + (size_t)(lsub + li) * bstride;
       v4i acc[NU];
 #pragma unroll
       for (int u = 0; u < NU; u++) acc[u] = (v4i){0, 0, 0, 0};
 #pragma unroll 1
-      for (int kc = 0; kc < nkb; kc += KC) {
-        v4i a[KC][P], b[KC][P];
-#pragma unroll
-        for (int j = 0; j < KC; j++)
-#pragma unroll
-          for (int s = 0; s < P; s++)
-          {
-            const v4i *q = reinterpret_cast<const v4i *>(pa + (size_t)(kc + j) * (P * 1024) + s * 1024);
-            a[j][s] = NT ? __builtin_nontemporal_load(q) : *q;
-          }
-#pragma unroll
-        for (int j = 0; j < KC; j++)
-#pragma unroll
-          for (int t = 0; t < P; t++)
-            b[j][t] = *reinterpret_cast<const v4i *>(pb + (size_t)(kc + j) * qtile + (size_t)t * 4 * qg.ncm * 16);
+      for (int ch = 0; ch < nch; ch++) {
+        if (!XPF) load(lsub + li, ch);
 #pragma unroll
         for (int j = 0; j < KC; j++)
 #pragma unroll
@@ -437,6 +446,11 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
 #pragma unroll
             for (int t = 0; t < P; t++)
               acc[s + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[j][s], b[j][t] & bmask, acc[s + t], 0, 0, 0);
+        if (XPF) {
+          const bool last = ch + 1 == nch;
+          const int nl = lsub + li + (last ? 1 : 0);
+          if (nl < kLevelsPerWG) load(nl, last ? 0 : ch + 1);
+        }
       }
 #pragma unroll
       for (int r = 0; r < 4; r++) {
@@ -530,6 +544,7 @@ void launch_qfrag64(const u64 *qntt, int8_t *qfrag, size_t q_stride, int B, cons
 struct ScanShape {
   bool direct;
   int CB, KC, LV, SC;
+  bool xpf;
 };
 
 template <typename Epi>
@@ -560,14 +575,14 @@ static void run_lds(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mi
     k_scan<CB, LV, false, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
 }
 
-template <int KC, int LV, int SC, typename Epi, typename SchemeT>
+template <int KC, int LV, int SC, bool XPF, typename Epi, typename SchemeT>
 static void run_direct(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
                        const DbGeom &g, const SchemeT &S, hipStream_t st) {
   const ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, 1);
   if (A.ncb == 1)
-    k_scan_direct<KC, LV, SC, true, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+    k_scan_direct<KC, LV, SC, true, XPF, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
   else
-    k_scan_direct<KC, LV, SC, false, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+    k_scan_direct<KC, LV, SC, false, XPF, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
 }
 
 template <typename Epi, typename SchemeT>
@@ -582,24 +597,25 @@ struct ScanEntry {
 
 typedef k2::Scheme64 S64;
 static const ScanEntry<EpiK1, SchemeConst> kScanK1[] = {
-    {{true, 1, 4, 16, 2}, run_direct<4, 16, 2, EpiK1, SchemeConst>},
-    {{true, 1, 4, 16, 16}, run_direct<4, 16, 16, EpiK1, SchemeConst>},
-    {{true, 1, 2, 16, 16}, run_direct<2, 16, 16, EpiK1, SchemeConst>},
-    {{true, 1, 1, 16, 16}, run_direct<1, 16, 16, EpiK1, SchemeConst>},
-    {{false, 1, 1, 4, 16}, run_lds<1, 4, EpiK1, SchemeConst>},
-    {{false, 2, 1, 4, 32}, run_lds<2, 4, EpiK1, SchemeConst>},
-    {{false, 2, 1, 2, 32}, run_lds<2, 2, EpiK1, SchemeConst>},
-    {{false, 4, 1, 2, 64}, run_lds<4, 2, EpiK1, SchemeConst>},
+    {{true, 1, 4, 16, 2, true}, run_direct<4, 16, 2, true, EpiK1, SchemeConst>},
+    {{true, 1, 4, 16, 16, true}, run_direct<4, 16, 16, true, EpiK1, SchemeConst>},
+    {{true, 1, 2, 16, 16, false}, run_direct<2, 16, 16, false, EpiK1, SchemeConst>},
+    {{true, 1, 1, 16, 16, false}, run_direct<1, 16, 16, false, EpiK1, SchemeConst>},
+    {{true, 1, 4, 16, 2, false}, run_direct<4, 16, 2, false, EpiK1, SchemeConst>},
+    {{false, 1, 1, 4, 16, false}, run_lds<1, 4, EpiK1, SchemeConst>},
+    {{false, 2, 1, 4, 32, false}, run_lds<2, 4, EpiK1, SchemeConst>},
+    {{false, 2, 1, 2, 32, false}, run_lds<2, 2, EpiK1, SchemeConst>},
 };
 
 static const ScanEntry<EpiK2, S64> kScanK2[] = {
-    {{true, 1, 2, 16, 2}, run_direct<2, 16, 2, EpiK2, S64>},
-    {{true, 1, 1, 16, 2}, run_direct<1, 16, 2, EpiK2, S64>},
-    {{true, 1, 2, 8, 16}, run_direct<2, 8, 16, EpiK2, S64>},
-    {{true, 1, 1, 8, 16}, run_direct<1, 8, 16, EpiK2, S64>},
-    {{false, 1, 1, 2, 16}, run_lds<1, 2, EpiK2, S64>},
-    {{false, 2, 1, 2, 32}, run_lds<2, 2, EpiK2, S64>},
-    {{false, 2, 1, 1, 32}, run_lds<2, 1, EpiK2, S64>},
+    {{true, 1, 2, 16, 2, false}, run_direct<2, 16, 2, false, EpiK2, S64>},
+    {{true, 1, 1, 16, 2, false}, run_direct<1, 16, 2, false, EpiK2, S64>},
+    {{true, 1, 2, 8, 16, false}, run_direct<2, 8, 16, false, EpiK2, S64>},
+    {{true, 1, 1, 8, 16, false}, run_direct<1, 8, 16, false, EpiK2, S64>},
+    {{false, 1, 1, 2, 16, false}, run_lds<1, 2, EpiK2, S64>},
+    {{false, 2, 1, 2, 32, false}, run_lds<2, 2, EpiK2, S64>},
+    {{false, 2, 1, 1, 32, false}, run_lds<2, 1, EpiK2, S64>},
+    {{true, 1, 1, 16, 2, true}, run_direct<1, 16, 2, true, EpiK2, S64>},
 };
 
 int scan_variant_count(int P) { return P == 4 ? (int)std::size(kScanK1) : (int)std::size(kScanK2); }
@@ -612,7 +628,7 @@ static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
 // Default shapes per batch size, in order of preference (measured on MI355X
 // at the 256 x 1024 and 512 x 4096 layouts): direct streaming while the scan
 // is HBM-bound, B fragments shared through LDS once it is MFMA-bound.
-static const int kPrefK1[3][4] = {{3, 0, 4, -1}, {3, 2, 4, -1}, {5, 4, -1, -1}};
+static const int kPrefK1[3][4] = {{0, 3, 5, -1}, {1, 3, 5, -1}, {6, 5, -1, -1}};
 static const int kPrefK2[3][4] = {{1, 3, 4, -1}, {3, 4, -1, -1}, {4, 5, -1, -1}};
 
 template <typename Epi, typename SchemeT, size_t NV>
