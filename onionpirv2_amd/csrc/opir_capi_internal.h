@@ -73,6 +73,7 @@ struct opir_db {
   int8_t *qfrag = nullptr;
   u64 *e64_a = nullptr, *e64_b = nullptr, *q64 = nullptr, *sel64 = nullptr, *m64_a = nullptr, *m64_b = nullptr;
   u64 *q_u64 = nullptr, *r_u64 = nullptr;
+  u64 *part64 = nullptr;  // partial products of split two-prime tree levels (kSplitTargetBlocks64 cts)
   size_t exp_cts = 0;
   // autotuned scan shape per batch-size class (ceil(log2(2B))), -1 = not tuned
   int tuned_scan[16] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};

@@ -51,7 +51,7 @@ u64 *run_expand(opir_db *db, size_t B, const k2::KeyPtrs64 &P) {
   u64 *in = db->e64_a, *out = db->e64_b;
   for (size_t L = 0; L < db->h; L++) {
     k2::launch_k2_expand_level(in, out, stride, (int)nodes_at_level(db->u, db->h, L), (int)B, (int)L, P,
-                               (size_t)ctx->G64.l_ks * kCt2, ctx->G64, ctx->S64, ctx->stream);
+                               (size_t)ctx->G64.l_ks * kCt2, ctx->G64, ctx->S64, ctx->stream, db->part64);
     std::swap(in, out);
   }
   return in;
@@ -70,12 +70,12 @@ u64 *run_other_dim(opir_db *db, size_t B, const u64 *sel) {
   const size_t perfect = size_t{1} << (hs - 1);
   u64 *in = db->m64_a, *out = db->m64_b;
   k2::launch_k2_mux(in, out, stride, (int)perfect, (int)offset, 0, (int)B, sel, sel_stride, ctx->G64, ctx->S64,
-                    ctx->stream);
+                    ctx->stream, db->part64);
   std::swap(in, out);
   for (size_t a = 1; a < hs; a++) {
     const size_t half = (size_t{1} << (hs - a)) >> 1;
     k2::launch_k2_mux(in, out, stride, (int)half, -1, (int)half, (int)B, sel + a * sel_vals, sel_stride, ctx->G64,
-                      ctx->S64, ctx->stream);
+                      ctx->S64, ctx->stream, db->part64);
     std::swap(in, out);
   }
   return in;
@@ -171,11 +171,12 @@ size_t k2_ws_bytes(const opir_db *db, size_t B) {
   const size_t nsel = db->dims - 1;
   return qfrag_bytes(make_qfrag_geom(db->N0, (int)B, 8, (int)N2)) + 2 * B * db->exp_cts * ct + B * db->N0 * ct +
          B * (nsel ? nsel : 1) * 2 * db->ctx->G64.l_ep * ct + 2 * B * db->Nrest * ct + B * 4 * N2 * sizeof(u64) +
-         B * 2 * N2 * sizeof(u64);
+         B * 2 * N2 * sizeof(u64) + k2::kSplitTargetBlocks64 * ct;
 }
 
 static void free_ws(opir_db *db) {
-  for (u64 **b : {&db->e64_a, &db->e64_b, &db->q64, &db->sel64, &db->m64_a, &db->m64_b, &db->q_u64, &db->r_u64}) {
+  for (u64 **b : {&db->e64_a, &db->e64_b, &db->q64, &db->sel64, &db->m64_a, &db->m64_b, &db->q_u64, &db->r_u64,
+                  &db->part64}) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
   }
@@ -198,6 +199,7 @@ int k2_ensure_ws(opir_db *db, size_t B) {
   HIP_TRY(ctx, hipMalloc((void **)&db->m64_b, B * db->Nrest * ct));
   HIP_TRY(ctx, hipMalloc((void **)&db->q_u64, B * 4 * N2 * sizeof(u64)));
   HIP_TRY(ctx, hipMalloc((void **)&db->r_u64, B * 2 * N2 * sizeof(u64)));
+  HIP_TRY(ctx, hipMalloc((void **)&db->part64, k2::kSplitTargetBlocks64 * ct));
   HIP_TRY(ctx, hipMalloc((void **)&db->qfrag, qfrag_bytes(make_qfrag_geom(db->N0, (int)B, 8, (int)N2))));
   db->ws_B = B;
   return OPIR_OK;

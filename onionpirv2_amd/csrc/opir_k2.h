@@ -52,10 +52,14 @@ struct KeyPtrs64 {
 // NTT of `count` polynomials of one limb each: data[p] is limb (p % 2) of
 // polynomial p / 2 in [poly][limb][N] order. In place.
 void launch_k2_ntt(u64 *data, size_t count, bool inverse, const Scheme64 &S, hipStream_t st);
+// Scratch for split (narrow) tree levels: at most kSplitTargetBlocks64
+// ciphertexts of partial products; narrow levels spread each node over
+// several workgroups when `part` is given and the level fits.
+constexpr size_t kSplitTargetBlocks64 = 512;
 // Expansion level: nodes [0, n) of in -> children 2r, 2r+1 of out (coefficient form).
 void launch_k2_expand_level(const u64 *in, u64 *out, size_t stride, int nnodes, int B, int level,
                             const KeyPtrs64 &P, size_t key_level_values, const Gadget64 &G, const Scheme64 &S,
-                            hipStream_t st);
+                            hipStream_t st, u64 *part);
 // Selector completion: nrows expanded rows (coefficient form) -> sel [nsel][2 l_ep][ct] (NTT form).
 void launch_k2_complete(const u64 *rows, u64 *sel, size_t row_stride, size_t sel_stride, int nrows, int B,
                         const KeyPtrs64 &P, const Gadget64 &G, const Scheme64 &S, hipStream_t st);
@@ -64,7 +68,7 @@ void launch_k2_ct_ntt(const u64 *in, u64 *out, size_t in_stride, size_t out_stri
                       const Scheme64 &S, hipStream_t st);
 // MUX level on NTT-form candidates (same slot rules as the K = 1 path).
 void launch_k2_mux(const u64 *in, u64 *out, size_t stride, int nslots, int offset, int half, int B, const u64 *sel,
-                   size_t sel_stride, const Gadget64 &G, const Scheme64 &S, hipStream_t st);
+                   size_t sel_stride, const Gadget64 &G, const Scheme64 &S, hipStream_t st, u64 *part);
 // INTT, drop q1 (round half up), centred rescale to small_q: res [B][ct] -> out [B][2][N].
 void launch_k2_final(const u64 *res, u64 *out, size_t res_stride, int B, const Scheme64 &S, hipStream_t st);
 // Plaintext coefficients (< t) [count][N] -> NTT per limb into out [limb][first + i][N] (of num_pt).

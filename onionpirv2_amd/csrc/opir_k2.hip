@@ -220,10 +220,12 @@ __device__ __forceinline__ void mac_row(u64 (&acc)[8], const u64 v[8], const u64
 // limb and are multiplied into acc0/acc1 with rows p*l + (l-1-i) of g
 // ([rows][2 outputs][2 limbs][N], NTT form).
 __device__ __forceinline__ void ep_poly(const u128 *xs, int p, int l, int b, const u64 *__restrict__ g, int limb,
-                                        int t_in, const Limb64 &L, u64 *lds, u64 (&acc0)[8], u64 (&acc1)[8]) {
+                                        int t_in, const Limb64 &L, u64 *lds, u64 (&acc0)[8], u64 (&acc1)[8],
+                                        int i0 = 0, int i1 = -1) {
   const u64 mask = ((u64)1 << b) - 1;
+  if (i1 < 0) i1 = l;
 #pragma unroll 1
-  for (int i = 0; i < l; i++) {
+  for (int i = i0; i < i1; i++) {
     // opaque copy of t: keeps per-thread addresses inside the loop (register pressure)
     int t = t_in;
     asm volatile("" : "+v"(t));
@@ -545,6 +547,195 @@ __global__ __launch_bounds__(1024) void k2_ext_prod(const u64 *gsw, int l, int b
 }
 
 // ---------------------------------------------------------------------------
+// Split tree levels for levels with few nodes: a node's digits are spread
+// over several workgroups, each writing its partial external product (NTT
+// form, reduced mod q) as one ciphertext block of `part`; a combine kernel
+// adds them up. Part block of (query qi, node or muxed slot m, group):
+//   MUX:       qi * (nmux * 2G) + (m * 2 + p) * G + g
+//   expansion: qi * (nnodes * G) + node * G + g   ([delta_b, delta_a][limb][N])
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mux_operands64(const MuxArgs64 &A, int m, int &xi, int &yi) {
+  if (A.offset >= 0) {
+    xi = A.offset + 2 * m;
+    yi = xi + 1;
+  } else {
+    xi = m;
+    yi = m + A.half;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k2_mux_part(MuxArgs64 A, u64 *__restrict__ part, int G, Scheme64 S) {
+  __shared__ u128 xs[kN];
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  const int qi = blockIdx.y;
+  const int g = blockIdx.x % G, p = (blockIdx.x / G) & 1, m = blockIdx.x / (2 * G);
+  int xi, yi;
+  mux_operands64(A, m, xi, yi);
+  const u64 *in = A.in + qi * A.stride;
+  const u64 *x = in + (size_t)xi * kCt, *y = in + (size_t)yi * kCt;
+  u64 v[8], w[8];
+  load3(v, y + (size_t)(p * 2 + limb) * kN, t);
+  load3(w, x + (size_t)(p * 2 + limb) * kN, t);
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = submod(v[c], w[c], L.q);
+  ntt_inv(v, lds[limb], t, L);
+  stage_compose(v, lds, limb, t, S, xs);
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  const int dg = (A.l + G - 1) / G, i0 = g * dg, i1 = min(A.l, i0 + dg);
+  ep_poly(xs, p, A.l, A.b, A.sel + qi * A.sel_stride, limb, t, L, lds[limb], acc0, acc1, i0, i1);
+  u64 *dst = part + ((size_t)qi * gridDim.x + blockIdx.x) * kCt;
+  store3(dst + (size_t)(0 * 2 + limb) * kN, acc0, t);
+  store3(dst + (size_t)(1 * 2 + limb) * kN, acc1, t);
+}
+
+// out[s] = in[s] for ragged slots s < offset, else x + the node's 2G partials.
+// 32 workgroups per slot, one 16-byte word per thread.
+__global__ __launch_bounds__(256) void k2_mux_fin(MuxArgs64 A, const u64 *__restrict__ part, int G, int nmux,
+                                                  Scheme64 S) {
+  constexpr int kWords = (int)kCt / 2;  // 16-byte words per ciphertext
+  const int s = blockIdx.x >> 5, qi = blockIdx.y;
+  const int e = (blockIdx.x & 31) * 256 + threadIdx.x;
+  const u64 *in = A.in + qi * A.stride;
+  ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(A.out + qi * A.stride + (size_t)s * kCt);
+  if (A.offset >= 0 && s < A.offset) {
+    dst[e] = reinterpret_cast<const ulonglong2 *>(in + (size_t)s * kCt)[e];
+    return;
+  }
+  const int m = A.offset >= 0 ? s - A.offset : s;
+  int xi, yi;
+  mux_operands64(A, m, xi, yi);
+  const ulonglong2 a = reinterpret_cast<const ulonglong2 *>(in + (size_t)xi * kCt)[e];
+  const ulonglong2 *pp =
+      reinterpret_cast<const ulonglong2 *>(part + ((size_t)qi * nmux * 2 * G + (size_t)m * 2 * G) * kCt);
+  constexpr int kMaxParts = 16;
+  ulonglong2 b[kMaxParts];
+#pragma unroll
+  for (int j = 0; j < kMaxParts; j++)
+    if (j < 2 * G) b[j] = pp[(size_t)j * kWords + e];
+  const Limb64 &L = S.limb[((2 * e) / kN) & 1];  // polynomial index = output * 2 + limb
+  u64 s0 = a.x, s1 = a.y;
+#pragma unroll
+  for (int j = 0; j < kMaxParts; j++)
+    if (j < 2 * G) {
+      s0 = addmod(s0, b[j].x, L.q);
+      s1 = addmod(s1, b[j].y, L.q);
+    }
+  dst[e] = make_ulonglong2(s0, s1);
+}
+
+__global__ __launch_bounds__(1024) void k2_expand_part(ExpandArgs64 A, u64 *__restrict__ part, int G, Scheme64 S) {
+  __shared__ u128 xs[kN];
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  const int node = blockIdx.x / G, g = blockIdx.x % G, qi = blockIdx.y;
+  const u64 *c = A.in + qi * A.stride + (size_t)node * kCt;
+  const u64 *c1 = c + (size_t)(2 + limb) * kN;
+  u64 v[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const int j = map0(t, cc);
+    const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+    const u64 x = c1[i0 < kN ? i0 : i0 - kN];
+    v[cc] = (i0 < kN || !x) ? x : L.q - x;
+  }
+  stage_compose(v, lds, limb, t, S, xs);
+  {
+    const u128 Q = Q128(S), hQ = hQ128(S);
+    for (int j = threadIdx.x; j < kN; j += 1024)
+      if (xs[j] > hQ) xs[j] -= Q;
+    __syncthreads();
+  }
+  u64 accb[8], acca[8];
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
+  const int b = A.b_ks;
+  const i128 h = ((i128)1 << (b - 1)) - 1;
+  const u64 *key = A.bv[qi] + (size_t)A.level * A.key_level_values;
+  const int dg = (A.l_ks + G - 1) / G, d0 = g * dg, d1 = min(A.l_ks, d0 + dg);
+  i128 H = 0;
+  for (int i = 0; i < d0; i++) H += h << (b * i);
+  const int t_in = t;
+  for (int i = d0; i < d1; i++) {
+    int t = t_in;
+    asm volatile("" : "+v"(t));
+    u64 d[8];
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) {
+      const i128 di = ((i128)xs[map0(t, cc)] + H) >> (b * i);
+      const int64_t r = (int64_t)(((u64)(di + h)) & (((u64)1 << b) - 1)) - (int64_t)h;
+      d[cc] = sres(r, L.q);
+    }
+    H += h << (b * i);
+    ntt_fwd(d, lds[limb], t, L, false);
+    const u64 *row = key + (size_t)i * kCt;
+    mac_row(accb, d, row + (size_t)(0 * 2 + limb) * kN, t, L);
+    mac_row(acca, d, row + (size_t)(1 * 2 + limb) * kN, t, L);
+  }
+  u64 *dst = part + ((size_t)qi * gridDim.x + blockIdx.x) * kCt;
+  store3(dst + (size_t)(0 * 2 + limb) * kN, accb, t);
+  store3(dst + (size_t)(1 * 2 + limb) * kN, acca, t);
+}
+
+// Workgroup (node, h), lane block = limb: delta_h = INTT(sum of the G
+// partials), then child polynomials h as in k2_expand_level.
+__global__ __launch_bounds__(1024) void k2_expand_fin(ExpandArgs64 A, const u64 *__restrict__ part, int G, int nnodes,
+                                                      Scheme64 S) {
+  __shared__ u64 lds[2][kLds];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
+  const Limb64 &L = S.limb[limb];
+  const int node = blockIdx.x >> 1, h = blockIdx.x & 1, qi = blockIdx.y;
+  const u64 *c = A.in + qi * A.stride + (size_t)node * kCt;
+  const u64 *pp = part + ((size_t)qi * nnodes * G + (size_t)node * G) * kCt + (size_t)(h * 2 + limb) * kN;
+  u64 w[8];
+  load3(w, pp, t);
+  for (int j = 1; j < G; j++) {
+    u64 u[8];
+    load3(u, pp + (size_t)j * kCt, t);
+#pragma unroll
+    for (int cc = 0; cc < 8; cc++) w[cc] = addmod(w[cc], u[cc], L.q);
+  }
+  ntt_inv(w, lds[limb], t, L);  // delta_h, coefficient form (map0)
+  u64 *left = A.out + qi * A.stride + (size_t)(2 * node) * kCt;
+  u64 *right = left + kCt;
+  const int k = 1 << A.level;
+  const u64 *cp = c + (size_t)(h * 2 + limb) * kN;
+  u64 cv[8], dv[8];
+  load0(cv, cp, t);
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    u64 ww = w[cc];
+    if (h == 0) {  // c'_0 = sigma(c0) + delta_b
+      const int j = map0(t, cc);
+      const int i0 = (int)(((unsigned)j * (unsigned)A.kinv) & (2 * kN - 1));
+      const u64 x = cp[i0 < kN ? i0 : i0 - kN];
+      ww = addmod(ww, (i0 < kN || !x) ? x : L.q - x, L.q);
+    }
+    left[(size_t)(h * 2 + limb) * kN + map0(t, cc)] = addmod(cv[cc], ww, L.q);
+    dv[cc] = submod(cv[cc], ww, L.q);
+  }
+  __syncthreads();  // the INTT's last reads of lds are done
+  store0(lds[limb], dv, t);
+  __syncthreads();
+#pragma unroll
+  for (int cc = 0; cc < 8; cc++) {
+    const int j = map0(t, cc), s = j + k;
+    const u64 x = lds[limb][s < kN ? s : s - kN];
+    right[(size_t)(h * 2 + limb) * kN + j] = (s < kN || !x) ? x : L.q - x;
+  }
+}
+
+// Digit groups of a split level: one digit per workgroup, at most 8 groups.
+static int split_groups64(int l) {
+  const int dg = (l + 7) / 8;
+  return (l + dg - 1) / dg;
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 void launch_k2_ntt(u64 *data, size_t count, bool inverse, const Scheme64 &S, hipStream_t st) {
@@ -553,7 +744,7 @@ void launch_k2_ntt(u64 *data, size_t count, bool inverse, const Scheme64 &S, hip
 
 void launch_k2_expand_level(const u64 *in, u64 *out, size_t stride, int nnodes, int B, int level,
                             const KeyPtrs64 &P, size_t key_level_values, const Gadget64 &G, const Scheme64 &S,
-                            hipStream_t st) {
+                            hipStream_t st, u64 *part) {
   if (nnodes <= 0 || B <= 0) return;
   ExpandArgs64 A;
   A.in = in; A.out = out; A.stride = stride; A.bv = P.bv; A.key_level_values = key_level_values;
@@ -562,6 +753,12 @@ void launch_k2_expand_level(const u64 *in, u64 *out, size_t stride, int nnodes, 
   unsigned inv = 1;  // gk^-1 mod 2N = gk^(N-1): the unit group mod 2N = 2^13 has order N
   for (int e = 0; e < kN - 1; e++) inv = (inv * gk) & (2 * kN - 1);
   A.kinv = (int)inv;
+  const int Gs = split_groups64(G.l_ks);
+  if (part && (size_t)nnodes * Gs * B <= kSplitTargetBlocks64) {
+    k2_expand_part<<<dim3(nnodes * Gs, B), 1024, 0, st>>>(A, part, Gs, S);
+    k2_expand_fin<<<dim3(nnodes * 2, B), 1024, 0, st>>>(A, part, Gs, nnodes, S);
+    return;
+  }
   k2_expand_level<<<dim3(nnodes, B), 1024, 0, st>>>(A, S);
 }
 
@@ -581,11 +778,21 @@ void launch_k2_ct_ntt(const u64 *in, u64 *out, size_t in_stride, size_t out_stri
 }
 
 void launch_k2_mux(const u64 *in, u64 *out, size_t stride, int nslots, int offset, int half, int B, const u64 *sel,
-                   size_t sel_stride, const Gadget64 &G, const Scheme64 &S, hipStream_t st) {
+                   size_t sel_stride, const Gadget64 &G, const Scheme64 &S, hipStream_t st, u64 *part) {
   if (nslots <= 0 || B <= 0) return;
   MuxArgs64 A;
   A.in = in; A.out = out; A.stride = stride; A.sel = sel; A.sel_stride = sel_stride;
   A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
+  const int nmux = offset >= 0 ? nslots - offset : nslots;
+  const size_t pairs = (size_t)nmux * 2 * B;
+  if (part && nmux > 0 && pairs <= kSplitTargetBlocks64) {
+    int Gs = (int)std::min<size_t>(split_groups64(G.l_ep), kSplitTargetBlocks64 / pairs);
+    const int dg = (G.l_ep + Gs - 1) / Gs;
+    Gs = (G.l_ep + dg - 1) / dg;  // no empty groups
+    k2_mux_part<<<dim3(nmux * 2 * Gs, B), 1024, 0, st>>>(A, part, Gs, S);
+    k2_mux_fin<<<dim3(nslots * 32, B), 256, 0, st>>>(A, part, Gs, nmux, S);
+    return;
+  }
   k2_mux<<<dim3(nslots, B), 1024, 0, st>>>(A, S);
 }
 
