@@ -291,6 +291,7 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   ctx->S.q1inv_s = shoup_h((u32)q1inv, (u32)q2);
   ctx->S.q2_mod_q1 = (u32)(q2 % q1);
   ctx->S.small_q = sch->small_q;
+  ctx->S.q_recip = (u64)(((u128)1 << 120) / Q);
   ctx->G = GadgetCfg{(int)sch->l_ep, (int)sch->base_log2, (int)sch->l_key, (int)sch->base_log2_key, (int)sch->l_ks,
                      (int)sch->bv_base_log2};
   *out = ctx;

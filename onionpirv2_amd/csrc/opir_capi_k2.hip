@@ -143,6 +143,12 @@ int k2_ctx_init(opir_ctx *ctx, const opir_scheme *sch) {
   ctx->S64.Q_lo = (u64)Q; ctx->S64.Q_hi = (u64)(Q >> 64);
   ctx->S64.hQ_lo = (u64)(Q >> 1); ctx->S64.hQ_hi = (u64)(Q >> 65);
   ctx->S64.small_q = sch->small_q;
+  for (int i = 0; i < 2; i++) {  // k = 63 + bit length of q: floor(2^k / q) in [2^63, 2^64)
+    const u64 q = i ? q1 : q0;
+    const int k = 63 + (64 - __builtin_clzll(q));
+    ctx->S64.rm[i] = (u64)(((u128)1 << k) / q);
+    ctx->S64.rk[i] = k;
+  }
   ctx->G64 = k2::Gadget64{(int)sch->l_ep, (int)sch->base_log2, (int)sch->l_key, (int)sch->base_log2_key,
                           (int)sch->l_ks, (int)sch->bv_base_log2};
   ctx->k2 = true;

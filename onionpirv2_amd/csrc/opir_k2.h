@@ -36,6 +36,8 @@ struct Scheme64 {
   u64 Q_lo, Q_hi;         // Q = q0 * q1
   u64 hQ_lo, hQ_hi;       // Q >> 1
   u64 small_q;
+  u64 rm[2];              // Barrett reciprocals floor(2^rk[i] / q_i), in [2^63, 2^64)
+  int rk[2];
 };
 
 struct Gadget64 {

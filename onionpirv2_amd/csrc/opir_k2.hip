@@ -68,16 +68,21 @@ __device__ __forceinline__ u128 hQ128(const Scheme64 &S) { return ((u128)S.hQ_hi
 // Signed residue r (|r| < 2^63) mod q.
 __device__ __forceinline__ u64 sres(int64_t r, u64 q) { return r >= 0 ? (u64)r : q - (u64)(-r); }
 
-// num / d for a quotient known to be < 2^bits (restoring division).
-__device__ __forceinline__ u64 div_small(u128 num, u64 d, int bits) {
-  u64 quo = 0;
-  for (int bit = bits - 1; bit >= 0; bit--) {
-    const u128 s = (u128)d << bit;
-    if (num >= s) {
-      num -= s;
-      quo |= (u64)1 << bit;
+// floor(num / d) for num < 2^121 by Barrett: m = floor(2^k / d) in
+// [2^63, 2^64) (k = 63 + bit length of d). floor(num * m / 2^k), keeping only
+// the high 64 bits of the low product, is the quotient or up to two less;
+// two conditional corrections make it exact.
+__device__ __forceinline__ u64 div_barrett(u128 num, u64 d, u64 m, int k) {
+  const u64 nh = (u64)(num >> 64), nl = (u64)num;
+  const u128 t = (u128)nh * m + __umul64hi(nl, m);  // about num * m / 2^64
+  u64 quo = (u64)(t >> (k - 64));
+  u128 r = num - (u128)quo * d;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+    if (r >= d) {
+      r -= d;
+      quo++;
     }
-  }
   return quo;
 }
 
@@ -254,8 +259,8 @@ __device__ __forceinline__ u64 rescale_q0(u64 a, const Scheme64 &S) {
   const u64 q0 = S.limb[0].q, h = q0 / 2;
   const bool neg = a >= h;  // v >= inp/2 -> v - inp
   const u64 mag = neg ? q0 - a : a;
-  const u64 quo = div_small((u128)mag * S.small_q + h, q0, 64);
-  const u64 r = quo % S.small_q;
+  const u64 quo = div_barrett((u128)mag * S.small_q + h, q0, S.rm[0], S.rk[0]);
+  const u64 r = quo < S.small_q ? quo : quo % S.small_q;  // mag <= q0/2 keeps quo below small_q
   return neg ? (r ? S.small_q - r : 0) : r;
 }
 
@@ -263,7 +268,7 @@ __device__ __forceinline__ u64 rescale_q0(u64 a, const Scheme64 &S) {
 __device__ __forceinline__ u64 final_coeff(u64 r0, u64 r1, const Scheme64 &S) {
   const u64 q0 = S.limb[0].q, q1 = S.limb[1].q;
   const u128 x = crt2(r0, r1, S);
-  u64 o = div_small(x + (q1 >> 1), q1, 64);
+  u64 o = div_barrett(x + (q1 >> 1), q1, S.rm[1], S.rk[1]);
   if (o >= q0) o -= q0;
   return rescale_q0(o, S);
 }

@@ -197,21 +197,23 @@ __device__ __forceinline__ u64 rescale_to_small(u64 x, const SchemeConst &S) {
   if (v >= (i64)(S.Q / 2)) v -= (i64)S.Q;
   const bool neg = v < 0;
   const u64 a = neg ? (u64)(-v) : (u64)v;
-  // floor((a * small_q + floor(Q/2)) / Q) by restoring division (quotient < 2^32)
-  unsigned __int128 num = (unsigned __int128)a * S.small_q + (S.Q >> 1);
-  u64 quo = 0;
-  for (int bit = 31; bit >= 0; bit--) {
-    const unsigned __int128 d = (unsigned __int128)S.Q << bit;
-    if (num >= d) {
-      num -= d;
-      quo |= (u64)1 << bit;
+  // floor((a * small_q + floor(Q/2)) / Q), numerator < 2^81. Barrett with
+  // m = floor(2^120 / Q): floor(num * m / 2^120), taking the high 64 bits of
+  // the low product only, is the quotient or up to two less; then correct.
+  const unsigned __int128 num = (unsigned __int128)a * S.small_q + (S.Q >> 1);
+  const u64 nh = (u64)(num >> 64), nl = (u64)num;
+  const unsigned __int128 t = (unsigned __int128)nh * S.q_recip + __umul64hi(nl, S.q_recip);
+  u64 quo = (u64)(t >> 56);
+  unsigned __int128 r = num - (unsigned __int128)quo * S.Q;
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+    if (r >= S.Q) {
+      r -= S.Q;
+      quo++;
     }
-  }
-  const i64 r = neg ? -(i64)quo : (i64)quo;
-  const i64 sq = (i64)S.small_q;
-  i64 m = r % sq;
-  if (m < 0) m += sq;
-  return (u64)m;
+  // a <= Q/2 gives quo <= (small_q + 1) / 2 < small_q, so the signed result
+  // lies in (-small_q, small_q)
+  return neg ? (quo ? S.small_q - quo : 0) : quo;
 }
 
 __global__ __launch_bounds__(512) void k_final(const u32 *res, u64 *out, size_t res_stride, SchemeConst S) {
