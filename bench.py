@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--serve-budget-gib", type=float, default=96.0,
                     help="HBM budget for client keys and batch workspaces of the serving loop")
     ap.add_argument("--no-k2", action="store_true", help="skip the CONFIG_N4096_K2_MP leg")
+    ap.add_argument("--no-paper-row", action="store_true",
+                    help="skip the with_layout({349526,10}) leg (the reference's 1 GB row, 5.3 GiB NTT form)")
     ap.add_argument("--k2-target", type=int, default=2**21, help="n=4096 layout: target plaintexts")
     ap.add_argument("--k2-height", type=int, default=10)
     ap.add_argument("--k2-batch", type=int, default=32)
@@ -195,6 +197,86 @@ def serving(args, H, params, server, log, k2=None):
     return {"databases": names, "clients_n2048": nc,
             "clients_n4096": min(args.serve_k2_entries, len(k2["queries"])) if k2 is not None else 0,
             "hbm_budget_gib": args.serve_budget_gib, "batch_limits": limits, "rates": rows}
+
+
+def paper_row(args, H, G, torch, log):
+    """The reference's 1 GB row: with_layout({349526, 10}) = 512 x 683
+    plaintexts (1.0 GiB logical, 5.3 GiB NTT form, ragged MUX tree), data
+    generated in HBM from a seed; plain path and one batch of 64."""
+    import ctypes
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    p = H.PirParams().with_layout(349526, 10, True)
+    B = 64
+    t0 = time.time()
+    server = H.PirServer(p)
+    idxs = distinct_indices(B, p.num_pt, args.seed + 5)
+    idxs = [i % p.target_num_pt for i in idxs]
+    server.gen_data_device(idxs[:2], args.seed + 5)
+
+    def mk(j):
+        c = H.PirClient(p, 9000 + j, args.seed + 17 * j + 3)
+        return c, c.create_bv_galois_keys(), c.generate_gsw_from_key(), c.fast_generate_query(p, idxs[j])
+
+    with ThreadPoolExecutor(8) as ex:
+        cl = list(ex.map(mk, range(B)))
+    for j, (c, bv, gsw, _) in enumerate(cl):
+        server.set_client_bv_galois_key(9000 + j, bv)
+        server.set_client_gsw_key(9000 + j, gsw)
+    queries = np.stack([x[3] for x in cl])
+    log(f"[1gb] layout N0={p.fst_dim_sz} Nrest={p.other_dim_sz} dims={p.num_dims}, {B} clients, "
+        f"setup {time.time() - t0:.1f}s")
+    for j in range(2):
+        pt, budget = cl[j][0].decrypt_mod_q(server.make_query(9000 + j, queries[j].copy()))
+        assert np.array_equal(pt, server.direct_get_original_plaintext(idxs[j])), "wrong 1 GB-row PIR answer"
+    db_h, ctx_h = server.db_handle(), server.ctx_handle()
+    stream = torch.cuda.current_stream()
+    assert G.ctx_call_raw("opir_ctx_set_stream", ctx_h, ctypes.c_void_p(stream.cuda_stream)) == 0
+    assert G.ctx_call_raw("opir_ctx_set_scan_variant", ctx_h, scan_shape_arg(args)) == 0
+    dev = torch.device("cuda:0")
+    d_q = torch.from_numpy(queries.view(np.int64)).to(dev)
+    d_r = torch.empty_like(d_q)
+    ids = np.arange(9000, 9000 + B, dtype=np.uint64)
+
+    def answer(b, off=0):
+        rc = G.answer_batch_device_raw(db_h, ids[off:off + b], d_q[off:off + b].data_ptr(), d_r[off:off + b].data_ptr())
+        if rc:
+            raise RuntimeError(G.last_error_raw(ctx_h))
+
+    def timed(b, steps):
+        for i in range(2):
+            answer(b)
+        torch.cuda.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        evs[0].record()
+        for i in range(steps):
+            answer(b, (i * b) % B if b < B else 0)
+            evs[i + 1].record()
+        torch.cuda.synchronize()
+        per = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+        return sum(per) / steps, per
+
+    ms_plain, per = timed(1, args.steps)
+    ms_b, per_b = timed(B, 4)
+    answer(B)
+    torch.cuda.synchronize()
+    rb = d_r.cpu().numpy().view(np.uint64)
+    assert np.array_equal(rb[B - 1], server.make_query(9000 + B - 1, queries[B - 1].copy())), "1 GB row batch != plain"
+    gib = p.target_num_pt * p.pt_size / 2**30
+    log(f"[1gb] plain {ms_plain:.3f} ms -> {gib / (ms_plain * 1e-3):.0f} DB GiB/s; batch {B}: {ms_b:.2f} ms -> "
+        f"{B * 1e3 / ms_b:.0f} q/s ({B * ms_plain / ms_b:.1f}x plain)")
+    res = {"workload": "OnionPIRv2 make_query, CONFIG_N2048_K1_COMP, with_layout({349526,10}) = 512 x 683 "
+                       "plaintexts (1.0 GiB logical, 5.3 GiB NTT form), synthetic seeded data generated in HBM",
+           "fst_dim_sz": p.fst_dim_sz, "other_dim_sz": p.other_dim_sz, "num_dims": p.num_dims,
+           "db_gib_per_s_plain": round(gib / (ms_plain * 1e-3), 2), "ms_per_query_plain": round(ms_plain, 4),
+           "queries_per_s_plain": round(1e3 / ms_plain, 2),
+           "latency_ms_plain": {"p50": round(pct(per, 50), 4), "p99": round(pct(per, 99), 4)},
+           "batched": {"batch": B, "ms_per_batch": round(ms_b, 3), "queries_per_s": round(B * 1e3 / ms_b, 2),
+                       "speedup_vs_plain_qps": round(B * ms_plain / ms_b, 3)},
+           "scan_shape_plain": tuned_shape(G, db_h, 1), "scan_shape_batched": tuned_shape(G, db_h, B)}
+    del d_q, d_r, server
+    torch.cuda.synchronize()
+    return res
 
 
 def k2_setup(args, H, log):
@@ -508,6 +590,8 @@ def main():
             log(f"[sweep] B={b}: {ms_s:.2f} ms/batch -> {b * 1e3 / ms_s:.1f} q/s")
         if rows:
             result["batch_sweep"] = rows
+    if not args.no_paper_row:
+        result["paper_1gb_row"] = paper_row(args, H, G, torch, log)
     k2 = k2_setup(args, H, log) if not args.no_k2 else None
     if not args.no_serve:
         torch.cuda.synchronize()
