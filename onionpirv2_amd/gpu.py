@@ -194,6 +194,18 @@ class GpuContext:
         gsw = _u64(gsw)
         self.check(self.lib.opir_client_set_keys(self.h, client_id, _ptr(bv), bv.shape[0], _ptr(gsw)))
 
+    def key_bytes(self, h):
+        """Device bytes of one client's keys at expansion height h."""
+        out = ctypes.c_size_t(0)
+        self.check(self.lib.opir_client_key_bytes(self.h, h, ctypes.byref(out)))
+        return out.value
+
+    def resident_key_bytes(self):
+        """Device bytes of all client keys resident on this context."""
+        out = ctypes.c_size_t(0)
+        self.check(self.lib.opir_ctx_resident_key_bytes(self.h, ctypes.byref(out)))
+        return out.value
+
     def remove_client(self, client_id):
         self.check(self.lib.opir_client_remove(self.h, client_id))
 

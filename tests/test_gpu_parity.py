@@ -257,3 +257,20 @@ def test_autotuned_scan_matches_default(O, comp_params, gpu_ctx):
     assert 0 <= s.db.tuned_scan_variant(1) < n
     assert 0 <= s.db.tuned_scan_variant(9) < n
     assert np.array_equal(want1, s.server.make_query(s.cid, qs[0]))
+
+
+def test_resident_key_bytes_track_clients(O, comp_params, gpu_ctx):
+    """opir_ctx_resident_key_bytes (serving admission) counts each resident
+    client's keys once, at its expansion height."""
+    say("[gpu] resident key bytes")
+    p = comp_params.with_layout(4096, 9, True)
+    c = O.PirClient(p, 71, 7)
+    bv, gsw = c.create_bv_galois_keys(), c.generate_gsw_from_key()
+    before = gpu_ctx.resident_key_bytes()
+    gpu_ctx.set_client_keys(9071, bv, gsw)
+    gpu_ctx.set_client_keys(9072, bv, gsw)
+    gpu_ctx.set_client_keys(9072, bv, gsw)  # replacing keeps one copy
+    assert gpu_ctx.resident_key_bytes() - before == 2 * gpu_ctx.key_bytes(bv.shape[0])
+    gpu_ctx.remove_client(9071)
+    gpu_ctx.remove_client(9072)
+    assert gpu_ctx.resident_key_bytes() == before
