@@ -126,6 +126,9 @@ int k2_ctx_init(opir_ctx *ctx, const opir_scheme *sch) {
     L.q = q; L.q2 = 2 * q;
     L.r64 = (u64)(((u128)1 << 64) % q); L.r64s = shoup64_h(L.r64, q);
     L.one_s = (u64)(((u128)1 << 64) / q);
+    u64 qi = q;  // q^-1 mod 2^64 by Newton steps (q odd: exact to 3 bits, doubling each step)
+    for (int it = 0; it < 5; it++) qi *= 2 - q * qi;
+    L.qninv = 0 - qi;
     L.ninv = ni; L.ninv_s = shoup64_h(ni, q);
   }
   HIP_TRY(nullptr, hipMalloc((void **)&ctx->d_tables64, tab.size() * sizeof(u64)));

@@ -3,8 +3,9 @@
 // Arithmetic: each limb is a ~60-bit NTT prime, so values are u64 and every
 // product is a 64x64->128 product. Modular multiplication by a known
 // constant uses Shoup's method with a 64-bit companion (floor(w*2^64/q));
-// products of two variables are reduced from 128 bits as
-// hi*(2^64 mod q) + lo, both halves by Shoup. NTT butterflies are
+// the multiply-accumulates against key rows (products of two variables) use
+// Montgomery reduction, with the 2^-64 factor of the whole sum removed by one
+// Shoup multiplication by 2^64 mod q at the end. NTT butterflies are
 // Harvey-lazy in [0, 4q) (4q < 2^62).
 //
 // Thread layout: a 4096-point transform is a lane block of 512 threads with
@@ -44,15 +45,15 @@ __device__ __forceinline__ u64 submod(u64 a, u64 b, u64 q) { return a >= b ? a -
 // a * w mod q lazily in [0, 2q), any a < 2^64, w < q.
 __device__ __forceinline__ u64 shoup(u64 a, u64 w, u64 ws, u64 q) { return a * w - __umul64hi(a, ws) * q; }
 
-// (hi * 2^64 + lo) mod q, exact.
-__device__ __forceinline__ u64 reduce128(u64 hi, u64 lo, const Limb64 &L) {
-  const u64 a = shoup(hi, L.r64, L.r64s, L.q);
-  const u64 b = lo - __umul64hi(lo, L.one_s) * L.q;
-  return csub(csub(a + b, L.q2), L.q);
+// Montgomery product a * b * 2^-64 mod q, lazily in [0, 2q), for a * b < q * 2^64
+// (a < 4q, b < q): m = (a * b) * (-q^-1) mod 2^64 makes a * b + m * q a
+// multiple of 2^64, and (a * b + m * q) / 2^64 < 2q. The low halves sum to 0
+// or to 2^64, so the carry is (lo != 0).
+__device__ __forceinline__ u64 mont_mul(u64 a, u64 b, const Limb64 &L) {
+  const u128 T = (u128)a * b;
+  const u64 lo = (u64)T, hi = (u64)(T >> 64);
+  return hi + __umul64hi(lo * L.qninv, L.q) + (lo != 0);
 }
-
-// a * b mod q for any a < 2^64, b < 2^64.
-__device__ __forceinline__ u64 mulmod(u64 a, u64 b, const Limb64 &L) { return reduce128(__umul64hi(a, b), a * b, L); }
 
 // CRT (r0 mod q0, r1 mod q1) -> X in [0, Q) (Garner, server.cpp:529-553 form).
 __device__ __forceinline__ u128 crt2(u64 r0, u64 r1, const Scheme64 &S) {
@@ -208,15 +209,21 @@ __device__ __forceinline__ void store3(u64 *p, const u64 v[8], int t) {
   for (int c = 0; c < 4; c++) q[c] = make_ulonglong2(v[2 * c], v[2 * c + 1]);
 }
 
-// acc[c] += v[c] * key[8t + c] mod q, two key values at a time.
+// acc[c] += v[c] * key[8t + c] * 2^-64 mod q, two key values at a time
+// (v < 4q, key < q); acc stays in [0, 2q). mac_finish turns the sum into
+// sum(v * key) mod q in [0, q) by one multiplication with 2^64 mod q.
 __device__ __forceinline__ void mac_row(u64 (&acc)[8], const u64 v[8], const u64 *key, int t, const Limb64 &L) {
   const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(key + 8 * t);
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const ulonglong2 k = kp[c];
-    acc[2 * c] = addmod(acc[2 * c], mulmod(v[2 * c], k.x, L), L.q);
-    acc[2 * c + 1] = addmod(acc[2 * c + 1], mulmod(v[2 * c + 1], k.y, L), L.q);
+    acc[2 * c] = csub(acc[2 * c] + mont_mul(v[2 * c], k.x, L), L.q2);
+    acc[2 * c + 1] = csub(acc[2 * c + 1] + mont_mul(v[2 * c + 1], k.y, L), L.q2);
   }
+}
+__device__ __forceinline__ void mac_finish(u64 (&acc)[8], const Limb64 &L) {
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc[c] = csub(shoup(acc[c], L.r64, L.r64s, L.q), L.q);
 }
 
 // One polynomial's external-product contribution (gsw.cpp:119-182 with the
@@ -354,6 +361,8 @@ __global__ __launch_bounds__(1024) void k2_expand_level(ExpandArgs64 A, Scheme64
     mac_row(accb, d, row + (size_t)(0 * 2 + limb) * kN, t, L);
     mac_row(acca, d, row + (size_t)(1 * 2 + limb) * kN, t, L);
   }
+  mac_finish(accb, L);
+  mac_finish(acca, L);
   ntt_inv(accb, lds[limb], t, L);  // delta_b, coefficient form (map0)
   ntt_inv(acca, lds[limb], t, L);  // delta_a
   // c' = (sigma(c0) + delta_b, delta_a); left = c + c', right = (c - c') * X^-k
@@ -419,6 +428,8 @@ __global__ __launch_bounds__(1024) void k2_complete(CompleteArgs64 A, Scheme64 S
     ep_poly(xs, p, A.l_key, A.b_key, A.gsw[qi], limb, t, L, lds[limb], acc0, acc1);
     __syncthreads();  // xs is rewritten for the next polynomial
   }
+  mac_finish(acc0, L);
+  mac_finish(acc1, L);
   u64 *bot = selb + (size_t)(A.l_ep + r) * kCt;
   store3(bot + (size_t)(0 * 2 + limb) * kN, acc0, t);
   store3(bot + (size_t)(1 * 2 + limb) * kN, acc1, t);
@@ -484,6 +495,8 @@ __global__ __launch_bounds__(1024) void k2_mux(MuxArgs64 A, Scheme64 S) {
     ep_poly(xs, p, A.l, A.b, g, limb, t, L, lds[limb], acc0, acc1);
     __syncthreads();  // xs is rewritten for the next polynomial
   }
+  mac_finish(acc0, L);
+  mac_finish(acc1, L);
   u64 x0[8], x1[8];
   load3(x0, x + (size_t)(0 * 2 + limb) * kN, t);
   load3(x1, x + (size_t)(1 * 2 + limb) * kN, t);
@@ -547,6 +560,8 @@ __global__ __launch_bounds__(1024) void k2_ext_prod(const u64 *gsw, int l, int b
     ep_poly(xs, p, l, b, gsw, limb, t, L, lds[limb], acc0, acc1);
     __syncthreads();  // xs is rewritten for the next polynomial
   }
+  mac_finish(acc0, L);
+  mac_finish(acc1, L);
   store3(out + (size_t)(0 * 2 + limb) * kN, acc0, t);
   store3(out + (size_t)(1 * 2 + limb) * kN, acc1, t);
 }
@@ -592,6 +607,8 @@ __global__ __launch_bounds__(1024) void k2_mux_part(MuxArgs64 A, u64 *__restrict
   for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
   const int dg = (A.l + G - 1) / G, i0 = g * dg, i1 = min(A.l, i0 + dg);
   ep_poly(xs, p, A.l, A.b, A.sel + qi * A.sel_stride, limb, t, L, lds[limb], acc0, acc1, i0, i1);
+  mac_finish(acc0, L);
+  mac_finish(acc1, L);
   u64 *dst = part + ((size_t)qi * gridDim.x + blockIdx.x) * kCt;
   store3(dst + (size_t)(0 * 2 + limb) * kN, acc0, t);
   store3(dst + (size_t)(1 * 2 + limb) * kN, acc1, t);
@@ -681,6 +698,8 @@ __global__ __launch_bounds__(1024) void k2_expand_part(ExpandArgs64 A, u64 *__re
     mac_row(accb, d, row + (size_t)(0 * 2 + limb) * kN, t, L);
     mac_row(acca, d, row + (size_t)(1 * 2 + limb) * kN, t, L);
   }
+  mac_finish(accb, L);
+  mac_finish(acca, L);
   u64 *dst = part + ((size_t)qi * gridDim.x + blockIdx.x) * kCt;
   store3(dst + (size_t)(0 * 2 + limb) * kN, accb, t);
   store3(dst + (size_t)(1 * 2 + limb) * kN, acca, t);
