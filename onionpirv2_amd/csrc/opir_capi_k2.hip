@@ -95,10 +95,11 @@ int k2_validate_scheme(const opir_scheme *sch) {
     if (powmod_h(roots[m], N2, qs[m]) != qs[m] - 1)
       return set_err(nullptr, OPIR_EINVAL, "ntt_root / ntt_root2 is not a primitive 2N-th root of its prime");
   // unsigned digits must stay below both primes; the signed BV digits' last
-  // shift b*(l_ks-1) must fit a 128-bit value with its sign
+  // shift b*(l_ks-1) must fit a 128-bit value with its sign; a 128-bit
+  // multiply-accumulate sum holds at most 64 products (2 l_ep, 2 l_key, l_ks)
   if (sch->l_ep == 0 || sch->l_key == 0 || sch->l_ks == 0 || sch->base_log2 == 0 || sch->base_log2 > 32 ||
       sch->base_log2_key == 0 || sch->base_log2_key > 32 || sch->bv_base_log2 < 2 || sch->bv_base_log2 > 32 ||
-      (u64)sch->bv_base_log2 * (sch->l_ks - 1) > 126 || sch->l_ep > 64 || sch->l_key > 64)
+      (u64)sch->bv_base_log2 * (sch->l_ks - 1) > 126 || sch->l_ep > 32 || sch->l_key > 32)
     return set_err(nullptr, OPIR_EINVAL, "invalid gadget parameters");
   if (sch->small_q < 2 || sch->small_q >= q0) return set_err(nullptr, OPIR_EINVAL, "small_q must be below q0");
   return OPIR_OK;
@@ -126,9 +127,6 @@ int k2_ctx_init(opir_ctx *ctx, const opir_scheme *sch) {
     L.q = q; L.q2 = 2 * q;
     L.r64 = (u64)(((u128)1 << 64) % q); L.r64s = shoup64_h(L.r64, q);
     L.one_s = (u64)(((u128)1 << 64) / q);
-    u64 qi = q;  // q^-1 mod 2^64 by Newton steps (q odd: exact to 3 bits, doubling each step)
-    for (int it = 0; it < 5; it++) qi *= 2 - q * qi;
-    L.qninv = 0 - qi;
     L.ninv = ni; L.ninv_s = shoup64_h(ni, q);
   }
   HIP_TRY(nullptr, hipMalloc((void **)&ctx->d_tables64, tab.size() * sizeof(u64)));

@@ -25,7 +25,6 @@ struct Limb64 {
   u64 q, q2;              // q, 2q
   u64 r64, r64s;          // 2^64 mod q and its Shoup companion
   u64 one_s;              // floor(2^64 / q)
-  u64 qninv;              // -q^-1 mod 2^64 (Montgomery reduction)
   u64 ninv, ninv_s;       // N^-1 mod q and Shoup companion
   const u64 *psi, *psi_s;     // forward twiddles, bit-reversed powers
   const u64 *ipsi, *ipsi_s;   // inverse twiddles

@@ -3,10 +3,10 @@
 // Arithmetic: each limb is a ~60-bit NTT prime, so values are u64 and every
 // product is a 64x64->128 product. Modular multiplication by a known
 // constant uses Shoup's method with a 64-bit companion (floor(w*2^64/q));
-// the multiply-accumulates against key rows (products of two variables) use
-// Montgomery reduction, with the 2^-64 factor of the whole sum removed by one
-// Shoup multiplication by 2^64 mod q at the end. NTT butterflies are
-// Harvey-lazy in [0, 4q) (4q < 2^62).
+// the multiply-accumulates against key rows (products of two variables) sum
+// exact 128-bit products and reduce each sum once, as hi*(2^64 mod q) + lo
+// with both halves by Shoup. NTT butterflies are Harvey-lazy in [0, 4q)
+// (4q < 2^62).
 //
 // Thread layout: a 4096-point transform is a lane block of 512 threads with
 // 8 values each, four radix-8 rounds and three LDS exchanges; value c of
@@ -45,15 +45,6 @@ __device__ __forceinline__ u64 submod(u64 a, u64 b, u64 q) { return a >= b ? a -
 // a * w mod q lazily in [0, 2q), any a < 2^64, w < q.
 __device__ __forceinline__ u64 shoup(u64 a, u64 w, u64 ws, u64 q) { return a * w - __umul64hi(a, ws) * q; }
 
-// Montgomery product a * b * 2^-64 mod q, lazily in [0, 2q), for a * b < q * 2^64
-// (a < 4q, b < q): m = (a * b) * (-q^-1) mod 2^64 makes a * b + m * q a
-// multiple of 2^64, and (a * b + m * q) / 2^64 < 2q. The low halves sum to 0
-// or to 2^64, so the carry is (lo != 0).
-__device__ __forceinline__ u64 mont_mul(u64 a, u64 b, const Limb64 &L) {
-  const u128 T = (u128)a * b;
-  const u64 lo = (u64)T, hi = (u64)(T >> 64);
-  return hi + __umul64hi(lo * L.qninv, L.q) + (lo != 0);
-}
 
 // CRT (r0 mod q0, r1 mod q1) -> X in [0, Q) (Garner, server.cpp:529-553 form).
 __device__ __forceinline__ u128 crt2(u64 r0, u64 r1, const Scheme64 &S) {
@@ -209,21 +200,29 @@ __device__ __forceinline__ void store3(u64 *p, const u64 v[8], int t) {
   for (int c = 0; c < 4; c++) q[c] = make_ulonglong2(v[2 * c], v[2 * c + 1]);
 }
 
-// acc[c] += v[c] * key[8t + c] * 2^-64 mod q, two key values at a time
-// (v < 4q, key < q); acc stays in [0, 2q). mac_finish turns the sum into
-// sum(v * key) mod q in [0, q) by one multiplication with 2^64 mod q.
-__device__ __forceinline__ void mac_row(u64 (&acc)[8], const u64 v[8], const u64 *key, int t, const Limb64 &L) {
+// acc[c] += v[c] * key[8t + c] as exact 128-bit sums, two key values at a
+// time; mac_finish reduces each sum once. With v < 4q and key < q < 2^60
+// every product is below 2^122, so a sum of up to 64 products cannot wrap
+// (the scheme validation bounds every digit count by that).
+__device__ __forceinline__ void mac_row(u128 (&acc)[8], const u64 v[8], const u64 *key, int t, const Limb64 &L) {
   const ulonglong2 *kp = reinterpret_cast<const ulonglong2 *>(key + 8 * t);
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     const ulonglong2 k = kp[c];
-    acc[2 * c] = csub(acc[2 * c] + mont_mul(v[2 * c], k.x, L), L.q2);
-    acc[2 * c + 1] = csub(acc[2 * c + 1] + mont_mul(v[2 * c + 1], k.y, L), L.q2);
+    acc[2 * c] += (u128)v[2 * c] * k.x;
+    acc[2 * c + 1] += (u128)v[2 * c + 1] * k.y;
   }
 }
-__device__ __forceinline__ void mac_finish(u64 (&acc)[8], const Limb64 &L) {
+// (hi * 2^64 + lo) mod q in [0, q): hi * (2^64 mod q) and lo * 1 by Shoup, each in [0, 2q).
+__device__ __forceinline__ u64 reduce128(u128 x, const Limb64 &L) {
+  const u64 hi = (u64)(x >> 64), lo = (u64)x;
+  const u64 a = shoup(hi, L.r64, L.r64s, L.q);
+  const u64 b = lo - __umul64hi(lo, L.one_s) * L.q;
+  return csub(csub(a + b, L.q2), L.q);
+}
+__device__ __forceinline__ void mac_finish(const u128 (&acc)[8], u64 (&out)[8], const Limb64 &L) {
 #pragma unroll
-  for (int c = 0; c < 8; c++) acc[c] = csub(shoup(acc[c], L.r64, L.r64s, L.q), L.q);
+  for (int c = 0; c < 8; c++) out[c] = reduce128(acc[c], L);
 }
 
 // One polynomial's external-product contribution (gsw.cpp:119-182 with the
@@ -232,7 +231,7 @@ __device__ __forceinline__ void mac_finish(u64 (&acc)[8], const Limb64 &L) {
 // limb and are multiplied into acc0/acc1 with rows p*l + (l-1-i) of g
 // ([rows][2 outputs][2 limbs][N], NTT form).
 __device__ __forceinline__ void ep_poly(const u128 *xs, int p, int l, int b, const u64 *__restrict__ g, int limb,
-                                        int t_in, const Limb64 &L, u64 *lds, u64 (&acc0)[8], u64 (&acc1)[8],
+                                        int t_in, const Limb64 &L, u64 *lds, u128 (&acc0)[8], u128 (&acc1)[8],
                                         int i0 = 0, int i1 = -1) {
   const u64 mask = ((u64)1 << b) - 1;
   if (i1 < 0) i1 = l;
@@ -337,9 +336,9 @@ __global__ __launch_bounds__(1024) void k2_expand_level(ExpandArgs64 A, Scheme64
       if (xs[j] > hQ) xs[j] -= Q;  // wraps to the two's complement of X - Q
     __syncthreads();
   }
-  u64 accb[8], acca[8];
+  u128 sb[8], sa[8];
 #pragma unroll
-  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
+  for (int cc = 0; cc < 8; cc++) sb[cc] = sa[cc] = 0;
   const int b = A.b_ks;
   const i128 h = ((i128)1 << (b - 1)) - 1;
   const u64 *key = A.bv[qi] + (size_t)A.level * A.key_level_values;
@@ -358,11 +357,12 @@ __global__ __launch_bounds__(1024) void k2_expand_level(ExpandArgs64 A, Scheme64
     H += h << (b * i);
     ntt_fwd(d, lds[limb], t, L, false);
     const u64 *row = key + (size_t)i * kCt;  // [b (-> c0), a (-> c1)][limb][N]
-    mac_row(accb, d, row + (size_t)(0 * 2 + limb) * kN, t, L);
-    mac_row(acca, d, row + (size_t)(1 * 2 + limb) * kN, t, L);
+    mac_row(sb, d, row + (size_t)(0 * 2 + limb) * kN, t, L);
+    mac_row(sa, d, row + (size_t)(1 * 2 + limb) * kN, t, L);
   }
-  mac_finish(accb, L);
-  mac_finish(acca, L);
+  u64 accb[8], acca[8];
+  mac_finish(sb, accb, L);
+  mac_finish(sa, acca, L);
   ntt_inv(accb, lds[limb], t, L);  // delta_b, coefficient form (map0)
   ntt_inv(acca, lds[limb], t, L);  // delta_a
   // c' = (sigma(c0) + delta_b, delta_a); left = c + c', right = (c - c') * X^-k
@@ -415,9 +415,9 @@ __global__ __launch_bounds__(1024) void k2_complete(CompleteArgs64 A, Scheme64 S
   const int a = g / A.l_ep, r = g % A.l_ep;
   const u64 *row = A.rows + qi * A.row_stride + (size_t)g * kCt;
   u64 *selb = A.sel + qi * A.sel_stride + (size_t)a * (2 * A.l_ep) * kCt;
-  u64 acc0[8], acc1[8];
+  u128 s0[8], s1[8];
 #pragma unroll
-  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  for (int c = 0; c < 8; c++) s0[c] = s1[c] = 0;
 #pragma unroll 1
   for (int p = 0; p < 2; p++) {
     u64 v[8];
@@ -425,11 +425,12 @@ __global__ __launch_bounds__(1024) void k2_complete(CompleteArgs64 A, Scheme64 S
     stage_compose(v, lds, limb, t, S, xs);
     ntt_fwd(v, lds[limb], t, L);
     store3(selb + (size_t)r * kCt + (size_t)(p * 2 + limb) * kN, v, t);
-    ep_poly(xs, p, A.l_key, A.b_key, A.gsw[qi], limb, t, L, lds[limb], acc0, acc1);
+    ep_poly(xs, p, A.l_key, A.b_key, A.gsw[qi], limb, t, L, lds[limb], s0, s1);
     __syncthreads();  // xs is rewritten for the next polynomial
   }
-  mac_finish(acc0, L);
-  mac_finish(acc1, L);
+  u64 acc0[8], acc1[8];
+  mac_finish(s0, acc0, L);
+  mac_finish(s1, acc1, L);
   u64 *bot = selb + (size_t)(A.l_ep + r) * kCt;
   store3(bot + (size_t)(0 * 2 + limb) * kN, acc0, t);
   store3(bot + (size_t)(1 * 2 + limb) * kN, acc1, t);
@@ -480,7 +481,7 @@ __global__ __launch_bounds__(1024) void k2_mux(MuxArgs64 A, Scheme64 S) {
   }
   const u64 *x = in + (size_t)xi * kCt, *y = in + (size_t)yi * kCt;
   const u64 *g = A.sel + qi * A.sel_stride;
-  u64 acc0[8], acc1[8];
+  u128 acc0[8], acc1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
 #pragma unroll 1
@@ -495,15 +496,16 @@ __global__ __launch_bounds__(1024) void k2_mux(MuxArgs64 A, Scheme64 S) {
     ep_poly(xs, p, A.l, A.b, g, limb, t, L, lds[limb], acc0, acc1);
     __syncthreads();  // xs is rewritten for the next polynomial
   }
-  mac_finish(acc0, L);
-  mac_finish(acc1, L);
+  u64 e0[8], e1[8];
+  mac_finish(acc0, e0, L);
+  mac_finish(acc1, e1, L);
   u64 x0[8], x1[8];
   load3(x0, x + (size_t)(0 * 2 + limb) * kN, t);
   load3(x1, x + (size_t)(1 * 2 + limb) * kN, t);
 #pragma unroll
   for (int c = 0; c < 8; c++) {
-    x0[c] = addmod(x0[c], acc0[c], L.q);
-    x1[c] = addmod(x1[c], acc1[c], L.q);
+    x0[c] = addmod(x0[c], e0[c], L.q);
+    x1[c] = addmod(x1[c], e1[c], L.q);
   }
   store3(dst + (size_t)(0 * 2 + limb) * kN, x0, t);
   store3(dst + (size_t)(1 * 2 + limb) * kN, x1, t);
@@ -549,19 +551,20 @@ __global__ __launch_bounds__(1024) void k2_ext_prod(const u64 *gsw, int l, int b
   __shared__ u64 lds[2][kLds];
   const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9), t = threadIdx.x & 511;
   const Limb64 &L = S.limb[limb];
-  u64 acc0[8], acc1[8];
+  u128 s0[8], s1[8];
 #pragma unroll
-  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  for (int c = 0; c < 8; c++) s0[c] = s1[c] = 0;
 #pragma unroll 1
   for (int p = 0; p < 2; p++) {
     u64 v[8];
     load0(v, ct + (size_t)(p * 2 + limb) * kN, t);
     stage_compose(v, lds, limb, t, S, xs);
-    ep_poly(xs, p, l, b, gsw, limb, t, L, lds[limb], acc0, acc1);
+    ep_poly(xs, p, l, b, gsw, limb, t, L, lds[limb], s0, s1);
     __syncthreads();  // xs is rewritten for the next polynomial
   }
-  mac_finish(acc0, L);
-  mac_finish(acc1, L);
+  u64 acc0[8], acc1[8];
+  mac_finish(s0, acc0, L);
+  mac_finish(s1, acc1, L);
   store3(out + (size_t)(0 * 2 + limb) * kN, acc0, t);
   store3(out + (size_t)(1 * 2 + limb) * kN, acc1, t);
 }
@@ -602,13 +605,14 @@ __global__ __launch_bounds__(1024) void k2_mux_part(MuxArgs64 A, u64 *__restrict
   for (int c = 0; c < 8; c++) v[c] = submod(v[c], w[c], L.q);
   ntt_inv(v, lds[limb], t, L);
   stage_compose(v, lds, limb, t, S, xs);
-  u64 acc0[8], acc1[8];
+  u128 s0[8], s1[8];
 #pragma unroll
-  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  for (int c = 0; c < 8; c++) s0[c] = s1[c] = 0;
   const int dg = (A.l + G - 1) / G, i0 = g * dg, i1 = min(A.l, i0 + dg);
-  ep_poly(xs, p, A.l, A.b, A.sel + qi * A.sel_stride, limb, t, L, lds[limb], acc0, acc1, i0, i1);
-  mac_finish(acc0, L);
-  mac_finish(acc1, L);
+  ep_poly(xs, p, A.l, A.b, A.sel + qi * A.sel_stride, limb, t, L, lds[limb], s0, s1, i0, i1);
+  u64 acc0[8], acc1[8];
+  mac_finish(s0, acc0, L);
+  mac_finish(s1, acc1, L);
   u64 *dst = part + ((size_t)qi * gridDim.x + blockIdx.x) * kCt;
   store3(dst + (size_t)(0 * 2 + limb) * kN, acc0, t);
   store3(dst + (size_t)(1 * 2 + limb) * kN, acc1, t);
@@ -672,9 +676,9 @@ __global__ __launch_bounds__(1024) void k2_expand_part(ExpandArgs64 A, u64 *__re
       if (xs[j] > hQ) xs[j] -= Q;
     __syncthreads();
   }
-  u64 accb[8], acca[8];
+  u128 sb[8], sa[8];
 #pragma unroll
-  for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
+  for (int cc = 0; cc < 8; cc++) sb[cc] = sa[cc] = 0;
   const int b = A.b_ks;
   const i128 h = ((i128)1 << (b - 1)) - 1;
   const u64 *key = A.bv[qi] + (size_t)A.level * A.key_level_values;
@@ -695,11 +699,12 @@ __global__ __launch_bounds__(1024) void k2_expand_part(ExpandArgs64 A, u64 *__re
     H += h << (b * i);
     ntt_fwd(d, lds[limb], t, L, false);
     const u64 *row = key + (size_t)i * kCt;
-    mac_row(accb, d, row + (size_t)(0 * 2 + limb) * kN, t, L);
-    mac_row(acca, d, row + (size_t)(1 * 2 + limb) * kN, t, L);
+    mac_row(sb, d, row + (size_t)(0 * 2 + limb) * kN, t, L);
+    mac_row(sa, d, row + (size_t)(1 * 2 + limb) * kN, t, L);
   }
-  mac_finish(accb, L);
-  mac_finish(acca, L);
+  u64 accb[8], acca[8];
+  mac_finish(sb, accb, L);
+  mac_finish(sa, acca, L);
   u64 *dst = part + ((size_t)qi * gridDim.x + blockIdx.x) * kCt;
   store3(dst + (size_t)(0 * 2 + limb) * kN, accb, t);
   store3(dst + (size_t)(1 * 2 + limb) * kN, acca, t);
