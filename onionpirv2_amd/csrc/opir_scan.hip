@@ -153,10 +153,12 @@ __global__ __launch_bounds__(256) void k_qfrag(const V *__restrict__ qntt, int8_
 // ---------------------------------------------------------------------------
 // Epilogues: sum_u (256^u mod q) G_u  ->  residue mod q
 // ---------------------------------------------------------------------------
+// The weights are held as int32 so that each term is one signed 32 x 32 ->
+// 64-bit multiply-add (v_mad_i64_i32).
 struct EpiK1 {
   static constexpr int P = 4;
   typedef u32 Res;
-  u32 wu[2 * P - 1];
+  int wu[2 * P - 1];  // 256^u mod q < 2^29
   u64 pos_off;  // q * ceil(2^59 / q): makes the int64 sum positive (|sum| < 2^56)
   const LimbConst *L;
   __device__ void init(const SchemeConst &S, int limb) {
@@ -164,7 +166,10 @@ struct EpiK1 {
     u64 x = 1;
 #pragma unroll
     for (int u = 0; u < 2 * P - 1; u++) {
-      wu[u] = (u32)x;
+      // readfirstlane (the value is uniform) hides the small known range of the
+      // first weights, which would otherwise select an unsigned multiply with
+      // a separate sign correction
+      wu[u] = __builtin_amdgcn_readfirstlane((int)x);
       x = (x << 8) % L->q;
     }
     pos_off = ((((u64)1 << 59) / L->q) + 1) * L->q;
@@ -180,7 +185,7 @@ struct EpiK1 {
 struct EpiK2 {
   static constexpr int P = 8;
   typedef u64 Res;
-  u32 wlo[2 * P - 1], whi[2 * P - 1];  // 256^u mod q split in 32-bit halves
+  int wlo[2 * P - 1], whi[2 * P - 1];  // 256^u mod q = whi * 2^30 + wlo, both < 2^30
   u64 off_lo, off_hi;                  // q * ceil(2^92 / q)
   const k2::Limb64 *L;
   __device__ void init(const k2::Scheme64 &S, int limb) {
@@ -189,8 +194,8 @@ struct EpiK2 {
     u64 x = 1;
 #pragma unroll
     for (int u = 0; u < 2 * P - 1; u++) {
-      wlo[u] = (u32)x;
-      whi[u] = (u32)(x >> 32);
+      wlo[u] = __builtin_amdgcn_readfirstlane((int)(x & ((1u << 30) - 1)));  // as in EpiK1
+      whi[u] = __builtin_amdgcn_readfirstlane((int)(x >> 30));
       x = (u64)(((u128)x << 8) % q);
     }
     const u128 off = ((((u128)1 << 92) / q) + 1) * (u128)q;
@@ -198,14 +203,14 @@ struct EpiK2 {
     off_hi = (u64)(off >> 64);
   }
   __device__ Res combine(const int (&g)[2 * P - 1]) const {
-    // |S_lo| < 15 * 2^32 * 2^26 < 2^63, |S_hi| < 15 * 2^28 * 2^26; total < 2^91
+    // |S_lo|, |S_hi| < 15 * 2^30 * 2^26 < 2^60; total < 2^91
     int64_t slo = 0, shi = 0;
 #pragma unroll
     for (int u = 0; u < 2 * P - 1; u++) {
       slo += (int64_t)wlo[u] * (int64_t)g[u];
       shi += (int64_t)whi[u] * (int64_t)g[u];
     }
-    const __int128 tot = (__int128)slo + ((__int128)shi << 32) + (__int128)(((u128)off_hi << 64) | off_lo);
+    const __int128 tot = (__int128)slo + ((__int128)shi << 30) + (__int128)(((u128)off_hi << 64) | off_lo);
     const u128 t = (u128)tot;
     const u64 hi = (u64)(t >> 64), lo = (u64)t;
     // (hi * 2^64 + lo) mod q, both halves by Shoup
@@ -366,9 +371,10 @@ __global__ __launch_bounds__(256) void k_scan(ScanArgs<Epi> A, SchemeT S) {
 // other reader: the database then passes through once and should not
 // displace the query fragments in L2), with the tile's B fragments (small,
 // L2-resident), then runs their MFMAs. Lanes
-// past the stored query columns load column 0 and multiply by zero, so every
-// load is unconditional and the compiler's in-order load counter lets the
-// MFMAs of the first k block start before the last load has landed.
+// past the stored query columns load column 0 (their result columns are
+// never stored), so every load is unconditional and the compiler's in-order
+// load counter lets the MFMAs of the first k block start before the last
+// load has landed.
 // SC: result columns staged per row (>= stored columns of the tile: 2 for
 // one query). For the HBM-bound small batches.
 // XPF: load the next batch of k blocks right after the current batch's MFMAs
@@ -399,8 +405,6 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
   const size_t rta = rt < (int)g.nrt ? (size_t)rt : g.nrt - 1;
   const int n = lane & 15, grp = lane >> 4;
   const bool has_col = n < qg.ncm;
-  int bmask = has_col ? -1 : 0;
-  asm volatile("" : "+v"(bmask));  // opaque: keeps the B loads unconditional
   const size_t qtile = (size_t)(P * 4 * qg.ncm * 16);
   const int nkb = (int)g.nkb;
   const int lv0 = lvb * kLevelsPerWG;
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
           for (int s = 0; s < P; s++)
 #pragma unroll
             for (int t = 0; t < P; t++)
-              acc[s + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[j][s], b[j][t] & bmask, acc[s + t], 0, 0, 0);
+              acc[s + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[j][s], b[j][t], acc[s + t], 0, 0, 0);
         if (XPF) {
           const bool last = ch + 1 == nch;
           const int nl = lsub + li + (last ? 1 : 0);
