@@ -57,6 +57,18 @@ __device__ __forceinline__ void balanced_digits(u64 a, int (&d)[P]) {
   d[P - 1] = (int)x;
 }
 
+// Digit s of the same representation on its own: a + 128 * (1 + 256 + ...
+// + 256^(P-2)) has the base-256 digits d_s + 128 (s < P - 1) and the top
+// digit d_{P-1}, since every d_s + 128 lies in [0, 256).
+template <int P>
+__device__ __forceinline__ int balanced_digit(u64 a, int s) {
+  u64 bias = 0;
+#pragma unroll
+  for (int i = 0; i < P - 1; i++) bias |= (u64)128 << (8 * i);
+  const u64 y = a + bias;
+  return s < P - 1 ? (int)((y >> (8 * s)) & 255) - 128 : (int)(y >> (8 * (P - 1)));
+}
+
 __device__ __forceinline__ size_t db_byte_offset(const DbGeom &g, int limb, int lv, size_t row, size_t k, int s) {
   const size_t rt = row >> 4, lr = row & 15, kb = k >> 6, grp = (k & 63) >> 4, j = k & 15;
   const size_t blk = ((((size_t)limb * g.nlv + lv) * g.nrt + rt) * g.nkb + kb) * g.P + s;
@@ -108,45 +120,55 @@ __global__ void k_db_from_ref(const u32 *__restrict__ ref, int8_t *__restrict__ 
 
 // Query columns -> B fragments. qntt [B][N0][2 polys][2 limbs][nlv] (NTT form)
 // -> qfrag [limb][level][ct][kb][plane][grp][n < ncm][16 bytes], ct = column
-// tile of 16 query columns (c = 2*query + poly), ncm = min(16, 2B).
-// Workgroup: one (query, poly, limb, 16-k block, 64-level block); thread
-// (level ll, plane group) packs 16 k values' digits of its planes.
+// tile of 16 query columns (c = 2*query + poly), ncm = min(16, 2B); the 16
+// bytes of (grp, n) are digit `plane` of k = 64*kb + 16*grp + j, j < 16
+// (zero for k >= N0; columns c >= 2B of the last tile are zero).
+// Workgroup: one (limb, column tile, 16-k group, 32-level block). It stages
+// the 16 columns x 16 k x 32 levels through LDS (read as 32-level runs of
+// each column's k), then each thread writes all P planes of one (level,
+// column): for a fixed (level, plane) the tile's columns are one contiguous
+// run of ncm * 16 bytes.
+constexpr int kQfLv = 32;                 // levels per workgroup
+constexpr int kQfNs = 16 * (kQfLv + 1) + 4;  // LDS pitch of a column (conflict-free reads)
 template <int P, typename V>
 __global__ __launch_bounds__(256) void k_qfrag(const V *__restrict__ qntt, int8_t *__restrict__ qfrag,
-                                               size_t q_stride, QfragGeom qg) {
-  __shared__ V tile[16][65];
-  const int col = blockIdx.x;  // 2*query + poly
-  const int qi = col >> 1, p = col & 1;
-  const int k0 = blockIdx.y * 16;
-  const int lv0 = (blockIdx.z >> 1) * 64, limb = blockIdx.z & 1;
-  const V *src = qntt + (size_t)qi * q_stride;
-  for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-    const int kk = e / 64, ll = e % 64;
-    const int k = k0 + kk;
-    tile[kk][ll] = k < qg.N0 ? src[(((size_t)k * 2 + p) * 2 + limb) * qg.nlv + lv0 + ll] : 0;
+                                               size_t q_stride, QfragGeom qg, int c_end) {
+  __shared__ V tile[16 * kQfNs];
+  const int ct = blockIdx.x / (qg.nkb * 4), kg = blockIdx.x % (qg.nkb * 4);  // kg = kb * 4 + grp
+  const int kb = kg >> 2, grp = kg & 3;
+  const int lv0 = blockIdx.y * kQfLv, limb = blockIdx.z;
+  const int ncols = qg.ncm;  // columns stored per tile; c_end = 2B columns in all
+  for (int e = threadIdx.x; e < 16 * 16 * kQfLv; e += 256) {
+    const int lv = e % kQfLv, row = e / kQfLv, n = row >> 4, j = row & 15;
+    const int c = ct * 16 + n, k = kg * 16 + j;
+    V x = 0;
+    if (n < ncols && c < c_end && k < qg.N0)
+      x = qntt[(size_t)(c >> 1) * q_stride + (((size_t)k * 2 + (c & 1)) * 2 + limb) * qg.nlv + lv0 + lv];
+    tile[n * kQfNs + j * (kQfLv + 1) + lv] = x;
   }
   __syncthreads();
-  const int ll = threadIdx.x & 63, tq = threadIdx.x >> 6;  // tq: planes tq, tq+4, ...
-  const int lv = lv0 + ll;
-  const int ct = col >> 4, n = col & 15, kb = k0 >> 6, grp = (k0 & 63) >> 4;
-  const size_t base = ((((size_t)limb * qg.nlv + lv) * qg.nct + ct) * qg.nkb + kb) * (size_t)(P * 4 * qg.ncm * 16);
+  const size_t blk = (((size_t)limb * qg.nlv + lv0) * qg.nct + ct) * qg.nkb + kb;  // level lv0
+  const size_t lvstep = (size_t)qg.nct * qg.nkb;  // blocks between levels
+  const size_t bsz = (size_t)P * 4 * ncols * 16;   // bytes of one block
+  for (int e = threadIdx.x; e < kQfLv * ncols; e += 256) {
+    const int n = e % ncols, lv = e / ncols;
+    V x[16];
 #pragma unroll
-  for (int pp = 0; pp < P / 4; pp++) {
-    const int s = tq + 4 * pp;
-    u32 w[4];
+    for (int j = 0; j < 16; j++) x[j] = tile[n * kQfNs + j * (kQfLv + 1) + lv];
+    int8_t *o = qfrag + (blk + lv * lvstep) * bsz + ((size_t)grp * ncols + n) * 16;
 #pragma unroll
-    for (int q4 = 0; q4 < 4; q4++) {
-      u32 packed = 0;
+    for (int s = 0; s < P; s++) {
+      u32 w[4];
 #pragma unroll
-      for (int b = 0; b < 4; b++) {
-        int d[P];
-        balanced_digits<P>((u64)tile[q4 * 4 + b][ll], d);
-        packed |= ((u32)(uint8_t)(int8_t)d[s]) << (8 * b);
+      for (int q4 = 0; q4 < 4; q4++) {
+        u32 packed = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+          packed |= ((u32)(uint8_t)(int8_t)balanced_digit<P>((u64)x[q4 * 4 + b], s)) << (8 * b);
+        w[q4] = packed;
       }
-      w[q4] = packed;
+      *reinterpret_cast<uint4 *>(o + (size_t)s * 4 * ncols * 16) = make_uint4(w[0], w[1], w[2], w[3]);
     }
-    *reinterpret_cast<uint4 *>(qfrag + base + ((size_t)(s * 4 + grp) * qg.ncm + n) * 16) =
-        make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
@@ -531,14 +553,14 @@ void launch_db_from_ref(const u32 *ref, int8_t *db, const DbGeom &g, hipStream_t
 void launch_qfrag(const u32 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st) {
   const QfragGeom qg = make_qfrag_geom(g.N0, B, 4, g.nlv);
   // every stored column and every k block is written (zeros for k >= N0)
-  dim3 pg(2 * B, cdivu(qg.nkb * 64, 16), (g.nlv / 64) * 2);
-  k_qfrag<4, u32><<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg);
+  dim3 pg(qg.nct * qg.nkb * 4, g.nlv / kQfLv, 2);
+  k_qfrag<4, u32><<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg, 2 * B);
 }
 
 void launch_qfrag64(const u64 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st) {
   const QfragGeom qg = make_qfrag_geom(g.N0, B, 8, g.nlv);
-  dim3 pg(2 * B, cdivu(qg.nkb * 64, 16), (g.nlv / 64) * 2);
-  k_qfrag<8, u64><<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg);
+  dim3 pg(qg.nct * qg.nkb * 4, g.nlv / kQfLv, 2);
+  k_qfrag<8, u64><<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg, 2 * B);
 }
 
 // Scan variants: (kind, column tiles per workgroup CB, k blocks per load
