@@ -123,16 +123,18 @@ __global__ void k_db_from_ref(const u32 *__restrict__ ref, int8_t *__restrict__ 
 // tile of 16 query columns (c = 2*query + poly), ncm = min(16, 2B); the 16
 // bytes of (grp, n) are digit `plane` of k = 64*kb + 16*grp + j, j < 16
 // (zero for k >= N0; columns c >= 2B of the last tile are zero).
-// Workgroup: one (limb, column tile, 16-k group, 32-level block). It stages
-// the 16 columns x 16 k x 32 levels through LDS (read as 32-level runs of
-// each column's k), then each thread writes all P planes of one (level,
-// column): for a fixed (level, plane) the tile's columns are one contiguous
-// run of ncm * 16 bytes.
-constexpr int kQfLv = 32;                 // levels per workgroup
-constexpr int kQfNs = 16 * (kQfLv + 1) + 4;  // LDS pitch of a column (conflict-free reads)
+// Workgroup: one (limb, column tile, 16-k group, block of kQfLv levels). It
+// stages the 16 columns x 16 k x kQfLv levels through LDS (read as 128-byte
+// level runs of each column's k), then each thread writes all P planes of
+// one (level, column): for a fixed (level, plane) the tile's columns are one
+// contiguous run of ncm * 16 bytes.
+template <typename V>
+constexpr int qf_levels() { return 128 / (int)sizeof(V); }  // levels per workgroup
 template <int P, typename V>
 __global__ __launch_bounds__(256) void k_qfrag(const V *__restrict__ qntt, int8_t *__restrict__ qfrag,
                                                size_t q_stride, QfragGeom qg, int c_end) {
+  constexpr int kQfLv = qf_levels<V>();
+  constexpr int kQfNs = 16 * (kQfLv + 1) + 4;  // LDS pitch of a column (conflict-free reads)
   __shared__ V tile[16 * kQfNs];
   const int ct = blockIdx.x / (qg.nkb * 4), kg = blockIdx.x % (qg.nkb * 4);  // kg = kb * 4 + grp
   const int kb = kg >> 2, grp = kg & 3;
@@ -553,13 +555,13 @@ void launch_db_from_ref(const u32 *ref, int8_t *db, const DbGeom &g, hipStream_t
 void launch_qfrag(const u32 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st) {
   const QfragGeom qg = make_qfrag_geom(g.N0, B, 4, g.nlv);
   // every stored column and every k block is written (zeros for k >= N0)
-  dim3 pg(qg.nct * qg.nkb * 4, g.nlv / kQfLv, 2);
+  dim3 pg(qg.nct * qg.nkb * 4, g.nlv / qf_levels<u32>(), 2);
   k_qfrag<4, u32><<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg, 2 * B);
 }
 
 void launch_qfrag64(const u64 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st) {
   const QfragGeom qg = make_qfrag_geom(g.N0, B, 8, g.nlv);
-  dim3 pg(qg.nct * qg.nkb * 4, g.nlv / kQfLv, 2);
+  dim3 pg(qg.nct * qg.nkb * 4, g.nlv / qf_levels<u64>(), 2);
   k_qfrag<8, u64><<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg, 2 * B);
 }
 
