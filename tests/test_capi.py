@@ -60,3 +60,8 @@ def test_k2_scheme_validation_without_gpu(O):
     assert create(bv_base_log2=20) == gpu.OPIR_EINVAL      # last signed digit shift beyond 126 bits
     assert create(small_q=q0 + 1) == gpu.OPIR_EINVAL
     assert create(poly_degree=2048) == gpu.OPIR_EINVAL
+    # a 128-bit multiply-accumulate sum holds at most 64 products of
+    # residues (< 4q times < q, q < 2^60): 2 * l_ep and 2 * l_key <= 64
+    assert create(l_ep=33, base_log2=3) == gpu.OPIR_EINVAL
+    assert create(l_key=33, base_log2_key=3) == gpu.OPIR_EINVAL
+    assert create(l_ep=32, base_log2=3) in (gpu.OPIR_OK, gpu.OPIR_EHIP)
