@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--serve-budget-gib", type=float, default=96.0,
                     help="HBM budget for client keys and batch workspaces of the serving loop")
     ap.add_argument("--no-k2", action="store_true", help="skip the CONFIG_N4096_K2_MP leg")
+    ap.add_argument("--no-graphs", action="store_true", help="skip the HIP-graph replay leg")
     ap.add_argument("--no-paper-row", action="store_true",
                     help="skip the with_layout({349526,10}) leg (the reference's 1 GB row, 5.3 GiB NTT form)")
     ap.add_argument("--k2-target", type=int, default=2**21, help="n=4096 layout: target plaintexts")
@@ -119,6 +120,38 @@ def committed_traffic(args):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":")[1]), os.path.relpath(path, ROOT)
     return None, None
+
+
+def graph_leg(args, G, ctx_h, answer, torch, B, d_r, server, queries, log):
+    """The same plain and batched answers with the launch sequence captured
+    as a HIP graph on first use and replayed (fixed buffers, so one graph per
+    key-table slot); the eager numbers above are the headline."""
+    import numpy as np
+    assert G.ctx_call_raw("opir_ctx_set_graphs", ctx_h, 1) == 0
+    out = {}
+    try:
+        for b, steps in ((1, args.steps), (B, max(3, args.steps // 4))):
+            if b == B and args.no_batch:
+                continue
+            for _ in range(10):  # capture every key-table slot's graph
+                answer(b)
+            torch.cuda.synchronize()
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+            evs[0].record()
+            for i in range(steps):
+                answer(b)
+                evs[i + 1].record()
+            torch.cuda.synchronize()
+            per = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+            ms = evs[0].elapsed_time(evs[-1]) / steps
+            out[f"b{b}"] = {"ms_per_batch": round(ms, 4), "p50_ms": round(pct(per, 50), 4),
+                            "p99_ms": round(pct(per, 99), 4), "steps": steps}
+            log(f"[graphs] B={b}: {ms:.3f} ms/batch (captured launch sequence replayed)")
+        rb = d_r.cpu().numpy().view(np.uint64)
+        assert np.array_equal(rb[0], server.make_query(1000, queries[0].copy())), "graph replay != plain"
+    finally:
+        assert G.ctx_call_raw("opir_ctx_set_graphs", ctx_h, 0) == 0
+    return out
 
 
 def serving(args, H, params, server, log, k2=None):
@@ -590,6 +623,8 @@ def main():
             log(f"[sweep] B={b}: {ms_s:.2f} ms/batch -> {b * 1e3 / ms_s:.1f} q/s")
         if rows:
             result["batch_sweep"] = rows
+    if not args.no_graphs:
+        result["graphs"] = graph_leg(args, G, ctx_h, answer, torch, B, d_r, server, queries, log)
     if not args.no_paper_row:
         result["paper_1gb_row"] = paper_row(args, H, G, torch, log)
     k2 = k2_setup(args, H, log) if not args.no_k2 else None

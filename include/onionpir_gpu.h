@@ -156,6 +156,16 @@ int opir_ctx_set_scan_variant(opir_ctx *ctx, int variant);
 int opir_scan_variant_count(const opir_ctx *ctx, int *out);
 /* The shape autotuning picked for B queries on db (-1 if not tuned yet). */
 int opir_db_tuned_scan_variant(const opir_db *db, size_t B, int *out);
+/* Launch mode of the answer calls that replace PirServer::make_query
+ * (src/server.cpp:895-947): HIP graphs for the batched form's
+ * per-dimension launches. on != 0: the first answer for a given batch size, key-pointer
+ * table slot, query/response buffers and scan shape captures its whole
+ * launch sequence (expansion, completion on the auxiliary stream, scan, MUX
+ * levels, mod switch) into a graph; later calls with the same ones replay
+ * it (up to 16 graphs per database, least recently used dropped). Answers
+ * are identical either way. Profiling and a scan shape still being
+ * autotuned run without a graph. Off by default. */
+int opir_ctx_set_graphs(opir_ctx *ctx, int on);
 
 /* ---- Stage entry points (host buffers), one per reference stage ---- */
 

@@ -32,6 +32,7 @@ int ensure_ws(opir_db *db, size_t B) {
   opir_ctx *ctx = db->ctx;
   if (db->ws_B >= B) return OPIR_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // queued work may still use the old buffers
+  drop_graphs(db);
   u32 **bufs[] = {&db->exp_a, &db->exp_b, &db->qntt, &db->sel, &db->mid_a, &db->mid_b, &db->part};
   for (auto b : bufs) {
     if (*b) (void)hipFree(*b);
@@ -124,26 +125,27 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
     if (k.h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
   }
   const size_t exp_stride = db->exp_cts * kCtWords;
-  rec(ctx, 0);
-  // root ct per query -> exp_a[q][0]
-  launch_split_u64_batch(d_q, 2 * kN, db->exp_a, exp_stride, 2, (int)B, ctx->S, ctx->stream);
-  LAUNCH_CHECK(ctx);
-  u32 *leaves = run_expand(db, B, P);
-  LAUNCH_CHECK(ctx);
-  rec(ctx, 1);
-  // selector completion on the auxiliary stream, concurrent with the scan
-  fork_aux(ctx);
-  const size_t nsel = db->dims - 1;
-  const size_t sel_stride = nsel * 2 * (size_t)ctx->G.l_ep * kCtWords;
-  launch_complete(leaves + db->N0 * kCtWords, db->sel, exp_stride, sel_stride, (int)(nsel * ctx->G.l_ep), (int)B, P,
-                  ctx->G, ctx->S, ctx->aux_stream);
-  LAUNCH_CHECK(ctx);
-  rec(ctx, 2, true);
-  launch_ct_ntt(leaves, db->qntt, exp_stride, db->N0 * kCtWords, (int)db->N0, (int)B, ctx->S, ctx->stream);
-  launch_qfrag(db->qntt, db->qfrag, db->N0 * kCtWords, (int)B, db->geom, ctx->stream);
-  rec(ctx, 5);
-  const int sv = pick_scan_variant(db, B, 4, [&](int v) {
-    launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream, v);
+  return run_answer(db, B, 4, P.bv_keys, d_q, d_r, [&]() -> int {
+    rec(ctx, 0);
+    // root ct per query -> exp_a[q][0]
+    launch_split_u64_batch(d_q, 2 * kN, db->exp_a, exp_stride, 2, (int)B, ctx->S, ctx->stream);
+    LAUNCH_CHECK(ctx);
+    u32 *leaves = run_expand(db, B, P);
+    LAUNCH_CHECK(ctx);
+    rec(ctx, 1);
+    // selector completion on the auxiliary stream, concurrent with the scan
+    fork_aux(ctx);
+    const size_t nsel = db->dims - 1;
+    const size_t sel_stride = nsel * 2 * (size_t)ctx->G.l_ep * kCtWords;
+    launch_complete(leaves + db->N0 * kCtWords, db->sel, exp_stride, sel_stride, (int)(nsel * ctx->G.l_ep), (int)B, P,
+                    ctx->G, ctx->S, ctx->aux_stream);
+    LAUNCH_CHECK(ctx);
+    rec(ctx, 2, true);
+    launch_ct_ntt(leaves, db->qntt, exp_stride, db->N0 * kCtWords, (int)db->N0, (int)B, ctx->S, ctx->stream);
+    launch_qfrag(db->qntt, db->qfrag, db->N0 * kCtWords, (int)B, db->geom, ctx->stream);
+    rec(ctx, 5);
+    const int sv = pick_scan_variant(db, B, 4, [&](int v) {
+      launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream, v);
   });
   if (launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream, sv))
     return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
@@ -158,6 +160,7 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   LAUNCH_CHECK(ctx);
   rec(ctx, 7);
   return OPIR_OK;
+  });
 }
 
 }  // namespace
@@ -350,6 +353,12 @@ int opir_ctx_set_scan_variant(opir_ctx *ctx, int variant) {
   return OPIR_OK;
 }
 
+int opir_ctx_set_graphs(opir_ctx *ctx, int on) {
+  if (!ctx) return OPIR_EINVAL;
+  ctx->graphs = on != 0;
+  return OPIR_OK;
+}
+
 int opir_db_tuned_scan_variant(const opir_db *db, size_t B, int *out) {
   if (!db || !out || B == 0) return OPIR_EINVAL;
   *out = db->tuned_scan[scan_class(B)];
@@ -419,6 +428,7 @@ int opir_db_create(opir_ctx *ctx, size_t N0, size_t Nrest, size_t dims, size_t h
 void opir_db_destroy(opir_db *db) {
   if (!db) return;
   (void)hipStreamSynchronize(db->ctx->stream);
+  drop_graphs(db);
   k2_db_free(db);
   for (u32 *p : {db->exp_a, db->exp_b, db->qntt, db->sel, db->mid_a, db->mid_b, db->part})
     if (p) (void)hipFree(p);

@@ -27,6 +27,17 @@ struct ClientKeys {
 // Small ring of pinned staging slots for per-batch key-pointer tables.
 constexpr int kRing = 8;
 
+// One captured answer: the launch sequence for B queries with these key
+// table, query and response pointers and this scan shape.
+struct GraphEntry {
+  size_t B = 0;
+  const void *table = nullptr, *dq = nullptr, *dr = nullptr;
+  int scan = -1;
+  hipGraphExec_t exec = nullptr;
+  uint64_t last_use = 0;
+};
+constexpr size_t kMaxGraphs = 16;  // per database
+
 }  // namespace capi
 
 struct opir_ctx {
@@ -45,6 +56,7 @@ struct opir_ctx {
   std::string err;
   bool profiling = false;
   int scan_variant = -1;    // first-dimension scan tile shape: -1 table default, -2 autotune, >= 0 forced
+  bool graphs = false;      // replay each answer's launch sequence as a captured HIP graph
   hipEvent_t ev[8] = {};
   // auxiliary stream (default priority): selector completion runs on it
   // concurrently with the first-dimension scan
@@ -77,6 +89,9 @@ struct opir_db {
   size_t exp_cts = 0;
   // autotuned scan shape per batch-size class (ceil(log2(2B))), -1 = not tuned
   int tuned_scan[16] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  // captured answers (opir_ctx_set_graphs), least recently used dropped first
+  std::vector<capi::GraphEntry> graphs;
+  uint64_t graph_tick = 0;
 };
 
 namespace capi {
@@ -156,6 +171,60 @@ inline void fork_aux(opir_ctx *ctx) {
 inline void join_aux(opir_ctx *ctx) {
   (void)hipEventRecord(ctx->ev_join, ctx->aux_stream);
   (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+}
+
+inline void drop_graphs(opir_db *db) {
+  for (auto &g : db->graphs) (void)hipGraphExecDestroy(g.exec);
+  db->graphs.clear();
+}
+
+// Runs launch() -- an answer's whole launch sequence on ctx->stream, which
+// may fork to the auxiliary stream and joins it back -- directly, or with
+// graphs on as a captured HIP graph: captured on the first call for the same
+// (B, key table, query, response pointers, scan shape), replayed after. The
+// sequence holds no memset nodes and no host synchronisation; an autotuned
+// scan shape that is not settled yet (its timing runs synchronise) and
+// profiling (its event records) run directly. Workspace reallocation drops
+// the cached graphs.
+template <typename Launch>
+int run_answer(opir_db *db, size_t B, int P, const void *table, const void *dq, const void *dr, Launch launch) {
+  opir_ctx *ctx = db->ctx;
+  if (!ctx->graphs || ctx->profiling) return launch();
+  int scan = ctx->scan_variant;
+  if (scan == -2) {
+    scan = db->tuned_scan[scan_class(B)];
+    if (scan < 0 || !scan_variant_ok(P, scan, (int)B, db->geom)) return launch();
+  }
+  GraphEntry *hit = nullptr;
+  for (auto &g : db->graphs)
+    if (g.B == B && g.table == table && g.dq == dq && g.dr == dr && g.scan == scan) hit = &g;
+  if (!hit) {
+    if (db->graphs.size() >= kMaxGraphs) {
+      auto lru = std::min_element(db->graphs.begin(), db->graphs.end(),
+                                  [](const GraphEntry &a, const GraphEntry &b) { return a.last_use < b.last_use; });
+      (void)hipGraphExecDestroy(lru->exec);
+      db->graphs.erase(lru);
+    }
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launch();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ec != hipSuccess) return set_err(ctx, OPIR_EHIP, std::string("graph capture: ") + hipGetErrorString(ec));
+    GraphEntry e;
+    const hipError_t ei = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return set_err(ctx, OPIR_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    e.B = B; e.table = table; e.dq = dq; e.dr = dr; e.scan = scan;
+    db->graphs.push_back(e);
+    hit = &db->graphs.back();
+  }
+  hit->last_use = ++db->graph_tick;
+  HIP_TRY(ctx, hipGraphLaunch(hit->exec, ctx->stream));
+  return OPIR_OK;
 }
 
 inline u64 mulmod_h(u64 a, u64 b, u64 m) { return (u64)((u128)a * b % m); }

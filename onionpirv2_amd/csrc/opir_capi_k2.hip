@@ -196,6 +196,7 @@ int k2_ensure_ws(opir_db *db, size_t B) {
   opir_ctx *ctx = db->ctx;
   if (db->ws_B >= B) return OPIR_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // queued work may still use the old buffers
+  drop_graphs(db);
   free_ws(db);
   const size_t ct = kCt2 * sizeof(u64), nsel = db->dims - 1;
   HIP_TRY(ctx, hipMalloc((void **)&db->e64_a, B * db->exp_cts * ct));
@@ -294,26 +295,27 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
     if (ctx->clients[ids[i]].h < db->h)
       return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
   const size_t exp_stride = db->exp_cts * kCt2;
-  rec(ctx, 0);
-  // root ct of query i -> e64_a[i][0]
-  HIP_TRY(ctx, hipMemcpy2DAsync(db->e64_a, exp_stride * sizeof(u64), d_q, kCt2 * sizeof(u64), kCt2 * sizeof(u64), B,
-                                hipMemcpyDeviceToDevice, ctx->stream));
-  u64 *leaves = run_expand(db, B, P);
-  LAUNCH_CHECK(ctx);
-  rec(ctx, 1);
-  // selector completion on the auxiliary stream, concurrent with the scan
-  fork_aux(ctx);
-  const size_t nsel = db->dims - 1;
-  const size_t sel_stride = nsel * 2 * (size_t)ctx->G64.l_ep * kCt2;
-  k2::launch_k2_complete(leaves + db->N0 * kCt2, db->sel64, exp_stride, sel_stride, (int)(nsel * ctx->G64.l_ep),
-                         (int)B, P, ctx->G64, ctx->S64, ctx->aux_stream);
-  LAUNCH_CHECK(ctx);
-  rec(ctx, 2, true);
-  k2::launch_k2_ct_ntt(leaves, db->q64, exp_stride, db->N0 * kCt2, (int)db->N0, (int)B, ctx->S64, ctx->stream);
-  launch_qfrag64(db->q64, db->qfrag, db->N0 * kCt2, (int)B, db->geom, ctx->stream);
-  rec(ctx, 5);
-  const int sv = pick_scan_variant(db, B, 8, [&](int v) {
-    launch_scan64(db->d_db, db->qfrag, db->m64_a, db->Nrest * kCt2, (int)B, db->geom, ctx->S64, ctx->stream, v);
+  return run_answer(db, B, 8, P.bv, d_q, d_r, [&]() -> int {
+    rec(ctx, 0);
+    // root ct of query i -> e64_a[i][0]
+    HIP_TRY(ctx, hipMemcpy2DAsync(db->e64_a, exp_stride * sizeof(u64), d_q, kCt2 * sizeof(u64), kCt2 * sizeof(u64), B,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+    u64 *leaves = run_expand(db, B, P);
+    LAUNCH_CHECK(ctx);
+    rec(ctx, 1);
+    // selector completion on the auxiliary stream, concurrent with the scan
+    fork_aux(ctx);
+    const size_t nsel = db->dims - 1;
+    const size_t sel_stride = nsel * 2 * (size_t)ctx->G64.l_ep * kCt2;
+    k2::launch_k2_complete(leaves + db->N0 * kCt2, db->sel64, exp_stride, sel_stride, (int)(nsel * ctx->G64.l_ep),
+                           (int)B, P, ctx->G64, ctx->S64, ctx->aux_stream);
+    LAUNCH_CHECK(ctx);
+    rec(ctx, 2, true);
+    k2::launch_k2_ct_ntt(leaves, db->q64, exp_stride, db->N0 * kCt2, (int)db->N0, (int)B, ctx->S64, ctx->stream);
+    launch_qfrag64(db->q64, db->qfrag, db->N0 * kCt2, (int)B, db->geom, ctx->stream);
+    rec(ctx, 5);
+    const int sv = pick_scan_variant(db, B, 8, [&](int v) {
+      launch_scan64(db->d_db, db->qfrag, db->m64_a, db->Nrest * kCt2, (int)B, db->geom, ctx->S64, ctx->stream, v);
   });
   if (launch_scan64(db->d_db, db->qfrag, db->m64_a, db->Nrest * kCt2, (int)B, db->geom, ctx->S64, ctx->stream, sv))
     return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
@@ -328,6 +330,7 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
   LAUNCH_CHECK(ctx);
   rec(ctx, 7);
   return OPIR_OK;
+  });
 }
 
 // ---------------------------------------------------------------------------

@@ -26,7 +26,7 @@ EXPORTED = [
     "opir_client_set_keys", "opir_client_remove",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
-    "opir_ctx_set_profiling", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
+    "opir_ctx_set_profiling", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_ctx_set_graphs", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
     "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
     "opir_ext_prod", "opir_mod_switch",
 ]
@@ -93,6 +93,7 @@ def load_library():
         "opir_ctx_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
         "opir_last_stage_ms": ([P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "opir_ctx_set_scan_variant": ([P, ctypes.c_int], ctypes.c_int),
+        "opir_ctx_set_graphs": ([P, ctypes.c_int], ctypes.c_int),
         "opir_scan_variant_count": ([P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "opir_db_tuned_scan_variant": ([P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "opir_ntt_fwd": ([P, P, sz], ctypes.c_int),
@@ -177,6 +178,12 @@ class GpuContext:
         size), SCAN_AUTOTUNE (-2, timed on first use per batch-size class), or
         a forced shape index."""
         self.check(self.lib.opir_ctx_set_scan_variant(self.h, int(variant)))
+
+    def set_graphs(self, on=True):
+        """Capture each answer's launch sequence as a HIP graph on first use
+        (per batch size, key-table slot, buffers and scan shape) and replay it
+        afterwards."""
+        self.check(self.lib.opir_ctx_set_graphs(self.h, 1 if on else 0))
 
     def scan_variant_count(self):
         out = ctypes.c_int(0)

@@ -259,6 +259,45 @@ def test_autotuned_scan_matches_default(O, comp_params, gpu_ctx):
     assert np.array_equal(want1, s.server.make_query(s.cid, qs[0]))
 
 
+def test_graph_replays_match_eager(O, comp_params, gpu_ctx):
+    """With graphs on, each answer's launch sequence is captured on first use
+    (per batch size and key-table slot) and replayed after: every replay
+    equals the eager answer, with eager calls in between, for single queries
+    and for batches whose client order changes between replays (the key
+    table is read at replay time)."""
+    say("[gpu] graph replays")
+    s = Setup(O, comp_params, (4096, 9, True), seed=6161, gpu_ctx=gpu_ctx, cid=61)
+    p = s.p
+    c2 = O.PirClient(p, 62, 6262)
+    gpu_ctx.set_client_keys(62, c2.create_bv_galois_keys(), c2.generate_gsw_from_key())
+    idxs = [(29 * j + 5) % p.target_num_pt for j in range(4)]
+    qa = np.stack([s.client.fast_generate_query(p, i) for i in idxs])
+    qb = np.stack([c2.fast_generate_query(p, i) for i in idxs])
+    want_a = [s.db.answer(61, q) for q in qa]
+    want_b = [s.db.answer(62, q) for q in qb]
+    mix_ids = [61, 62, 61, 62]
+    mix_q = np.stack([qa[0], qb[1], qa[2], qb[3]])
+    want_mix = s.db.answer_batch(mix_ids, mix_q)
+    assert np.array_equal(want_mix[1], want_b[1])
+    try:
+        gpu_ctx.set_graphs(True)
+        for r in range(20):  # more calls than key-table slots: every slot's graph is replayed
+            i = r % 4
+            assert np.array_equal(s.db.answer(61, qa[i]), want_a[i]), (r, "a")
+            assert np.array_equal(s.db.answer(62, qb[i]), want_b[i]), (r, "b")
+            if r % 3 == 0:
+                perm = [(k + r) % 4 for k in range(4)]
+                got = s.db.answer_batch([mix_ids[k] for k in perm], mix_q[perm])
+                assert np.array_equal(got, want_mix[perm]), (r, "mix")
+            if r % 5 == 4:  # eager work between replays
+                gpu_ctx.set_graphs(False)
+                assert np.array_equal(s.db.answer(62, qb[i]), want_b[i]), (r, "eager")
+                gpu_ctx.set_graphs(True)
+    finally:
+        gpu_ctx.set_graphs(False)
+        gpu_ctx.remove_client(62)
+
+
 def test_resident_key_bytes_track_clients(O, comp_params, gpu_ctx):
     """opir_ctx_resident_key_bytes (serving admission) counts each resident
     client's keys once, at its expansion height."""

@@ -182,6 +182,33 @@ def test_k2_batch_equals_single(O, k2p, ctx2):
     assert np.array_equal(batch[0], s.server.make_query(31, qs[0]))
 
 
+def test_k2_graph_replays_match_eager(O, k2p, ctx2):
+    """The two-prime answer captured as a HIP graph and replayed: every
+    replay equals the eager answer, with eager calls in between and a
+    changing client order in the batch."""
+    say("[gpu] K=2 graph replays")
+    s = Setup(O, k2p, (1024, 8, True), seed=91, ctx=ctx2, cid=35)
+    other = O.PirClient(s.p, 36, 92)
+    ctx2.set_client_keys(36, other.create_bv_galois_keys(), other.generate_gsw_from_key())
+    qa = s.client.fast_generate_query(s.p, 7)
+    qb = other.fast_generate_query(s.p, 1000)
+    want_a, want_b = s.db.answer(35, qa), s.db.answer(36, qb)
+    try:
+        ctx2.set_graphs(True)
+        for r in range(10):  # more calls than key-table slots
+            assert np.array_equal(s.db.answer(35, qa), want_a), (r, "a")
+            ids, qs, want = ([35, 36], [qa, qb], [want_a, want_b]) if r % 2 else ([36, 35], [qb, qa], [want_b, want_a])
+            got = s.db.answer_batch(ids, np.stack(qs))
+            assert np.array_equal(got, np.stack(want)), (r, "batch")
+            if r % 4 == 3:
+                ctx2.set_graphs(False)
+                assert np.array_equal(s.db.answer(36, qb), want_b), (r, "eager")
+                ctx2.set_graphs(True)
+    finally:
+        ctx2.set_graphs(False)
+        ctx2.remove_client(36)
+
+
 @pytest.mark.parametrize("layout", LAYOUTS[:4], ids=lambda l: "x".join(map(str, l)))
 def test_k2_every_scan_shape_matches_oracle(O, k2p, ctx2, layout):
     """Every tile shape of the two-prime scan gives the oracle's first
