@@ -35,6 +35,8 @@
 //    and shared by the 4 waves, each of which multiplies its A fragments
 //    (prefetched one k block ahead) into CB x (2P-1) accumulator tiles.
 // The shape is picked per batch size from a table (launch_scan).
+#include <type_traits>
+
 #include "opir_k2.h"
 #include "opir_kernels.h"
 
@@ -260,6 +262,46 @@ struct ScanArgs {
 
 constexpr int kLevelsPerWG = 16;
 
+// Write one wave's staged results: for each (row rr < 16, column nn < ncols)
+// of its tile, the LV consecutive levels st[0..LV)[rr][nn] go to
+// mid[q][row][poly][limb][lvo ..], c = col0 + nn = 2q + poly, row = 16 rt + rr.
+// With W = 16 / sizeof(Res) dividing LV, as 16-byte stores, consecutive
+// lanes taking consecutive pieces of a run (its index, a multiple of W,
+// makes each piece 16-byte aligned); rows past Nrest are not stored.
+template <int LV, int SW, typename Res>
+__device__ __forceinline__ void flush_stage(const Res (*st)[16][SW], int lane, int ncols, int rt, int col0, int limb,
+                                            int lvo, Res *mid, size_t mid_stride, size_t ct_vals, int nlv,
+                                            size_t Nrest) {
+  constexpr int W = 16 / (int)sizeof(Res);
+  constexpr int CH = LV % W == 0 ? LV / W : 0;  // 16-byte pieces per run
+  if constexpr (CH > 0) {
+    for (int e = lane; e < 16 * ncols * CH; e += 64) {
+      const int run = e / CH, ch = e - run * CH;
+      const int rr = run / ncols, nn = run % ncols;
+      const size_t row = (size_t)rt * 16 + rr;
+      if (row >= Nrest) continue;
+      const int c = col0 + nn, qi = c >> 1, pp = c & 1;
+      Res v[W];
+#pragma unroll
+      for (int w = 0; w < W; w++) v[w] = st[ch * W + w][rr][nn];
+      uint4 x;
+      __builtin_memcpy(&x, v, 16);
+      *reinterpret_cast<uint4 *>(mid + qi * mid_stride + row * ct_vals + (size_t)(pp * 2 + limb) * nlv + lvo +
+                                 ch * W) = x;
+    }
+  } else {
+    for (int e = lane; e < 16 * ncols; e += 64) {
+      const int rr = e / ncols, nn = e % ncols;
+      const size_t row = (size_t)rt * 16 + rr;
+      if (row >= Nrest) continue;
+      const int c = col0 + nn, qi = c >> 1, pp = c & 1;
+      Res *dst = mid + qi * mid_stride + row * ct_vals + (size_t)(pp * 2 + limb) * nlv + lvo;
+#pragma unroll
+      for (int li = 0; li < LV; li++) dst[li] = st[li][rr][nn];
+    }
+  }
+}
+
 // NT: nontemporal A loads, for grids where each DB tile has one reader
 // (one column block); with several, the tile's later readers find it in L2.
 template <int CB, int LV, bool NT, typename Epi, typename SchemeT>
@@ -375,17 +417,8 @@ __global__ __launch_bounds__(256) void k_scan(ScanArgs<Epi> A, SchemeT S) {
     }
     __syncthreads();
     // (row, column) pairs of this wave's row tile: LV consecutive levels each
-    const int ncols = min(CB * 16, 2 * A.B - ct0 * 16);
-    for (int e = lane; e < 16 * ncols; e += 64) {
-      const int rr = e / ncols, nn = e % ncols;
-      const size_t row = (size_t)rt * 16 + rr;
-      if (row >= g.Nrest) continue;
-      const int c = ct0 * 16 + nn, qi = c >> 1, pp = c & 1;
-      Res *dst = A.mid + qi * A.mid_stride + row * A.ct_vals + (size_t)(pp * 2 + limb) * g.nlv +
-                 lvb * kLevelsPerWG + lsub;
-#pragma unroll
-      for (int li = 0; li < LV; li++) dst[li] = stage[wave][li][rr][nn];
-    }
+    flush_stage<LV>(stage[wave], lane, min(CB * 16, 2 * A.B - ct0 * 16), rt, ct0 * 16, limb,
+                    lvb * kLevelsPerWG + lsub, A.mid, A.mid_stride, A.ct_vals, g.nlv, g.Nrest);
   }
 }
 
@@ -490,18 +523,115 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
       }
     }
     __syncthreads();
-    const int ncols = min(16, 2 * A.B - ct * 16);
-    for (int e = lane; e < 16 * ncols; e += 64) {
-      const int rr = e / ncols, nn = e % ncols;
-      const size_t row = (size_t)rt * 16 + rr;
-      if (row >= g.Nrest) continue;
-      const int c = ct * 16 + nn, qi = c >> 1, pp = c & 1;
-      Res *dst = A.mid + qi * A.mid_stride + row * A.ct_vals + (size_t)(pp * 2 + limb) * g.nlv + lv0 + lsub;
-#pragma unroll
-      for (int li = 0; li < LV; li++) dst[li] = stage[wave][li][rr][nn];
-    }
+    flush_stage<LV>(stage[wave], lane, min(16, 2 * A.B - ct * 16), rt, ct * 16, limb, lv0 + lsub, A.mid,
+                    A.mid_stride, A.ct_vals, g.nlv, g.Nrest);
     __syncthreads();
   }
+}
+
+// Ping-pong variant of the direct shape: two register buffers of KC k blocks
+// each. Step k of the workgroup's (level, batch) sequence first issues the
+// loads of step k + 1 into the other buffer, then runs its MFMAs, so the
+// wave's next loads are in flight while it multiplies and runs a level's
+// epilogue (the direct shape with XPF issues them only after its MFMAs).
+template <int KC, int LV, int SC, bool NT, typename Epi, typename SchemeT>
+__global__ __launch_bounds__(256) void k_scan_pp(ScanArgs<Epi> A, SchemeT S) {
+  constexpr int P = Epi::P, NU = 2 * P - 1;
+  typedef typename Epi::Res Res;
+  __shared__ Res stage[4][LV][16][SC + 1];
+  const DbGeom &g = A.g;
+  const QfragGeom &qg = A.qg;
+  const unsigned W = gridDim.x, pblk = blockIdx.x;
+  const unsigned Lg = (W % 8 == 0) ? (pblk % 8) * (W / 8) + pblk / 8 : pblk;
+  const int ct = Lg % A.ncb;
+  unsigned rest = Lg / A.ncb;
+  const int rb = rest % A.nrb;
+  rest /= A.nrb;
+  const int nlvb = g.nlv / kLevelsPerWG;
+  const int lvb = rest % nlvb;
+  const int limb = rest / nlvb;
+  Epi epi;
+  epi.init(S, limb);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rt = rb * 4 + wave;
+  const size_t rta = rt < (int)g.nrt ? (size_t)rt : g.nrt - 1;
+  const int n = lane & 15, grp = lane >> 4;
+  const bool has_col = n < qg.ncm;
+  const size_t qtile = (size_t)(P * 4 * qg.ncm * 16);
+  const int nkb = (int)g.nkb;
+  const int lv0 = lvb * kLevelsPerWG;
+  const int8_t *abase = A.db + (((size_t)limb * g.nlv + lv0) * g.nrt + rta) * g.nkb * (size_t)(P * 1024) + lane * 16;
+  const size_t astride = (size_t)g.nrt * g.nkb * (P * 1024);
+  const int8_t *bbase = A.qfrag + (((size_t)limb * g.nlv + lv0) * qg.nct + ct) * qg.nkb * qtile +
+                        ((size_t)grp * qg.ncm + (has_col ? n : 0)) * 16;
+  const size_t bstride = (size_t)qg.nct * qg.nkb * qtile;
+  const int nch = nkb / KC;                 // batches per level
+  const int nsteps = kLevelsPerWG * nch;    // even: kLevelsPerWG is
+  typedef v4i Buf[KC][P];
+  auto load = [&](int k, Buf &a, Buf &b) {
+    const int lrel = k / nch, ch = k - lrel * nch;
+    const int8_t *pa = abase + (size_t)lrel * astride + (size_t)ch * KC * (P * 1024);
+    const int8_t *pb = bbase + (size_t)lrel * bstride + (size_t)ch * KC * qtile;
+#pragma unroll
+    for (int j = 0; j < KC; j++)
+#pragma unroll
+      for (int s = 0; s < P; s++) {
+        const v4i *q = reinterpret_cast<const v4i *>(pa + (size_t)j * (P * 1024) + s * 1024);
+        a[j][s] = NT ? __builtin_nontemporal_load(q) : *q;
+      }
+#pragma unroll
+    for (int j = 0; j < KC; j++)
+#pragma unroll
+      for (int t = 0; t < P; t++)
+        b[j][t] = *reinterpret_cast<const v4i *>(pb + (size_t)j * qtile + (size_t)t * 4 * qg.ncm * 16);
+  };
+  const int ncols = min(16, 2 * A.B - ct * 16);
+  v4i acc[NU];
+#pragma unroll
+  for (int u = 0; u < NU; u++) acc[u] = (v4i){0, 0, 0, 0};
+  // loads are unconditional inside the loop (the last step is peeled), so
+  // the compiler's load counter waits only for the step's own buffer
+  auto step = [&](auto do_load, int k, const Buf &a, const Buf &b, Buf &an, Buf &bn) {
+    if constexpr (decltype(do_load)::value) load(k + 1, an, bn);
+#pragma unroll
+    for (int j = 0; j < KC; j++)
+#pragma unroll
+      for (int s = 0; s < P; s++)
+#pragma unroll
+        for (int t = 0; t < P; t++)
+          acc[s + t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[j][s], b[j][t], acc[s + t], 0, 0, 0);
+    const int lrel = k / nch;
+    if (k - lrel * nch != nch - 1) return;
+    // last batch of level lrel: epilogue, then a flush every LV levels
+    const int li = lrel % LV;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      int gs[NU];
+#pragma unroll
+      for (int u = 0; u < NU; u++) gs[u] = acc[u][r];
+      const Res v = epi.combine(gs);
+      if (n < SC) stage[wave][li][grp * 4 + r][n] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < NU; u++) acc[u] = (v4i){0, 0, 0, 0};
+    if (li != LV - 1) return;
+    const int lsub = lrel - li;
+    __syncthreads();
+    flush_stage<LV>(stage[wave], lane, ncols, rt, ct * 16, limb, lv0 + lsub, A.mid, A.mid_stride, A.ct_vals, g.nlv,
+                    g.Nrest);
+    __syncthreads();
+  };
+  Buf a0, b0, a1, b1;
+  load(0, a0, b0);
+  const std::true_type next;
+  const std::false_type last;
+#pragma unroll 1
+  for (int k = 0; k < nsteps - 2; k += 2) {
+    step(next, k, a0, b0, a1, b1);
+    step(next, k + 1, a1, b1, a0, b0);
+  }
+  step(next, nsteps - 2, a0, b0, a1, b1);
+  step(last, nsteps - 1, a1, b1, a0, b0);
 }
 
 }  // namespace
@@ -566,9 +696,10 @@ void launch_qfrag64(const u64 *qntt, int8_t *qfrag, size_t q_stride, int B, cons
 }
 
 // Scan variants: (kind, column tiles per workgroup CB, k blocks per load
-// batch KC, levels staged per flush LV, staged columns SC). A direct variant
-// takes one column tile per workgroup, needs nkb % KC == 0 and stages
-// min(16, 2B) <= SC columns.
+// batch KC, levels staged per flush LV, staged columns SC, next loads issued
+// early). A direct variant (k_scan_direct, or k_scan_pp with two register
+// buffers) takes one column tile per workgroup, needs nkb % KC == 0 and
+// stages min(16, 2B) <= SC columns.
 struct ScanShape {
   bool direct;
   int CB, KC, LV, SC;
@@ -613,6 +744,16 @@ static void run_direct(const int8_t *db, const int8_t *qfrag, typename Epi::Res 
     k_scan_direct<KC, LV, SC, false, XPF, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
 }
 
+template <int KC, int LV, int SC, typename Epi, typename SchemeT>
+static void run_pp(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
+                   const DbGeom &g, const SchemeT &S, hipStream_t st) {
+  const ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, 1);
+  if (A.ncb == 1)
+    k_scan_pp<KC, LV, SC, true, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+  else
+    k_scan_pp<KC, LV, SC, false, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+}
+
 template <typename Epi, typename SchemeT>
 using ScanFn = void (*)(const int8_t *, const int8_t *, typename Epi::Res *, size_t, int, const DbGeom &,
                         const SchemeT &, hipStream_t);
@@ -633,6 +774,7 @@ static const ScanEntry<EpiK1, SchemeConst> kScanK1[] = {
     {{false, 1, 1, 4, 16, false}, run_lds<1, 4, EpiK1, SchemeConst>},
     {{false, 2, 1, 4, 32, false}, run_lds<2, 4, EpiK1, SchemeConst>},
     {{false, 2, 1, 2, 32, false}, run_lds<2, 2, EpiK1, SchemeConst>},
+    {{true, 1, 2, 16, 2, true}, run_pp<2, 16, 2, EpiK1, SchemeConst>},
 };
 
 static const ScanEntry<EpiK2, S64> kScanK2[] = {
@@ -644,6 +786,7 @@ static const ScanEntry<EpiK2, S64> kScanK2[] = {
     {{false, 2, 1, 2, 32, false}, run_lds<2, 2, EpiK2, S64>},
     {{false, 2, 1, 1, 32, false}, run_lds<2, 1, EpiK2, S64>},
     {{true, 1, 1, 16, 2, true}, run_direct<1, 16, 2, true, EpiK2, S64>},
+    {{true, 1, 1, 16, 2, true}, run_pp<1, 16, 2, EpiK2, S64>},
 };
 
 int scan_variant_count(int P) { return P == 4 ? (int)std::size(kScanK1) : (int)std::size(kScanK2); }
@@ -653,11 +796,11 @@ static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
   return g.nkb % s.KC == 0 && (2 * B < 16 ? 2 * B : 16) <= s.SC;
 }
 
-// Default shapes per batch size, in order of preference (measured on MI355X
-// at the 256 x 1024 and 512 x 4096 layouts): direct streaming while the scan
-// is HBM-bound, B fragments shared through LDS once it is MFMA-bound.
-static const int kPrefK1[3][4] = {{0, 3, 5, -1}, {1, 3, 5, -1}, {6, 5, -1, -1}};
-static const int kPrefK2[3][4] = {{1, 3, 4, -1}, {3, 4, -1, -1}, {4, 5, -1, -1}};
+// Default shapes per batch-size class (2B <= 2, <= 16, larger), in order of
+// preference, the fastest measured on MI355X at the 256 x 1024 and
+// 512 x 4096 layouts (B = 1, 8 and 64 / 32).
+static const int kPrefK1[3][4] = {{0, 3, 5, -1}, {1, 3, 5, -1}, {1, 6, 5, -1}};
+static const int kPrefK2[3][4] = {{8, 1, 3, 4}, {3, 4, -1, -1}, {3, 4, 5, -1}};
 
 template <typename Epi, typename SchemeT, size_t NV>
 static int dispatch(const ScanEntry<Epi, SchemeT> (&tab)[NV], const int (&pref)[3][4], int variant,
