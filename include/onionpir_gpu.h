@@ -190,6 +190,14 @@ int opir_other_dim(opir_db *db, const uint64_t *mid, const uint64_t *sel, uint64
  * ct [2][N] coefficient form -> out [2][N] NTT form. */
 int opir_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t base_log2, const uint64_t *ct,
                   uint64_t *out);
+/* PirServer::ext_prod_mux (src/server.cpp:652-722, declared in
+ * src/includes/server.h:96-99): out = x + INTT(gsw [x] (y - x)), the
+ * homomorphic select of y (gsw encrypts 1) or x (gsw encrypts 0). gsw
+ * [2l][2][N] NTT form; x, y, out [2][N] coefficient form. x and y are left
+ * unchanged (the reference uses y as scratch; its contents after the call
+ * are not part of the result). */
+int opir_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t base_log2, const uint64_t *x,
+                      const uint64_t *y, uint64_t *out);
 /* PirServer::mod_switch_inplace (src/server.cpp:1025-1043): ct [2][N]
  * coefficient form mod Q -> out [2][N] mod small_q. */
 int opir_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out);

@@ -839,6 +839,41 @@ int opir_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const 
   return OPIR_OK;
 }
 
+int opir_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const uint64_t *x, const uint64_t *y,
+                      uint64_t *out) {
+  if (!ctx || !gsw || !x || !y || !out) return OPIR_EINVAL;
+  if (ctx->k2) return k2_ext_prod_mux(ctx, gsw, l, b, x, y, out);
+  if (l == 0 || l > 64 || b == 0 || b > 32) return set_err(ctx, OPIR_EINVAL, "invalid gadget");
+  int rc = validate_mod(ctx, gsw, 2 * l * 2 * kN, ctx->S.Q, "RGSW");
+  if (rc) return rc;
+  if ((rc = validate_mod(ctx, x, 2 * kN, ctx->S.Q, "ciphertext x"))) return rc;
+  if ((rc = validate_mod(ctx, y, 2 * kN, ctx->S.Q, "ciphertext y"))) return rc;
+  DevBuf g64, g32, xd, yd, c32, o32, o64;
+  DEV_ALLOC(ctx, g64, 2 * l * 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, g32, 2 * l * kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, xd, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, yd, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, c32, kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, o32, kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, o64, 2 * kN * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(g64.p, gsw, 2 * l * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(xd.p, x, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(yd.p, y, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  const u64 Q = ctx->S.Q;
+  // y - x mod Q, external product (NTT-form output), INTT, + x (server.cpp:696-721)
+  launch_addsub_mod((u64 *)yd.p, (u64 *)xd.p, (u64 *)yd.p, 2 * kN, kN, 1, Q, Q, true, ctx->stream);
+  launch_split_u64((u64 *)g64.p, (u32 *)g32.p, 2 * l * 2, ctx->S, ctx->stream);
+  launch_split_u64((u64 *)yd.p, (u32 *)c32.p, 2, ctx->S, ctx->stream);
+  launch_ext_prod((u32 *)g32.p, (int)l, (int)b, (u32 *)c32.p, (u32 *)o32.p, ctx->S, ctx->stream);
+  launch_join_u64((u32 *)o32.p, (u64 *)o64.p, 2, ctx->S, ctx->stream);
+  launch_ntt_u64((u64 *)o64.p, 2, true, ctx->S, ctx->stream);
+  launch_addsub_mod((u64 *)xd.p, (u64 *)o64.p, (u64 *)o64.p, 2 * kN, kN, 1, Q, Q, false, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
 int opir_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out) {
   if (!ctx || !ct || !out) return OPIR_EINVAL;
   if (ctx->k2) return k2_mod_switch(ctx, ct, out);

@@ -355,6 +355,8 @@ int k2_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *se
 int k2_first_dim(opir_db *db, const uint64_t *query, uint64_t *out);
 int k2_other_dim(opir_db *db, const uint64_t *mid, const uint64_t *sel, uint64_t *out);
 int k2_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const uint64_t *ct, uint64_t *out);
+int k2_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const uint64_t *x, const uint64_t *y,
+                    uint64_t *out);
 int k2_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out);
 
 }  // namespace capi

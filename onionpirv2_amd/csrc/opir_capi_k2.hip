@@ -450,6 +450,32 @@ int k2_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const ui
   return OPIR_OK;
 }
 
+int k2_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const uint64_t *x, const uint64_t *y,
+                    uint64_t *out) {
+  if (l == 0 || l > 64 || b == 0 || b > 32) return set_err(ctx, OPIR_EINVAL, "invalid gadget");
+  int rc = validate_rns(ctx, gsw, 2 * l * 2, "RGSW");
+  if (rc) return rc;
+  if ((rc = validate_rns(ctx, x, 2, "ciphertext x"))) return rc;
+  if ((rc = validate_rns(ctx, y, 2, "ciphertext y"))) return rc;
+  DevBuf g, xd, yd, o;
+  DEV_ALLOC(ctx, g, 2 * l * kCt2 * sizeof(u64));
+  DEV_ALLOC(ctx, xd, kCt2 * sizeof(u64));
+  DEV_ALLOC(ctx, yd, kCt2 * sizeof(u64));
+  DEV_ALLOC(ctx, o, kCt2 * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(g.p, gsw, 2 * l * kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(xd.p, x, kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(yd.p, y, kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  const u64 q0 = ctx->S64.limb[0].q, q1 = ctx->S64.limb[1].q;
+  opir::launch_addsub_mod((u64 *)yd.p, (u64 *)xd.p, (u64 *)yd.p, kCt2, N2, 2, q0, q1, true, ctx->stream);
+  k2::launch_k2_ext_prod((const u64 *)g.p, (int)l, (int)b, (const u64 *)yd.p, (u64 *)o.p, ctx->S64, ctx->stream);
+  k2::launch_k2_ntt((u64 *)o.p, 2, true, ctx->S64, ctx->stream);
+  opir::launch_addsub_mod((u64 *)xd.p, (u64 *)o.p, (u64 *)o.p, kCt2, N2, 2, q0, q1, false, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, kCt2 * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
 int k2_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out) {
   int rc = validate_rns(ctx, ct, 2, "ciphertext");
   if (rc) return rc;

@@ -143,5 +143,10 @@ void launch_ct_to_coef_u64(const u32 *in, u64 *out, size_t count, const SchemeCo
 // Stand-alone external product (test entry point).
 void launch_ext_prod(const u32 *gsw, int l, int b, const u32 *ct_coef, u32 *out_ntt, const SchemeConst &S,
                      hipStream_t st);
+// out = a - b (sub) or a + b per residue, n values laid out as [..][K][limb_len]
+// with modulus q[k] for limb k (K = 1 or 2; a, b already reduced, q < 2^63).
+// out may alias a or b.
+void launch_addsub_mod(const u64 *a, const u64 *b, u64 *out, size_t n, size_t limb_len, int K, u64 q0, u64 q1,
+                       bool sub, hipStream_t st);
 
 }  // namespace opir

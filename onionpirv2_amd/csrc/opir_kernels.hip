@@ -84,6 +84,25 @@ __global__ void k_join_u64(const u32 *__restrict__ in, u64 *__restrict__ out, si
   out[i] = crt_compose(in[(2 * p) * kN + j], in[(2 * p + 1) * kN + j], S);
 }
 
+// Residue-wise a -/+ b, the EltwiseSubMod / EltwiseAddMod steps of the
+// homomorphic MUX (src/server.cpp:657-683). Limb k of every [K][limb_len]
+// group is reduced by q_k.
+__global__ void k_addsub_mod(const u64 *a, const u64 *b, u64 *out, size_t n, size_t limb_len, int K, u64 q0,
+                             u64 q1, int sub) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u64 q = ((i / limb_len) % (size_t)K) ? q1 : q0;
+  const u64 x = a[i], y = b[i];
+  u64 r;
+  if (sub) {
+    r = x >= y ? x - y : x + (q - y);
+  } else {
+    r = x + y;
+    if (r >= q) r -= q;
+  }
+  out[i] = r;
+}
+
 // In-place NTT of u64 (mod Q) polynomials; one workgroup (2 lane blocks,
 // one per limb) per polynomial.
 __global__ __launch_bounds__(512) void k_ntt_u64(u64 *data, int inverse, SchemeConst S) {
@@ -290,6 +309,10 @@ void launch_split_u64_batch(const u64 *in, size_t in_stride, u32 *out, size_t ou
 }
 void launch_join_u64(const u32 *in, u64 *out, size_t npoly, const SchemeConst &S, hipStream_t st) {
   k_join_u64<<<cdiv(npoly * kN, 256), 256, 0, st>>>(in, out, npoly, S);
+}
+void launch_addsub_mod(const u64 *a, const u64 *b, u64 *out, size_t n, size_t limb_len, int K, u64 q0, u64 q1,
+                       bool sub, hipStream_t st) {
+  if (n) k_addsub_mod<<<cdiv(n, 256), 256, 0, st>>>(a, b, out, n, limb_len, K, q0, q1, sub ? 1 : 0);
 }
 void launch_ntt_u64(u64 *data, size_t npoly, bool inverse, const SchemeConst &S, hipStream_t st) {
   if (npoly) k_ntt_u64<<<(unsigned)npoly, 512, 0, st>>>(data, inverse ? 1 : 0, S);

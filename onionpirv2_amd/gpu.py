@@ -28,7 +28,7 @@ EXPORTED = [
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
     "opir_ctx_set_profiling", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_ctx_set_graphs", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
     "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
-    "opir_ext_prod", "opir_mod_switch",
+    "opir_ext_prod", "opir_ext_prod_mux", "opir_mod_switch",
 ]
 
 
@@ -103,6 +103,7 @@ def load_library():
         "opir_first_dim": ([P, P, P], ctypes.c_int),
         "opir_other_dim": ([P, P, P, P], ctypes.c_int),
         "opir_ext_prod": ([P, P, sz, sz, P, P], ctypes.c_int),
+        "opir_ext_prod_mux": ([P, P, sz, sz, P, P, P], ctypes.c_int),
         "opir_mod_switch": ([P, P, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -231,6 +232,12 @@ class GpuContext:
         c = _u64(ct)
         out = np.zeros((2, self.KN), dtype=np.uint64)
         self.check(self.lib.opir_ext_prod(self.h, _ptr(g), l, base_log2, _ptr(c), _ptr(out)))
+        return out
+
+    def ext_prod_mux(self, gsw, l, base_log2, x, y):
+        g, xa, ya = _u64(gsw), _u64(x), _u64(y)
+        out = np.zeros((2, self.KN), dtype=np.uint64)
+        self.check(self.lib.opir_ext_prod_mux(self.h, _ptr(g), l, base_log2, _ptr(xa), _ptr(ya), _ptr(out)))
         return out
 
     def mod_switch(self, ct):

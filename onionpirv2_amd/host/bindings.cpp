@@ -235,6 +235,13 @@ PYBIND11_MODULE(_opir_host, m) {
              }
              return a;
            })
+      .def("ext_prod_mux",
+           [](PirServer &s, Arr x, Arr y, Arr sel) {
+             RlweCt cx = arr_ct(x), cy = arr_ct(y), r;
+             GSWCt g = arr_gsw(sel);
+             s.ext_prod_mux(cx, cy, g, r);
+             return ct_arr(r);
+           })
       .def("save_resp_to_stream",
            [](PirServer &s, Arr ct) {
              std::stringstream ss;

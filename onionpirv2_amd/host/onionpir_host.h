@@ -238,6 +238,9 @@ class PirServer {
   std::vector<GSWCt> complete_selectors(size_t client_id, const std::vector<RlweCt> &expanded);
   std::vector<RlweCt> evaluate_first_dim(std::vector<RlweCt> &selection_vector);
   RlweCt evaluate_other_dim(std::vector<RlweCt> &intermediate_db, std::vector<GSWCt> &selectors);
+  // Homomorphic select: result = x when selection_cipher encrypts 0, y when
+  // it encrypts 1 (server.h:96-99). x and y stay unchanged; result may be x.
+  void ext_prod_mux(RlweCt &x, RlweCt &y, GSWCt &selection_cipher, RlweCt &result);
   size_t save_resp_to_stream(const RlweCt &response, std::stringstream &resp_stream);
   void set_client_bv_galois_key(size_t client_id, bvks::BvGaloisKeys bv_keys);
   void set_client_gsw_key(size_t client_id, GSWCt gsw_key);

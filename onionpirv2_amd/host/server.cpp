@@ -309,6 +309,21 @@ RlweCt PirServer::evaluate_other_dim(std::vector<RlweCt> &mid, std::vector<GSWCt
   return unflatten(out.data(), N, false);
 }
 
+void PirServer::ext_prod_mux(RlweCt &x, RlweCt &y, GSWCt &sel, RlweCt &result) {
+  // server.cpp:652-722: result = x + INTT(sel [x] (y - x)), all on the GPU
+  const size_t N = params_.get_poly_degree() * params_.K(), l = params_.get_l();
+  if (sel.size() != 2 * l) throw std::invalid_argument("selection RGSW must hold 2*l rows");
+  auto fx = flatten(x, N), fy = flatten(y, N);
+  std::vector<uint64_t> g(2 * l * 2 * N), out(2 * N);
+  for (size_t r = 0; r < 2 * l; r++) {
+    if (sel[r].size() != 2 * N) throw std::invalid_argument("selection RGSW row has the wrong size");
+    std::memcpy(&g[r * 2 * N], sel[r].data(), 2 * N * 8);
+  }
+  scheme_->check(
+      opir_ext_prod_mux(scheme_->ctx(), g.data(), l, params_.get_base_log2(), fx.data(), fy.data(), out.data()));
+  result = unflatten(out.data(), N, x.ntt_form);
+}
+
 size_t PirServer::save_resp_to_stream(const RlweCt &resp, std::stringstream &stream) {
   // server.cpp:950-1009: c0 then c1, ceil(log2 q') bits each, LSB-first
   const size_t N = params_.get_poly_degree();

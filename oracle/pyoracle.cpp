@@ -353,6 +353,12 @@ PYBIND11_MODULE(onionpir_oracle, m) {
            [](PirServer &s, size_t id, U64Arr bv, U64Arr gsw) {
              s.set_client_keys(id, arr_to_bv(bv, s.params.N()), arr_to_gsw(gsw));
            })
+      .def("ext_prod_mux",
+           [](const PirServer &s, U64Arr x, U64Arr y, U64Arr sel) {
+             RlweCt cx = arr_to_ct(x), cy = arr_to_ct(y), r;
+             s.ext_prod_mux(cx, cy, arr_to_gsw(sel), r);
+             return ct_to_arr(r);
+           })
       .def("direct_get_original_plaintext", [](const PirServer &s, size_t i) {
         auto it = s.recorded.find(i);
         if (it == s.recorded.end()) throw std::invalid_argument("Plaintext index was not recorded during database loading");

@@ -99,6 +99,35 @@ def test_external_product_matches_oracle(O, comp_params, gpu_ctx, which):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("sel", ["zero", "one", "random"])
+def test_ext_prod_mux_matches_oracle(O, comp_params, gpu_ctx, sel):
+    """PirServer::ext_prod_mux (src/server.cpp:652-722) through the C ABI and
+    through the host PirServer, against the oracle, bit-exact."""
+    import onionpirv2_amd.host as H
+    say(f"[gpu] ext_prod_mux ({sel} selector)")
+    p = comp_params
+    Q = p.rns_mods[0]
+    l, b = p.l_ep, p.base_log2
+    rng = np.random.default_rng(19)
+    if sel == "random":
+        gsw = rng.integers(0, Q, size=(2 * l, 2 * p.N), dtype=np.uint64)
+    else:
+        sk, _ = O.gen_sk_and_encrypt(p, np.zeros(p.N, np.uint64), 5)
+        pt = np.zeros(p.N, np.uint64); pt[0] = 1 if sel == "one" else 0
+        gsw = O.plain_to_gsw(p, pt, sk, l, b, 9)
+    x = rng.integers(0, Q, size=(2, p.N), dtype=np.uint64)
+    y = rng.integers(0, Q, size=(2, p.N), dtype=np.uint64)
+    # subtraction and addition wrap-around edges
+    x[0, :4] = [0, Q - 1, 5, Q - 1]
+    y[0, :4] = [Q - 1, 0, 5, Q - 1]
+    exp = O.PirServer(p.with_layout(1, 0)).ext_prod_mux(x.copy(), y.copy(), gsw)
+    x0, y0 = x.copy(), y.copy()
+    assert np.array_equal(gpu_ctx.ext_prod_mux(gsw, l, b, x, y), exp)
+    assert np.array_equal(x, x0) and np.array_equal(y, y0)
+    hp = H.PirParams("CONFIG_N2048_K1_COMP").with_layout(43, 5)
+    assert np.array_equal(H.PirServer(hp).ext_prod_mux(x, y, gsw), exp)
+
+
 def test_mod_switch_matches_oracle(O, comp_params, gpu_ctx):
     say("[gpu] mod switch")
     p = comp_params

@@ -112,6 +112,35 @@ def test_k2_external_product_matches_oracle(O, k2p, ctx2, which):
     assert np.array_equal(ctx2.ext_prod(gsw, l, b, ct), exp)
 
 
+@pytest.mark.parametrize("sel", ["one", "random"])
+def test_k2_ext_prod_mux_matches_oracle(O, k2p, ctx2, sel):
+    """PirServer::ext_prod_mux (src/server.cpp:652-722) with two 60-bit limbs,
+    through the C ABI and the host PirServer, bit-exact against the oracle."""
+    import onionpirv2_amd.host as H
+    say(f"[gpu] K=2 ext_prod_mux ({sel} selector)")
+    p = k2p
+    l, b = p.l_ep, p.base_log2
+    rng = np.random.default_rng(23)
+    if sel == "random":
+        gsw = rand_rns(rng, p, (2 * l, 2)).reshape(2 * l, -1)
+    else:
+        sk, _ = O.gen_sk_and_encrypt(p, np.zeros(p.N, np.uint64), 5)
+        one = np.zeros(p.N, np.uint64); one[0] = 1
+        gsw = O.plain_to_gsw(p, one, sk, l, b, 9)
+    x = rand_rns(rng, p, (2,))
+    y = rand_rns(rng, p, (2,))
+    q0, q1 = p.rns_mods
+    x[0, 0, :3] = [0, q0 - 1, 7]
+    y[0, 0, :3] = [q0 - 1, 0, 7]
+    x[1, 1, :2] = [q1 - 1, 0]
+    y[1, 1, :2] = [q1 - 1, q1 - 1]
+    x, y = x.reshape(2, -1), y.reshape(2, -1)
+    exp = O.PirServer(p.with_layout(43, 5)).ext_prod_mux(x.copy(), y.copy(), gsw)
+    assert np.array_equal(ctx2.ext_prod_mux(gsw, l, b, x, y), exp)
+    hp = H.PirParams("CONFIG_N4096_K2_MP").with_layout(43, 5)
+    assert np.array_equal(H.PirServer(hp).ext_prod_mux(x, y, gsw), exp)
+
+
 def test_k2_mod_switch_matches_oracle(O, k2p, ctx2):
     say("[gpu] K=2 mod switch")
     p = k2p

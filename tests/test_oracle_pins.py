@@ -299,6 +299,27 @@ def test_external_product_selects(O, comp_params):
         assert np.array_equal(dec, expect) and budget > 0
 
 
+def test_ext_prod_mux_selects(O, comp_params):
+    """tests/test_ext_prod_mux.cpp: mux(RGSW(1), a, b) decrypts to b (and,
+    completing the identity x + b*(y - x), RGSW(0) keeps a)."""
+    p = comp_params.with_layout(43, 5)
+    t = p.plain_mod
+    c = O.PirClient(p, 1, 41)
+    sk = c.secret_key()
+    rng = np.random.default_rng(42)
+    a = rng.integers(0, t, size=N, dtype=np.uint64)
+    b = rng.integers(0, t, size=N, dtype=np.uint64)
+    a[:3] = [5, 7, t - 1]
+    b[:3] = [3, 9, 1]
+    ca, cb = c.encrypt_bfv(a, 43), c.encrypt_bfv(b, 44)
+    srv = O.PirServer(p)
+    for bit, want in [(1, b), (0, a)]:
+        g = O.plain_to_gsw(p, np.eye(1, N, 0, np.uint64)[0] * np.uint64(bit), sk, p.l_ep, p.base_log2, 45 + bit)
+        out = srv.ext_prod_mux(ca, cb, g)
+        dec, budget = O.decrypt_and_budget(p, sk, out, False)
+        assert np.array_equal(dec, want) and budget > 0
+
+
 def test_bv_keyswitch_is_automorphism(O, comp_params):
     p = comp_params
     t = p.plain_mod

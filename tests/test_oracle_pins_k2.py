@@ -117,6 +117,26 @@ def test_k2_bv_keyswitch_is_automorphism(O, p2):
         assert np.array_equal(O.decrypt_rns(p2, sk, out, False), O.automorphism_coeff(msg, k, t))
 
 
+def test_k2_ext_prod_mux_selects(O, p2):
+    """tests/test_ext_prod_mux.cpp with two RNS limbs: RGSW(1) selects y,
+    RGSW(0) keeps x."""
+    p = p2.with_layout(43, 5)
+    N, t = p.N, p.plain_mod
+    c = O.PirClient(p, 1, 51)
+    sk = c.secret_key()
+    rng = np.random.default_rng(52)
+    a = rng.integers(0, t, size=N, dtype=np.uint64)
+    b = rng.integers(0, t, size=N, dtype=np.uint64)
+    a[:3] = [5, 7, t - 1]
+    b[:3] = [3, 9, 1]
+    ca, cb = c.encrypt_bfv(a, 53), c.encrypt_bfv(b, 54)
+    srv = O.PirServer(p)
+    for bit, want in [(1, b), (0, a)]:
+        g = O.plain_to_gsw(p, np.eye(1, N, 0, np.uint64)[0] * np.uint64(bit), sk, p.l_ep, p.base_log2, 55 + bit)
+        out = srv.ext_prod_mux(ca, cb, g)
+        assert np.array_equal(O.decrypt_rns(p, sk, out, False), want)
+
+
 def test_k2_mod_switch_round_trip(O, p2):
     p = p2.with_layout(43, 5)
     c = O.PirClient(p, 1, 31)
