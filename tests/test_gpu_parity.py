@@ -262,6 +262,32 @@ def test_every_scan_shape_matches_oracle(O, comp_params, gpu_ctx, layout):
         assert np.array_equal(b, want), f"scan shape {v}"
 
 
+def test_every_scan_shape_two_row_tiles_64_queries(O, comp_params, gpu_ctx):
+    """256 x 32 layout: four full k blocks per level (N0 = 256), two row
+    tiles, and a 64-query batch (128 query columns, eight full column tiles). Every shape gives the oracle's evaluate_first_dim output
+    and the same batch answers; one answer is checked against the oracle."""
+    say("[gpu] scan shapes, 256 x 32 layout, 64 queries")
+    s = Setup(O, comp_params, (8192, 9, True), seed=4242, gpu_ctx=gpu_ctx, cid=42)
+    p = s.p
+    assert (p.fst_dim_sz, p.other_dim_sz) == (256, 32)
+    query = s.client.fast_generate_query(p, s.indices[-1])
+    fq = s.server.fast_expand_qry(s.cid, query)[: p.fst_dim_sz]
+    mid_o = s.server.evaluate_first_dim(fq)
+    mids = _each_scan_shape(gpu_ctx, lambda: s.db.first_dim(fq))
+    assert len(mids) == gpu_ctx.scan_variant_count()
+    for v, m in mids.items():
+        assert np.array_equal(m, mid_o), f"scan shape {v}"
+    idxs = [(29 * j + 5) % p.target_num_pt for j in range(64)]
+    qs = np.stack([s.client.fast_generate_query(p, i) for i in idxs])
+    ids = [s.cid] * len(idxs)
+    want = s.db.answer_batch(ids, qs)
+    assert np.array_equal(want[63], s.server.make_query(s.cid, qs[63]))
+    got = _each_scan_shape(gpu_ctx, lambda: s.db.answer_batch(ids, qs))
+    assert len(got) >= 8  # the shapes that stage 2 columns take one query only
+    for v, b in got.items():
+        assert np.array_equal(b, want), f"scan shape {v}"
+
+
 def test_autotuned_scan_matches_default(O, comp_params, gpu_ctx):
     """Autotune mode times the applicable shapes once per batch-size class,
     keeps one, and answers exactly as the default shape does."""
