@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=20251142)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (continuous batching) leg")
     ap.add_argument("--serve-lambdas", default="10,50,200,1000,2000", help="Poisson arrival rates (1/s)")
+    ap.add_argument("--serve-k2-cus", type=int, default=224,
+                    help="serving: compute units the n=4096 database's context may use during the serving leg "
+                         "(0 = all); the rest stay free for the n=2048 databases' short kernels")
     ap.add_argument("--serve-k2-entries", type=int, default=2,
                     help="pool entries for the n=4096 database in the serving mix (of 128 + this many)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per rate")
@@ -191,7 +194,17 @@ def serving(args, H, params, server, log, k2=None):
         qs = list(ex.map(mq, entries))
     ids = [5000 + j for _, j in entries]
     names = ["256x1024 n=2048 (4 GiB NTT form)", "256x256 n=2048 (1 GiB NTT form)"]
+    k2_masked = None
     if k2 is not None and args.serve_k2_entries > 0:
+        if args.serve_k2_cus > 0:
+            import ctypes
+            from onionpirv2_amd import gpu as G
+            words = [0] * ((args.serve_k2_cus + 31) // 32)
+            for c in range(args.serve_k2_cus):
+                words[c // 32] |= 1 << (c % 32)
+            arr = (ctypes.c_uint32 * len(words))(*words)
+            k2_masked = k2["server"].ctx_handle()
+            assert G.ctx_call_raw("opir_ctx_set_cu_mask", k2_masked, arr, len(words)) == 0
         servers.append(k2["server"])
         names.append("512x4096 n=4096 K=2 (128 GiB NTT form)")
         for j in range(min(args.serve_k2_entries, len(k2["queries"]))):
@@ -227,7 +240,9 @@ def serving(args, H, params, server, log, k2=None):
         limits = list(loop.batch_limits)
     finally:
         loop.stop()
-    return {"databases": names, "clients_n2048": nc,
+        if k2_masked is not None:  # the n=4096 leg runs unrestricted
+            assert G.ctx_call_raw("opir_ctx_set_cu_mask", k2_masked, None, 0) == 0
+    return {"databases": names, "clients_n2048": nc, "n4096_cus": args.serve_k2_cus if k2_masked else None,
             "clients_n4096": min(args.serve_k2_entries, len(k2["queries"])) if k2 is not None else 0,
             "hbm_budget_gib": args.serve_budget_gib, "batch_limits": limits, "rates": rows}
 

@@ -72,6 +72,15 @@ const char *opir_last_error(const opir_ctx *ctx);
 /* Use an existing HIP stream (hipStream_t) for all work of this context;
  * NULL reverts to the context's own stream. */
 int opir_ctx_set_stream(opir_ctx *ctx, void *hip_stream);
+/* Restrict the context's own streams to a set of compute units (bit i of
+ * mask word i/32 = CU i, as hipExtStreamCreateWithCUMask takes it; words = 0
+ * restores unrestricted streams). For a server that shares the device
+ * between databases: a context whose answers stream a very large database
+ * can leave some CUs to the others, so their short kernels need not wait for
+ * its long-running workgroups. No counterpart in the reference (one CPU
+ * server per process). Masked streams follow the default stream semantics
+ * of hipExtStreamCreateWithCUMask. */
+int opir_ctx_set_cu_mask(opir_ctx *ctx, const uint32_t *mask, size_t words);
 /* Blocks until all work queued by the context has finished. */
 int opir_ctx_synchronize(opir_ctx *ctx);
 
@@ -80,6 +89,8 @@ int opir_ctx_synchronize(opir_ctx *ctx);
  * (src/pir.cpp:100-134): N0 = fst_dim_sz, Nrest = other_dim_sz. */
 int opir_db_create(opir_ctx *ctx, size_t fst_dim_sz, size_t other_dim_sz, size_t num_dims,
                    size_t expansion_height, opir_db **out);
+/* A database uses its context's stream and tables: destroy it before the
+ * context. */
 void opir_db_destroy(opir_db *db);
 /* Replaces PirServer::load_data's NTT + transpose (src/server.cpp:118-171):
  * `count` coefficient-form plaintexts (N values < t each, pt-major),
@@ -124,8 +135,11 @@ int opir_answer_batch_device(opir_db *db, size_t B, const uint64_t *client_ids, 
  * one query per make_query call; a serving loop uses these to admit batches
  * by HBM budget. workspace_bytes: device bytes of the answer workspace for
  * batches of up to B queries against db. reserve: allocate that workspace
- * now (it is kept and only grows). key_bytes: device bytes of one client's
- * keys at expansion height h. */
+ * now (it is kept and only grows), with the pinned host staging of
+ * opir_answer_batch for B queries, and in autotune mode settle the scan tile
+ * shape of every batch-size class up to B; answers of up to B queries then
+ * neither allocate nor stop to time shapes. key_bytes: device bytes of one
+ * client's keys at expansion height h. */
 int opir_db_workspace_bytes(const opir_db *db, size_t B, size_t *out);
 int opir_db_reserve(opir_db *db, size_t B);
 int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out);

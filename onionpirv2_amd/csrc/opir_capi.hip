@@ -163,6 +163,14 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   });
 }
 
+int tune_scan(opir_db *db, size_t B) {
+  opir_ctx *ctx = db->ctx;
+  const int sv = pick_scan_variant(db, B, 4, [&](int v) {
+    launch_scan(db->d_db, db->qfrag, db->mid_a, db->Nrest * kCtWords, (int)B, db->geom, ctx->S, ctx->stream, v);
+  });
+  return sv < 0 ? set_err(ctx, OPIR_EHIP, "scan autotune failed") : OPIR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -330,6 +338,36 @@ int opir_ctx_set_stream(opir_ctx *ctx, void *s) {
   if (!ctx) return OPIR_EINVAL;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  return OPIR_OK;
+}
+
+int opir_ctx_set_cu_mask(opir_ctx *ctx, const uint32_t *mask, size_t words) {
+  if (!ctx || (words && !mask) || words > 64) return OPIR_EINVAL;
+  if (words) {
+    bool any = false;
+    for (size_t i = 0; i < words; i++) any = any || mask[i];
+    if (!any) return set_err(ctx, OPIR_EINVAL, "CU mask selects no compute unit");
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->aux_stream));
+  // captured answer graphs hold no stream: they stay valid on the new ones
+  hipStream_t own = nullptr, aux = nullptr;
+  auto make = [&](hipStream_t *s) {
+    return words ? hipExtStreamCreateWithCUMask(s, (uint32_t)words, mask)
+                 : hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+  };
+  if (make(&own) != hipSuccess) return set_err(ctx, OPIR_EHIP, "stream create");
+  if (make(&aux) != hipSuccess) {
+    (void)hipStreamDestroy(own);
+    return set_err(ctx, OPIR_EHIP, "stream create");
+  }
+  const bool own_current = ctx->stream == ctx->own_stream;
+  (void)hipStreamDestroy(ctx->own_stream);
+  (void)hipStreamDestroy(ctx->aux_stream);
+  ctx->own_stream = own;
+  ctx->aux_stream = aux;
+  if (own_current) ctx->stream = own;
   return OPIR_OK;
 }
 
@@ -615,7 +653,24 @@ int opir_db_workspace_bytes(const opir_db *db, size_t B, size_t *out) {
 int opir_db_reserve(opir_db *db, size_t B) {
   if (!db) return OPIR_EINVAL;
   if (!B) return OPIR_OK;
-  return db->ctx->k2 ? k2_ensure_ws(db, B) : ensure_ws(db, B);
+  opir_ctx *ctx = db->ctx;
+  int rc = ctx->k2 ? k2_ensure_ws(db, B) : ensure_ws(db, B);
+  if (rc) return rc;
+  // the host staging of opir_answer_batch and the key-pointer ring
+  const size_t vals = (ctx->k2 ? 4 : 2) * ctx->N;
+  if ((rc = ensure_pin(ctx, B * vals * sizeof(u64)))) return rc;
+  if ((rc = ensure_ring(ctx, B))) return rc;
+  if (ctx->scan_variant == -2) {
+    // settle the scan shape of every batch-size class up to B (the tuner
+    // times the shapes on the reserved, not yet meaningful, operands)
+    for (size_t b = 1;; b = std::min(2 * b, B)) {
+      rc = ctx->k2 ? k2_tune_scan(db, b) : tune_scan(db, b);
+      if (rc) return rc;
+      if (b == B) break;
+    }
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return OPIR_OK;
 }
 
 int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out) {

@@ -270,7 +270,8 @@ inline int ensure_pin(opir_ctx *ctx, size_t bytes) {
 
 // Upload per-query key pointers for a batch through the pinned ring:
 // table[0..B) = BV keys, table[B..2B) = RGSW(s) keys.
-inline int upload_key_ptrs(opir_ctx *ctx, const uint64_t *ids, size_t B, const void *const **table) {
+// Grow the pinned key-pointer ring to hold batches of B queries.
+inline int ensure_ring(opir_ctx *ctx, size_t B) {
   if (ctx->ring_cap < B) {
     for (int i = 0; i < kRing; i++) {
       if (ctx->ring_ev[i]) (void)hipEventSynchronize(ctx->ring_ev[i]);
@@ -288,6 +289,11 @@ inline int upload_key_ptrs(opir_ctx *ctx, const uint64_t *ids, size_t B, const v
     }
     ctx->ring_cap = cap;
   }
+  return OPIR_OK;
+}
+
+inline int upload_key_ptrs(opir_ctx *ctx, const uint64_t *ids, size_t B, const void *const **table) {
+  if (int rc = ensure_ring(ctx, B)) return rc;
   const int slot = ctx->ring_pos;
   ctx->ring_pos = (ctx->ring_pos + 1) % kRing;
   HIP_TRY(ctx, hipEventSynchronize(ctx->ring_ev[slot]));
@@ -344,6 +350,7 @@ void k2_ctx_free(opir_ctx *ctx);
 int k2_db_init(opir_db *db);
 size_t k2_ws_bytes(const opir_db *db, size_t B);
 int k2_ensure_ws(opir_db *db, size_t B);
+int k2_tune_scan(opir_db *db, size_t B);  // settle the autotuned scan shape for B queries
 void k2_db_free(opir_db *db);
 int k2_ingest(opir_db *db, const u64 *d_pts, u64 *d_tmp, size_t first_pt, size_t n);
 int k2_load_plain(opir_db *db, const uint64_t *coeffs, size_t first_pt, size_t count);

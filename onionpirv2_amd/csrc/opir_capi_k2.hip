@@ -282,6 +282,14 @@ int k2_set_keys(opir_ctx *ctx, const uint64_t *bv, size_t h, const uint64_t *gsw
   return OPIR_OK;
 }
 
+int k2_tune_scan(opir_db *db, size_t B) {
+  opir_ctx *ctx = db->ctx;
+  const int sv = pick_scan_variant(db, B, 8, [&](int v) {
+    launch_scan64(db->d_db, db->qfrag, db->m64_a, db->Nrest * kCt2, (int)B, db->geom, ctx->S64, ctx->stream, v);
+  });
+  return sv < 0 ? set_err(ctx, OPIR_EHIP, "scan autotune failed") : OPIR_OK;
+}
+
 int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u64 *d_r) {
   opir_ctx *ctx = db->ctx;
   int rc = check_answerable(db);

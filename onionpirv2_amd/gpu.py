@@ -4,6 +4,7 @@ This is the boundary the GPU parity tests and bench.py call. Loading fails
 loudly when libonionpir_gpu.so has not been built: there is no CPU fallback.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -26,7 +27,7 @@ EXPORTED = [
     "opir_client_set_keys", "opir_client_remove",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
-    "opir_ctx_set_profiling", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_ctx_set_graphs", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
+    "opir_ctx_set_profiling", "opir_ctx_set_cu_mask", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_ctx_set_graphs", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
     "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
     "opir_ext_prod", "opir_ext_prod_mux", "opir_mod_switch",
 ]
@@ -93,6 +94,7 @@ def load_library():
         "opir_ctx_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
         "opir_last_stage_ms": ([P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "opir_ctx_set_scan_variant": ([P, ctypes.c_int], ctypes.c_int),
+        "opir_ctx_set_cu_mask": ([P, P, sz], ctypes.c_int),
         "opir_ctx_set_graphs": ([P, ctypes.c_int], ctypes.c_int),
         "opir_scan_variant_count": ([P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "opir_db_tuned_scan_variant": ([P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -141,6 +143,7 @@ class GpuContext:
         if rc != OPIR_OK:
             self._raise(rc, self.lib.opir_last_error(None))
         self.h = h
+        self._dbs = weakref.WeakSet()  # databases to destroy before the context
 
     def _raise(self, rc, msg):
         msg = msg.decode() if isinstance(msg, bytes) else str(msg)
@@ -156,6 +159,8 @@ class GpuContext:
 
     def close(self):
         if getattr(self, "h", None):
+            for db in list(getattr(self, "_dbs", ())):
+                db.close()
             self.lib.opir_ctx_destroy(self.h)
             self.h = None
 
@@ -179,6 +184,18 @@ class GpuContext:
         size), SCAN_AUTOTUNE (-2, timed on first use per batch-size class), or
         a forced shape index."""
         self.check(self.lib.opir_ctx_set_scan_variant(self.h, int(variant)))
+
+    def set_cu_mask(self, cus=None):
+        """Run this context's work on the compute units listed in `cus`
+        (indices); None restores unrestricted streams."""
+        if cus is None:
+            self.check(self.lib.opir_ctx_set_cu_mask(self.h, None, 0))
+            return
+        cus = list(cus)
+        words = np.zeros(max(cus, default=0) // 32 + 1, dtype=np.uint32)
+        for c in cus:
+            words[c // 32] |= np.uint32(1 << (c % 32))
+        self.check(self.lib.opir_ctx_set_cu_mask(self.h, words.ctypes.data_as(ctypes.c_void_p), len(words)))
 
     def set_graphs(self, on=True):
         """Capture each answer's launch sequence as a HIP graph on first use
@@ -261,6 +278,7 @@ class GpuDatabase:
         ctx.check(self.lib.opir_db_create(ctx.h, fst_dim_sz, other_dim_sz, num_dims, expansion_height,
                                           ctypes.byref(h)))
         self.h = h
+        ctx._dbs.add(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -304,6 +322,11 @@ class GpuDatabase:
         self.ctx.check(self.lib.opir_answer_batch_device(self.h, len(ids), _ptr(ids),
                                                          ctypes.c_void_p(d_queries_ptr),
                                                          ctypes.c_void_p(d_resps_ptr)))
+
+    def reserve(self, batch):
+        """Prepare answers of up to `batch` queries: workspace, host staging
+        and, in autotune mode, the scan shape of every batch-size class."""
+        self.ctx.check(self.lib.opir_db_reserve(self.h, batch))
 
     def tuned_scan_variant(self, batch):
         """Scan shape the autotuner picked for this batch size (-1: not tuned)."""
