@@ -265,6 +265,9 @@ constexpr int kLevelsPerWG = 16;
 // Write one wave's staged results: for each (row rr < 16, column nn < ncols)
 // of its tile, the LV consecutive levels st[0..LV)[rr][nn] go to
 // mid[q][row][poly][limb][lvo ..], c = col0 + nn = 2q + poly, row = 16 rt + rr.
+// The stores are nontemporal: the next kernel reads the results, and
+// keeping them out of the L2 leaves it to the database stream (the plain
+// scan ran 0.707 against 0.734 ms with cached stores).
 // With W = 16 / sizeof(Res) dividing LV, as 16-byte stores, consecutive
 // lanes taking consecutive pieces of a run (its index, a multiple of W,
 // makes each piece 16-byte aligned); rows past Nrest are not stored.
@@ -284,10 +287,11 @@ __device__ __forceinline__ void flush_stage(const Res (*st)[16][SW], int lane, i
       Res v[W];
 #pragma unroll
       for (int w = 0; w < W; w++) v[w] = st[ch * W + w][rr][nn];
-      uint4 x;
+      v4i x;
       __builtin_memcpy(&x, v, 16);
-      *reinterpret_cast<uint4 *>(mid + qi * mid_stride + row * ct_vals + (size_t)(pp * 2 + limb) * nlv + lvo +
-                                 ch * W) = x;
+      // nontemporal: the results are read by the next kernel, not by this one
+      __builtin_nontemporal_store(x, reinterpret_cast<v4i *>(mid + qi * mid_stride + row * ct_vals +
+                                                             (size_t)(pp * 2 + limb) * nlv + lvo + ch * W));
     }
   } else {
     for (int e = lane; e < 16 * ncols; e += 64) {
@@ -297,7 +301,7 @@ __device__ __forceinline__ void flush_stage(const Res (*st)[16][SW], int lane, i
       const int c = col0 + nn, qi = c >> 1, pp = c & 1;
       Res *dst = mid + qi * mid_stride + row * ct_vals + (size_t)(pp * 2 + limb) * nlv + lvo;
 #pragma unroll
-      for (int li = 0; li < LV; li++) dst[li] = st[li][rr][nn];
+      for (int li = 0; li < LV; li++) __builtin_nontemporal_store(st[li][rr][nn], dst + li);
     }
   }
 }
