@@ -142,11 +142,12 @@ __global__ __launch_bounds__(256) void k_qfrag(const V *__restrict__ qntt, int8_
   const int kb = kg >> 2, grp = kg & 3;
   const int lv0 = blockIdx.y * kQfLv, limb = blockIdx.z;
   const int ncols = qg.ncm;  // columns stored per tile; c_end = 2B columns in all
-  for (int e = threadIdx.x; e < 16 * 16 * kQfLv; e += 256) {
+  // only the stored columns are staged (the writes below read no others)
+  for (int e = threadIdx.x; e < ncols * 16 * kQfLv; e += 256) {
     const int lv = e % kQfLv, row = e / kQfLv, n = row >> 4, j = row & 15;
     const int c = ct * 16 + n, k = kg * 16 + j;
     V x = 0;
-    if (n < ncols && c < c_end && k < qg.N0)
+    if (c < c_end && k < qg.N0)
       x = qntt[(size_t)(c >> 1) * q_stride + (((size_t)k * 2 + (c & 1)) * 2 + limb) * qg.nlv + lv0 + lv];
     tile[n * kQfNs + j * (kQfLv + 1) + lv] = x;
   }
