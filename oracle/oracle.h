@@ -185,6 +185,7 @@ struct BvGaloisKeys {
 };
 void signed_gadget_decompose(u64 val, size_t base_log2, u64 q, u64 *out, size_t nd);          // bv_keyswitch.cpp:20-40
 void signed_gadget_decompose_mp(u128 val, u128 Q, size_t base_log2, int64_t *out, size_t nd); // bv_keyswitch.cpp:42-64
+u64 drop_q1(u64 r0, u64 r1, u64 q0, u64 q1, u64 q0_inv_mod_q1);                     // server.cpp:1053-1061
 BvKey gen_bv_ks_key(const PirParams &p, const RlweSk &sk, uint32_t galois_k, std::mt19937_64 &rng); // 222-259
 BvGaloisKeys gen_bv_galois_keys(const PirParams &p, const RlweSk &sk, u64 seed);                     // 261-276
 void bv_apply_galois_inplace(RlweCt &ct, uint32_t galois_k, const BvKey &key, const PirParams &p);  // 296-500

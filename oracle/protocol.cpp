@@ -413,6 +413,16 @@ RlweCt PirServer::evaluate_other_dim(std::vector<RlweCt> &mid, std::vector<GSWCt
   return mid[0];
 }
 
+u64 drop_q1(u64 r0, u64 r1, u64 q0, u64 q1, u64 q0_inv_mod_q1) {
+  // server.cpp:1053-1061: Garner CRT of (r0 mod q0, r1 mod q1), then
+  // round-half-up division by q1, wrapped into [0, q0)
+  const u64 diff = (r1 + q1 - r0 % q1) % q1;
+  const u64 s = mulmod(diff, q0_inv_mod_q1, q1);
+  const u128 x = (u128)q0 * s + r0;
+  const u64 o = (u64)((x + ((u128)q1 >> 1)) / q1);
+  return o >= q0 ? o - q0 : o;
+}
+
 void PirServer::mod_switch_inplace(RlweCt &ct, u64 sq) const {
   // server.cpp:1025-1075
   const size_t N = params.N(), K = params.K();
@@ -425,16 +435,9 @@ void PirServer::mod_switch_inplace(RlweCt &ct, u64 sq) const {
     return;
   }
   const u64 q0 = qs[0], q1 = qs[1];
-  auto drop = [&](u64 r0, u64 r1) -> u64 {
-    const u64 diff = (r1 + q1 - r0 % q1) % q1;
-    const u64 s = mulmod(diff, params.q0_inv_mod_q1, q1);
-    const u128 x = (u128)q0 * s + r0;
-    const u64 o = (u64)((x + ((u128)q1 >> 1)) / q1);
-    return o >= q0 ? o - q0 : o;
-  };
   for (size_t i = 0; i < N; i++) {
-    ct.c0[i] = rescale(drop(ct.c0[i], ct.c0[N + i]), q0, sq);
-    ct.c1[i] = rescale(drop(ct.c1[i], ct.c1[N + i]), q0, sq);
+    ct.c0[i] = rescale(drop_q1(ct.c0[i], ct.c0[N + i], q0, q1, params.q0_inv_mod_q1), q0, sq);
+    ct.c1[i] = rescale(drop_q1(ct.c1[i], ct.c1[N + i], q0, q1, params.q0_inv_mod_q1), q0, sq);
   }
   ct.c0.resize(N);
   ct.c1.resize(N);

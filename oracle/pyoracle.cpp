@@ -206,6 +206,13 @@ PYBIND11_MODULE(onionpir_oracle, m) {
     signed_gadget_decompose(v, b, q, out.data(), nd);
     return out;
   });
+  // two-prime gadget digits: val and Q as (lo, hi) u64 halves of u128 values
+  m.def("signed_gadget_decompose_mp", [](u64 v_lo, u64 v_hi, u64 Q_lo, u64 Q_hi, size_t b, size_t nd) {
+    std::vector<int64_t> out(nd);
+    signed_gadget_decompose_mp(((u128)v_hi << 64) | v_lo, ((u128)Q_hi << 64) | Q_lo, b, out.data(), nd);
+    return out;
+  });
+  m.def("drop_q1", &drop_q1);
   m.def("level_mat_mat_32", [](U32Arr A, U32Arr B, size_t m_, size_t n, size_t levels, u64 q) {
     if ((size_t)A.size() != m_ * n * levels || (size_t)B.size() != n * 2 * levels)
       throw std::invalid_argument("shape");

@@ -178,3 +178,96 @@ def test_k2_end_to_end_decrypts(O, p2, layout):
         assert len(wire) == (2 * p.N * 50 + 7) // 8
         pt, budget = c.decrypt_mod_q(c.load_resp_from_stream(wire))
         assert np.array_equal(pt, srv.direct_get_original_plaintext(i)) and budget >= 1
+
+
+def test_k2_constants_exact(p2):
+    """The CONFIG_N4096_K2_MP values re-derived from the reference's prime and
+    plaintext-modulus rules (database_constants.h:111-124, pir.cpp:14-98,
+    utils.cpp:136-287): the two largest 60-bit primes 1 mod 2N, t the smallest
+    prime above 2^39, small_q the largest 50-bit prime 1 mod 2N."""
+    q0, q1 = p2.rns_mods
+    assert (q0, q1) == (1152921504606830593, 1152921504606748673)
+    assert p2.plain_mod == 549755813911
+    assert p2.small_q == 1125899906826241
+    assert p2.bv_base_log2 == 16 and p2.l_ks == 8
+
+
+def _centred(x, Q):
+    return x if x <= Q // 2 else x - Q
+
+
+@pytest.mark.parametrize("b", [16, 8])
+def test_k2_signed_mp_digits_balanced_and_exact(O, p2, b):
+    """bv_keyswitch.cpp:42-64: the balanced digits of the centred CRT value lie
+    in (-B/2, B/2] -- a tie goes to +B/2 -- and recompose it exactly over the
+    integers. With enough digits this representation is unique, so range +
+    recomposition pin every digit. Ties are forced at every digit position and
+    at the centring boundary +-Q/2."""
+    q0, q1 = p2.rns_mods
+    Q = q0 * q1
+    B, nd = 1 << b, -(-Q.bit_length() // b) + 1
+    rng = np.random.default_rng(77)
+    vals = [0, 1, Q - 1, Q // 2, Q // 2 + 1, Q // 2 - 1, B // 2, Q - B // 2, B // 2 - 1, Q - B // 2 - 1]
+    for pos in range(nd - 1):
+        for k in (0, 1, 5):
+            vals.append((k * B + B // 2) * B ** pos)                    # digit pos is a tie
+            vals.append(Q - (k * B + B // 2) * B ** pos)                # negative tie
+            vals.append(((k * B + B // 2) * B + B // 2) * B ** pos % Q)  # two ties in a row
+    vals += [int(x) % Q for x in rng.integers(0, 1 << 62, size=40, dtype=np.uint64) * (1 << 58)]
+    vals += [int(rng.integers(0, 1 << 60)) * (1 << 60) % Q + int(rng.integers(0, 1 << 60)) for _ in range(40)]
+    ties = 0
+    for x in vals:
+        x %= Q
+        d = O.signed_gadget_decompose_mp(x & ((1 << 64) - 1), x >> 64, Q & ((1 << 64) - 1), Q >> 64, b, nd)
+        assert all(-B // 2 < di <= B // 2 for di in d), (x, d)
+        assert sum(di * B ** i for i, di in enumerate(d)) == _centred(x, Q), x
+        ties += sum(di == B // 2 for di in d)
+    assert ties >= 3 * (nd - 1)
+
+
+def test_k2_drop_q1_round_half_up_exact(O, p2):
+    """server.cpp:1053-1061: x = CRT(r0, r1) in [0, Q), out = round(x / q1)
+    (nearest, half up) reduced into [0, q0). Checked with exact rationals at
+    the rounding boundary (x mod q1 around q1/2) and at the top of [0, Q),
+    where round(x / q1) = q0 wraps to 0."""
+    from fractions import Fraction
+    import math
+    q0, q1 = p2.rns_mods
+    Q = q0 * q1
+    inv = pow(q0, -1, q1)
+    assert p2.q0_inv_mod_q1 == inv
+    xs = [0, 1, q1 - 1, q1, Q - 1, Q - q1 // 2, Q - q1 // 2 - 1, Q - q1 // 2 - 2, Q - q1]
+    rng = np.random.default_rng(5)
+    for k in [0, 1, 12345, q0 - 2, q0 - 1] + [int(v) for v in rng.integers(0, q0, size=10, dtype=np.uint64)]:
+        for r in (q1 // 2 - 1, q1 // 2, q1 // 2 + 1, 0, q1 - 1):
+            xs.append(k * q1 + r)
+    wraps = 0
+    for x in xs:
+        assert 0 <= x < Q
+        want = math.floor(Fraction(x, q1) + Fraction(1, 2)) % q0
+        wraps += math.floor(Fraction(x, q1) + Fraction(1, 2)) == q0
+        assert O.drop_q1(x % q0, x % q1, q0, q1, inv) == want, x
+    assert wraps >= 2
+
+
+def test_k2_mod_switch_is_drop_then_rescale(O, p2):
+    """mod_switch_inplace's two-prime branch (server.cpp:1044-1075) on crafted
+    coefficients: each output is rescale(drop_q1(...)), both pinned above and
+    in the single-modulus pins."""
+    p = p2.with_layout(1, 0)
+    q0, q1 = p2.rns_mods
+    N, sq = p.N, p.small_q
+    rng = np.random.default_rng(6)
+    ct = np.empty((2, 2 * N), np.uint64)
+    ct[:, :N] = rng.integers(0, q0, size=(2, N), dtype=np.uint64)
+    ct[:, N:] = rng.integers(0, q1, size=(2, N), dtype=np.uint64)
+    Q = q0 * q1
+    special = [Q - 1, Q // 2, Q // 2 + 1, 5 * q1 + q1 // 2, 5 * q1 + q1 // 2 + 1, 0]
+    for i, x in enumerate(special):
+        ct[0, i], ct[0, N + i] = x % q0, x % q1
+    out = O.PirServer(p).mod_switch(ct)
+    inv = pow(q0, -1, q1)
+    for poly in range(2):
+        for i in list(range(len(special))) + [100, N - 1]:
+            d = O.drop_q1(int(ct[poly, i]), int(ct[poly, N + i]), q0, q1, inv)
+            assert int(out[poly, i]) == O.rescale(d, q0, sq)
