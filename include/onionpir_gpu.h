@@ -94,12 +94,14 @@ int opir_db_create(opir_ctx *ctx, size_t fst_dim_sz, size_t other_dim_sz, size_t
 void opir_db_destroy(opir_db *db);
 /* Replaces PirServer::load_data's NTT + transpose (src/server.cpp:118-171):
  * `count` coefficient-form plaintexts (N values < t each, pt-major),
- * logical indices [first_pt, first_pt + count). Validation of the values
- * (< t) is the caller's; this call rejects ranges outside the layout. */
+ * logical indices [first_pt, first_pt + count). A coefficient >= t or a
+ * range outside the layout -> OPIR_EINVAL (the reference throws
+ * std::invalid_argument, src/server.cpp:129-137); nothing is loaded then. */
 int opir_db_load_plain(opir_db *db, const uint16_t *coeffs, size_t first_pt, size_t count);
 /* Loads an already preprocessed database in the reference's composite
  * layout: lo[coeff * num_pt + pt] = NTT_Q(pt)[coeff] mod q1, hi the same
- * mod q2 (src/server.cpp:162-170). */
+ * mod q2 (src/server.cpp:162-170). A residue >= its limb prime ->
+ * OPIR_EINVAL. */
 int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi);
 /* Same as opir_db_load_plain with 64-bit coefficients (< t), for plaintext
  * moduli beyond 16 bits (CONFIG_N4096_K2_MP: t ~ 2^39). */
@@ -212,6 +214,12 @@ int opir_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t base_log2
  * are not part of the result). */
 int opir_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t base_log2, const uint64_t *x,
                       const uint64_t *y, uint64_t *out);
+/* bvks::bv_apply_galois_inplace (src/bv_keyswitch.cpp:296-367 for K = 1,
+ * 373-481 for K = 2): ct [2][K*N] coefficient form; key: the l_ks rows of one
+ * BvKeySwitchKey for galois element k, [l_ks][a, b][K*N] NTT form
+ * (src/includes/bv_keyswitch.h:48-60); out [2][K*N] coefficient form, the
+ * keyswitched sigma_k(ct) under the original secret. k must be odd and < 2N. */
+int opir_bv_ks(opir_ctx *ctx, const uint64_t *ct, uint32_t galois_k, const uint64_t *key, uint64_t *out);
 /* PirServer::mod_switch_inplace (src/server.cpp:1025-1043): ct [2][N]
  * coefficient form mod Q -> out [2][N] mod small_q. */
 int opir_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out);

@@ -26,8 +26,8 @@ HIP_FLAGS = [
 ]
 GPU_SRC = ["csrc/opir_kernels.hip", "csrc/opir_tree.hip", "csrc/opir_scan.hip", "csrc/opir_k2.hip",
            "csrc/opir_capi.hip", "csrc/opir_capi_k2.hip"]
-HOST_SRC = ["host/params.cpp", "host/ring.cpp", "host/client.cpp", "host/server.cpp", "host/serving.cpp",
-            "host/bindings.cpp"]
+HOST_SRC = ["host/params.cpp", "host/ring.cpp", "host/codec.cpp", "host/client.cpp", "host/server.cpp",
+            "host/serving.cpp", "host/bindings.cpp"]
 
 
 def _run(cmd):
@@ -36,9 +36,18 @@ def _run(cmd):
 
 
 def build_gpu():
+    """Each translation unit compiles on its own (in parallel), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, os.path.basename(src) + ".o") for src in GPU_SRC]
+    jobs = max(1, min(len(GPU_SRC), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: _run([HIPCC] + cflags + ["-c", so[0], "-o", so[1]]), zip(GPU_SRC, objs)))
     tmp = GPU_LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + GPU_SRC + ["-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp])
     os.replace(tmp, GPU_LIB)
 
 

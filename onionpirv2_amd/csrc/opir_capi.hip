@@ -302,7 +302,12 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   ctx->S.q1inv_s = shoup_h((u32)q1inv, (u32)q2);
   ctx->S.q2_mod_q1 = (u32)(q2 % q1);
   ctx->S.small_q = sch->small_q;
-  ctx->S.q_recip = (u64)(((u128)1 << 120) / Q);
+  {
+    int bits = 0;
+    while (bits < 64 && (Q >> bits)) bits++;
+    ctx->S.q_recip = (u64)(((u128)1 << (63 + bits)) / Q);
+    ctx->S.q_shift = bits - 1;
+  }
   ctx->G = GadgetCfg{(int)sch->l_ep, (int)sch->base_log2, (int)sch->l_key, (int)sch->base_log2_key, (int)sch->l_ks,
                      (int)sch->bv_base_log2};
   *out = ctx;
@@ -486,6 +491,13 @@ int opir_db_load_plain(opir_db *db, const uint16_t *coeffs, size_t first_pt, siz
   }
   if (first_pt > db->num_pt || count > db->num_pt - first_pt)
     return set_err(ctx, OPIR_EINVAL, "PirServer::load_data logical plaintext count exceeds its layout");
+  // server.cpp:129-137: every coefficient must be below t
+  {
+    const u64 t = ctx->sch.plain_mod;
+    const size_t nv = count * kN;
+    for (size_t i = 0; i < nv; i++)
+      if (coeffs[i] >= t) return set_err(ctx, OPIR_EINVAL, "plaintext coefficient not reduced mod t");
+  }
   const size_t chunk = 8192;
   uint16_t *d_pts = nullptr;
   u32 *d_tmp = nullptr;
@@ -553,6 +565,11 @@ int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi) {
   opir_ctx *ctx = db->ctx;
   if (ctx->k2) return set_err(ctx, OPIR_EINVAL, "the composite (lo, hi) database layout exists only for K=1");
   const size_t n = (size_t)kN * db->num_pt;
+  // residues must be reduced: the digit planes assume values below 2^30
+  const u32 q1 = ctx->S.limb[0].q, q2 = ctx->S.limb[1].q;
+  for (size_t i = 0; i < n; i++)
+    if (lo[i] >= q1 || hi[i] >= q2)
+      return set_err(ctx, OPIR_EINVAL, "database residue not reduced mod its limb prime");
   u32 *d_ref = nullptr;
   HIP_TRY(ctx, hipMalloc((void **)&d_ref, 2 * n * sizeof(u32)));
   int rc = OPIR_OK;
@@ -923,6 +940,49 @@ int opir_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, co
   launch_join_u64((u32 *)o32.p, (u64 *)o64.p, 2, ctx->S, ctx->stream);
   launch_ntt_u64((u64 *)o64.p, 2, true, ctx->S, ctx->stream);
   launch_addsub_mod((u64 *)xd.p, (u64 *)o64.p, (u64 *)o64.p, 2 * kN, kN, 1, Q, Q, false, ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
+int opir_bv_ks(opir_ctx *ctx, const uint64_t *ct, uint32_t galois_k, const uint64_t *key, uint64_t *out) {
+  if (!ctx || !ct || !key || !out) return OPIR_EINVAL;
+  const size_t N = ctx->N;
+  if (!(galois_k & 1) || galois_k >= 2 * N) return set_err(ctx, OPIR_EINVAL, "galois element must be odd and < 2N");
+  // k^-1 mod 2N: the odd residues mod 2N = 2^(log N + 1) form a group of order N
+  unsigned kinv = 1;
+  for (size_t e = 0; e + 1 < N; e++) kinv = (unsigned)((kinv * (u64)galois_k) & (2 * N - 1));
+  if (ctx->k2) return k2_bv_ks(ctx, ct, (int)kinv, key, out);
+  const size_t L = ctx->G.l_ks;
+  int rc = validate_mod(ctx, ct, 2 * kN, ctx->S.Q, "ciphertext");
+  if (rc) return rc;
+  if ((rc = validate_mod(ctx, key, L * 2 * kN, ctx->S.Q, "BV key"))) return rc;
+  const u64 q1 = ctx->S.limb[0].q, q2 = ctx->S.limb[1].q;
+  // key in: [l_ks][a, b][N] mod Q; device: [l_ks][b (-> c0), a (-> c1)][2 limbs][N]
+  std::vector<u32> hk(L * kCtWords);
+  for (size_t r = 0; r < L; r++)
+    for (int ab = 0; ab < 2; ab++) {
+      const uint64_t *src = key + (r * 2 + ab) * kN;
+      u32 *dst = hk.data() + r * kCtWords + (size_t)(ab == 0 ? 1 : 0) * 2 * kN;
+      for (size_t j = 0; j < (size_t)kN; j++) {
+        dst[j] = (u32)(src[j] % q1);
+        dst[kN + j] = (u32)(src[j] % q2);
+      }
+    }
+  DevBuf kd, kp, c64, c32, o32, o64;
+  DEV_ALLOC(ctx, kd, hk.size() * sizeof(u32));
+  DEV_ALLOC(ctx, kp, sizeof(void *));
+  DEV_ALLOC(ctx, c64, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, c32, kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, o32, kCtWords * sizeof(u32));
+  DEV_ALLOC(ctx, o64, 2 * kN * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(kd.p, hk.data(), hk.size() * sizeof(u32), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(kp.p, &kd.p, sizeof(void *), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(c64.p, ct, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  launch_split_u64((u64 *)c64.p, (u32 *)c32.p, 2, ctx->S, ctx->stream);
+  launch_bv_apply((const u32 *)c32.p, (u32 *)o32.p, (const u32 *const *)kp.p, (int)kinv, ctx->G, ctx->S, ctx->stream);
+  launch_join_u64((u32 *)o32.p, (u64 *)o64.p, 2, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));

@@ -365,5 +365,6 @@ int k2_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const ui
 int k2_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const uint64_t *x, const uint64_t *y,
                     uint64_t *out);
 int k2_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out);
+int k2_bv_ks(opir_ctx *ctx, const uint64_t *ct, int kinv, const uint64_t *key, uint64_t *out);
 
 }  // namespace capi

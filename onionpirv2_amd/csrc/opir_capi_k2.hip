@@ -484,6 +484,39 @@ int k2_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, cons
   return OPIR_OK;
 }
 
+int k2_bv_ks(opir_ctx *ctx, const uint64_t *ct, int kinv, const uint64_t *key, uint64_t *out) {
+  const size_t L = ctx->G64.l_ks;
+  int rc = validate_rns(ctx, ct, 2, "ciphertext");
+  if (rc) return rc;
+  const u64 q[2] = {ctx->S64.limb[0].q, ctx->S64.limb[1].q};
+  // key in: [l_ks][a, b][limb][N]; device: [l_ks][b (-> c0), a (-> c1)][limb][N]
+  std::vector<u64> hk(L * kCt2);
+  for (size_t r = 0; r < L; r++)
+    for (int ab = 0; ab < 2; ab++)
+      for (int m = 0; m < 2; m++) {
+        const uint64_t *src = key + ((r * 2 + ab) * 2 + m) * N2;
+        u64 *dst = hk.data() + r * kCt2 + ((size_t)(ab == 0 ? 1 : 0) * 2 + m) * N2;
+        for (size_t j = 0; j < N2; j++) {
+          if (src[j] >= q[m]) return set_err(ctx, OPIR_EINVAL, "BV key value not reduced mod its RNS prime");
+          dst[j] = src[j];
+        }
+      }
+  DevBuf kd, kp, c, o;
+  DEV_ALLOC(ctx, kd, hk.size() * sizeof(u64));
+  DEV_ALLOC(ctx, kp, sizeof(void *));
+  DEV_ALLOC(ctx, c, kCt2 * sizeof(u64));
+  DEV_ALLOC(ctx, o, kCt2 * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(kd.p, hk.data(), hk.size() * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(kp.p, &kd.p, sizeof(void *), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(c.p, ct, kCt2 * sizeof(u64), hipMemcpyHostToDevice));
+  k2::launch_k2_bv_apply((const u64 *)c.p, (u64 *)o.p, (const u64 *const *)kp.p, kinv, ctx->G64, ctx->S64,
+                         ctx->stream);
+  LAUNCH_CHECK(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, kCt2 * sizeof(u64), hipMemcpyDeviceToHost));
+  return OPIR_OK;
+}
+
 int k2_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out) {
   int rc = validate_rns(ctx, ct, 2, "ciphertext");
   if (rc) return rc;

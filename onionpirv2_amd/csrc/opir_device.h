@@ -54,7 +54,8 @@ struct SchemeConst {
   u32 q1inv, q1inv_s; // q1^-1 mod q2 and Shoup companion (mod q2)
   u32 q2_mod_q1;
   u64 small_q;
-  u64 q_recip;        // floor(2^120 / Q): Barrett reciprocal for the final rescale
+  u64 q_recip;        // floor(2^k / Q), k = 63 + bit length of Q: Barrett reciprocal for the final rescale
+  int q_shift;        // k - 64
 };
 
 __device__ __forceinline__ u32 mul_shoup(u32 a, u32 w, u32 ws, u32 q) {

@@ -62,6 +62,12 @@ constexpr size_t kSplitTargetBlocks64 = 512;
 void launch_k2_expand_level(const u64 *in, u64 *out, size_t stride, int nnodes, int B, int level,
                             const KeyPtrs64 &P, size_t key_level_values, const Gadget64 &G, const Scheme64 &S,
                             hipStream_t st, u64 *part);
+// Stand-alone BV keyswitch (bv_keyswitch.cpp:373-481) of one coefficient-form
+// ct: out = (sigma_k(c0) + sum d_i b_i, sum d_i a_i); key_ptr: device array
+// holding one pointer to l_ks device key rows [b (-> c0), a (-> c1)][limb][N];
+// kinv = k^-1 mod 2N.
+void launch_k2_bv_apply(const u64 *in, u64 *out, const u64 *const *key_ptr, int kinv, const Gadget64 &G,
+                        const Scheme64 &S, hipStream_t st);
 // Selector completion: nrows expanded rows (coefficient form) -> sel [nsel][2 l_ep][ct] (NTT form).
 void launch_k2_complete(const u64 *rows, u64 *sel, size_t row_stride, size_t sel_stride, int nrows, int B,
                         const KeyPtrs64 &P, const Gadget64 &G, const Scheme64 &S, hipStream_t st);

@@ -308,6 +308,7 @@ struct ExpandArgs64 {
   const u64 *const *bv;
   size_t key_level_values;
   int level, kinv, l_ks, b_ks;
+  int bv_only = 0;  // 1: out node 0 receives the keyswitched c' alone (no children)
 };
 
 // Expansion level (server.cpp:786-810, bv_keyswitch.cpp:373-481).
@@ -382,9 +383,10 @@ __global__ __launch_bounds__(1024) void k2_expand_level(ExpandArgs64 A, Scheme64
         const u64 x = c0[i0 < kN ? i0 : i0 - kN];
         w = addmod(w, (i0 < kN || !x) ? x : L.q - x, L.q);
       }
-      left[(size_t)(p * 2 + limb) * kN + map0(t, cc)] = addmod(cv[cc], w, L.q);
+      left[(size_t)(p * 2 + limb) * kN + map0(t, cc)] = A.bv_only ? w : addmod(cv[cc], w, L.q);
       dv[cc] = submod(cv[cc], w, L.q);
     }
+    if (A.bv_only) continue;  // uniform over the workgroup
     store0(lds[limb], dv, t);
     __syncthreads();
 #pragma unroll
@@ -769,6 +771,15 @@ static int split_groups64(int l) {
 // ---------------------------------------------------------------------------
 void launch_k2_ntt(u64 *data, size_t count, bool inverse, const Scheme64 &S, hipStream_t st) {
   if (count) k2_ntt<<<(unsigned)count, 1024, 0, st>>>(data, inverse ? 1 : 0, S);
+}
+
+void launch_k2_bv_apply(const u64 *in, u64 *out, const u64 *const *key_ptr, int kinv, const Gadget64 &G,
+                        const Scheme64 &S, hipStream_t st) {
+  ExpandArgs64 A;
+  A.in = in; A.out = out; A.stride = 0; A.bv = key_ptr; A.key_level_values = 0;
+  A.level = 0; A.l_ks = G.l_ks; A.b_ks = G.b_ks; A.kinv = kinv;
+  A.bv_only = 1;
+  k2_expand_level<<<dim3(1, 1), 1024, 0, st>>>(A, S);
 }
 
 void launch_k2_expand_level(const u64 *in, u64 *out, size_t stride, int nnodes, int B, int level,

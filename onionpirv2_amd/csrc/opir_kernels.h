@@ -28,6 +28,7 @@ struct ExpandArgs {
   int level;                     // k = 2^level, galois = N/k + 1
   int kinv;                      // galois^-1 mod 2N
   int l_ks, b_ks;
+  int bv_only = 0;               // 1: out node 0 receives the keyswitched c' alone (no children)
 };
 
 struct CompleteArgs {
@@ -99,6 +100,12 @@ constexpr size_t kMuxSplitMaxBlocks = 1024;
 void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int nnodes, int B,
                          int level, const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st,
                          u32 *part);
+// Stand-alone BV keyswitch (bv_keyswitch.cpp:296-367) of one coefficient-form
+// RNS ct: out = (sigma_k(c0) + sum d_i b_i, sum d_i a_i), d_i the balanced
+// digits of sigma_k(c1); key_ptr: device array holding one pointer to l_ks
+// device key rows [b (-> c0), a (-> c1)][2 limbs][N]; kinv = k^-1 mod 2N.
+void launch_bv_apply(const u32 *in, u32 *out, const u32 *const *key_ptr, int kinv, const GadgetCfg &G,
+                     const SchemeConst &S, hipStream_t st);
 // Forward NTT of ciphertexts (both polys): in coef RNS cts -> out NTT RNS cts.
 void launch_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int count, int B,
                    const SchemeConst &S, hipStream_t st);

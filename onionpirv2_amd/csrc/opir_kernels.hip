@@ -216,13 +216,14 @@ __device__ __forceinline__ u64 rescale_to_small(u64 x, const SchemeConst &S) {
   if (v >= (i64)(S.Q / 2)) v -= (i64)S.Q;
   const bool neg = v < 0;
   const u64 a = neg ? (u64)(-v) : (u64)v;
-  // floor((a * small_q + floor(Q/2)) / Q), numerator < 2^81. Barrett with
-  // m = floor(2^120 / Q): floor(num * m / 2^120), taking the high 64 bits of
-  // the low product only, is the quotient or up to two less; then correct.
+  // floor((a * small_q + floor(Q/2)) / Q), numerator < Q * 2^32. Barrett
+  // with m = floor(2^k / Q) in [2^63, 2^64), k = 63 + bit length of Q:
+  // floor(num * m / 2^k), taking the high 64 bits of the low product only,
+  // is the quotient or up to two less (num / 2^k < 2^-31); then correct.
   const unsigned __int128 num = (unsigned __int128)a * S.small_q + (S.Q >> 1);
   const u64 nh = (u64)(num >> 64), nl = (u64)num;
   const unsigned __int128 t = (unsigned __int128)nh * S.q_recip + __umul64hi(nl, S.q_recip);
-  u64 quo = (u64)(t >> 56);
+  u64 quo = (u64)(t >> S.q_shift);
   unsigned __int128 r = num - (unsigned __int128)quo * S.Q;
 #pragma unroll
   for (int k = 0; k < 2; k++)

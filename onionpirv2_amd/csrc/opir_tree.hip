@@ -161,6 +161,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         keep[cc] = add_mod(keep[cc], (i0 < kN || !x) ? x : L.q - x, L.q);
       }
     }
+    if (A.bv_only) {  // uniform over the workgroup: every thread skips the same barriers
+      store_map0(left + (h * 2 + limb) * kN, keep, t);
+      continue;
+    }
     u32 l_out[8];
 #pragma unroll
     for (int cc = 0; cc < 8; cc++) {
@@ -571,6 +575,19 @@ void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_s
   }
   dim3 grid(nnodes, B);
   k_expand_serial<<<grid, 512, 0, st>>>(A, S);
+}
+
+void launch_bv_apply(const u32 *in, u32 *out, const u32 *const *key_ptr, int kinv, const GadgetCfg &G,
+                     const SchemeConst &S, hipStream_t st) {
+  ExpandArgs A;
+  A.in = in; A.out = out; A.in_stride = 0; A.out_stride = 0;
+  A.bv_keys = key_ptr;
+  A.key_level_words = 0;
+  A.level = 0;
+  A.kinv = kinv;
+  A.l_ks = G.l_ks; A.b_ks = G.b_ks;
+  A.bv_only = 1;
+  k_expand_serial<<<dim3(1, 1), 512, 0, st>>>(A, S);
 }
 
 void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,

@@ -29,7 +29,7 @@ EXPORTED = [
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
     "opir_ctx_set_profiling", "opir_ctx_set_cu_mask", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_ctx_set_graphs", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
     "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
-    "opir_ext_prod", "opir_ext_prod_mux", "opir_mod_switch",
+    "opir_ext_prod", "opir_ext_prod_mux", "opir_mod_switch", "opir_bv_ks",
 ]
 
 
@@ -107,6 +107,7 @@ def load_library():
         "opir_ext_prod": ([P, P, sz, sz, P, P], ctypes.c_int),
         "opir_ext_prod_mux": ([P, P, sz, sz, P, P, P], ctypes.c_int),
         "opir_mod_switch": ([P, P, P], ctypes.c_int),
+        "opir_bv_ks": ([P, P, ctypes.c_uint32, P, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -122,8 +123,15 @@ def _ptr(a):
 
 
 def _u64(a, shape=None):
+    """Contiguous uint64 copy/view of a; with `shape`, the element count must
+    match it exactly (the C ABI reads exactly that many values)."""
+    if isinstance(a, np.ndarray) and a.dtype.kind == "i" and a.size and int(a.min()) < 0:
+        raise InvalidArgument("negative value where a residue is expected")
     a = np.ascontiguousarray(a, dtype=np.uint64)
-    if shape is not None and a.shape != tuple(shape):
+    if shape is not None:
+        shape = tuple(int(x) for x in shape)
+        if a.size != int(np.prod(shape)):
+            raise InvalidArgument(f"array of {a.size} values where {shape} is expected")
         a = a.reshape(shape)
     return a
 
@@ -215,8 +223,11 @@ class GpuContext:
         return dict(zip(names, [float(x) for x in out]))
 
     def set_client_keys(self, client_id, bv, gsw):
-        bv = _u64(bv)
-        gsw = _u64(gsw)
+        sc = self.scheme
+        bv = np.asarray(bv)
+        h = bv.shape[0] if bv.ndim else 0
+        bv = _u64(bv, (h, sc["l_ks"], 2, self.KN))
+        gsw = _u64(gsw, (2 * sc["l_key"], 2 * self.KN))
         self.check(self.lib.opir_client_set_keys(self.h, client_id, _ptr(bv), bv.shape[0], _ptr(gsw)))
 
     def key_bytes(self, h):
@@ -245,20 +256,28 @@ class GpuContext:
         return a
 
     def ext_prod(self, gsw, l, base_log2, ct):
-        g = _u64(gsw)
-        c = _u64(ct)
+        g = _u64(gsw, (2 * l, 2 * self.KN))
+        c = _u64(ct, (2, self.KN))
         out = np.zeros((2, self.KN), dtype=np.uint64)
         self.check(self.lib.opir_ext_prod(self.h, _ptr(g), l, base_log2, _ptr(c), _ptr(out)))
         return out
 
     def ext_prod_mux(self, gsw, l, base_log2, x, y):
-        g, xa, ya = _u64(gsw), _u64(x), _u64(y)
+        g, xa, ya = _u64(gsw, (2 * l, 2 * self.KN)), _u64(x, (2, self.KN)), _u64(y, (2, self.KN))
         out = np.zeros((2, self.KN), dtype=np.uint64)
         self.check(self.lib.opir_ext_prod_mux(self.h, _ptr(g), l, base_log2, _ptr(xa), _ptr(ya), _ptr(out)))
         return out
 
+    def bv_ks(self, ct, galois_k, key_rows):
+        """bv_apply_galois_inplace: key_rows [l_ks, 2 (a, b), K*N] NTT form."""
+        c = _u64(ct, (2, self.KN))
+        k = _u64(key_rows, (self.scheme["l_ks"], 2, self.KN))
+        out = np.zeros((2, self.KN), dtype=np.uint64)
+        self.check(self.lib.opir_bv_ks(self.h, _ptr(c), int(galois_k), _ptr(k), _ptr(out)))
+        return out
+
     def mod_switch(self, ct):
-        c = _u64(ct)
+        c = _u64(ct, (2, self.KN))
         out = np.zeros((2, self.N), dtype=np.uint64)
         self.check(self.lib.opir_mod_switch(self.h, _ptr(c), _ptr(out)))
         return out
@@ -292,6 +311,12 @@ class GpuDatabase:
             pass
 
     def load_plain(self, pts, first_pt=0):
+        pts = np.asarray(pts)
+        if pts.size % self.N:
+            raise InvalidArgument("plaintexts must hold a whole number of N-coefficient polynomials")
+        # server.cpp:129-137, checked before any narrowing so no value wraps
+        if pts.size and (int(pts.max()) >= self.ctx.scheme["plain_mod"] or int(pts.min()) < 0):
+            raise InvalidArgument("plaintext coefficient not reduced mod t")
         if self.ctx.scheme["plain_mod"] <= 1 << 16:
             a = np.ascontiguousarray(pts, dtype=np.uint16)
             self.ctx.check(self.lib.opir_db_load_plain(self.h, _ptr(a), first_pt, a.size // self.N))
@@ -300,19 +325,25 @@ class GpuDatabase:
             self.ctx.check(self.lib.opir_db_load_plain_u64(self.h, _ptr(a), first_pt, a.size // self.N))
 
     def load_ntt(self, lo, hi):
+        n = self.N * self.N0 * self.Nrest
+        lo, hi = np.asarray(lo), np.asarray(hi)
+        if lo.size != n or hi.size != n:
+            raise InvalidArgument(f"reference-layout database needs {n} residues per limb")
+        if int(lo.max()) >= 1 << 32 or int(hi.max()) >= 1 << 32:
+            raise InvalidArgument("database residue does not fit 32 bits")
         lo = np.ascontiguousarray(lo, dtype=np.uint32)
         hi = np.ascontiguousarray(hi, dtype=np.uint32)
         self.ctx.check(self.lib.opir_db_load_ntt(self.h, _ptr(lo), _ptr(hi)))
 
     def answer(self, client_id, query):
-        q = _u64(query)
+        q = _u64(query, (2, self.KN))
         out = np.zeros((2, self.N), dtype=np.uint64)
         self.ctx.check(self.lib.opir_answer(self.h, client_id, _ptr(q), _ptr(out)))
         return out
 
     def answer_batch(self, client_ids, queries):
         ids = np.ascontiguousarray(client_ids, dtype=np.uint64)
-        q = _u64(queries)
+        q = _u64(queries, (len(ids), 2, self.KN))
         out = np.zeros((len(ids), 2, self.N), dtype=np.uint64)
         self.ctx.check(self.lib.opir_answer_batch(self.h, len(ids), _ptr(ids), _ptr(q), _ptr(out)))
         return out
@@ -335,27 +366,28 @@ class GpuDatabase:
         return out.value
 
     def expand(self, client_id, query):
-        q = _u64(query)
+        q = _u64(query, (2, self.KN))
         out = np.zeros((self.u, 2, self.KN), dtype=np.uint64)
         self.ctx.check(self.lib.opir_expand(self.h, client_id, _ptr(q), _ptr(out)))
         return out
 
     def complete(self, client_id, expanded):
-        e = _u64(expanded)
+        e = _u64(expanded, (self.u, 2, self.KN))
         ns = self.dims - 1
         out = np.zeros((ns, 2 * self.ctx.scheme["l_ep"], 2 * self.KN), dtype=np.uint64)
         self.ctx.check(self.lib.opir_complete(self.h, client_id, _ptr(e), _ptr(out)))
         return out
 
     def first_dim(self, query_cts):
-        q = _u64(query_cts)
+        q = _u64(query_cts, (self.N0, 2, self.KN))
         out = np.zeros((self.Nrest, 2, self.KN), dtype=np.uint64)
         self.ctx.check(self.lib.opir_first_dim(self.h, _ptr(q), _ptr(out)))
         return out
 
     def other_dim(self, mid, sel):
-        m = _u64(mid)
-        s = _u64(sel) if sel is not None else np.zeros(1, dtype=np.uint64)
+        m = _u64(mid, (self.Nrest, 2, self.KN))
+        ns = self.dims - 1
+        s = _u64(sel, (ns, 2 * self.ctx.scheme["l_ep"], 2 * self.KN)) if ns else np.zeros(1, dtype=np.uint64)
         out = np.zeros((2, self.KN), dtype=np.uint64)
         self.ctx.check(self.lib.opir_other_dim(self.h, _ptr(m), _ptr(s), _ptr(out)))
         return out

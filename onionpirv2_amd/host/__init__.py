@@ -11,6 +11,7 @@ _LIB = _os.path.join(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__
 if _LIB not in _sys.path:
     _sys.path.insert(0, _LIB)
 try:
-    from _opir_host import GpuScheme, PirClient, PirParams, PirServer, ServingLoop, plan_batch_limits  # noqa: F401
+    from _opir_host import (GpuScheme, PirClient, PirParams, PirServer, ServingLoop, pack_bits,  # noqa: F401
+                            plan_batch_limits, unpack_bits)
 except ImportError as e:  # pragma: no cover
     raise ImportError(f"onionpirv2_amd native host module is not built in {_LIB}: {e}") from e

@@ -8,6 +8,7 @@
 
 #include <cstring>
 
+#include "codec.h"
 #include "onionpir_host.h"
 #include "serving.h"
 
@@ -71,6 +72,18 @@ static bvks::BvGaloisKeys arr_bv(const Arr &a, size_t N) {
 
 PYBIND11_MODULE(_opir_host, m) {
   m.doc() = "Host API of the MI355X OnionPIRv2 server (PirParams, PirClient, PirServer)";
+  m.def("pack_bits", [](Arr v, size_t width) {
+    std::string out;
+    codec::pack_bits(v.data(), (size_t)v.size(), width, out);
+    return py::bytes(out);
+  });
+  m.def("unpack_bits", [](py::bytes b, size_t n, size_t width) {
+    std::string s = b;
+    size_t pos = 0;
+    Arr a((py::ssize_t)n);
+    codec::unpack_bits(s, pos, n, width, a.mutable_data());
+    return py::make_tuple(a, pos);
+  });
   py::register_exception_translator([](std::exception_ptr p) {
     try {
       if (p) std::rethrow_exception(p);

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "codec.h"
 #include "onionpir_host.h"
 
 namespace onionpir {
@@ -276,23 +277,16 @@ RlweCt PirClient::load_resp_from_stream(std::stringstream &stream) const {
   // client.cpp:315-361: c0 then c1, ceil(log2 q') bits per value, LSB-first
   const size_t N = params_.get_poly_degree();
   const size_t w = std::bit_width(params_.get_small_q() - 1);
+  const size_t need = codec::packed_bytes(2 * N, w);
+  std::string buf(need, '\0');
+  stream.read(buf.data(), (std::streamsize)need);
+  if ((size_t)stream.gcount() != need) throw std::runtime_error("unexpected end of response stream");
+  std::vector<u64> v(2 * N);
+  size_t pos = 0;
+  codec::unpack_bits(buf, pos, 2 * N, w, v.data());
   RlweCt r;
-  r.c0.assign(N, 0);
-  r.c1.assign(N, 0);
-  u64 acc = 0;
-  size_t have = 0;
-  for (size_t p = 0; p < 2; p++)
-    for (size_t i = 0; i < N; i++) {
-      while (have < w) {
-        const int ch = stream.get();
-        if (ch == EOF) throw std::runtime_error("unexpected end of response stream");
-        acc |= (u64)(uint8_t)ch << have;
-        have += 8;
-      }
-      r.data(p)[i] = acc & ((u64{1} << w) - 1);
-      acc >>= w;
-      have -= w;
-    }
+  r.c0.assign(v.begin(), v.begin() + N);
+  r.c1.assign(v.begin() + N, v.end());
   return r;
 }
 

@@ -8,6 +8,7 @@
 #include <unordered_set>
 
 #include "onionpir_gpu.h"
+#include "codec.h"
 #include "onionpir_host.h"
 
 namespace onionpir {
@@ -328,24 +329,13 @@ size_t PirServer::save_resp_to_stream(const RlweCt &resp, std::stringstream &str
   // server.cpp:950-1009: c0 then c1, ceil(log2 q') bits each, LSB-first
   const size_t N = params_.get_poly_degree();
   const size_t w = std::bit_width(params_.get_small_q() - 1);
-  const u64 mask = (u64{1} << w) - 1;
-  u64 acc = 0;
-  size_t have = 0, bytes = 0;
-  for (size_t p = 0; p < 2; p++)
-    for (size_t i = 0; i < N; i++) {
-      acc |= (resp.data(p)[i] & mask) << have;
-      have += w;
-      while (have >= 8) {
-        stream.put((char)(acc & 0xFF));
-        acc >>= 8;
-        have -= 8;
-        bytes++;
-      }
-    }
-  if (have) {
-    stream.put((char)(acc & 0xFF));
-    bytes++;
-  }
+  if (resp.c0.size() < N || resp.c1.size() < N) throw std::invalid_argument("response must hold N coefficients per poly");
+  std::vector<u64> v(2 * N);
+  std::copy(resp.c0.begin(), resp.c0.begin() + N, v.begin());
+  std::copy(resp.c1.begin(), resp.c1.begin() + N, v.begin() + N);
+  std::string out;
+  const size_t bytes = codec::pack_bits(v.data(), 2 * N, w, out);
+  stream.write(out.data(), (std::streamsize)out.size());
   return bytes;
 }
 

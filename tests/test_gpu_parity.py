@@ -430,3 +430,78 @@ def test_resident_key_bytes_track_clients(O, comp_params, gpu_ctx):
     gpu_ctx.remove_client(9071)
     gpu_ctx.remove_client(9072)
     assert gpu_ctx.resident_key_bytes() == before
+
+
+def test_loaders_reject_unreduced_values(comp_params, gpu_ctx):
+    """server.cpp:129-137: a plaintext coefficient >= t is rejected with
+    invalid_argument, by the Python wrapper before any narrowing and by the C
+    ABI itself; a reference-layout residue >= its limb prime likewise."""
+    import ctypes
+    from onionpirv2_amd.gpu import GpuDatabase, InvalidArgument, OPIR_EINVAL
+    say("[gpu] loader value checks")
+    p = comp_params.with_layout(43, 5)
+    db = GpuDatabase(gpu_ctx, p.fst_dim_sz, p.other_dim_sz, p.num_dims, p.expansion_height)
+    pts = np.zeros((p.num_pt, p.N), np.uint16)
+    pts[3, 7] = p.plain_mod
+    with pytest.raises(InvalidArgument):
+        db.load_plain(pts)
+    with pytest.raises(InvalidArgument):
+        db.load_plain(np.full((1, p.N), 1 << 16, np.uint32))  # would wrap to 0 as uint16
+    rc = db.lib.opir_db_load_plain(db.h, ctypes.c_void_p(pts.ctypes.data), 0, p.num_pt)
+    assert rc == OPIR_EINVAL and "mod t" in gpu_ctx.lib.opir_last_error(gpu_ctx.h).decode()
+    n = p.N * p.num_pt
+    lo = np.zeros(n, np.uint32)
+    hi = np.zeros(n, np.uint32)
+    lo[n - 1] = p.q1
+    rc = db.lib.opir_db_load_ntt(db.h, ctypes.c_void_p(lo.ctypes.data), ctypes.c_void_p(hi.ctypes.data))
+    assert rc == OPIR_EINVAL
+    lo[n - 1] = 0
+    hi[5] = p.q2
+    with pytest.raises(InvalidArgument):
+        db.load_ntt(lo, hi)
+    db.close()
+
+
+def test_mod_switch_small_composite_modulus_exact(O):
+    """The final rescale on a composite modulus far below the reference's
+    2^58 (two 20-bit limb primes): every output equals the exact centred
+    rescale (utils.cpp:445-462, pinned against exact rationals)."""
+    from onionpirv2_amd.gpu import GpuContext
+    say("[gpu] mod switch, 40-bit composite modulus")
+    N = 2048
+    q1, q2 = O.generate_ntt_friendly_primes([20, 20], N)
+    Q = q1 * q2
+    root = O.crt_combine(O.minimal_primitive_root(2 * N, q1), q1, O.minimal_primitive_root(2 * N, q2), q2)
+    sq = 12289
+    sch = dict(poly_degree=N, composite=1, q1=q1, q2=q2, ntt_root=root, plain_mod=257, small_q=sq,
+               l_ep=4, base_log2=10, l_key=4, base_log2_key=10, l_ks=4, bv_base_log2=10)
+    ctx = GpuContext(sch)
+    try:
+        rng = np.random.default_rng(41)
+        ct = rng.integers(0, Q, size=(2, N), dtype=np.uint64)
+        ct[0, :7] = [0, 1, Q // 2 - 1, Q // 2, Q // 2 + 1, Q - 1, Q - 2]
+        got = ctx.mod_switch(ct)
+        want = np.array([[O.rescale(int(x), Q, sq) for x in row] for row in ct], np.uint64)
+        assert np.array_equal(got, want)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("k", [2049, 1025, 5, 3, 4095])
+def test_bv_keyswitch_matches_oracle(O, comp_params, gpu_ctx, k):
+    """bvks::bv_apply_galois_inplace (bv_keyswitch.cpp:296-367) alone through
+    opir_bv_ks, against the oracle's bv_apply on a random ciphertext (every
+    digit path, including the centring edges), bit-exact."""
+    from onionpirv2_amd.gpu import InvalidArgument
+    say(f"[gpu] BV keyswitch k={k}")
+    p = comp_params
+    Q = p.rns_mods[0]
+    sk, _ = O.gen_sk_and_encrypt(p, np.zeros(p.N, np.uint64), 15)
+    key = O.gen_bv_ks_key(p, sk, k, 16 + k)[0]
+    rng = np.random.default_rng(k)
+    ct = rng.integers(0, Q, size=(2, p.N), dtype=np.uint64)
+    ct[1, :6] = [0, 1, Q // 2, Q // 2 + 1, Q - 1, Q // 2 - 1]
+    want = O.bv_apply(p, ct, k, key[None])
+    assert np.array_equal(gpu_ctx.bv_ks(ct, k, key), want)
+    with pytest.raises(InvalidArgument):
+        gpu_ctx.bv_ks(ct, 2, key)

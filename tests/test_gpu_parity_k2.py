@@ -262,3 +262,21 @@ def test_k2_every_scan_shape_matches_oracle(O, k2p, ctx2, layout):
     assert len(got) >= 4
     for v, b in got.items():
         assert np.array_equal(b, want), f"scan shape {v}"
+
+
+@pytest.mark.parametrize("k", [4097, 1025, 3, 8191])
+def test_k2_bv_keyswitch_matches_oracle(O, k2p, ctx2, k):
+    """bv_apply_galois_inplace_k2 (bv_keyswitch.cpp:373-481: signed MP digits,
+    ties to +B/2) alone through opir_bv_ks, against the oracle, bit-exact."""
+    say(f"[gpu] K=2 BV keyswitch k={k}")
+    p = k2p
+    N = p.N
+    q0, q1 = p.rns_mods
+    sk, _ = O.gen_sk_and_encrypt(p, np.zeros(N, np.uint64), 25)
+    key = O.gen_bv_ks_key(p, sk, k, 26 + k)[0]
+    rng = np.random.default_rng(k)
+    ct = np.empty((2, 2 * N), np.uint64)
+    ct[:, :N] = rng.integers(0, q0, size=(2, N), dtype=np.uint64)
+    ct[:, N:] = rng.integers(0, q1, size=(2, N), dtype=np.uint64)
+    want = O.bv_apply(p, ct, k, key[None])
+    assert np.array_equal(ctx2.bv_ks(ct, k, key), want)
