@@ -214,6 +214,18 @@ int opir_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t base_log2
  * are not part of the result). */
 int opir_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t base_log2, const uint64_t *x,
                       const uint64_t *y, uint64_t *out);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the homomorphic
+ * select of opir_ext_prod_mux with every operand in device memory (d_gsw
+ * [2l][2][K*N] NTT form; d_x, d_y, d_out [2][K*N] coefficient form, values
+ * reduced; d_out may alias d_x). It calls no allocator: its intermediates use
+ * a fixed 2 MiB device array per GPU, so l is limited to 16 rows (composite
+ * scheme) or 32 (two primes), larger -> OPIR_EINVAL. The work is queued on
+ * the context stream and the call returns without waiting. All calls on one
+ * GPU share that array: calls that may overlap on the device (other streams,
+ * other contexts on the same GPU) race. */
+int opir_ext_prod_mux_device_noalloc(opir_ctx *ctx, const uint64_t *d_gsw, size_t l, size_t base_log2,
+                                     const uint64_t *d_x, const uint64_t *d_y, uint64_t *d_out);
 /* bvks::bv_apply_galois_inplace (src/bv_keyswitch.cpp:296-367 for K = 1,
  * 373-481 for K = 2): ct [2][K*N] coefficient form; key: the l_ks rows of one
  * BvKeySwitchKey for galois element k, [l_ks][a, b][K*N] NTT form

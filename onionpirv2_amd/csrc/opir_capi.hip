@@ -198,7 +198,8 @@ static int ctx_create_k2(const opir_scheme *sch, int device, opir_ctx **out) {
   if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
     return fail(OPIR_EHIP, "event create");
-  const int rc = k2_ctx_init(ctx, sch);
+  int rc = k2_ctx_init(ctx, sch);
+  if (!rc) rc = noalloc_init(ctx);
   if (rc) {
     const std::string m = g_create_error;
     return fail(rc, m);
@@ -245,6 +246,7 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
     return OPIR_EHIP;
   };
   if (hipSetDevice(device) != hipSuccess) return fail("hipSetDevice failed");
+  if (noalloc_init(ctx)) return fail(g_create_error);
   if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) return fail("stream create");
   ctx->stream = ctx->own_stream;
   if (hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking) != hipSuccess) return fail("stream create");

@@ -71,6 +71,8 @@ struct opir_ctx {
   // host staging for synchronous calls
   void *pin = nullptr;
   size_t pin_bytes = 0;
+  // this device's copy of the allocation-free entry point's scratch array
+  void *scratch = nullptr;
 };
 
 struct opir_db {
@@ -341,6 +343,10 @@ struct DevBuf {
   }
 };
 #define DEV_ALLOC(ctx, buf, bytes) HIP_TRY(ctx, hipMalloc(&(buf).p, (bytes)))
+
+// Resolves ctx->scratch for ctx->device (the device must be current); once
+// per device, at context creation (opir_noalloc.hip).
+int noalloc_init(opir_ctx *ctx);
 
 // K = 2 implementations behind the same entry points (opir_capi_k2.hip).
 int k2_validate_scheme(const opir_scheme *sch);  // host-only checks, before any device call

@@ -29,7 +29,7 @@ EXPORTED = [
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
     "opir_ctx_set_profiling", "opir_ctx_set_cu_mask", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_ctx_set_graphs", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
     "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
-    "opir_ext_prod", "opir_ext_prod_mux", "opir_mod_switch", "opir_bv_ks",
+    "opir_ext_prod", "opir_ext_prod_mux", "opir_mod_switch", "opir_bv_ks", "opir_ext_prod_mux_device_noalloc",
 ]
 
 
@@ -108,6 +108,7 @@ def load_library():
         "opir_ext_prod_mux": ([P, P, sz, sz, P, P, P], ctypes.c_int),
         "opir_mod_switch": ([P, P, P], ctypes.c_int),
         "opir_bv_ks": ([P, P, ctypes.c_uint32, P, P], ctypes.c_int),
+        "opir_ext_prod_mux_device_noalloc": ([P, P, sz, sz, P, P, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
