@@ -82,7 +82,13 @@ double ServingLoop::now() const {
 uint64_t ServingLoop::submit(size_t db, size_t client_id, std::vector<u64> query) {
   if (db >= lanes_.size()) throw std::out_of_range("unknown database index");
   const PirParams &p = lanes_[db]->srv->get_params();
-  if (query.size() != 2 * p.get_poly_degree() * p.K()) throw std::invalid_argument("query must hold [c0 K*N][c1 K*N]");
+  const size_t N = p.get_poly_degree(), K = p.K();
+  if (query.size() != 2 * N * K) throw std::invalid_argument("query must hold [c0 K*N][c1 K*N]");
+  // values must be reduced (limb by limb): a malformed query fails here, on
+  // its own, rather than the batch it would have joined
+  const auto &mods = p.get_rns_mods();
+  for (size_t i = 0; i < query.size(); i++)
+    if (query[i] >= mods[(i / N) % K]) throw std::invalid_argument("query value not reduced mod its modulus");
   std::lock_guard<std::mutex> g(mu_);
   if (stop_) throw std::runtime_error("serving loop is stopped");
   const uint64_t t = next_ticket_++;
@@ -144,10 +150,24 @@ void ServingLoop::worker(size_t d) {
     const double ts = now();
     int rc;
     std::string err;
+    std::vector<int> rcs(B, OPIR_OK);
+    std::vector<std::string> errs(B);
     {
       std::lock_guard<std::mutex> g(*lane.ctx_mu);
       rc = opir_answer_batch(srv.gpu_db(), B, ids.data(), q.data(), r.data());
       if (rc != OPIR_OK) err = opir_last_error(srv.gpu_scheme()->ctx());
+      if ((rc == OPIR_EINVAL || rc == OPIR_ERANGE) && B > 1) {
+        // one request (an unknown client, say) failed the whole batch: answer
+        // them one by one so that only the bad ones carry the error
+        rc = OPIR_OK;
+        for (size_t i = 0; i < B; i++) {
+          rcs[i] = opir_answer_batch(srv.gpu_db(), 1, &ids[i], &q[i * 2 * QN], &r[i * 2 * N]);
+          if (rcs[i] != OPIR_OK) errs[i] = opir_last_error(srv.gpu_scheme()->ctx());
+        }
+      } else if (rc != OPIR_OK) {
+        std::fill(rcs.begin(), rcs.end(), rc);
+        std::fill(errs.begin(), errs.end(), err);
+      }
     }
     const double td = now();
     std::vector<ServeResult> res(B);
@@ -160,10 +180,10 @@ void ServingLoop::worker(size_t d) {
       x.t_start = ts;
       x.t_done = td;
       x.batch = B;
-      if (rc == OPIR_OK)
+      if (rcs[i] == OPIR_OK)
         x.resp.assign(r.begin() + i * 2 * N, r.begin() + (i + 1) * 2 * N);
       else
-        x.error = err;
+        x.error = errs[i];
     }
     std::lock_guard<std::mutex> g(mu_);
     for (auto &x : res) done_.push_back(std::move(x));

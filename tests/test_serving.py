@@ -147,3 +147,36 @@ def test_mixed_schemes_served_equal_alone(served):
     assert seen_k2 > 0
     pt, budget = c2.decrypt_mod_q(a2[1])
     assert np.array_equal(pt, s2.direct_get_original_plaintext(1000)) and budget > 0
+
+
+@pytest.mark.gpu
+def test_bad_request_fails_alone(served):
+    """A request from an unkeyed client, batched with valid ones, fails on
+    its own: every other request still equals its answer alone. A query with
+    unreduced values is rejected at submit."""
+    H, servers, clients, pool, alone = served
+    print("[gpu] serving: one bad request in a burst", flush=True)
+    loop = H.ServingLoop(servers, max_batch=8)
+    try:
+        tickets = {}
+        bad = None
+        for k, (d, cid, _, q) in enumerate(pool):
+            tickets[loop.submit(d, cid, q)] = k
+            if k == 3:
+                bad = loop.submit(d, 987654, q)  # no keys for this client
+        with pytest.raises(ValueError):
+            q = pool[0][3].copy()
+            q[0, 0] = np.uint64(1 << 62)
+            loop.submit(0, pool[0][1], q)
+        loop.wait_idle()
+        res = loop.poll()
+    finally:
+        loop.stop()
+    assert len(res) == len(tickets) + 1
+    for r in res:
+        if r["ticket"] == bad:
+            assert "unknown client" in r["error"]
+        else:
+            assert r["error"] == "", r["error"]
+            assert np.array_equal(r["resp"], alone[tickets[r["ticket"]]])
+    assert max(r["batch"] for r in res) > 1
