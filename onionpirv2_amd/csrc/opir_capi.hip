@@ -9,6 +9,30 @@ thread_local std::string capi::g_create_error;
 
 namespace {
 
+// u64 values per polynomial at the ABI: N (composite, mod Q) or 2N (the
+// two-limb MP scheme, [limb][N] residues).
+inline size_t abi_kn(const opir_ctx *ctx) { return (size_t)ctx->K * kN; }
+
+// npoly ABI polynomials must hold reduced values: < Q (composite) or each
+// limb below its prime (MP).
+int validate_polys(opir_ctx *ctx, const uint64_t *v, size_t npoly, const char *what) {
+  if (!ctx->S.mp) return validate_mod(ctx, v, npoly * kN, ctx->S.Q, what);
+  const u64 q[2] = {ctx->S.limb[0].q, ctx->S.limb[1].q};
+  for (size_t i = 0; i < npoly * 2 * kN; i++)
+    if (v[i] >= q[(i / kN) & 1]) return set_err(ctx, OPIR_EINVAL, std::string(what) + " value not reduced mod its RNS prime");
+  return OPIR_OK;
+}
+
+// One ABI polynomial -> its two u32 limbs (composite: residues of the value
+// mod Q; MP: the limbs as given).
+inline void poly_to_rns(const opir_ctx *ctx, const uint64_t *src, u32 *dst) {
+  const u64 q1 = ctx->S.limb[0].q, q2 = ctx->S.limb[1].q;
+  for (size_t j = 0; j < (size_t)kN; j++) {
+    dst[j] = (u32)(ctx->S.mp ? src[j] : src[j] % q1);
+    dst[kN + j] = (u32)(ctx->S.mp ? src[kN + j] : src[j] % q2);
+  }
+}
+
 int upload_k1(opir_ctx *ctx, const uint64_t *ids, size_t B, BatchPtrs &P) {
   const void *const *t = nullptr;
   const int rc = upload_key_ptrs(ctx, ids, B, &t);
@@ -24,7 +48,7 @@ size_t ws_bytes(const opir_db *db, size_t B) {
   const size_t ct = kCtWords * sizeof(u32);
   const size_t nsel = db->dims - 1;
   return qfrag_bytes(make_qfrag_geom(db->N0, (int)B)) + 2 * B * db->exp_cts * ct + B * db->N0 * ct +
-         B * (nsel ? nsel : 1) * 2 * db->ctx->G.l_ep * ct + 2 * B * db->Nrest * ct + 2 * B * 2 * kN * sizeof(u64) +
+         B * (nsel ? nsel : 1) * 2 * db->ctx->G.l_ep * ct + 2 * B * db->Nrest * ct + B * 2 * (abi_kn(db->ctx) + kN) * sizeof(u64) +
          kMuxSplitMaxBlocks * ct;
 }
 
@@ -54,7 +78,7 @@ int ensure_ws(opir_db *db, size_t B) {
   HIP_TRY(ctx, hipMalloc((void **)&db->mid_a, B * db->Nrest * ct));
   HIP_TRY(ctx, hipMalloc((void **)&db->mid_b, B * db->Nrest * ct));
   HIP_TRY(ctx, hipMalloc((void **)&db->part, kMuxSplitMaxBlocks * ct));
-  HIP_TRY(ctx, hipMalloc((void **)&db->q_u64, B * 2 * kN * sizeof(u64)));
+  HIP_TRY(ctx, hipMalloc((void **)&db->q_u64, B * 2 * abi_kn(ctx) * sizeof(u64)));
   HIP_TRY(ctx, hipMalloc((void **)&db->r_u64, B * 2 * kN * sizeof(u64)));
   db->ws_B = B;
   return OPIR_OK;
@@ -128,7 +152,7 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   return run_answer(db, B, 4, P.bv_keys, d_q, d_r, [&]() -> int {
     rec(ctx, 0);
     // root ct per query -> exp_a[q][0]
-    launch_split_u64_batch(d_q, 2 * kN, db->exp_a, exp_stride, 2, (int)B, ctx->S, ctx->stream);
+    launch_split_u64_batch(d_q, 2 * abi_kn(ctx), db->exp_a, exp_stride, 2, (int)B, ctx->S, ctx->stream);
     LAUNCH_CHECK(ctx);
     u32 *leaves = run_expand(db, B, P);
     LAUNCH_CHECK(ctx);
@@ -211,7 +235,11 @@ static int ctx_create_k2(const opir_scheme *sch, int device, opir_ctx **out) {
 int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   if (!sch || !out) return set_err(nullptr, OPIR_EINVAL, "null argument");
   *out = nullptr;
-  if (!sch->composite) return ctx_create_k2(sch, device, out);
+  // two RNS primes below 2^30 at N = 2048 (CONFIG_N2048_K2_MP) run on the
+  // composite scheme's 32-bit kernels with the MP gadgets and mod switch
+  const bool mp = !sch->composite && sch->poly_degree == (uint32_t)kN && sch->q1 < (1ULL << 30) &&
+                  sch->q2 < (1ULL << 30);
+  if (!sch->composite && !mp) return ctx_create_k2(sch, device, out);
   if (sch->poly_degree != (uint32_t)kN)
     return set_err(nullptr, OPIR_EINVAL, "the composite scheme runs at N=2048 (CONFIG_N2048_K1_COMP)");
   const u64 q1 = sch->q1, q2 = sch->q2;
@@ -236,10 +264,14 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   }
   if (sch->small_q < 2 || sch->small_q >= (1ULL << 32))
     return set_err(nullptr, OPIR_EINVAL, "small_q must fit 32 bits");
+  if (mp && sch->small_q >= std::min(q1, q2))
+    return set_err(nullptr, OPIR_EINVAL, "small_q must be below the RNS primes");
   opir_ctx *ctx = new (std::nothrow) opir_ctx();
   if (!ctx) return set_err(nullptr, OPIR_ENOMEM, "out of host memory");
   ctx->sch = *sch;
   ctx->device = device;
+  ctx->K = mp ? 2 : 1;
+  ctx->S.mp = mp ? 1 : 0;
   auto fail = [&](const std::string &m) {
     g_create_error = m;
     opir_ctx_destroy(ctx);
@@ -260,7 +292,8 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   const u64 Q = q1 * q2;
   for (int l = 0; l < 2; l++) {
     const u64 q = l == 0 ? q1 : q2;
-    const u64 w = sch->ntt_root % q;
+    // composite: one root of Z_Q seen mod each prime; MP: a root per prime
+    const u64 w = (mp && l == 1 ? sch->ntt_root2 : sch->ntt_root) % q;
     if (powmod_h(w, kN, q) != q - 1) {
       opir_ctx_destroy(ctx);
       return set_err(nullptr, OPIR_EINVAL, "ntt_root is not a primitive 2N-th root of unity");
@@ -565,7 +598,7 @@ int opir_db_gen_random(opir_db *db, uint64_t seed) {
 int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi) {
   if (!db || !lo || !hi) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;
-  if (ctx->k2) return set_err(ctx, OPIR_EINVAL, "the composite (lo, hi) database layout exists only for K=1");
+  if (ctx->k2 || ctx->S.mp) return set_err(ctx, OPIR_EINVAL, "the composite (lo, hi) database layout exists only for K=1");
   const size_t n = (size_t)kN * db->num_pt;
   // residues must be reduced: the digit planes assume values below 2^30
   const u32 q1 = ctx->S.limb[0].q, q2 = ctx->S.limb[1].q;
@@ -609,32 +642,19 @@ int opir_client_set_keys(opir_ctx *ctx, uint64_t id, const uint64_t *bv, size_t 
     return OPIR_OK;
   }
   if (h > 11) return set_err(ctx, OPIR_EINVAL, "BV key count out of range");
-  const size_t L = ctx->G.l_ks, lk = ctx->G.l_key;
-  const u64 Q = ctx->S.Q, q1 = ctx->S.limb[0].q, q2 = ctx->S.limb[1].q;
-  int rc = h ? validate_mod(ctx, bv, h * L * 2 * kN, Q, "BV key") : OPIR_OK;
+  const size_t L = ctx->G.l_ks, lk = ctx->G.l_key, KN = abi_kn(ctx);
+  int rc = h ? validate_polys(ctx, bv, h * L * 2, "BV key") : OPIR_OK;
   if (rc) return rc;
-  rc = validate_mod(ctx, gsw, 2 * lk * 2 * kN, Q, "RGSW key");
+  rc = validate_polys(ctx, gsw, 2 * lk * 2, "RGSW key");
   if (rc) return rc;
   std::vector<u32> hbv(h * L * kCtWords), hg(2 * lk * kCtWords);
   for (size_t r = 0; r < h * L; r++)
     for (int ab = 0; ab < 2; ab++) {      // input order a, b
       const int out = ab == 0 ? 1 : 0;    // device order b (-> c0), a (-> c1)
-      const uint64_t *src = bv + (r * 2 + ab) * kN;
-      u32 *dst = hbv.data() + r * kCtWords + (size_t)out * 2 * kN;
-      for (size_t j = 0; j < (size_t)kN; j++) {
-        dst[j] = (u32)(src[j] % q1);
-        dst[kN + j] = (u32)(src[j] % q2);
-      }
+      poly_to_rns(ctx, bv + (r * 2 + ab) * KN, hbv.data() + r * kCtWords + (size_t)out * 2 * kN);
     }
   for (size_t r = 0; r < 2 * lk; r++)
-    for (int p = 0; p < 2; p++) {
-      const uint64_t *src = gsw + (r * 2 + p) * kN;
-      u32 *dst = hg.data() + r * kCtWords + (size_t)p * 2 * kN;
-      for (size_t j = 0; j < (size_t)kN; j++) {
-        dst[j] = (u32)(src[j] % q1);
-        dst[kN + j] = (u32)(src[j] % q2);
-      }
-    }
+    for (int p = 0; p < 2; p++) poly_to_rns(ctx, gsw + (r * 2 + p) * KN, hg.data() + r * kCtWords + (size_t)p * 2 * kN);
   ClientKeys k;
   k.h = h;
   if (h) HIP_TRY(ctx, hipMalloc((void **)&k.bv, hbv.size() * sizeof(u32)));
@@ -676,7 +696,7 @@ int opir_db_reserve(opir_db *db, size_t B) {
   int rc = ctx->k2 ? k2_ensure_ws(db, B) : ensure_ws(db, B);
   if (rc) return rc;
   // the host staging of opir_answer_batch and the key-pointer ring
-  const size_t vals = (ctx->k2 ? 4 : 2) * ctx->N;
+  const size_t vals = (ctx->k2 || ctx->S.mp ? 4 : 2) * ctx->N;
   if ((rc = ensure_pin(ctx, B * vals * sizeof(u64)))) return rc;
   if ((rc = ensure_ring(ctx, B))) return rc;
   if (ctx->scan_variant == -2) {
@@ -721,8 +741,8 @@ int opir_answer_batch(opir_db *db, size_t B, const uint64_t *ids, const uint64_t
   if (!db || !ids || !queries || !resps) return OPIR_EINVAL;
   if (B == 0) return OPIR_OK;
   opir_ctx *ctx = db->ctx;
-  const size_t N = ctx->N, qvals = (ctx->k2 ? 4 : 2) * N, rvals = 2 * N;
-  int rc = ctx->k2 ? k2_validate_rns(ctx, queries, B * 2, "query") : validate_mod(ctx, queries, B * qvals, ctx->S.Q, "query");
+  const size_t N = ctx->N, qvals = (ctx->k2 || ctx->S.mp ? 4 : 2) * N, rvals = 2 * N;
+  int rc = ctx->k2 ? k2_validate_rns(ctx, queries, B * 2, "query") : validate_polys(ctx, queries, B * 2, "query");
   if (rc) return rc;
   for (size_t i = 0; i < B; i++)
     if (!ctx->clients.count(ids[i])) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(ids[i]));
@@ -753,15 +773,15 @@ static int ntt_entry(opir_ctx *ctx, uint64_t *data, size_t count, bool inverse) 
   if (!ctx || (!data && count)) return OPIR_EINVAL;
   if (!count) return OPIR_OK;
   if (ctx->k2) return k2_ntt(ctx, data, count, inverse);
-  int rc = validate_mod(ctx, data, count * kN, ctx->S.Q, "NTT input");
+  int rc = validate_polys(ctx, data, count, "NTT input");
   if (rc) return rc;
   DevBuf d;
-  DEV_ALLOC(ctx, d, count * kN * sizeof(u64));
-  HIP_TRY(ctx, hipMemcpy(d.p, data, count * kN * sizeof(u64), hipMemcpyHostToDevice));
+  DEV_ALLOC(ctx, d, count * abi_kn(ctx) * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(d.p, data, count * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
   launch_ntt_u64((u64 *)d.p, count, inverse, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_TRY(ctx, hipMemcpy(data, d.p, count * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(data, d.p, count * abi_kn(ctx) * sizeof(u64), hipMemcpyDeviceToHost));
   return OPIR_OK;
 }
 
@@ -772,7 +792,7 @@ int opir_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out) 
   if (!db || !query || !out) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;
   if (ctx->k2) return k2_expand(db, id, query, out);
-  int rc = validate_mod(ctx, query, 2 * kN, ctx->S.Q, "query");
+  int rc = validate_polys(ctx, query, 2, "query");
   if (rc) return rc;
   if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
   if (ctx->clients[id].h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
@@ -781,16 +801,16 @@ int opir_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out) 
   BatchPtrs P;
   rc = upload_k1(ctx, &id, 1, P);
   if (rc) return rc;
-  HIP_TRY(ctx, hipMemcpy(db->q_u64, query, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(db->q_u64, query, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
   launch_split_u64(db->q_u64, db->exp_a, 2, ctx->S, ctx->stream);
   u32 *leaves = run_expand(db, 1, P);
   LAUNCH_CHECK(ctx);
   DevBuf o;
-  DEV_ALLOC(ctx, o, db->u * 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, o, db->u * 2 * abi_kn(ctx) * sizeof(u64));
   launch_join_u64(leaves, (u64 *)o.p, db->u * 2, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_TRY(ctx, hipMemcpy(out, o.p, db->u * 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, db->u * 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyDeviceToHost));
   return OPIR_OK;
 }
 
@@ -801,7 +821,7 @@ int opir_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *
   const size_t nsel = db->dims - 1;
   if (!nsel) return OPIR_OK;
   if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
-  int rc = validate_mod(ctx, expanded, db->u * 2 * kN, ctx->S.Q, "expanded");
+  int rc = validate_polys(ctx, expanded, db->u * 2, "expanded");
   if (rc) return rc;
   rc = ensure_ws(db, 1);
   if (rc) return rc;
@@ -809,19 +829,19 @@ int opir_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *
   rc = upload_k1(ctx, &id, 1, P);
   if (rc) return rc;
   DevBuf e;
-  DEV_ALLOC(ctx, e, db->u * 2 * kN * sizeof(u64));
-  HIP_TRY(ctx, hipMemcpy(e.p, expanded, db->u * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  DEV_ALLOC(ctx, e, db->u * 2 * abi_kn(ctx) * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(e.p, expanded, db->u * 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
   launch_split_u64((u64 *)e.p, db->exp_a, db->u * 2, ctx->S, ctx->stream);
   const size_t sel_stride = nsel * 2 * (size_t)ctx->G.l_ep * kCtWords;
   launch_complete(db->exp_a + db->N0 * kCtWords, db->sel, 0, sel_stride, (int)(nsel * ctx->G.l_ep), 1, P, ctx->G,
                   ctx->S, ctx->stream);
   const size_t npoly = nsel * 2 * ctx->G.l_ep * 2;
   DevBuf o;
-  DEV_ALLOC(ctx, o, npoly * kN * sizeof(u64));
+  DEV_ALLOC(ctx, o, npoly * abi_kn(ctx) * sizeof(u64));
   launch_join_u64(db->sel, (u64 *)o.p, npoly, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_TRY(ctx, hipMemcpy(sel, o.p, npoly * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(sel, o.p, npoly * abi_kn(ctx) * sizeof(u64), hipMemcpyDeviceToHost));
   return OPIR_OK;
 }
 
@@ -829,13 +849,13 @@ int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
   if (!db || !query || !out) return OPIR_EINVAL;
   opir_ctx *ctx = db->ctx;
   if (ctx->k2) return k2_first_dim(db, query, out);
-  int rc = validate_mod(ctx, query, db->N0 * 2 * kN, ctx->S.Q, "query");
+  int rc = validate_polys(ctx, query, db->N0 * 2, "query");
   if (rc) return rc;
   rc = ensure_ws(db, 1);
   if (rc) return rc;
   DevBuf e;
-  DEV_ALLOC(ctx, e, db->N0 * 2 * kN * sizeof(u64));
-  HIP_TRY(ctx, hipMemcpy(e.p, query, db->N0 * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  DEV_ALLOC(ctx, e, db->N0 * 2 * abi_kn(ctx) * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(e.p, query, db->N0 * 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
   launch_split_u64((u64 *)e.p, db->exp_a, db->N0 * 2, ctx->S, ctx->stream);
   launch_ct_ntt(db->exp_a, db->qntt, 0, 0, (int)db->N0, 1, ctx->S, ctx->stream);
   launch_qfrag(db->qntt, db->qfrag, 0, 1, db->geom, ctx->stream);
@@ -843,11 +863,11 @@ int opir_first_dim(opir_db *db, const uint64_t *query, uint64_t *out) {
                   ctx->scan_variant == -2 ? -1 : ctx->scan_variant))
     return set_err(ctx, OPIR_EINVAL, "scan variant cannot take this batch size");
   DevBuf o;
-  DEV_ALLOC(ctx, o, db->Nrest * 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, o, db->Nrest * 2 * abi_kn(ctx) * sizeof(u64));
   launch_ct_to_coef_u64(db->mid_a, (u64 *)o.p, db->Nrest, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_TRY(ctx, hipMemcpy(out, o.p, db->Nrest * 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, db->Nrest * 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyDeviceToHost));
   return OPIR_OK;
 }
 
@@ -858,31 +878,31 @@ int opir_other_dim(opir_db *db, const uint64_t *mid, const uint64_t *sel, uint64
   if (int e = check_answerable(db)) return e;
   const size_t nsel = db->dims - 1;
   const size_t sel_polys = nsel * 2 * ctx->G.l_ep * 2;
-  int rc = validate_mod(ctx, mid, db->Nrest * 2 * kN, ctx->S.Q, "candidates");
+  int rc = validate_polys(ctx, mid, db->Nrest * 2, "candidates");
   if (rc) return rc;
-  if (nsel) rc = validate_mod(ctx, sel, sel_polys * kN, ctx->S.Q, "selectors");
+  if (nsel) rc = validate_polys(ctx, sel, sel_polys, "selectors");
   if (rc) return rc;
   rc = ensure_ws(db, 1);
   if (rc) return rc;
   DevBuf m;
-  DEV_ALLOC(ctx, m, db->Nrest * 2 * kN * sizeof(u64));
-  HIP_TRY(ctx, hipMemcpy(m.p, mid, db->Nrest * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  DEV_ALLOC(ctx, m, db->Nrest * 2 * abi_kn(ctx) * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(m.p, mid, db->Nrest * 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
   // stage coefficient-form candidates in mid_b (Nrest cts), NTT into mid_a
   launch_split_u64((u64 *)m.p, db->mid_b, db->Nrest * 2, ctx->S, ctx->stream);
   launch_ct_ntt(db->mid_b, db->mid_a, 0, 0, (int)db->Nrest, 1, ctx->S, ctx->stream);
   DevBuf s;
   if (nsel) {
-    DEV_ALLOC(ctx, s, sel_polys * kN * sizeof(u64));
-    HIP_TRY(ctx, hipMemcpy(s.p, sel, sel_polys * kN * sizeof(u64), hipMemcpyHostToDevice));
+    DEV_ALLOC(ctx, s, sel_polys * abi_kn(ctx) * sizeof(u64));
+    HIP_TRY(ctx, hipMemcpy(s.p, sel, sel_polys * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
     launch_split_u64((u64 *)s.p, db->sel, sel_polys, ctx->S, ctx->stream);
   }
   u32 *res = run_other_dim(db, 1, db->sel);
   DevBuf o;
-  DEV_ALLOC(ctx, o, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, o, 2 * abi_kn(ctx) * sizeof(u64));
   launch_ct_to_coef_u64(res, (u64 *)o.p, 1, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_TRY(ctx, hipMemcpy(out, o.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(out, o.p, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyDeviceToHost));
   return OPIR_OK;
 }
 
@@ -890,26 +910,26 @@ int opir_ext_prod(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, const 
   if (!ctx || !gsw || !ct || !out) return OPIR_EINVAL;
   if (ctx->k2) return k2_ext_prod(ctx, gsw, l, b, ct, out);
   if (l == 0 || l > 64 || b == 0 || b > 32) return set_err(ctx, OPIR_EINVAL, "invalid gadget");
-  int rc = validate_mod(ctx, gsw, 2 * l * 2 * kN, ctx->S.Q, "RGSW");
+  int rc = validate_polys(ctx, gsw, 2 * l * 2, "RGSW");
   if (rc) return rc;
-  rc = validate_mod(ctx, ct, 2 * kN, ctx->S.Q, "ciphertext");
+  rc = validate_polys(ctx, ct, 2, "ciphertext");
   if (rc) return rc;
   DevBuf g64, g32, c64, c32, o32, o64;
-  DEV_ALLOC(ctx, g64, 2 * l * 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, g64, 2 * l * 2 * abi_kn(ctx) * sizeof(u64));
   DEV_ALLOC(ctx, g32, 2 * l * kCtWords * sizeof(u32));
-  DEV_ALLOC(ctx, c64, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, c64, 2 * abi_kn(ctx) * sizeof(u64));
   DEV_ALLOC(ctx, c32, kCtWords * sizeof(u32));
   DEV_ALLOC(ctx, o32, kCtWords * sizeof(u32));
-  DEV_ALLOC(ctx, o64, 2 * kN * sizeof(u64));
-  HIP_TRY(ctx, hipMemcpy(g64.p, gsw, 2 * l * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
-  HIP_TRY(ctx, hipMemcpy(c64.p, ct, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  DEV_ALLOC(ctx, o64, 2 * abi_kn(ctx) * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(g64.p, gsw, 2 * l * 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(c64.p, ct, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
   launch_split_u64((u64 *)g64.p, (u32 *)g32.p, 2 * l * 2, ctx->S, ctx->stream);
   launch_split_u64((u64 *)c64.p, (u32 *)c32.p, 2, ctx->S, ctx->stream);
   launch_ext_prod((u32 *)g32.p, (int)l, (int)b, (u32 *)c32.p, (u32 *)o32.p, ctx->S, ctx->stream);
   launch_join_u64((u32 *)o32.p, (u64 *)o64.p, 2, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyDeviceToHost));
   return OPIR_OK;
 }
 
@@ -918,33 +938,34 @@ int opir_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t b, co
   if (!ctx || !gsw || !x || !y || !out) return OPIR_EINVAL;
   if (ctx->k2) return k2_ext_prod_mux(ctx, gsw, l, b, x, y, out);
   if (l == 0 || l > 64 || b == 0 || b > 32) return set_err(ctx, OPIR_EINVAL, "invalid gadget");
-  int rc = validate_mod(ctx, gsw, 2 * l * 2 * kN, ctx->S.Q, "RGSW");
+  int rc = validate_polys(ctx, gsw, 2 * l * 2, "RGSW");
   if (rc) return rc;
-  if ((rc = validate_mod(ctx, x, 2 * kN, ctx->S.Q, "ciphertext x"))) return rc;
-  if ((rc = validate_mod(ctx, y, 2 * kN, ctx->S.Q, "ciphertext y"))) return rc;
+  if ((rc = validate_polys(ctx, x, 2, "ciphertext x"))) return rc;
+  if ((rc = validate_polys(ctx, y, 2, "ciphertext y"))) return rc;
   DevBuf g64, g32, xd, yd, c32, o32, o64;
-  DEV_ALLOC(ctx, g64, 2 * l * 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, g64, 2 * l * 2 * abi_kn(ctx) * sizeof(u64));
   DEV_ALLOC(ctx, g32, 2 * l * kCtWords * sizeof(u32));
-  DEV_ALLOC(ctx, xd, 2 * kN * sizeof(u64));
-  DEV_ALLOC(ctx, yd, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, xd, 2 * abi_kn(ctx) * sizeof(u64));
+  DEV_ALLOC(ctx, yd, 2 * abi_kn(ctx) * sizeof(u64));
   DEV_ALLOC(ctx, c32, kCtWords * sizeof(u32));
   DEV_ALLOC(ctx, o32, kCtWords * sizeof(u32));
-  DEV_ALLOC(ctx, o64, 2 * kN * sizeof(u64));
-  HIP_TRY(ctx, hipMemcpy(g64.p, gsw, 2 * l * 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
-  HIP_TRY(ctx, hipMemcpy(xd.p, x, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
-  HIP_TRY(ctx, hipMemcpy(yd.p, y, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
-  const u64 Q = ctx->S.Q;
-  // y - x mod Q, external product (NTT-form output), INTT, + x (server.cpp:696-721)
-  launch_addsub_mod((u64 *)yd.p, (u64 *)xd.p, (u64 *)yd.p, 2 * kN, kN, 1, Q, Q, true, ctx->stream);
+  DEV_ALLOC(ctx, o64, 2 * abi_kn(ctx) * sizeof(u64));
+  HIP_TRY(ctx, hipMemcpy(g64.p, gsw, 2 * l * 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(xd.p, x, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(yd.p, y, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
+  // per ABI limb: Q (composite) or the two primes (MP)
+  const u64 m0 = ctx->S.mp ? ctx->S.limb[0].q : ctx->S.Q, m1 = ctx->S.mp ? ctx->S.limb[1].q : ctx->S.Q;
+  // y - x, external product (NTT-form output), INTT, + x (server.cpp:696-721)
+  launch_addsub_mod((u64 *)yd.p, (u64 *)xd.p, (u64 *)yd.p, 2 * abi_kn(ctx), kN, ctx->K, m0, m1, true, ctx->stream);
   launch_split_u64((u64 *)g64.p, (u32 *)g32.p, 2 * l * 2, ctx->S, ctx->stream);
   launch_split_u64((u64 *)yd.p, (u32 *)c32.p, 2, ctx->S, ctx->stream);
   launch_ext_prod((u32 *)g32.p, (int)l, (int)b, (u32 *)c32.p, (u32 *)o32.p, ctx->S, ctx->stream);
   launch_join_u64((u32 *)o32.p, (u64 *)o64.p, 2, ctx->S, ctx->stream);
   launch_ntt_u64((u64 *)o64.p, 2, true, ctx->S, ctx->stream);
-  launch_addsub_mod((u64 *)xd.p, (u64 *)o64.p, (u64 *)o64.p, 2 * kN, kN, 1, Q, Q, false, ctx->stream);
+  launch_addsub_mod((u64 *)xd.p, (u64 *)o64.p, (u64 *)o64.p, 2 * abi_kn(ctx), kN, ctx->K, m0, m1, false, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyDeviceToHost));
   return OPIR_OK;
 }
 
@@ -957,51 +978,44 @@ int opir_bv_ks(opir_ctx *ctx, const uint64_t *ct, uint32_t galois_k, const uint6
   for (size_t e = 0; e + 1 < N; e++) kinv = (unsigned)((kinv * (u64)galois_k) & (2 * N - 1));
   if (ctx->k2) return k2_bv_ks(ctx, ct, (int)kinv, key, out);
   const size_t L = ctx->G.l_ks;
-  int rc = validate_mod(ctx, ct, 2 * kN, ctx->S.Q, "ciphertext");
+  int rc = validate_polys(ctx, ct, 2, "ciphertext");
   if (rc) return rc;
-  if ((rc = validate_mod(ctx, key, L * 2 * kN, ctx->S.Q, "BV key"))) return rc;
-  const u64 q1 = ctx->S.limb[0].q, q2 = ctx->S.limb[1].q;
-  // key in: [l_ks][a, b][N] mod Q; device: [l_ks][b (-> c0), a (-> c1)][2 limbs][N]
+  if ((rc = validate_polys(ctx, key, L * 2, "BV key"))) return rc;
+  // key in: [l_ks][a, b][K*N]; device: [l_ks][b (-> c0), a (-> c1)][2 limbs][N]
   std::vector<u32> hk(L * kCtWords);
   for (size_t r = 0; r < L; r++)
-    for (int ab = 0; ab < 2; ab++) {
-      const uint64_t *src = key + (r * 2 + ab) * kN;
-      u32 *dst = hk.data() + r * kCtWords + (size_t)(ab == 0 ? 1 : 0) * 2 * kN;
-      for (size_t j = 0; j < (size_t)kN; j++) {
-        dst[j] = (u32)(src[j] % q1);
-        dst[kN + j] = (u32)(src[j] % q2);
-      }
-    }
+    for (int ab = 0; ab < 2; ab++)
+      poly_to_rns(ctx, key + (r * 2 + ab) * abi_kn(ctx), hk.data() + r * kCtWords + (size_t)(ab == 0 ? 1 : 0) * 2 * kN);
   DevBuf kd, kp, c64, c32, o32, o64;
   DEV_ALLOC(ctx, kd, hk.size() * sizeof(u32));
   DEV_ALLOC(ctx, kp, sizeof(void *));
-  DEV_ALLOC(ctx, c64, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, c64, 2 * abi_kn(ctx) * sizeof(u64));
   DEV_ALLOC(ctx, c32, kCtWords * sizeof(u32));
   DEV_ALLOC(ctx, o32, kCtWords * sizeof(u32));
-  DEV_ALLOC(ctx, o64, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, o64, 2 * abi_kn(ctx) * sizeof(u64));
   HIP_TRY(ctx, hipMemcpy(kd.p, hk.data(), hk.size() * sizeof(u32), hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(kp.p, &kd.p, sizeof(void *), hipMemcpyHostToDevice));
-  HIP_TRY(ctx, hipMemcpy(c64.p, ct, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(c64.p, ct, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
   launch_split_u64((u64 *)c64.p, (u32 *)c32.p, 2, ctx->S, ctx->stream);
   launch_bv_apply((const u32 *)c32.p, (u32 *)o32.p, (const u32 *const *)kp.p, (int)kinv, ctx->G, ctx->S, ctx->stream);
   launch_join_u64((u32 *)o32.p, (u64 *)o64.p, 2, ctx->S, ctx->stream);
   LAUNCH_CHECK(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * kN * sizeof(u64), hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(out, o64.p, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyDeviceToHost));
   return OPIR_OK;
 }
 
 int opir_mod_switch(opir_ctx *ctx, const uint64_t *ct, uint64_t *out) {
   if (!ctx || !ct || !out) return OPIR_EINVAL;
   if (ctx->k2) return k2_mod_switch(ctx, ct, out);
-  int rc = validate_mod(ctx, ct, 2 * kN, ctx->S.Q, "ciphertext");
+  int rc = validate_polys(ctx, ct, 2, "ciphertext");
   if (rc) return rc;
   DevBuf c64, c32, n32, o64;
-  DEV_ALLOC(ctx, c64, 2 * kN * sizeof(u64));
+  DEV_ALLOC(ctx, c64, 2 * abi_kn(ctx) * sizeof(u64));
   DEV_ALLOC(ctx, c32, kCtWords * sizeof(u32));
   DEV_ALLOC(ctx, n32, kCtWords * sizeof(u32));
   DEV_ALLOC(ctx, o64, 2 * kN * sizeof(u64));
-  HIP_TRY(ctx, hipMemcpy(c64.p, ct, 2 * kN * sizeof(u64), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(c64.p, ct, 2 * abi_kn(ctx) * sizeof(u64), hipMemcpyHostToDevice));
   launch_split_u64((u64 *)c64.p, (u32 *)c32.p, 2, ctx->S, ctx->stream);
   launch_ct_ntt((u32 *)c32.p, (u32 *)n32.p, 0, 0, 1, 1, ctx->S, ctx->stream);
   launch_final((u32 *)n32.p, (u64 *)o64.p, 0, 1, ctx->S, ctx->stream);

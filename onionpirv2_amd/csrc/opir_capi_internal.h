@@ -44,7 +44,8 @@ struct opir_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr, stream = nullptr;
   opir_scheme sch{};
-  bool k2 = false;          // two-prime scheme (CONFIG_N4096_K2_MP)
+  bool k2 = false;          // two 60-bit primes at N = 4096 (CONFIG_N4096_K2_MP, opir_capi_k2.hip)
+  int K = 1;                // RNS limbs of a polynomial at the ABI (2: K2 schemes)
   size_t N = kN;            // ring degree
   SchemeConst S{};          // K = 1 constants
   GadgetCfg G{};

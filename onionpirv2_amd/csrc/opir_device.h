@@ -56,6 +56,11 @@ struct SchemeConst {
   u64 small_q;
   u64 q_recip;        // floor(2^k / Q), k = 63 + bit length of Q: Barrett reciprocal for the final rescale
   int q_shift;        // k - 64
+  // 0: composite Q = q1 * q2 seen as one modulus (CONFIG_N2048_K1_COMP);
+  // 1: the two primes are the ciphertext's own RNS limbs (CONFIG_N2048_K2_MP):
+  //    MP gadgets (src/gsw.cpp:22-85,184-250, src/bv_keyswitch.cpp:42-64) and
+  //    the drop-then-rescale mod switch (src/server.cpp:1044-1075)
+  int mp;
 };
 
 __device__ __forceinline__ u32 mul_shoup(u32 a, u32 w, u32 ws, u32 q) {
@@ -92,13 +97,25 @@ __device__ __forceinline__ u64 crt_compose(u32 lo, u32 hi, const SchemeConst &S)
 // Signed digit r in [-2^(b-1), 2^(b-1)) rendered mod q (q > 2^(b-1)).
 __device__ __forceinline__ u32 digit_mod(i64 r, u32 q) { return r >= 0 ? (u32)r : (u32)(r + (i64)q); }
 
-// Balanced digit extraction step (src/gsw.cpp:285-293, src/bv_keyswitch.cpp:32-39):
-// r = sign-extended low b bits of d, d <- (d - r) >> b.
-__device__ __forceinline__ i64 next_digit(i64 &d, int b) {
+// Digit extraction step, LSB-first: r = sext_b(d - delta) + delta, the
+// representative of d mod 2^b in [delta - 2^(b-1), delta + 2^(b-1)), then
+// d <- (d - r) >> b. delta = 0: balanced digits in [-B/2, B/2)
+// (src/gsw.cpp:285-293, src/bv_keyswitch.cpp:32-39); delta = 1: (-B/2, B/2]
+// (signed_gadget_decompose_mp, src/bv_keyswitch.cpp:42-64); delta = B/2: the
+// unsigned digits of d >= 0 (decomp_rlwe_mp, src/gsw.cpp:184-250).
+__device__ __forceinline__ i64 next_digit(i64 &d, int b, i64 delta = 0) {
   const int sh = 64 - b;
-  const i64 r = (i64)((u64)d << sh) >> sh;
+  const i64 r = ((i64)((u64)(d - delta) << sh) >> sh) + delta;
   d = (d - r) >> b;
   return r;
+}
+
+// The integer a gadget decomposes, from both limbs of one coefficient:
+// composite, or the MP BV gadget: the CRT value centred to (-Q/2, Q/2];
+// the MP data and key gadgets: the CRT value in [0, Q) itself.
+__device__ __forceinline__ i64 gadget_value(u32 lo, u32 hi, const SchemeConst &S, bool centred) {
+  const u64 x = crt_compose(lo, hi, S);
+  return centred ? (x > S.half_Q ? (i64)x - (i64)S.Q : (i64)x) : (i64)x;
 }
 
 // Centre a value in [0, Q) to (-Q/2, Q/2] the way the reference does

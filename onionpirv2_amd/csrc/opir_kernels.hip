@@ -24,14 +24,14 @@ struct Barrier {
 template <int NP, bool GSW_ROWS>
 __device__ __forceinline__ void gadget_mac(i64 (&D)[NP][8], int l, int b, const u32 *__restrict__ key, int sb,
                                            int t, const LimbConst &L, u32 *nttlds, u64 (&acc0)[8],
-                                           u64 (&acc1)[8]) {
+                                           u64 (&acc1)[8], i64 delta) {
   Barrier bar;
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     for (int i = 0; i < l; i++) {
       u32 v[8];
 #pragma unroll
-      for (int c = 0; c < 8; c++) v[c] = digit_mod(next_digit(D[p][c], b), L.q);
+      for (int c = 0; c < 8; c++) v[c] = digit_mod(next_digit(D[p][c], b, delta), L.q);
       ntt_fwd_2048(v, nttlds, t, L, bar);
       const int row = GSW_ROWS ? (p * l + (l - 1 - i)) : i;
       const u32 *r = key + (size_t)row * (4 * kN);
@@ -65,12 +65,18 @@ __device__ __forceinline__ void compose_centre(const u32 *lo, const u32 *hi, int
 // ---------------------------------------------------------------------------
 // Group blockIdx.y: npoly polynomials at in + y*in_stride (u64 words) ->
 // out + y*out_stride (u32 words).
+// Two-limb MP scheme: the u64 input already holds [npoly][2 limbs][N] residues.
 __global__ void k_split_u64(const u64 *__restrict__ in, u32 *__restrict__ out, size_t npoly, size_t in_stride,
                             size_t out_stride, SchemeConst S) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npoly * kN) return;
   in += blockIdx.y * in_stride;
   out += blockIdx.y * out_stride;
+  if (S.mp) {
+    out[2 * i] = (u32)in[2 * i];
+    out[2 * i + 1] = (u32)in[2 * i + 1];
+    return;
+  }
   const size_t p = i / kN, j = i % kN;
   const u64 x = in[i];
   out[(2 * p + 0) * kN + j] = reduce_u64(x, S.limb[0]);
@@ -80,6 +86,11 @@ __global__ void k_split_u64(const u64 *__restrict__ in, u32 *__restrict__ out, s
 __global__ void k_join_u64(const u32 *__restrict__ in, u64 *__restrict__ out, size_t npoly, SchemeConst S) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npoly * kN) return;
+  if (S.mp) {  // [npoly][2 limbs][N] residues out
+    out[2 * i] = in[2 * i];
+    out[2 * i + 1] = in[2 * i + 1];
+    return;
+  }
   const size_t p = i / kN, j = i % kN;
   out[i] = crt_compose(in[(2 * p) * kN + j], in[(2 * p + 1) * kN + j], S);
 }
@@ -110,9 +121,19 @@ __global__ __launch_bounds__(512) void k_ntt_u64(u64 *data, int inverse, SchemeC
   __shared__ u32 res[2][kN];
   const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
   const LimbConst &L = S.limb[sb];
-  u64 *poly = data + (size_t)blockIdx.x * kN;
+  u64 *poly = data + (size_t)blockIdx.x * (S.mp ? 2 : 1) * kN;
   u32 v[8];
   Barrier bar;
+  if (S.mp) {  // each lane block transforms its own limb's residues in place
+    u64 *lp = poly + (size_t)sb * kN;
+#pragma unroll
+    for (int c = 0; c < 8; c++) v[c] = reduce_u64(lp[inverse ? map3(t, c) : map0(t, c)], L);
+    if (!inverse) ntt_fwd_2048(v, nlds[sb], t, L, bar);
+    else ntt_inv_2048(v, nlds[sb], t, L, bar);
+#pragma unroll
+    for (int c = 0; c < 8; c++) lp[inverse ? map0(t, c) : map3(t, c)] = v[c];
+    return;
+  }
   if (!inverse) {
 #pragma unroll
     for (int c = 0; c < 8; c++) v[c] = reduce_u64(poly[map0(t, c)], L);
@@ -236,6 +257,22 @@ __device__ __forceinline__ u64 rescale_to_small(u64 x, const SchemeConst &S) {
   return neg ? (quo ? S.small_q - quo : 0) : quo;
 }
 
+// Two-limb MP scheme (server.cpp:1044-1075): drop the second limb's prime
+// with round half up, x = CRT(r0, r1) in [0, q0 q1) -> floor((x + floor(q1/2))
+// / q1) reduced into [0, q0), then the centred rescale q0 -> small_q
+// (utils.cpp:445-462). Both limbs are below 2^30, so every value fits u64.
+__device__ __forceinline__ u64 drop_and_rescale(u32 r0, u32 r1, const SchemeConst &S) {
+  const u64 q0 = S.limb[0].q, q1 = S.limb[1].q, sq = S.small_q;
+  const u64 x = crt_compose(r0, r1, S);
+  u64 o = (x + (q1 >> 1)) / q1;
+  if (o >= q0) o -= q0;
+  const u64 h = q0 / 2;
+  const bool neg = o >= h;  // v >= inp/2 -> v - inp
+  const u64 mag = neg ? q0 - o : o;
+  const u64 quo = (mag * sq + h) / q0;  // mag <= q0/2 keeps quo below small_q
+  return neg ? (quo ? sq - quo : 0) : quo;
+}
+
 __global__ __launch_bounds__(512) void k_final(const u32 *res, u64 *out, size_t res_stride, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 sh[2][kN];
@@ -248,7 +285,10 @@ __global__ __launch_bounds__(512) void k_final(const u32 *res, u64 *out, size_t 
   store_map0(sh[sb], v, t);
   __syncthreads();
   u64 *o = out + ((size_t)qi * 2 + p) * kN;
-  for (int j = threadIdx.x; j < kN; j += 512) o[j] = rescale_to_small(crt_compose(sh[0][j], sh[1][j], S), S);
+  if (S.mp)
+    for (int j = threadIdx.x; j < kN; j += 512) o[j] = drop_and_rescale(sh[0][j], sh[1][j], S);
+  else
+    for (int j = threadIdx.x; j < kN; j += 512) o[j] = rescale_to_small(crt_compose(sh[0][j], sh[1][j], S), S);
 }
 
 // NTT-form RNS cts -> coefficient-form u64 mod Q. grid (count*2).
@@ -263,6 +303,13 @@ __global__ __launch_bounds__(512) void k_ct_to_coef_u64(const u32 *in, u64 *out,
   ntt_inv_2048(v, nlds[sb], t, L, Barrier());
   store_map0(sh[sb], v, t);
   __syncthreads();
+  if (S.mp) {  // [count][2 polys][2 limbs][N] residues
+    for (int j = threadIdx.x; j < kN; j += 512) {
+      out[(poly * 2 + 0) * kN + j] = sh[0][j];
+      out[(poly * 2 + 1) * kN + j] = sh[1][j];
+    }
+    return;
+  }
   for (int j = threadIdx.x; j < kN; j += 512) out[poly * kN + j] = crt_compose(sh[0][j], sh[1][j], S);
 }
 
@@ -278,12 +325,12 @@ __global__ __launch_bounds__(512) void k_ext_prod(const u32 *gsw, int l, int b, 
 #pragma unroll
     for (int c = 0; c < 8; c++) {
       const int j = map0(t, c);
-      D[p][c] = centre(crt_compose(ct[(p * 2 + 0) * kN + j], ct[(p * 2 + 1) * kN + j], S), S);
+      D[p][c] = gadget_value(ct[(p * 2 + 0) * kN + j], ct[(p * 2 + 1) * kN + j], S, !S.mp);
     }
   u64 acc0[8], acc1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
-  gadget_mac<2, true>(D, l, b, gsw, sb, t, L, nlds[sb], acc0, acc1);
+  gadget_mac<2, true>(D, l, b, gsw, sb, t, L, nlds[sb], acc0, acc1, S.mp ? (i64)1 << (b - 1) : 0);
   u32 o0[8], o1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) {

@@ -73,22 +73,22 @@ extern "C" int opir_ext_prod_mux_device_noalloc(opir_ctx *ctx, const uint64_t *d
     return set_err(ctx, OPIR_EINVAL, "selector gadget of " + std::to_string(l) + " rows exceeds the " +
                                          std::to_string(kMaxL1) + "-row scratch limit");
   // scratch: diff (u64 ct) | o64 (u64 ct) | c32 | o32 | g32 (2l RNS rows)
-  const size_t ct64 = 2 * (size_t)kN;
+  const size_t ct64 = 2 * (size_t)ctx->K * kN;  // u64 values of an ABI ciphertext
   u64 *diff = reinterpret_cast<u64 *>(s);
   u64 *o64 = diff + ct64;
   u32 *c32 = reinterpret_cast<u32 *>(o64 + ct64);
   u32 *o32 = c32 + kCtWords;
   u32 *g32 = o32 + kCtWords;
-  static_assert(2 * 2 * (size_t)kN * sizeof(u64) + (2 + 2 * kMaxL1) * kCtWords * sizeof(u32) <= kScratchBytes,
-                "scratch too small for the composite layout");
-  const u64 Q = ctx->S.Q;
-  launch_addsub_mod(d_y, d_x, diff, ct64, kN, 1, Q, Q, true, st);
+  static_assert(2 * 4 * (size_t)kN * sizeof(u64) + (2 + 2 * kMaxL1) * kCtWords * sizeof(u32) <= kScratchBytes,
+                "scratch too small for the 32-bit limb layouts");
+  const u64 m0 = ctx->S.mp ? ctx->S.limb[0].q : ctx->S.Q, m1 = ctx->S.mp ? ctx->S.limb[1].q : ctx->S.Q;
+  launch_addsub_mod(d_y, d_x, diff, ct64, kN, ctx->K, m0, m1, true, st);
   launch_split_u64(d_gsw, g32, 2 * l * 2, ctx->S, st);
   launch_split_u64(diff, c32, 2, ctx->S, st);
   launch_ext_prod(g32, (int)l, (int)b, c32, o32, ctx->S, st);
   launch_join_u64(o32, o64, 2, ctx->S, st);
   launch_ntt_u64(o64, 2, true, ctx->S, st);
-  launch_addsub_mod(d_x, o64, d_out, ct64, kN, 1, Q, Q, false, st);
+  launch_addsub_mod(d_x, o64, d_out, ct64, kN, ctx->K, m0, m1, false, st);
   LAUNCH_CHECK(ctx);
   return OPIR_OK;
 }

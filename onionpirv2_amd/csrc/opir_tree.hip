@@ -33,24 +33,35 @@ inline int split_groups(int l) {
   return (l + dg - 1) / dg;
 }
 
-// Digit i of the balanced base-2^b expansion of D, as a residue mod q:
-// sext_b(floor((D + H_i) / 2^(b*i))), H_i = 2^(b-1) * sum_{j<i} 2^(b*j).
-__device__ __forceinline__ i64 digit_offset(int i, int b) {
+// Digit i of the base-2^b expansion of D whose digits lie in
+// [delta - 2^(b-1), delta + 2^(b-1)) (next_digit's rule), as a residue mod q:
+// sext_b(floor((D + H_i) / 2^(b*i)) - delta) + delta with
+// H_i = (2^(b-1) - delta) * sum_{j<i} 2^(b*j).
+__device__ __forceinline__ i64 digit_offset(int i, int b, i64 delta) {
   i64 H = 0;
-  for (int j = 0; j < i; j++) H += ((i64)1 << (b - 1)) << (b * j);
+  for (int j = 0; j < i; j++) H += (((i64)1 << (b - 1)) - delta) << (b * j);
   return H;
 }
-__device__ __forceinline__ u32 digit_residue(i64 D, i64 H, int shift, int b, u32 q) {
-  const int r = __builtin_amdgcn_sbfe((int)(u32)((D + H) >> shift), 0, b);
+__device__ __forceinline__ u32 digit_residue(i64 D, i64 H, int shift, int b, u32 q, int delta) {
+  const int r = __builtin_amdgcn_sbfe((int)(u32)((D + H) >> shift) - delta, 0, b) + delta;
   return (u32)r + (q & (u32)(r >> 31));
 }
 
+// Per-gadget digit rule: the composite scheme centres and takes balanced
+// digits (delta 0) everywhere; the MP scheme's BV gadget centres with ties to
+// +B/2 (delta 1), its data and key gadgets take unsigned digits of [0, Q)
+// (delta B/2).
+template <bool MP>
+__device__ __forceinline__ int ks_delta() { return MP ? 1 : 0; }
+template <bool MP>
+__device__ __forceinline__ int sel_delta(int b) { return MP ? 1 << (b - 1) : 0; }
+
 __device__ __forceinline__ void compose_centre8(const u32 *lo, const u32 *hi, int t, const SchemeConst &S,
-                                                i64 (&D)[8]) {
+                                                i64 (&D)[8], bool centred = true) {
 #pragma unroll
   for (int c = 0; c < 8; c++) {
     const int j = map0(t, c);
-    D[c] = centre(crt_compose(lo[j], hi[j], S), S);
+    D[c] = gadget_value(lo[j], hi[j], S, centred);
   }
 }
 
@@ -62,7 +73,7 @@ __device__ __forceinline__ void compose_centre8(const u32 *lo, const u32 *hi, in
 template <bool kKeyFirst = false>
 __device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int l, int b, bool rev, int row_base,
                                           const u32 *__restrict__ key, int limb, int t_in, const LimbConst &L,
-                                          u32 *nttlds, u64 (&acc0)[8], u64 (&acc1)[8]) {
+                                          u32 *nttlds, u64 (&acc0)[8], u64 (&acc1)[8], int delta) {
   Barrier bar;
 #pragma unroll 1
   for (int i = i0; i < i1; i++) {
@@ -70,7 +81,7 @@ __device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int
     // inside the loop; hoisted, they do not fit in 128 registers.
     int t = t_in;
     asm volatile("" : "+v"(t));
-    const i64 H = digit_offset(i, b);
+    const i64 H = digit_offset(i, b, delta);
     // key row: [2 outputs][2 limbs][N] words; byte offsets fit in 32 bits
     const unsigned row = (unsigned)(row_base + (rev ? l - 1 - i : i));
     const unsigned off = row * (unsigned)(kCtWords * 4) + (unsigned)limb * (kN * 4) + ((unsigned)t << 5);
@@ -81,7 +92,7 @@ __device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int
     }
     u32 v[8];
 #pragma unroll
-    for (int c = 0; c < 8; c++) v[c] = digit_residue(D[c], H, b * i, b, L.q);
+    for (int c = 0; c < 8; c++) v[c] = digit_residue(D[c], H, b * i, b, L.q, delta);
     ntt_fwd_2048<false>(v, nttlds, t, L, bar);  // [0, 4q): at most 10 products of < 2^60 per sum
     if (!kKeyFirst) {
       a0 = ld_off<uint4>(key, off), a1 = ld_off<uint4>(key, off + 16);
@@ -106,6 +117,7 @@ __device__ __forceinline__ void digit_mac(const i64 (&D)[8], int i0, int i1, int
 // Wide levels: one 512-thread workgroup per node (lane block = limb), all
 // l_ks digits, then delta_b and delta_a in turn; two nodes share a CU.
 // ---------------------------------------------------------------------------
+template <bool MP>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_expand_serial(ExpandArgs A,
                                                                                                SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
@@ -138,7 +150,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     compose_centre8(sig[0], sig[1], t, S, D);
     __syncthreads();  // sig is read; the digit transforms reuse its LDS
     const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
-    digit_mac<true>(D, 0, A.l_ks, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[limb], accb, acca);
+    digit_mac<true>(D, 0, A.l_ks, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[limb], accb, acca, ks_delta<MP>());
   }
   u32 *left = A.out + qi * A.out_stride + (size_t)(2 * node) * kCtWords;
   u32 *right = left + kCtWords;
@@ -191,6 +203,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 // qi * (nnodes * G) + node * G + g; k_expand_fin adds them up, runs the INTTs
 // and forms both children as k_expand_level does.
 // ---------------------------------------------------------------------------
+template <bool MP>
 __global__ __launch_bounds__(512) void k_expand_part(ExpandArgs A, u32 *__restrict__ part, int G, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 c1[2][kN];  // c1 per limb
@@ -222,7 +235,7 @@ __global__ __launch_bounds__(512) void k_expand_part(ExpandArgs A, u32 *__restri
   for (int cc = 0; cc < 8; cc++) accb[cc] = acca[cc] = 0;
   const int dg = (A.l_ks + G - 1) / G, i0 = g * dg, i1 = min(A.l_ks, i0 + dg);
   const u32 *key = A.bv_keys[qi] + (size_t)A.level * A.key_level_words;
-  digit_mac<true>(D, i0, i1, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[limb], accb, acca);
+  digit_mac<true>(D, i0, i1, A.l_ks, A.b_ks, false, 0, key, limb, t, L, nlds[limb], accb, acca, ks_delta<MP>());
   u32 ob[8], oa[8];
 #pragma unroll
   for (int cc = 0; cc < 8; cc++) {
@@ -309,6 +322,7 @@ __global__ __launch_bounds__(256) void k_expand_fin(ExpandArgs A, const u32 *__r
 // NTT(row); bottom row = external_product(RGSW(s), row), key gadget.
 // Lane block (limb, p): poly p of the row, all l_key digits of it.
 // ---------------------------------------------------------------------------
+template <bool MP>
 __global__ __launch_bounds__(1024) void k_complete(CompleteArgs A, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[4][kLdsPoly];
   __shared__ u32 sh[2][2][kN];
@@ -325,11 +339,12 @@ __global__ __launch_bounds__(1024) void k_complete(CompleteArgs A, SchemeConst S
   ntt_fwd_2048(v, nlds[lb], t, L, Barrier());  // its barriers also publish sh
   store_map3(selb + (size_t)r * kCtWords + (p * 2 + limb) * kN, v, t);
   i64 D[8];
-  compose_centre8(sh[p][0], sh[p][1], t, S, D);
+  compose_centre8(sh[p][0], sh[p][1], t, S, D, !MP);
   u64 acc0[8], acc1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
-  digit_mac<true>(D, 0, A.l_key, A.l_key, A.b_key, true, p * A.l_key, A.gsw_keys[qi], limb, t, L, nlds[lb], acc0, acc1);
+  digit_mac<true>(D, 0, A.l_key, A.l_key, A.b_key, true, p * A.l_key, A.gsw_keys[qi], limb, t, L, nlds[lb], acc0, acc1,
+                  sel_delta<MP>(A.b_key));
   u32 o0[8], o1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -360,6 +375,7 @@ __global__ __launch_bounds__(1024) void k_complete(CompleteArgs A, SchemeConst S
 // The same row with one 512-thread workgroup (lane block = limb) that handles
 // poly p = 0 and then p = 1. The sum is reduced between the two halves: each
 // half adds l_key products of < 2^60.
+template <bool MP>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_complete_serial(CompleteArgs A,
                                                                                                  SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
@@ -381,7 +397,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     ntt_fwd_2048(v, nlds[limb], t, L, Barrier());  // its barriers also publish sh
     store_map3(selb + (size_t)r * kCtWords + (p * 2 + limb) * kN, v, t);
     i64 D[8];
-    compose_centre8(sh[0], sh[1], t, S, D);
+    compose_centre8(sh[0], sh[1], t, S, D, !MP);
     if (p == 1) {
 #pragma unroll
       for (int c = 0; c < 8; c++) {
@@ -390,7 +406,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
       }
     }
     digit_mac<true>(D, 0, A.l_key, A.l_key, A.b_key, true, p * A.l_key, A.gsw_keys[qi], limb, t, L, nlds[limb], acc0,
-                    acc1);
+                    acc1, sel_delta<MP>(A.b_key));
   }
   u32 o0[8], o1[8];
 #pragma unroll
@@ -423,6 +439,7 @@ __device__ __forceinline__ void mux_operands(const MuxArgs &A, int m, int &xi, i
 // handles poly p = 0 and then p = 1 (INTT of poly p of y - x, then its l
 // digits), accumulating both into one sum; two nodes share a CU and their
 // barrier waits overlap.
+template <bool MP>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_mux_serial(MuxArgs A, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 sh[2][kN];
@@ -454,9 +471,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     store_map0(sh[limb], v, t);
     __syncthreads();
     i64 D[8];
-    compose_centre8(sh[0], sh[1], t, S, D);
+    compose_centre8(sh[0], sh[1], t, S, D, !MP);
     digit_mac<true>(D, 0, A.l, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[limb], acc0,
-                    acc1);
+                    acc1, sel_delta<MP>(A.b));
   }
   u32 x0[8], x1[8];
   load_map3(x0, x + (0 * 2 + limb) * kN, t);
@@ -481,6 +498,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
 //   qi * (nmux * 2G) + (m * 2 + p) * G + g, one ciphertext each.
 // ---------------------------------------------------------------------------
 
+template <bool MP>
 __global__ __launch_bounds__(512) void k_mux_part(MuxArgs A, u32 *__restrict__ part, int G, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 sh[2][kN];
@@ -501,13 +519,13 @@ __global__ __launch_bounds__(512) void k_mux_part(MuxArgs A, u32 *__restrict__ p
   store_map0(sh[limb], v, t);
   __syncthreads();
   i64 D[8];
-  compose_centre8(sh[0], sh[1], t, S, D);
+  compose_centre8(sh[0], sh[1], t, S, D, !MP);
   u64 acc0[8], acc1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
   const int dg = (A.l + G - 1) / G, i0 = g * dg, i1 = min(A.l, i0 + dg);
   digit_mac<true>(D, i0, i1, A.l, A.b, true, p * A.l, A.sel + qi * A.sel_stride, limb, t, L, nlds[limb], acc0,
-                  acc1);
+                  acc1, sel_delta<MP>(A.b));
   u32 o0[8], o1[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -569,12 +587,14 @@ void launch_expand_level(const u32 *in, u32 *out, size_t in_stride, size_t out_s
   A.l_ks = G.l_ks; A.b_ks = G.b_ks;
   const int Gs = split_groups(G.l_ks);
   if (part && (size_t)nnodes * Gs * B <= kMuxSplitMaxBlocks) {
-    k_expand_part<<<dim3(nnodes * Gs, B), 512, 0, st>>>(A, part, Gs, S);
+    if (S.mp) k_expand_part<true><<<dim3(nnodes * Gs, B), 512, 0, st>>>(A, part, Gs, S);
+    else k_expand_part<false><<<dim3(nnodes * Gs, B), 512, 0, st>>>(A, part, Gs, S);
     k_expand_fin<<<dim3(nnodes * 4, B), 256, 0, st>>>(A, part, Gs, nnodes, S);
     return;
   }
   dim3 grid(nnodes, B);
-  k_expand_serial<<<grid, 512, 0, st>>>(A, S);
+  if (S.mp) k_expand_serial<true><<<grid, 512, 0, st>>>(A, S);
+  else k_expand_serial<false><<<grid, 512, 0, st>>>(A, S);
 }
 
 void launch_bv_apply(const u32 *in, u32 *out, const u32 *const *key_ptr, int kinv, const GadgetCfg &G,
@@ -587,7 +607,8 @@ void launch_bv_apply(const u32 *in, u32 *out, const u32 *const *key_ptr, int kin
   A.kinv = kinv;
   A.l_ks = G.l_ks; A.b_ks = G.b_ks;
   A.bv_only = 1;
-  k_expand_serial<<<dim3(1, 1), 512, 0, st>>>(A, S);
+  if (S.mp) k_expand_serial<true><<<dim3(1, 1), 512, 0, st>>>(A, S);
+  else k_expand_serial<false><<<dim3(1, 1), 512, 0, st>>>(A, S);
 }
 
 void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
@@ -598,9 +619,11 @@ void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel
   A.gsw_keys = P.gsw_keys; A.l_ep = G.l_ep; A.l_key = G.l_key; A.b_key = G.b_key;
   dim3 grid(nrows, B);
   if ((size_t)nrows * B >= 256)  // enough rows to fill the device twice over
-    k_complete_serial<<<grid, 512, 0, st>>>(A, S);
+    if (S.mp) k_complete_serial<true><<<grid, 512, 0, st>>>(A, S);
+    else k_complete_serial<false><<<grid, 512, 0, st>>>(A, S);
   else
-    k_complete<<<grid, 1024, 0, st>>>(A, S);
+    if (S.mp) k_complete<true><<<grid, 1024, 0, st>>>(A, S);
+    else k_complete<false><<<grid, 1024, 0, st>>>(A, S);
 }
 
 void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, int half, int B,
@@ -610,7 +633,8 @@ void launch_mux(const u32 *in, u32 *out, size_t stride, int nslots, int offset, 
   A.in = in; A.out = out; A.stride = stride; A.sel = sel; A.sel_stride = sel_stride;
   A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
   dim3 grid(nslots, B);
-  k_mux_serial<<<grid, 512, 0, st>>>(A, S);
+  if (S.mp) k_mux_serial<true><<<grid, 512, 0, st>>>(A, S);
+  else k_mux_serial<false><<<grid, 512, 0, st>>>(A, S);
 }
 
 int mux_split_groups(int nslots, int offset, int B, const GadgetCfg &G) {
@@ -633,7 +657,8 @@ void launch_mux_split(const u32 *in, u32 *out, size_t stride, int nslots, int of
   A.in = in; A.out = out; A.stride = stride; A.sel = sel; A.sel_stride = sel_stride;
   A.offset = offset; A.half = half; A.l = G.l_ep; A.b = G.b_ep;
   const int nmux = offset >= 0 ? nslots - offset : nslots;
-  k_mux_part<<<dim3(nmux * 2 * Gs, B), 512, 0, st>>>(A, part, Gs, S);
+  if (S.mp) k_mux_part<true><<<dim3(nmux * 2 * Gs, B), 512, 0, st>>>(A, part, Gs, S);
+  else k_mux_part<false><<<dim3(nmux * 2 * Gs, B), 512, 0, st>>>(A, part, Gs, S);
   k_mux_fin<<<dim3(8 * nslots, B), 256, 0, st>>>(A, part, Gs, nmux, S);
 }
 

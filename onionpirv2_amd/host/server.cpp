@@ -29,7 +29,7 @@ GpuScheme::GpuScheme(const PirParams &p, int device) : params_(p) {
     s.ntt_root = utils::minimal_primitive_root(2 * N, s.q1);
     s.ntt_root2 = utils::minimal_primitive_root(2 * N, s.q2);
   } else {
-    throw std::invalid_argument("the GPU path runs CONFIG_N2048_K1_COMP and CONFIG_N4096_K2_MP");
+    throw std::invalid_argument("the GPU path runs CONFIG_N2048_K1_COMP, CONFIG_N2048_K2_MP and CONFIG_N4096_K2_MP");
   }
   s.plain_mod = p.get_plain_mod();
   s.small_q = p.get_small_q();

@@ -1,5 +1,9 @@
-"""GPU parity of the two-prime path (CONFIG_N4096_K2_MP: N = 4096, two ~60-bit
-RNS limbs) against the CPU oracle, bit-exact, through the C ABI.
+"""GPU parity of the two-prime schemes against the CPU oracle, bit-exact,
+through the C ABI. Every test runs for both:
+  CONFIG_N4096_K2_MP: N = 4096, two ~60-bit RNS limbs (u64 kernels);
+  CONFIG_N2048_K2_MP: N = 2048, two 29-bit RNS limbs (the 32-bit kernels of
+    the composite scheme with the MP gadgets and the drop-then-rescale mod
+    switch, database_constants.h:96-109).
 
 Layouts (with_layout, K = 2 gadgets l_ep = 5):
   (43, 5)        N0=16,  dims=3, Nrest=3   ragged deepest level (offset 1)
@@ -23,9 +27,17 @@ def say(*a):
     sys.stdout.flush()
 
 
+K2_CONFIGS = ["CONFIG_N4096_K2_MP", "CONFIG_N2048_K2_MP"]
+
+
+@pytest.fixture(scope="module", params=K2_CONFIGS)
+def k2name(request):
+    return request.param
+
+
 @pytest.fixture(scope="module")
-def k2p(O):
-    return O.PirParams("CONFIG_N4096_K2_MP")
+def k2p(O, k2name):
+    return O.PirParams(k2name)
 
 
 def scheme_k2(O, p):
@@ -36,9 +48,9 @@ def scheme_k2(O, p):
 
 
 @pytest.fixture(scope="module")
-def ctx2(O, k2p):
+def ctx2(O, k2p, k2name):
     from onionpirv2_amd.gpu import GpuContext
-    say("[gpu] creating K=2 context")
+    say(f"[gpu] creating K=2 context ({k2name})")
     ctx = GpuContext(scheme_k2(O, k2p))
     yield ctx
     ctx.close()
@@ -113,7 +125,7 @@ def test_k2_external_product_matches_oracle(O, k2p, ctx2, which):
 
 
 @pytest.mark.parametrize("sel", ["one", "random"])
-def test_k2_ext_prod_mux_matches_oracle(O, k2p, ctx2, sel):
+def test_k2_ext_prod_mux_matches_oracle(O, k2p, ctx2, k2name, sel):
     """PirServer::ext_prod_mux (src/server.cpp:652-722) with two 60-bit limbs,
     through the C ABI and the host PirServer, bit-exact against the oracle."""
     import onionpirv2_amd.host as H
@@ -137,7 +149,7 @@ def test_k2_ext_prod_mux_matches_oracle(O, k2p, ctx2, sel):
     x, y = x.reshape(2, -1), y.reshape(2, -1)
     exp = O.PirServer(p.with_layout(43, 5)).ext_prod_mux(x.copy(), y.copy(), gsw)
     assert np.array_equal(ctx2.ext_prod_mux(gsw, l, b, x, y), exp)
-    hp = H.PirParams("CONFIG_N4096_K2_MP").with_layout(43, 5)
+    hp = H.PirParams(k2name).with_layout(43, 5)
     assert np.array_equal(H.PirServer(hp).ext_prod_mux(x, y, gsw), exp)
 
 
@@ -190,7 +202,7 @@ def test_k2_answer_bit_exact_and_decrypts(O, k2p, ctx2, layout):
         resp_g = s.db.answer(s.cid, query)
         assert np.array_equal(resp_g, resp_o), f"index {idx}"
         wire = s.server.save_resp_to_stream(resp_g)
-        assert len(wire) == (2 * p.N * 50 + 7) // 8
+        assert len(wire) == (2 * p.N * int(p.small_q - 1).bit_length() + 7) // 8
         pt, budget = s.client.decrypt_mod_q(s.client.load_resp_from_stream(wire))
         assert np.array_equal(pt, s.server.direct_get_original_plaintext(idx)) and budget > 0
 
@@ -250,7 +262,7 @@ def test_k2_every_scan_shape_matches_oracle(O, k2p, ctx2, layout):
     fq = s.server.fast_expand_qry(s.cid, query)[: p.fst_dim_sz]
     mid_o = s.server.evaluate_first_dim(fq)
     mids = _each_scan_shape(ctx2, lambda: s.db.first_dim(fq))
-    assert len(mids) >= 5
+    assert len(mids) >= 4
     for v, m in mids.items():
         assert np.array_equal(m, mid_o), f"scan shape {v}"
     idxs = [(s.indices[-1] + 13 * j) % p.target_num_pt for j in range(9)]
@@ -264,13 +276,14 @@ def test_k2_every_scan_shape_matches_oracle(O, k2p, ctx2, layout):
         assert np.array_equal(b, want), f"scan shape {v}"
 
 
-@pytest.mark.parametrize("k", [4097, 1025, 3, 8191])
-def test_k2_bv_keyswitch_matches_oracle(O, k2p, ctx2, k):
+@pytest.mark.parametrize("kexpr", ["N+1", "N//4+1", "3", "2*N-1"])
+def test_k2_bv_keyswitch_matches_oracle(O, k2p, ctx2, kexpr):
     """bv_apply_galois_inplace_k2 (bv_keyswitch.cpp:373-481: signed MP digits,
     ties to +B/2) alone through opir_bv_ks, against the oracle, bit-exact."""
-    say(f"[gpu] K=2 BV keyswitch k={k}")
     p = k2p
     N = p.N
+    k = eval(kexpr, {"N": N})
+    say(f"[gpu] K=2 BV keyswitch k={k}")
     q0, q1 = p.rns_mods
     sk, _ = O.gen_sk_and_encrypt(p, np.zeros(N, np.uint64), 25)
     key = O.gen_bv_ks_key(p, sk, k, 26 + k)[0]

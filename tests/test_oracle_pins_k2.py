@@ -180,6 +180,44 @@ def test_k2_end_to_end_decrypts(O, p2, layout):
         assert np.array_equal(pt, srv.direct_get_original_plaintext(i)) and budget >= 1
 
 
+def _is_prime(n):
+    """Deterministic Miller-Rabin for n < 2^64 (first 12 prime bases)."""
+    if n < 2:
+        return False
+    bases = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37]
+    for b in bases:
+        if n % b == 0:
+            return n == b
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d, r = d // 2, r + 1
+    for a in bases:
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(r - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def _largest_prime_1_mod(bits, m, skip=()):
+    x = ((1 << bits) - 1) // m * m + 1
+    while x not in skip and not _is_prime(x) or x in skip:
+        x -= m
+    return x
+
+
+def _smallest_prime_above(v):
+    x = v + 1
+    while not _is_prime(x):
+        x += 1
+    return x
+
+
 def test_k2_constants_exact(p2):
     """The CONFIG_N4096_K2_MP values re-derived from the reference's prime and
     plaintext-modulus rules (database_constants.h:111-124, pir.cpp:14-98,
@@ -187,8 +225,9 @@ def test_k2_constants_exact(p2):
     prime above 2^39, small_q the largest 50-bit prime 1 mod 2N."""
     q0, q1 = p2.rns_mods
     assert (q0, q1) == (1152921504606830593, 1152921504606748673)
-    assert p2.plain_mod == 549755813911
-    assert p2.small_q == 1125899906826241
+    assert q0 == _largest_prime_1_mod(60, 2 * p2.N) and q1 == _largest_prime_1_mod(60, 2 * p2.N, skip=(q0,))
+    assert p2.plain_mod == 549755813911 == _smallest_prime_above(1 << 39)
+    assert p2.small_q == 1125899906826241 == _largest_prime_1_mod(50, 2 * p2.N)
     assert p2.bv_base_log2 == 16 and p2.l_ks == 8
 
 
@@ -271,3 +310,75 @@ def test_k2_mod_switch_is_drop_then_rescale(O, p2):
         for i in list(range(len(special))) + [100, N - 1]:
             d = O.drop_q1(int(ct[poly, i]), int(ct[poly, N + i]), q0, q1, inv)
             assert int(out[poly, i]) == O.rescale(d, q0, sq)
+
+
+# ---------------------------------------------------------------------------
+# CONFIG_N2048_K2_MP (database_constants.h:96-109): the same two-prime
+# algorithms at N = 2048 with two 29-bit primes.
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def p2s(O):
+    return O.PirParams("CONFIG_N2048_K2_MP")
+
+
+def test_k2_n2048_constants_exact(O, p2s):
+    """Re-derived from the reference's rules: the two largest 29-bit primes
+    1 mod 2N (the composite scheme's q1, q2), t the smallest prime above
+    2^9, small_q the largest 22-bit prime 1 mod 2N; gadget widths
+    ceil(58 / l)."""
+    assert p2s.N == 2048 and p2s.rns_mods == [536813569, 536752129]
+    q0 = _largest_prime_1_mod(29, 2 * p2s.N)
+    assert p2s.rns_mods == [q0, _largest_prime_1_mod(29, 2 * p2s.N, skip=(q0,))]
+    assert p2s.plain_mod == 521 == _smallest_prime_above(1 << 9)
+    assert p2s.small_q == 4169729 == _largest_prime_1_mod(22, 2 * p2s.N)
+    assert (p2s.l_ep, p2s.base_log2, p2s.l_key, p2s.base_log2_key, p2s.l_ks, p2s.bv_base_log2) == (5, 12, 8, 8, 8, 8)
+
+
+def test_k2_n2048_ntt_brute_force(O, p2s):
+    N = p2s.N
+    rng = np.random.default_rng(2)
+    for q in p2s.rns_mods:
+        psi = O.ntt_root(N, q)
+        assert pow(psi, N, q) == q - 1
+        a = rng.integers(0, q, size=N, dtype=np.uint64)
+        got = O.ntt_fwd(a, N, q)
+        ai = [int(x) for x in a]
+        for j in [0, 1, 777, N - 1]:
+            w = pow(psi, 2 * brv(j, 11) + 1, q)
+            assert int(got[j]) == sum(x * pow(w, i, q) for i, x in enumerate(ai)) % q
+
+
+def test_k2_n2048_digits_and_drop_exact(O, p2s):
+    """The MP gadget digits and drop_q1 with this scheme's Q = q0 * q1 < 2^58."""
+    from fractions import Fraction
+    import math
+    q0, q1 = p2s.rns_mods
+    Q = q0 * q1
+    b = p2s.bv_base_log2
+    B, nd = 1 << b, p2s.l_ks
+    for x in [0, 1, Q - 1, Q // 2, Q // 2 + 1, B // 2, Q - B // 2, (3 * B + B // 2) * B, Q - (B // 2) * B ** 3,
+              123456789012345]:
+        d = O.signed_gadget_decompose_mp(x, 0, Q, 0, b, nd)
+        assert all(-B // 2 < di <= B // 2 for di in d)
+        assert sum(di * B ** i for i, di in enumerate(d)) == _centred(x, Q)
+    inv = pow(q0, -1, q1)
+    assert p2s.q0_inv_mod_q1 == inv
+    for x in [0, Q - 1, Q - q1 // 2, Q - q1 // 2 - 1, 77 * q1 + q1 // 2, 77 * q1 + q1 // 2 + 1]:
+        want = math.floor(Fraction(x, q1) + Fraction(1, 2)) % q0
+        assert O.drop_q1(x % q0, x % q1, q0, q1, inv) == want
+
+
+@pytest.mark.parametrize("layout", [(43, 5, True), (700, 8, True)])
+def test_k2_n2048_end_to_end_decrypts(O, p2s, layout):
+    p = p2s.with_layout(*layout)
+    c = O.PirClient(p, 9, 99 + layout[0])
+    srv = O.PirServer(p)
+    idx = p.target_num_pt - 1
+    srv.load_data(p.target_num_pt, "random", 18, [idx, 0])
+    srv.set_client_keys(9, c.create_bv_galois_keys(), c.generate_gsw_from_key())
+    for i in [idx, 0]:
+        resp = srv.make_query(9, c.fast_generate_query(p, i))
+        wire = srv.save_resp_to_stream(resp)
+        assert len(wire) == (2 * p.N * 22 + 7) // 8
+        pt, budget = c.decrypt_mod_q(c.load_resp_from_stream(wire))
+        assert np.array_equal(pt, srv.direct_get_original_plaintext(i)) and budget >= 1
