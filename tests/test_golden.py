@@ -19,8 +19,9 @@ FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "answer_*.npz")))
 def load(path):
     d = np.load(path)  # plain arrays, allow_pickle stays False
     target, height, pow2, db_seed, cid, cseed, index = (int(x) for x in d["case"])
+    cfg = str(d["config"]) if "config" in d.files else "CONFIG_N2048_K1_COMP"
     return dict(target=target, height=height, pow2=bool(pow2), db_seed=db_seed, cid=cid, cseed=cseed,
-                index=index, query=d["query"], resp=d["resp"], wire=d["wire"].tobytes())
+                index=index, query=d["query"], resp=d["resp"], wire=d["wire"].tobytes(), config=cfg)
 
 
 def db_plaintexts(O, f, N, t):
@@ -29,13 +30,15 @@ def db_plaintexts(O, f, N, t):
 
 
 def test_fixtures_present():
-    assert len(FIXTURES) >= 3
+    assert len(FIXTURES) >= 6
+    assert {load(f)["config"] for f in FIXTURES} == {"CONFIG_N2048_K1_COMP", "CONFIG_N2048_K2_MP",
+                                                     "CONFIG_N4096_K2_MP"}
 
 
 @pytest.mark.parametrize("path", FIXTURES, ids=os.path.basename)
-def test_oracle_reproduces_fixture(O, comp_params, path):
+def test_oracle_reproduces_fixture(O, path):
     f = load(path)
-    p = comp_params.with_layout(f["target"], f["height"], f["pow2"])
+    p = O.PirParams(f["config"]).with_layout(f["target"], f["height"], f["pow2"])
     client = O.PirClient(p, f["cid"], f["cseed"])
     # the client's seeded stream runs keys first, then the query
     bv, gsw = client.create_bv_galois_keys(), client.generate_gsw_from_key()
@@ -52,7 +55,7 @@ def test_oracle_reproduces_fixture(O, comp_params, path):
 def test_host_client_reproduces_fixture_query(path):
     import onionpirv2_amd.host as H
     f = load(path)
-    hp = H.PirParams().with_layout(f["target"], f["height"], f["pow2"])
+    hp = H.PirParams(f["config"]).with_layout(f["target"], f["height"], f["pow2"])
     c = H.PirClient(hp, f["cid"], f["cseed"])
     c.create_bv_galois_keys()
     c.generate_gsw_from_key()
@@ -65,7 +68,7 @@ def test_gpu_answer_matches_fixture(O, path):
     import onionpirv2_amd.host as H
     f = load(path)
     print(f"[gpu] golden {os.path.basename(path)}", flush=True)
-    hp = H.PirParams().with_layout(f["target"], f["height"], f["pow2"])
+    hp = H.PirParams(f["config"]).with_layout(f["target"], f["height"], f["pow2"])
     server = H.PirServer(hp)
     server.load_data_array(db_plaintexts(O, f, hp.N, hp.plain_mod), [f["index"]])
     c = H.PirClient(hp, f["cid"], f["cseed"])
