@@ -118,6 +118,18 @@ int opir_db_gen_random(opir_db *db, uint64_t seed);
  * mod Q in NTT form, h >= expansion height of every db used with it;
  * gsw: [2*l_key][2][N] mod Q in NTT form. Re-setting an id replaces the keys. */
 int opir_client_set_keys(opir_ctx *ctx, uint64_t client_id, const uint64_t *bv, size_t h, const uint64_t *gsw);
+/* Seed-compressed form of opir_client_set_keys (the reference models these
+ * sizes, src/includes/pir.h:267-291, with no encoder): bv_b [h][l_ks][K*N]
+ * holds the b half of every BV key row and gsw_c0 [2*l_key][K*N] the c0 half
+ * of every RGSW(s) row, NTT form; the uniform halves (a, c1) are regenerated
+ * on the device from the client's 32-byte key seed: row (level, r) of the BV
+ * keys is uniform(seed, domain 4, level*l_ks + r, limb), RGSW(s) row r is
+ * uniform(seed, domain 6, r, limb), with uniform(...) the ChaCha20 stream of
+ * onionpirv2_amd/host/prg.h (coefficient k = the 128-bit draw k mod the
+ * limb's modulus, Q for the composite scheme). Re-setting an id replaces
+ * the keys. */
+int opir_client_set_keys_seeded(opir_ctx *ctx, uint64_t client_id, const uint8_t *seed, const uint64_t *bv_b,
+                                size_t h, const uint64_t *gsw_c0);
 int opir_client_remove(opir_ctx *ctx, uint64_t client_id);
 
 /* Replaces PirServer::make_query (src/server.cpp:895-947): query is the

@@ -154,6 +154,7 @@ int k2_ctx_init(opir_ctx *ctx, const opir_scheme *sch) {
                           (int)sch->l_ks, (int)sch->bv_base_log2};
   ctx->k2 = true;
   ctx->N = N2;
+  ctx->K = 2;
   return OPIR_OK;
 }
 

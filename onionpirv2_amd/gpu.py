@@ -24,7 +24,7 @@ SCAN_AUTOTUNE = -2
 EXPORTED = [
     "opir_ctx_create", "opir_ctx_destroy", "opir_last_error", "opir_ctx_set_stream", "opir_ctx_synchronize",
     "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt", "opir_db_load_plain_u64", "opir_db_gen_random",
-    "opir_client_set_keys", "opir_client_remove",
+    "opir_client_set_keys", "opir_client_set_keys_seeded", "opir_client_remove",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
     "opir_ctx_set_profiling", "opir_ctx_set_cu_mask", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_ctx_set_graphs", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
@@ -84,6 +84,7 @@ def load_library():
         "opir_db_gen_random": ([P, ctypes.c_uint64], ctypes.c_int),
         "opir_client_set_keys": ([P, ctypes.c_uint64, P, sz, P], ctypes.c_int),
         "opir_client_remove": ([P, ctypes.c_uint64], ctypes.c_int),
+        "opir_client_set_keys_seeded": ([P, ctypes.c_uint64, P, P, sz, P], ctypes.c_int),
         "opir_answer": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
         "opir_answer_batch": ([P, sz, P, P, P], ctypes.c_int),
         "opir_answer_batch_device": ([P, sz, P, P, P], ctypes.c_int),
@@ -230,6 +231,22 @@ class GpuContext:
         bv = _u64(bv, (h, sc["l_ks"], 2, self.KN))
         gsw = _u64(gsw, (2 * sc["l_key"], 2 * self.KN))
         self.check(self.lib.opir_client_set_keys(self.h, client_id, _ptr(bv), bv.shape[0], _ptr(gsw)))
+
+    def set_client_keys_seeded(self, client_id, seed, bv_b, gsw_c0):
+        """Seed-compressed keys: bv_b [h, l_ks, K*N] (b halves), gsw_c0
+        [2 l_key, K*N] (c0 halves), NTT form; the uniform halves are
+        regenerated on the device from the 32-byte seed."""
+        sc = self.scheme
+        seed = bytes(seed)
+        if len(seed) != 32:
+            raise InvalidArgument("the key seed is 32 bytes")
+        bv_b = np.asarray(bv_b)
+        h = bv_b.shape[0] if bv_b.ndim else 0
+        b = _u64(bv_b, (h, sc["l_ks"], self.KN))
+        g = _u64(gsw_c0, (2 * sc["l_key"], self.KN))
+        sb = (ctypes.c_uint8 * 32).from_buffer_copy(seed)
+        self.check(self.lib.opir_client_set_keys_seeded(self.h, client_id, ctypes.cast(sb, ctypes.c_void_p),
+                                                        _ptr(b), h, _ptr(g)))
 
     def key_bytes(self, h):
         """Device bytes of one client's keys at expansion height h."""

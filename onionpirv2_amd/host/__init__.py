@@ -11,7 +11,9 @@ _LIB = _os.path.join(_os.path.dirname(_os.path.dirname(_os.path.abspath(__file__
 if _LIB not in _sys.path:
     _sys.path.insert(0, _LIB)
 try:
-    from _opir_host import (GpuScheme, PirClient, PirParams, PirServer, ServingLoop, pack_bits,  # noqa: F401
-                            plan_batch_limits, unpack_bits)
+    from _opir_host import (GpuScheme, PirClient, PirParams, PirServer, ServingLoop, chacha20_block,  # noqa: F401
+                            decode_keys, decode_query, encode_keys, encode_query, expand_keys, gaussian_cdt,
+                            keys_wire_bytes, pack_bits, plan_batch_limits, prg_gaussian, prg_ternary, prg_uniform,
+                            query_seed, query_wire_bytes, unpack_bits)
 except ImportError as e:  # pragma: no cover
     raise ImportError(f"onionpirv2_amd native host module is not built in {_LIB}: {e}") from e

@@ -70,8 +70,70 @@ static bvks::BvGaloisKeys arr_bv(const Arr &a, size_t N) {
   return k;
 }
 
+static prg::Seed to_seed(const py::bytes &b) {
+  const std::string s = b;
+  if (s.size() != 32) throw std::invalid_argument("a seed is 32 bytes");
+  prg::Seed r;
+  std::memcpy(r.data(), s.data(), 32);
+  return r;
+}
+static py::bytes seed_bytes(const prg::Seed &s) { return py::bytes(reinterpret_cast<const char *>(s.data()), 32); }
+
 PYBIND11_MODULE(_opir_host, m) {
   m.doc() = "Host API of the MI355X OnionPIRv2 server (PirParams, PirClient, PirServer)";
+  // counter-based sampler (prg.h)
+  m.def("chacha20_block", [](py::bytes key, uint32_t counter, std::array<uint32_t, 3> nonce) {
+    uint32_t w[16];
+    prg::chacha20_block(to_seed(key), counter, nonce.data(), w);
+    return py::bytes(reinterpret_cast<const char *>(w), 64);
+  });
+  m.def("prg_uniform", [](py::bytes key, uint32_t domain, uint32_t i, uint32_t j, uint64_t q, size_t n) {
+    Arr a((py::ssize_t)n);
+    prg::uniform(to_seed(key), domain, i, j, q, n, a.mutable_data());
+    return a;
+  });
+  m.def("prg_ternary", [](py::bytes key, size_t n) {
+    py::array_t<int8_t> a((py::ssize_t)n);
+    prg::ternary(to_seed(key), n, a.mutable_data());
+    return a;
+  });
+  m.def("gaussian_cdt", &prg::gaussian_cdt);
+  m.def("prg_gaussian", [](py::bytes key, uint32_t domain, uint32_t i, double sigma, size_t n) {
+    py::array_t<int64_t> a((py::ssize_t)n);
+    prg::gaussian(to_seed(key), domain, i, prg::gaussian_cdt(sigma), n, a.mutable_data());
+    return a;
+  });
+  m.def("query_seed", [](py::bytes key, uint64_t ctr) { return seed_bytes(prg::query_seed(to_seed(key), ctr)); });
+  // seed-compressed wire codec (codec.h)
+  m.def("query_wire_bytes", &codec::query_wire_bytes);
+  m.def("keys_wire_bytes", &codec::keys_wire_bytes);
+  m.def("encode_query", [](const PirParams &p, py::bytes qseed, Arr q) {
+    return py::bytes(codec::encode_query(p, to_seed(qseed), arr_ct(q)));
+  });
+  m.def("decode_query", [](const PirParams &p, py::bytes w) { return ct_arr(codec::decode_query(p, std::string(w))); });
+  m.def("encode_keys", [](const PirParams &p, py::bytes key, Arr bv, Arr gsw) {
+    return py::bytes(codec::encode_keys(p, to_seed(key), arr_bv(bv, p.get_poly_degree()), arr_gsw(gsw)));
+  });
+  m.def("decode_keys", [](const PirParams &p, py::bytes w) {
+    const codec::SeededKeys k = codec::decode_keys(p, std::string(w));
+    const size_t KN = p.get_poly_degree() * p.K();
+    Arr b({(py::ssize_t)k.h, (py::ssize_t)p.get_l_ks(), (py::ssize_t)KN});
+    Arr g({(py::ssize_t)(2 * p.get_l_key()), (py::ssize_t)KN});
+    if (!k.bv_b.empty()) std::memcpy(b.mutable_data(), k.bv_b.data(), k.bv_b.size() * 8);
+    std::memcpy(g.mutable_data(), k.gsw_c0.data(), k.gsw_c0.size() * 8);
+    py::dict d;
+    d["key"] = seed_bytes(k.key);
+    d["h"] = k.h;
+    d["bv_b"] = b;
+    d["gsw_c0"] = g;
+    return d;
+  });
+  m.def("expand_keys", [](const PirParams &p, py::bytes w) {
+    bvks::BvGaloisKeys bv;
+    GSWCt g;
+    codec::expand_keys(p, codec::decode_keys(p, std::string(w)), bv, g);
+    return py::make_tuple(bv_arr(bv), gsw_arr(g));
+  });
   m.def("pack_bits", [](Arr v, size_t width) {
     std::string out;
     codec::pack_bits(v.data(), (size_t)v.size(), width, out);
@@ -128,7 +190,14 @@ PYBIND11_MODULE(_opir_host, m) {
 
   py::class_<PirClient>(m, "PirClient")
       .def(py::init<const PirParams &, size_t, uint64_t>(), py::arg("params"), py::arg("client_id"), py::arg("seed"))
+      .def(py::init([](const PirParams &p, size_t id, py::bytes key) {
+             return std::make_unique<PirClient>(p, id, to_seed(key));
+           }),
+           py::arg("params"), py::arg("client_id"), py::arg("key"))
       .def_property_readonly("client_id", &PirClient::get_client_id)
+      .def_property_readonly("seeded", &PirClient::seeded)
+      .def("key_seed", [](const PirClient &c) { return seed_bytes(c.key_seed()); })
+      .def("last_query_seed", [](const PirClient &c) { return seed_bytes(c.last_query_seed()); })
       .def("secret_key", [](const PirClient &c) {
         const auto &d = c.secret_key().data;
         Arr a((py::ssize_t)d.size());
@@ -222,6 +291,18 @@ PYBIND11_MODULE(_opir_host, m) {
              {
                py::gil_scoped_release nogil;
                r = s.make_query(id, ct);
+             }
+             return ct_arr(r);
+           })
+      .def("set_client_keys_wire",
+           [](PirServer &s, size_t id, py::bytes w) { s.set_client_keys_wire(id, std::string(w)); })
+      .def("make_query_wire",
+           [](PirServer &s, size_t id, py::bytes w) {
+             const std::string wire = w;
+             RlweCt r;
+             {
+               py::gil_scoped_release nogil;
+               r = s.make_query_wire(id, wire);
              }
              return ct_arr(r);
            })

@@ -12,6 +12,7 @@
 
 #include "codec.h"
 #include "onionpir_host.h"
+#include "prg.h"
 
 namespace onionpir {
 
@@ -80,6 +81,58 @@ PirClient::PirClient(const PirParams &params, size_t client_id, u64 seed)
   }
 }
 
+PirClient::PirClient(const PirParams &params, size_t client_id, const std::array<uint8_t, 32> &key)
+    : client_id_(client_id), params_(params), rng_(0), seed_(0), qs_(params.get_rns_mods()),
+      rings_(rings_for(params)), prg_(true), key_(key) {
+  const size_t N = params.get_poly_degree(), K = qs_.size();
+  cdt_ = prg::gaussian_cdt(params.get_noise_std_dev());
+  std::vector<int8_t> tern(N);
+  prg::ternary(key_, N, tern.data());
+  sk_.data.resize(K * N);
+  for (size_t k = 0; k < K; k++) {
+    for (size_t i = 0; i < N; i++) sk_.data[k * N + i] = tern[i] == 0 ? 0 : (tern[i] == 1 ? 1 : qs_[k] - 1);
+    rings_[k]->ntt_fwd(sk_.data.data() + k * N);
+  }
+}
+
+void PirClient::shared_error(uint32_t domain, uint32_t i, std::vector<u64> &e) const {
+  const size_t N = params_.get_poly_degree(), K = qs_.size();
+  std::vector<int64_t> v(N);
+  prg::gaussian(key_, domain, i, cdt_, N, v.data());
+  e.assign(K * N, 0);
+  for (size_t k = 0; k < K; k++)
+    for (size_t j = 0; j < N; j++) {
+      const u64 mag = (u64)(v[j] < 0 ? -v[j] : v[j]) % qs_[k];
+      e[k * N + j] = (v[j] < 0 && mag) ? qs_[k] - mag : mag;
+    }
+}
+
+void PirClient::encrypt_zero_seeded(RlweCt &ct, const std::array<uint8_t, 32> &qseed) {
+  // c1 = uniform(qseed, kQueryA, 0, limb), c0 = -(c1*s + e), coefficient form
+  const size_t N = params_.get_poly_degree(), K = qs_.size();
+  ct.c0.assign(K * N, 0);
+  ct.c1.assign(K * N, 0);
+  for (size_t k = 0; k < K; k++) prg::uniform(qseed, prg::kQueryA, 0, (uint32_t)k, qs_[k], N, ct.c1.data() + k * N);
+  std::vector<int64_t> ev(N);
+  prg::gaussian(qseed, prg::kQueryE, 0, cdt_, N, ev.data());
+  std::vector<u64> a(N);
+  for (size_t k = 0; k < K; k++) {
+    const u64 q = qs_[k];
+    u64 *c0 = ct.c0.data() + k * N;
+    std::memcpy(a.data(), ct.c1.data() + k * N, N * 8);
+    rings_[k]->ntt_fwd(a.data());
+    rings_[k]->mul(a.data(), sk_.data.data() + k * N, c0);
+    rings_[k]->ntt_inv(c0);
+    for (size_t i = 0; i < N; i++) {
+      const u64 mag = (u64)(ev[i] < 0 ? -ev[i] : ev[i]) % q;
+      const u64 e = (ev[i] < 0 && mag) ? q - mag : mag;
+      const u64 s = (c0[i] + e) % q;
+      c0[i] = s ? q - s : 0;
+    }
+  }
+  ct.ntt_form = false;
+}
+
 void PirClient::encrypt_zero(RlweCt &ct) {
   // rlwe.cpp:230-276: c1 = a (uniform per limb), c0 = -(a*s + e), coefficient form
   const size_t N = params_.get_poly_degree(), K = qs_.size();
@@ -138,7 +191,12 @@ RlweCt PirClient::fast_generate_query(const PirParams &qp, size_t pt_idx) {
   utils::try_invert_uint_mod(size_t{1} << h, t, inv);
   const size_t slot = utils::bit_reverse(idx[0], h);
   RlweCt q;
-  encrypt_zero(q);
+  if (prg_) {
+    last_qseed_ = prg::query_seed(key_, query_ctr_++);
+    encrypt_zero_seeded(q, last_qseed_);
+  } else {
+    encrypt_zero(q);
+  }
   if (K == 1) {
     const u64 Q = qs_[0];
     const u64 scaled = (u64)(((u128)Q * inv + (t >> 1)) / t) % Q;  // round(Q * inv / t)
@@ -212,14 +270,20 @@ bvks::BvGaloisKeys PirClient::create_bv_galois_keys() {
       auto &row = key.cts[r];
       row.a.assign(K * N, 0);
       row.b.assign(K * N, 0);
-      for (size_t k = 0; k < K; k++) uniform_poly(row.a.data() + k * N, N, qs_[k], rng);
-      gaussian_shared(e.data(), N, qs_, params_.get_noise_std_dev(), rng);
+      const uint32_t sid = (uint32_t)(lvl * L + r);
+      if (prg_) {  // a drawn directly in NTT form; the error in coefficient form
+        for (size_t k = 0; k < K; k++) prg::uniform(key_, prg::kBvA, sid, (uint32_t)k, qs_[k], N, row.a.data() + k * N);
+        shared_error(prg::kBvE, sid, e);
+      } else {
+        for (size_t k = 0; k < K; k++) uniform_poly(row.a.data() + k * N, N, qs_[k], rng);
+        gaussian_shared(e.data(), N, qs_, params_.get_noise_std_dev(), rng);
+      }
       for (size_t k = 0; k < K; k++) {
         const u64 q = qs_[k];
         u64 Br = 1 % q;
         for (size_t z = 0; z < r * b; z++) Br = (u64)(((u128)Br << 1) % q);
         u64 *a = row.a.data() + k * N, *bb = row.b.data() + k * N, *ek = e.data() + k * N;
-        rings_[k]->ntt_fwd(a);
+        if (!prg_) rings_[k]->ntt_fwd(a);
         rings_[k]->ntt_fwd(ek);
         for (size_t j = 0; j < N; j++) {
           // b = sigma(s) * B^r - a*s + e, NTT form
@@ -242,6 +306,32 @@ GSWCt PirClient::generate_gsw_from_key() {
   const u64 q0 = qs_[0];
   const auto gad = utils::gsw_gadget(l, params_.get_base_log2_key(), qs_);
   GSWCt out(2 * l, std::vector<u64>(2 * K * N));
+  if (prg_) {
+    // row = half * l + r: c1 = uniform(kGswA, row) in NTT form, c0 = -(c1*s + e)
+    // + g_r * (half ? s^2 : s): the same phase as the unseeded rows, whose c1
+    // is a + s*g_r (bottom half) with a uniform
+    std::vector<u64> e;
+    for (size_t half = 0; half < 2; half++)
+      for (size_t r = 0; r < l; r++) {
+        const uint32_t row_id = (uint32_t)(half * l + r);
+        auto &row = out[row_id];
+        shared_error(prg::kGswE, row_id, e);
+        for (size_t k = 0; k < K; k++) {
+          const u64 q = qs_[k];
+          u64 *c0 = row.data() + k * N, *c1 = row.data() + K * N + k * N, *ek = e.data() + k * N;
+          const u64 *s = sk_.data.data() + k * N;
+          prg::uniform(key_, prg::kGswA, row_id, (uint32_t)k, q, N, c1);
+          rings_[k]->ntt_fwd(ek);
+          for (size_t j = 0; j < N; j++) {
+            const u64 as = mmul(c1[j], s[j], q);
+            const u64 m = mmul(gad[k][r], half ? mmul(s[j], s[j], q) : s[j], q);
+            const u64 neg = (as + ek[j]) % q;
+            c0[j] = ((neg ? q - neg : 0) + m) % q;
+          }
+        }
+      }
+    return out;
+  }
   RlweCt ct;
   for (size_t half = 0; half < 2; half++)
     for (size_t r = 0; r < l; r++) {

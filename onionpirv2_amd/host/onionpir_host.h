@@ -12,12 +12,14 @@
 //     std::random_device), drawn with the same std:: distributions in the
 //     same order, so a given seed reproduces the same keys and queries.
 #pragma once
+#include <array>
 #include <cstddef>
 #include <cstdint>
 #include <functional>
 #include <map>
 #include <memory>
 #include <random>
+#include <set>
 #include <sstream>
 #include <string>
 #include <unordered_map>
@@ -177,6 +179,17 @@ using PlaintextSource = std::function<void(size_t logical_index, RlwePt &out)>;
 class PirClient {
  public:
   PirClient(const PirParams &params, size_t client_id, u64 seed);
+  // Seeded client: every random polynomial (secret, uniform halves, errors)
+  // comes from the counter-based sampler of prg.h keyed by `key`, so a server
+  // can regenerate the uniform halves of its keys and queries from seeds
+  // (the wire codec of codec.h) and the device can generate the same keys
+  // and queries (opir_client_keygen_device, opir_query_gen_device).
+  PirClient(const PirParams &params, size_t client_id, const std::array<uint8_t, 32> &key);
+  bool seeded() const { return prg_; }
+  const std::array<uint8_t, 32> &key_seed() const { return key_; }
+  // Seed of the last query a seeded client generated (its c1 is
+  // uniform(seed, kQueryA, 0, limb)).
+  const std::array<uint8_t, 32> &last_query_seed() const { return last_qseed_; }
   RlweCt fast_generate_query(size_t pt_idx);
   RlweCt fast_generate_query(const PirParams &query_params, size_t pt_idx);
   void add_gsw_to_query(const PirParams &qp, RlweCt &query, const std::vector<size_t> &idx);
@@ -191,6 +204,8 @@ class PirClient {
 
  private:
   void encrypt_zero(RlweCt &ct);
+  void encrypt_zero_seeded(RlweCt &ct, const std::array<uint8_t, 32> &qseed);
+  void shared_error(uint32_t domain, uint32_t i, std::vector<u64> &e) const;  // [K][N], same integer per limb
   size_t client_id_;
   PirParams params_;
   std::mt19937_64 rng_;
@@ -198,6 +213,10 @@ class PirClient {
   std::vector<u64> qs_;                           // RNS limb moduli (one logical Q for K = 1)
   std::vector<std::shared_ptr<RingMod>> rings_;   // one negacyclic NTT ring per limb
   RlweSk sk_;                                     // K*N values, NTT form per limb
+  bool prg_ = false;                              // seeded client (counter-based sampler)
+  std::array<uint8_t, 32> key_{}, last_qseed_{};
+  uint64_t query_ctr_ = 0;
+  std::vector<uint64_t> cdt_;                     // error table for the seeded sampler
 };
 
 // Owns one GPU context (scheme tables, device keys, stream); shareable by
@@ -245,6 +264,11 @@ class PirServer {
   void set_client_bv_galois_key(size_t client_id, bvks::BvGaloisKeys bv_keys);
   void set_client_gsw_key(size_t client_id, GSWCt gsw_key);
   void set_client_session_keys(size_t client_id, SharedPirSessionKeys keys);
+  // Seed-compressed wire forms (codec.h): keys whose uniform halves the
+  // device regenerates (they live on the device only), and a query whose c1
+  // is regenerated from its seed.
+  void set_client_keys_wire(size_t client_id, const std::string &wire);
+  RlweCt make_query_wire(size_t client_id, const std::string &wire);
   SharedPirSessionKeys client_session_keys(size_t client_id) const;
   RlwePt direct_get_original_plaintext(size_t index) const;
   const PirParams &get_params() const { return params_; }
@@ -259,6 +283,7 @@ class PirServer {
   std::map<size_t, bvks::BvGaloisKeys> bv_keys_;
   std::map<size_t, GSWCt> gsw_keys_;
   std::map<size_t, SharedPirSessionKeys> sessions_;
+  std::set<size_t> wire_keyed_;  // clients keyed through set_client_keys_wire
   std::unordered_map<size_t, RlwePt> recorded_;
 };
 

@@ -175,11 +175,13 @@ void PirServer::push_keys(size_t id) {
 }
 
 void PirServer::set_client_bv_galois_key(size_t id, bvks::BvGaloisKeys bv) {
+  wire_keyed_.erase(id);
   bv_keys_[id] = std::move(bv);
   push_keys(id);
 }
 
 void PirServer::set_client_gsw_key(size_t id, GSWCt gsw) {
+  wire_keyed_.erase(id);
   gsw_keys_[id] = std::move(gsw);
   push_keys(id);
 }
@@ -220,13 +222,28 @@ static RlweCt unflatten(const uint64_t *f, size_t N, bool ntt) {
 }
 
 static void require_keys(const std::map<size_t, bvks::BvGaloisKeys> &bv, const std::map<size_t, GSWCt> &g,
-                         const std::map<size_t, SharedPirSessionKeys> &s, size_t id) {
+                         const std::map<size_t, SharedPirSessionKeys> &s, const std::set<size_t> &w, size_t id) {
   // server.cpp:739-743,841-845: std::map::at on an unknown client
-  if (!s.count(id) && (!bv.count(id) || !g.count(id))) throw std::out_of_range("map::at");
+  if (!s.count(id) && !w.count(id) && (!bv.count(id) || !g.count(id))) throw std::out_of_range("map::at");
+}
+
+void PirServer::set_client_keys_wire(size_t id, const std::string &wire) {
+  const codec::SeededKeys k = codec::decode_keys(params_, wire);
+  if (k.h < params_.get_expan_height()) throw std::invalid_argument("PirServer session lacks keys for its expansion height");
+  scheme_->check(opir_client_set_keys_seeded(scheme_->ctx(), id, k.key.data(), k.bv_b.data(), k.h, k.gsw_c0.data()));
+  bv_keys_.erase(id);
+  gsw_keys_.erase(id);
+  sessions_.erase(id);
+  wire_keyed_.insert(id);
+}
+
+RlweCt PirServer::make_query_wire(size_t id, const std::string &wire) {
+  RlweCt q = codec::decode_query(params_, wire);
+  return make_query(id, q);
 }
 
 RlweCt PirServer::make_query(size_t id, RlweCt &query) {  // server.cpp:895-947
-  require_keys(bv_keys_, gsw_keys_, sessions_, id);
+  require_keys(bv_keys_, gsw_keys_, sessions_, wire_keyed_, id);
   const size_t N = params_.get_poly_degree();
   auto q = flatten(query, N * params_.K());
   std::vector<uint64_t> r(2 * N);
@@ -239,7 +256,7 @@ std::vector<RlweCt> PirServer::make_queries(const std::vector<size_t> &ids, cons
   const size_t N = params_.get_poly_degree(), B = ids.size(), KN = N * params_.K();
   std::vector<uint64_t> q(B * 2 * KN), r(B * 2 * N), id64(ids.begin(), ids.end());
   for (size_t i = 0; i < B; i++) {
-    require_keys(bv_keys_, gsw_keys_, sessions_, ids[i]);
+    require_keys(bv_keys_, gsw_keys_, sessions_, wire_keyed_, ids[i]);
     auto f = flatten(queries[i], KN);
     std::memcpy(&q[i * 2 * KN], f.data(), 2 * KN * 8);
   }
@@ -250,7 +267,7 @@ std::vector<RlweCt> PirServer::make_queries(const std::vector<size_t> &ids, cons
 }
 
 std::vector<RlweCt> PirServer::expand_query(size_t id, RlweCt &query) const {  // server.cpp:818-821
-  require_keys(bv_keys_, gsw_keys_, sessions_, id);
+  require_keys(bv_keys_, gsw_keys_, sessions_, wire_keyed_, id);
   const size_t N = params_.get_poly_degree() * params_.K();  // values per polynomial
   const size_t u = params_.get_fst_dim_sz() + params_.get_l() * (params_.get_num_dims() - 1);
   auto q = flatten(query, N);
@@ -267,7 +284,7 @@ std::vector<GSWCt> PirServer::complete_selectors(size_t id, const std::vector<Rl
   if (!ns) return sel;
   const size_t u = params_.get_fst_dim_sz() + l * ns;
   if (expanded.size() < u) throw std::invalid_argument("complete_selectors expects one expanded row group per selector");
-  require_keys(bv_keys_, gsw_keys_, sessions_, id);
+  require_keys(bv_keys_, gsw_keys_, sessions_, wire_keyed_, id);
   std::vector<uint64_t> e(u * 2 * N), out(ns * 2 * l * 2 * N);
   for (size_t i = 0; i < u; i++) {
     auto f = flatten(expanded[i], N);
