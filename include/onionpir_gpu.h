@@ -60,6 +60,8 @@ typedef struct opir_scheme {
   uint32_t l_key, base_log2_key;    /* RGSW(s) completion gadget */
   uint32_t l_ks, bv_base_log2;      /* BV keyswitch gadget */
   uint64_t ntt_root2;       /* two primes: primitive 2N-th root mod q2 (ignored when composite) */
+  double noise_std_dev;     /* error standard deviation of device-generated keys and queries
+                               (DBConsts::NoiseStdDev, database_constants.h:48); 0 -> 2.55 */
 } opir_scheme;
 
 /* Replaces the scheme set-up of PirParams::PirParams / init_composite_rns
@@ -131,6 +133,31 @@ int opir_client_set_keys(opir_ctx *ctx, uint64_t client_id, const uint64_t *bv, 
 int opir_client_set_keys_seeded(opir_ctx *ctx, uint64_t client_id, const uint8_t *seed, const uint64_t *bv_b,
                                 size_t h, const uint64_t *gsw_c0);
 int opir_client_remove(opir_ctx *ctx, uint64_t client_id);
+/* Reads a resident client's keys back in the layout opir_client_set_keys
+ * takes (bv [h][l_ks][a, b][K*N], gsw [2*l_key][2][K*N], NTT form); h must
+ * equal the resident BV key count. */
+int opir_client_get_keys(opir_ctx *ctx, uint64_t client_id, uint64_t *bv, size_t h, uint64_t *gsw);
+
+/* ---- Client-side helpers on the GPU (load generation) ----
+ * No counterpart on the reference's server: these generate, in HBM, what a
+ * seeded client (the host PirClient(params, id, key32), whose secret, uniform
+ * halves and errors come from the ChaCha20 sampler of
+ * onionpirv2_amd/host/prg.h) generates on its CPU -- the same values -- for
+ * PirClient::create_bv_galois_keys / generate_gsw_from_key
+ * (src/bv_keyswitch.cpp:191-276, src/client.cpp:25-38) and
+ * fast_generate_query (src/client.cpp:107-225). The 32-bit-limb schemes
+ * (N = 2048) only; a two-prime N = 4096 context returns OPIR_EINVAL.
+ * keygen: n clients (keys [n][32] bytes), h BV levels each, installed as
+ * resident keys under ids[i] (no key material crosses PCIe).
+ * query_gen: B queries [B][2][K*N] of clients keys[i] with query counter
+ * counters[i] (the host client's fast_generate_query numbers its queries
+ * from 0) for plaintext indices[i] of db's layout; _device writes device
+ * memory, the other host memory. */
+int opir_client_keygen_device(opir_ctx *ctx, size_t n, const uint64_t *ids, const uint8_t *keys, size_t h);
+int opir_query_gen(opir_db *db, size_t B, const uint8_t *keys, const uint64_t *counters, const uint64_t *indices,
+                   uint64_t *queries);
+int opir_query_gen_device(opir_db *db, size_t B, const uint8_t *keys, const uint64_t *counters,
+                          const uint64_t *indices, uint64_t *d_queries);
 
 /* Replaces PirServer::make_query (src/server.cpp:895-947): query is the
  * coefficient-form packed BFV query [2][N] mod Q; resp receives the

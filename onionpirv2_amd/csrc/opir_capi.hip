@@ -209,6 +209,7 @@ static int ctx_create_k2(const opir_scheme *sch, int device, opir_ctx **out) {
   if (!ctx) return set_err(nullptr, OPIR_ENOMEM, "out of host memory");
   ctx->sch = *sch;
   ctx->device = device;
+  if (sch->noise_std_dev > 0) ctx->noise_sigma = sch->noise_std_dev;
   auto fail = [&](int code, const std::string &m) {
     opir_ctx_destroy(ctx);
     return set_err(nullptr, code, m);
@@ -272,6 +273,7 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   ctx->device = device;
   ctx->K = mp ? 2 : 1;
   ctx->S.mp = mp ? 1 : 0;
+  if (sch->noise_std_dev > 0) ctx->noise_sigma = sch->noise_std_dev;
   auto fail = [&](const std::string &m) {
     g_create_error = m;
     opir_ctx_destroy(ctx);

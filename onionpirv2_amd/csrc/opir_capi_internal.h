@@ -46,6 +46,7 @@ struct opir_ctx {
   opir_scheme sch{};
   bool k2 = false;          // two 60-bit primes at N = 4096 (CONFIG_N4096_K2_MP, opir_capi_k2.hip)
   int K = 1;                // RNS limbs of a polynomial at the ABI (2: K2 schemes)
+  double noise_sigma = 2.55;  // error standard deviation of device-generated keys and queries
   size_t N = kN;            // ring degree
   SchemeConst S{};          // K = 1 constants
   GadgetCfg G{};

@@ -24,7 +24,8 @@ SCAN_AUTOTUNE = -2
 EXPORTED = [
     "opir_ctx_create", "opir_ctx_destroy", "opir_last_error", "opir_ctx_set_stream", "opir_ctx_synchronize",
     "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt", "opir_db_load_plain_u64", "opir_db_gen_random",
-    "opir_client_set_keys", "opir_client_set_keys_seeded", "opir_client_remove",
+    "opir_client_set_keys", "opir_client_set_keys_seeded", "opir_client_remove", "opir_client_get_keys",
+    "opir_client_keygen_device", "opir_query_gen", "opir_query_gen_device",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
     "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
     "opir_ctx_set_profiling", "opir_ctx_set_cu_mask", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_ctx_set_graphs", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
@@ -54,6 +55,7 @@ class OpirScheme(ctypes.Structure):
         ("l_key", ctypes.c_uint32), ("base_log2_key", ctypes.c_uint32),
         ("l_ks", ctypes.c_uint32), ("bv_base_log2", ctypes.c_uint32),
         ("ntt_root2", ctypes.c_uint64),
+        ("noise_std_dev", ctypes.c_double),
     ]
 
 
@@ -85,6 +87,10 @@ def load_library():
         "opir_client_set_keys": ([P, ctypes.c_uint64, P, sz, P], ctypes.c_int),
         "opir_client_remove": ([P, ctypes.c_uint64], ctypes.c_int),
         "opir_client_set_keys_seeded": ([P, ctypes.c_uint64, P, P, sz, P], ctypes.c_int),
+        "opir_client_get_keys": ([P, ctypes.c_uint64, P, sz, P], ctypes.c_int),
+        "opir_client_keygen_device": ([P, sz, P, P, sz], ctypes.c_int),
+        "opir_query_gen": ([P, sz, P, P, P, P], ctypes.c_int),
+        "opir_query_gen_device": ([P, sz, P, P, P, P], ctypes.c_int),
         "opir_answer": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
         "opir_answer_batch": ([P, sz, P, P, P], ctypes.c_int),
         "opir_answer_batch_device": ([P, sz, P, P, P], ctypes.c_int),
@@ -248,6 +254,23 @@ class GpuContext:
         self.check(self.lib.opir_client_set_keys_seeded(self.h, client_id, ctypes.cast(sb, ctypes.c_void_p),
                                                         _ptr(b), h, _ptr(g)))
 
+    def get_client_keys(self, client_id, h):
+        """The resident keys in set_client_keys's layout: (bv [h, l_ks, 2, K*N], gsw [2 l_key, 2*K*N])."""
+        sc = self.scheme
+        bv = np.zeros((h, sc["l_ks"], 2, self.KN), np.uint64)
+        gsw = np.zeros((2 * sc["l_key"], 2 * self.KN), np.uint64)
+        self.check(self.lib.opir_client_get_keys(self.h, client_id, _ptr(bv), h, _ptr(gsw)))
+        return bv, gsw
+
+    def keygen_device(self, client_ids, keys, h):
+        """Generate and install the keys of seeded clients on the device
+        (keys: one 32-byte seed per client)."""
+        ids = np.ascontiguousarray(client_ids, dtype=np.uint64)
+        kb = np.frombuffer(b"".join(bytes(k) for k in keys), dtype=np.uint8).copy()
+        if kb.size != 32 * len(ids):
+            raise InvalidArgument("one 32-byte key per client")
+        self.check(self.lib.opir_client_keygen_device(self.h, len(ids), _ptr(ids), _ptr(kb), h))
+
     def key_bytes(self, h):
         """Device bytes of one client's keys at expansion height h."""
         out = ctypes.c_size_t(0)
@@ -371,6 +394,17 @@ class GpuDatabase:
         self.ctx.check(self.lib.opir_answer_batch_device(self.h, len(ids), _ptr(ids),
                                                          ctypes.c_void_p(d_queries_ptr),
                                                          ctypes.c_void_p(d_resps_ptr)))
+
+    def query_gen(self, keys, counters, indices):
+        """Queries of seeded clients generated on the device: [B, 2, K*N]."""
+        kb = np.frombuffer(b"".join(bytes(k) for k in keys), dtype=np.uint8).copy()
+        c = np.ascontiguousarray(counters, dtype=np.uint64)
+        i = np.ascontiguousarray(indices, dtype=np.uint64)
+        if kb.size != 32 * len(c) or len(c) != len(i):
+            raise InvalidArgument("one key, counter and index per query")
+        out = np.zeros((len(c), 2, self.KN), np.uint64)
+        self.ctx.check(self.lib.opir_query_gen(self.h, len(c), _ptr(kb), _ptr(c), _ptr(i), _ptr(out)))
+        return out
 
     def reserve(self, batch):
         """Prepare answers of up to `batch` queries: workspace, host staging

@@ -39,6 +39,7 @@ GpuScheme::GpuScheme(const PirParams &p, int device) : params_(p) {
   s.base_log2_key = (uint32_t)p.get_base_log2_key();
   s.l_ks = (uint32_t)p.get_l_ks();
   s.bv_base_log2 = (uint32_t)p.get_bv_base_log2();
+  s.noise_std_dev = p.get_noise_std_dev();
   const int rc = opir_ctx_create(&s, device, &ctx_);
   if (rc != OPIR_OK) {
     const std::string msg = opir_last_error(nullptr);
