@@ -83,6 +83,17 @@ int opir_ctx_set_stream(opir_ctx *ctx, void *hip_stream);
  * server per process). Masked streams follow the default stream semantics
  * of hipExtStreamCreateWithCUMask. */
 int opir_ctx_set_cu_mask(opir_ctx *ctx, const uint32_t *mask, size_t words);
+/* Serving support, no counterpart in the reference: a clone is a context on
+ * the same device with the parent's scheme tables and client keys (read
+ * through the parent; key setters on a clone fail with OPIR_EINVAL) but its
+ * own streams, events and staging, so answers issued on a clone and on its
+ * parent from different host threads run concurrently on the device. A view
+ * is a database handle on such a context that shares src's resident
+ * database (no copy) with its own answer workspace. The parent context (and
+ * src) must outlive its clones (views); keys must not change while clones
+ * answer. */
+int opir_ctx_clone(opir_ctx *parent, opir_ctx **out);
+int opir_db_view(opir_db *src, opir_ctx *ctx, opir_db **out);
 /* Blocks until all work queued by the context has finished. */
 int opir_ctx_synchronize(opir_ctx *ctx);
 

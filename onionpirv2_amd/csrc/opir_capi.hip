@@ -145,7 +145,7 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
   rc = upload_k1(ctx, ids, B, P);
   if (rc) return rc;
   for (size_t i = 0; i < B; i++) {
-    const ClientKeys &k = ctx->clients[ids[i]];
+    const ClientKeys &k = ctx->clients()[ids[i]];
     if (k.h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
   }
   const size_t exp_stride = db->exp_cts * kCtWords;
@@ -355,7 +355,7 @@ void opir_ctx_destroy(opir_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
-  for (auto &kv : ctx->clients) {
+  for (auto &kv : ctx->own_clients) {
     (void)hipFree(kv.second.bv);
     (void)hipFree(kv.second.gsw);
   }
@@ -374,6 +374,67 @@ void opir_ctx_destroy(opir_ctx *ctx) {
   if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
+}
+
+int opir_ctx_clone(opir_ctx *parent, opir_ctx **out) {
+  if (!parent || !out) return OPIR_EINVAL;
+  *out = nullptr;
+  opir_ctx *ctx = new (std::nothrow) opir_ctx();
+  if (!ctx) return set_err(parent, OPIR_ENOMEM, "out of host memory");
+  // scheme constants and device tables are the parent's (it must outlive the clone)
+  ctx->device = parent->device;
+  ctx->sch = parent->sch;
+  ctx->k2 = parent->k2;
+  ctx->K = parent->K;
+  ctx->noise_sigma = parent->noise_sigma;
+  ctx->N = parent->N;
+  ctx->S = parent->S;
+  ctx->G = parent->G;
+  ctx->S64 = parent->S64;
+  ctx->G64 = parent->G64;
+  ctx->scan_variant = parent->scan_variant;
+  ctx->scratch = parent->scratch;
+  ctx->key_owner = parent->key_owner ? parent->key_owner : parent;
+  auto fail = [&](const char *m) {
+    opir_ctx_destroy(ctx);
+    return set_err(parent, OPIR_EHIP, m);
+  };
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail("hipSetDevice failed");
+  // the parent's compute-unit mask carries over to the clone's streams
+  ctx->cu_mask = parent->cu_mask;
+  auto make = [&](hipStream_t *st) {
+    return ctx->cu_mask.empty() ? hipStreamCreateWithFlags(st, hipStreamNonBlocking)
+                                : hipExtStreamCreateWithCUMask(st, (uint32_t)ctx->cu_mask.size(), ctx->cu_mask.data());
+  };
+  if (make(&ctx->own_stream) != hipSuccess) return fail("stream create");
+  ctx->stream = ctx->own_stream;
+  if (make(&ctx->aux_stream) != hipSuccess) return fail("stream create");
+  for (auto &e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail("event create");
+  if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail("event create");
+  *out = ctx;
+  return OPIR_OK;
+}
+
+int opir_db_view(opir_db *src, opir_ctx *ctx, opir_db **out) {
+  if (!src || !ctx || !out) return OPIR_EINVAL;
+  *out = nullptr;
+  const opir_ctx *a = src->ctx->key_owner ? src->ctx->key_owner : src->ctx;
+  const opir_ctx *b = ctx->key_owner ? ctx->key_owner : ctx;
+  if (a != b) return set_err(ctx, OPIR_EINVAL, "a database view needs a context of the same key family (opir_ctx_clone)");
+  opir_db *db = new (std::nothrow) opir_db();
+  if (!db) return set_err(ctx, OPIR_ENOMEM, "out of host memory");
+  db->ctx = ctx;
+  db->owns_db = false;
+  db->N0 = src->N0; db->Nrest = src->Nrest; db->dims = src->dims; db->h = src->h;
+  db->num_pt = src->num_pt; db->u = src->u; db->exp_cts = src->exp_cts;
+  db->geom = src->geom;
+  db->d_db = src->d_db;
+  for (int i = 0; i < 16; i++) db->tuned_scan[i] = src->tuned_scan[i];
+  *out = db;
+  return OPIR_OK;
 }
 
 int opir_ctx_set_stream(opir_ctx *ctx, void *s) {
@@ -410,6 +471,7 @@ int opir_ctx_set_cu_mask(opir_ctx *ctx, const uint32_t *mask, size_t words) {
   ctx->own_stream = own;
   ctx->aux_stream = aux;
   if (own_current) ctx->stream = own;
+  ctx->cu_mask.assign(mask, mask + words);
   return OPIR_OK;
 }
 
@@ -512,7 +574,7 @@ void opir_db_destroy(opir_db *db) {
   k2_db_free(db);
   for (u32 *p : {db->exp_a, db->exp_b, db->qntt, db->sel, db->mid_a, db->mid_b, db->part})
     if (p) (void)hipFree(p);
-  if (db->d_db) (void)hipFree(db->d_db);
+  if (db->d_db && db->owns_db) (void)hipFree(db->d_db);
   if (db->qfrag) (void)hipFree(db->qfrag);
   if (db->q_u64) (void)hipFree(db->q_u64);
   if (db->r_u64) (void)hipFree(db->r_u64);
@@ -623,17 +685,18 @@ int opir_db_load_ntt(opir_db *db, const uint32_t *lo, const uint32_t *hi) {
 }
 
 static void replace_client(opir_ctx *ctx, uint64_t id, const ClientKeys &k) {
-  auto it = ctx->clients.find(id);
-  if (it != ctx->clients.end()) {
+  auto it = ctx->clients().find(id);
+  if (it != ctx->clients().end()) {
     (void)hipStreamSynchronize(ctx->stream);  // queued answers may still read the old keys
     (void)hipFree(it->second.bv);
     (void)hipFree(it->second.gsw);
   }
-  ctx->clients[id] = k;
+  ctx->clients()[id] = k;
 }
 
 int opir_client_set_keys(opir_ctx *ctx, uint64_t id, const uint64_t *bv, size_t h, const uint64_t *gsw) {
   if (!ctx) return OPIR_EINVAL;
+  if (ctx->key_owner) return set_err(ctx, OPIR_EINVAL, "a cloned context reads its parent's client keys");
   if ((!bv && h) || !gsw) return set_err(ctx, OPIR_EINVAL, "PirServer session key bundle cannot be null");
   if (h > 12) return set_err(ctx, OPIR_EINVAL, "BV key count out of range");
   if (ctx->k2) {
@@ -676,12 +739,13 @@ int opir_client_set_keys(opir_ctx *ctx, uint64_t id, const uint64_t *bv, size_t 
 
 int opir_client_remove(opir_ctx *ctx, uint64_t id) {
   if (!ctx) return OPIR_EINVAL;
-  auto it = ctx->clients.find(id);
-  if (it == ctx->clients.end()) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  if (ctx->key_owner) return set_err(ctx, OPIR_EINVAL, "a cloned context reads its parent's client keys");
+  auto it = ctx->clients().find(id);
+  if (it == ctx->clients().end()) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   (void)hipFree(it->second.bv);
   (void)hipFree(it->second.gsw);
-  ctx->clients.erase(it);
+  ctx->clients().erase(it);
   return OPIR_OK;
 }
 
@@ -724,7 +788,7 @@ int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out) {
 int opir_ctx_resident_key_bytes(const opir_ctx *ctx, size_t *out) {
   if (!ctx || !out) return OPIR_EINVAL;
   size_t tot = 0;
-  for (const auto &kv : ctx->clients) {
+  for (const auto &kv : ctx->clients()) {
     size_t kb = 0;
     opir_client_key_bytes(ctx, kv.second.h, &kb);
     tot += kb;
@@ -747,7 +811,7 @@ int opir_answer_batch(opir_db *db, size_t B, const uint64_t *ids, const uint64_t
   int rc = ctx->k2 ? k2_validate_rns(ctx, queries, B * 2, "query") : validate_polys(ctx, queries, B * 2, "query");
   if (rc) return rc;
   for (size_t i = 0; i < B; i++)
-    if (!ctx->clients.count(ids[i])) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(ids[i]));
+    if (!ctx->clients().count(ids[i])) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(ids[i]));
   rc = ctx->k2 ? k2_ensure_ws(db, B) : ensure_ws(db, B);
   if (rc) return rc;
   const size_t qbytes = B * qvals * sizeof(u64), rbytes = B * rvals * sizeof(u64);
@@ -796,8 +860,8 @@ int opir_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out) 
   if (ctx->k2) return k2_expand(db, id, query, out);
   int rc = validate_polys(ctx, query, 2, "query");
   if (rc) return rc;
-  if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
-  if (ctx->clients[id].h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
+  if (!ctx->clients().count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  if (ctx->clients()[id].h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
   rc = ensure_ws(db, 1);
   if (rc) return rc;
   BatchPtrs P;
@@ -822,7 +886,7 @@ int opir_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *
   if (ctx->k2) return k2_complete(db, id, expanded, sel);
   const size_t nsel = db->dims - 1;
   if (!nsel) return OPIR_OK;
-  if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  if (!ctx->clients().count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
   int rc = validate_polys(ctx, expanded, db->u * 2, "expanded");
   if (rc) return rc;
   rc = ensure_ws(db, 1);

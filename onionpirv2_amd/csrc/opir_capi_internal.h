@@ -54,7 +54,13 @@ struct opir_ctx {
   k2::Scheme64 S64{};       // K = 2 constants
   k2::Gadget64 G64{};
   u64 *d_tables64 = nullptr;
-  std::unordered_map<uint64_t, capi::ClientKeys> clients;
+  // client keys; a clone (opir_ctx_clone) reads its parent's
+  std::unordered_map<uint64_t, capi::ClientKeys> own_clients;
+  opir_ctx *key_owner = nullptr;
+  std::unordered_map<uint64_t, capi::ClientKeys> &clients() { return key_owner ? key_owner->own_clients : own_clients; }
+  const std::unordered_map<uint64_t, capi::ClientKeys> &clients() const {
+    return key_owner ? key_owner->own_clients : own_clients;
+  }
   std::string err;
   bool profiling = false;
   int scan_variant = -1;    // first-dimension scan tile shape: -1 table default, -2 autotune, >= 0 forced
@@ -75,10 +81,12 @@ struct opir_ctx {
   size_t pin_bytes = 0;
   // this device's copy of the allocation-free entry point's scratch array
   void *scratch = nullptr;
+  std::vector<uint32_t> cu_mask;  // compute-unit mask of the own streams (empty: all CUs)
 };
 
 struct opir_db {
   opir_ctx *ctx = nullptr;
+  bool owns_db = true;     // false for a view (opir_db_view): d_db belongs to another handle
   size_t N0 = 0, Nrest = 0, dims = 0, h = 0, num_pt = 0, u = 0;
   DbGeom geom;
   int8_t *d_db = nullptr;  // digit planes in MFMA fragment order (opir_scan.hip), P = 4 (K = 1) or 8 (K = 2)
@@ -303,8 +311,8 @@ inline int upload_key_ptrs(opir_ctx *ctx, const uint64_t *ids, size_t B, const v
   HIP_TRY(ctx, hipEventSynchronize(ctx->ring_ev[slot]));
   const void **hp = ctx->pin_ptrs[slot];
   for (size_t i = 0; i < B; i++) {
-    auto it = ctx->clients.find(ids[i]);
-    if (it == ctx->clients.end())
+    auto it = ctx->clients().find(ids[i]);
+    if (it == ctx->clients().end())
       return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(ids[i]));
     hp[i] = it->second.bv;
     hp[B + i] = it->second.gsw;

@@ -217,7 +217,7 @@ int k2_ensure_ws(opir_db *db, size_t B) {
 void k2_db_free(opir_db *db) {
   if (!db->ctx->k2) return;
   free_ws(db);
-  if (db->d_db) (void)hipFree(db->d_db);
+  if (db->d_db && db->owns_db) (void)hipFree(db->d_db);
   db->d_db = nullptr;
 }
 
@@ -301,7 +301,7 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
   rc = upload_k2(ctx, ids, B, P);
   if (rc) return rc;
   for (size_t i = 0; i < B; i++)
-    if (ctx->clients[ids[i]].h < db->h)
+    if (ctx->clients()[ids[i]].h < db->h)
       return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
   const size_t exp_stride = db->exp_cts * kCt2;
   return run_answer(db, B, 8, P.bv, d_q, d_r, [&]() -> int {
@@ -362,8 +362,8 @@ int k2_expand(opir_db *db, uint64_t id, const uint64_t *query, uint64_t *out) {
   opir_ctx *ctx = db->ctx;
   int rc = validate_rns(ctx, query, 2, "query");
   if (rc) return rc;
-  if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
-  if (ctx->clients[id].h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
+  if (!ctx->clients().count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  if (ctx->clients()[id].h < db->h) return set_err(ctx, OPIR_EINVAL, "PirServer session lacks keys for its expansion height");
   rc = k2_ensure_ws(db, 1);
   if (rc) return rc;
   k2::KeyPtrs64 P;
@@ -381,7 +381,7 @@ int k2_complete(opir_db *db, uint64_t id, const uint64_t *expanded, uint64_t *se
   opir_ctx *ctx = db->ctx;
   const size_t nsel = db->dims - 1;
   if (!nsel) return OPIR_OK;
-  if (!ctx->clients.count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  if (!ctx->clients().count(id)) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
   int rc = validate_rns(ctx, expanded, db->u * 2, "expanded");
   if (rc) return rc;
   rc = k2_ensure_ws(db, 1);

@@ -379,6 +379,7 @@ extern "C" {
 
 int opir_client_keygen_device(opir_ctx *ctx, size_t n, const uint64_t *ids, const uint8_t *keys, size_t h) {
   if (!ctx || (n && (!ids || !keys))) return OPIR_EINVAL;
+  if (ctx->key_owner) return set_err(ctx, OPIR_EINVAL, "a cloned context reads its parent's client keys");
   if (int rc = check_k1(ctx)) return rc;
   if (h > 11) return set_err(ctx, OPIR_EINVAL, "BV key count out of range");
   if (!n) return OPIR_OK;
@@ -438,12 +439,12 @@ int opir_client_keygen_device(opir_ctx *ctx, size_t n, const uint64_t *ids, cons
     return set_err(ctx, OPIR_EHIP, std::string("device key generation: ") + hipGetErrorString(e));
   }
   for (size_t i = 0; i < n; i++) {
-    auto it = ctx->clients.find(ids[i]);
-    if (it != ctx->clients.end()) {
+    auto it = ctx->clients().find(ids[i]);
+    if (it != ctx->clients().end()) {
       (void)hipFree(it->second.bv);
       (void)hipFree(it->second.gsw);
     }
-    ctx->clients[ids[i]] = made[i];
+    ctx->clients()[ids[i]] = made[i];
   }
   return OPIR_OK;
 }
@@ -472,8 +473,8 @@ int opir_query_gen(opir_db *db, size_t B, const uint8_t *keys, const uint64_t *c
 
 int opir_client_get_keys(opir_ctx *ctx, uint64_t id, uint64_t *bv, size_t h, uint64_t *gsw) {
   if (!ctx || (!bv && h) || !gsw) return OPIR_EINVAL;
-  auto it = ctx->clients.find(id);
-  if (it == ctx->clients.end()) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
+  auto it = ctx->clients().find(id);
+  if (it == ctx->clients().end()) return set_err(ctx, OPIR_ERANGE, "unknown client id " + std::to_string(id));
   const ClientKeys &k = it->second;
   if (h != k.h) return set_err(ctx, OPIR_EINVAL, "BV key count differs from the resident keys");
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

@@ -109,7 +109,7 @@ extern "C" int opir_client_set_keys_seeded(opir_ctx *ctx, uint64_t id, const uin
   for (size_t r = 0; r < 2 * lk; r++) std::memcpy(&gsw[(r * 2 + 0) * KN], gsw_c0 + r * KN, KN * sizeof(uint64_t));
   int rc = opir_client_set_keys(ctx, id, bv.data(), h, gsw.data());
   if (rc) return rc;
-  const ClientKeys &k = ctx->clients[id];
+  const ClientKeys &k = ctx->clients()[id];
   const prg::Key key = key_words(seed);
   // device key rows: [b or c0 (-> c0), a or c1 (-> c1)][2 limbs][N]; the
   // uniform half is the second polynomial of each row

@@ -23,6 +23,7 @@ SCAN_AUTOTUNE = -2
 # Every symbol include/onionpir_gpu.h declares.
 EXPORTED = [
     "opir_ctx_create", "opir_ctx_destroy", "opir_last_error", "opir_ctx_set_stream", "opir_ctx_synchronize",
+    "opir_ctx_clone", "opir_db_view",
     "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt", "opir_db_load_plain_u64", "opir_db_gen_random",
     "opir_client_set_keys", "opir_client_set_keys_seeded", "opir_client_remove", "opir_client_get_keys",
     "opir_client_keygen_device", "opir_query_gen", "opir_query_gen_device",
@@ -78,6 +79,8 @@ def load_library():
         "opir_last_error": ([P], ctypes.c_char_p),
         "opir_ctx_set_stream": ([P, P], ctypes.c_int),
         "opir_ctx_synchronize": ([P], ctypes.c_int),
+        "opir_ctx_clone": ([P, ctypes.POINTER(P)], ctypes.c_int),
+        "opir_db_view": ([P, P, ctypes.POINTER(P)], ctypes.c_int),
         "opir_db_create": ([P, sz, sz, sz, sz, ctypes.POINTER(P)], ctypes.c_int),
         "opir_db_destroy": ([P], None),
         "opir_db_load_plain": ([P, P, sz, sz], ctypes.c_int),
@@ -147,19 +150,28 @@ def _u64(a, shape=None):
 class GpuContext:
     """One opir_ctx: scheme tables, client keys and a HIP stream on one device."""
 
-    def __init__(self, scheme: dict, device: int = 0):
+    def __init__(self, scheme: dict, device: int = 0, _parent=None):
         self.lib = load_library()
-        s = OpirScheme(**scheme)
         self.scheme = dict(scheme)
         self.N = scheme["poly_degree"]
         self.K = 1 if scheme.get("composite", 1) else 2  # RNS limbs per polynomial at the boundary
         self.KN = self.K * self.N
         h = ctypes.c_void_p()
-        rc = self.lib.opir_ctx_create(ctypes.byref(s), device, ctypes.byref(h))
-        if rc != OPIR_OK:
-            self._raise(rc, self.lib.opir_last_error(None))
+        if _parent is None:
+            s = OpirScheme(**scheme)
+            rc = self.lib.opir_ctx_create(ctypes.byref(s), device, ctypes.byref(h))
+            if rc != OPIR_OK:
+                self._raise(rc, self.lib.opir_last_error(None))
+        else:
+            _parent.check(self.lib.opir_ctx_clone(_parent.h, ctypes.byref(h)))
         self.h = h
+        self._parent = _parent  # a clone keeps its parent alive
         self._dbs = weakref.WeakSet()  # databases to destroy before the context
+
+    def clone(self):
+        """A context with this one's scheme and client keys but its own
+        streams (opir_ctx_clone)."""
+        return GpuContext(self.scheme, _parent=self)
 
     def _raise(self, rc, msg):
         msg = msg.decode() if isinstance(msg, bytes) else str(msg)
@@ -327,7 +339,7 @@ class GpuContext:
 class GpuDatabase:
     """One opir_db: a resident NTT-form database of one layout."""
 
-    def __init__(self, ctx: GpuContext, fst_dim_sz, other_dim_sz, num_dims, expansion_height):
+    def __init__(self, ctx: GpuContext, fst_dim_sz, other_dim_sz, num_dims, expansion_height, _view_of=None):
         self.ctx = ctx
         self.lib = ctx.lib
         self.N = ctx.N
@@ -335,10 +347,19 @@ class GpuDatabase:
         self.N0, self.Nrest, self.dims, self.h = fst_dim_sz, other_dim_sz, num_dims, expansion_height
         self.u = fst_dim_sz + ctx.scheme["l_ep"] * (num_dims - 1)
         h = ctypes.c_void_p()
-        ctx.check(self.lib.opir_db_create(ctx.h, fst_dim_sz, other_dim_sz, num_dims, expansion_height,
-                                          ctypes.byref(h)))
+        if _view_of is None:
+            ctx.check(self.lib.opir_db_create(ctx.h, fst_dim_sz, other_dim_sz, num_dims, expansion_height,
+                                              ctypes.byref(h)))
+        else:
+            ctx.check(self.lib.opir_db_view(_view_of.h, ctx.h, ctypes.byref(h)))
         self.h = h
+        self._view_of = _view_of  # a view keeps the database it shares alive
         ctx._dbs.add(self)
+
+    def view(self, ctx: "GpuContext"):
+        """A handle on `ctx` (a clone of this database's context) sharing this
+        resident database, with its own workspace (opir_db_view)."""
+        return GpuDatabase(ctx, self.N0, self.Nrest, self.dims, self.h, _view_of=self)
 
     def close(self):
         if getattr(self, "h", None):

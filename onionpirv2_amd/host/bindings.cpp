@@ -358,15 +358,17 @@ PYBIND11_MODULE(_opir_host, m) {
         py::arg("max_batch"));
 
   py::class_<ServingLoop>(m, "ServingLoop")
-      .def(py::init([](std::vector<PirServer *> dbs, size_t max_batch, size_t hbm_budget, size_t keyed_clients) {
+      .def(py::init([](std::vector<PirServer *> dbs, size_t max_batch, size_t hbm_budget, size_t keyed_clients,
+                       size_t lanes) {
              ServeConfig c;
              c.max_batch = max_batch;
              c.hbm_budget = hbm_budget;
              c.keyed_clients = keyed_clients;
+             c.lanes = lanes;
              return std::make_unique<ServingLoop>(std::move(dbs), c);
            }),
            py::arg("dbs"), py::arg("max_batch") = 64, py::arg("hbm_budget") = 0, py::arg("keyed_clients") = 0,
-           py::keep_alive<1, 2>())
+           py::arg("lanes") = 2, py::keep_alive<1, 2>())
       .def("submit",
            [](ServingLoop &l, size_t db, size_t cid, Arr q) {
              std::vector<u64> v(q.data(), q.data() + q.size());

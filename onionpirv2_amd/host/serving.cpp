@@ -36,6 +36,7 @@ std::vector<size_t> plan_batch_limits(size_t ndb, const std::function<size_t(siz
 
 ServingLoop::ServingLoop(std::vector<PirServer *> dbs, const ServeConfig &cfg) {
   t0_ = std::chrono::steady_clock::now();
+  if (cfg.lanes == 0) throw std::invalid_argument("lanes must be positive");
   // one mutex per distinct context: a context serves one host thread at a time
   std::map<opir_ctx *, std::mutex *> by_ctx;
   size_t fixed = 0;
@@ -57,20 +58,63 @@ ServingLoop::ServingLoop(std::vector<PirServer *> dbs, const ServeConfig &cfg) {
     }
     auto lane = std::make_unique<DbLane>();
     lane->srv = s;
-    lane->ctx_mu = by_ctx[ctx];
+    lane->lanes.resize(cfg.lanes);
+    lane->lanes[0].ctx = ctx;
+    lane->lanes[0].db = s->gpu_db();
+    lane->lanes[0].ctx_mu = by_ctx[ctx];
     lanes_.push_back(std::move(lane));
   }
-  auto ws = [&](size_t d, size_t B) {
-    size_t b = 0;
-    lanes_[d]->srv->gpu_scheme()->check(opir_db_workspace_bytes(lanes_[d]->srv->gpu_db(), B, &b));
-    return b;
+  auto release = [&]() {
+    for (auto &d : lanes_)
+      for (auto &l : d->lanes)
+        if (l.clone) {
+          if (l.db) opir_db_destroy(l.db);
+          if (l.ctx) opir_ctx_destroy(l.ctx);
+          l.db = nullptr;
+          l.ctx = nullptr;
+        }
   };
-  limits_ = plan_batch_limits(lanes_.size(), ws, fixed, cfg.hbm_budget, cfg.max_batch);
-  for (size_t d = 0; d < lanes_.size(); d++) {  // reserve every workspace before the first request
-    std::lock_guard<std::mutex> g(*lanes_[d]->ctx_mu);
-    lanes_[d]->srv->gpu_scheme()->check(opir_db_reserve(lanes_[d]->srv->gpu_db(), limits_[d]));
+  try {
+    // further lanes: a context clone (own streams, the parent's keys) and a
+    // view of the same resident database (own workspace)
+    for (auto &d : lanes_)
+      for (size_t k = 1; k < d->lanes.size(); k++) {
+        auto &l = d->lanes[k];
+        const GpuScheme &gs = *d->srv->gpu_scheme();
+        l.clone = true;
+        gs.check(opir_ctx_clone(d->lanes[0].ctx, &l.ctx));
+        ctx_mus_.push_back(std::make_unique<std::mutex>());
+        l.ctx_mu = ctx_mus_.back().get();
+        const int rc = opir_db_view(d->lanes[0].db, l.ctx, &l.db);
+        if (rc != OPIR_OK) {
+          const std::string msg = opir_last_error(l.ctx);
+          throw std::runtime_error("database view: " + msg);
+        }
+      }
+    // every lane of database d reserves a workspace of the admitted batch
+    auto ws = [&](size_t d, size_t B) {
+      size_t b = 0;
+      lanes_[d]->srv->gpu_scheme()->check(opir_db_workspace_bytes(lanes_[d]->srv->gpu_db(), B, &b));
+      return b * cfg.lanes;
+    };
+    limits_ = plan_batch_limits(lanes_.size(), ws, fixed, cfg.hbm_budget, cfg.max_batch);
+    for (size_t d = 0; d < lanes_.size(); d++)  // reserve every workspace before the first request
+      for (auto &l : lanes_[d]->lanes) {
+        std::lock_guard<std::mutex> g(*l.ctx_mu);
+        const int rc = opir_db_reserve(l.db, limits_[d]);
+        if (rc != OPIR_OK) {
+          const std::string msg = opir_last_error(l.ctx);
+          if (rc == OPIR_EINVAL) throw std::invalid_argument(msg);
+          throw std::runtime_error(msg);
+        }
+      }
+  } catch (...) {
+    release();
+    throw;
   }
-  for (size_t d = 0; d < lanes_.size(); d++) lanes_[d]->th = std::thread(&ServingLoop::worker, this, d);
+  for (size_t d = 0; d < lanes_.size(); d++)
+    for (size_t k = 0; k < lanes_[d]->lanes.size(); k++)
+      lanes_[d]->lanes[k].th = std::thread(&ServingLoop::worker, this, d, k);
 }
 
 ServingLoop::~ServingLoop() { stop(); }
@@ -117,13 +161,23 @@ void ServingLoop::stop() {
     stop_ = true;
     cv_work_.notify_all();
   }
-  for (auto &l : lanes_)
-    if (l->th.joinable()) l->th.join();
+  for (auto &d : lanes_)
+    for (auto &l : d->lanes)
+      if (l.th.joinable()) l.th.join();
+  for (auto &d : lanes_)
+    for (auto &l : d->lanes)
+      if (l.clone) {
+        if (l.db) opir_db_destroy(l.db);
+        if (l.ctx) opir_ctx_destroy(l.ctx);
+        l.db = nullptr;
+        l.ctx = nullptr;
+      }
 }
 
-void ServingLoop::worker(size_t d) {
+void ServingLoop::worker(size_t d, size_t li) {
   DbLane &lane = *lanes_[d];
   PirServer &srv = *lane.srv;
+  DbLane::Lane &me = lane.lanes[li];
   const size_t N = srv.get_params().get_poly_degree(), QN = N * srv.get_params().K();  // response / query values per poly
   const size_t limit = limits_[d];
   std::vector<uint64_t> ids, q, r;
@@ -153,16 +207,16 @@ void ServingLoop::worker(size_t d) {
     std::vector<int> rcs(B, OPIR_OK);
     std::vector<std::string> errs(B);
     {
-      std::lock_guard<std::mutex> g(*lane.ctx_mu);
-      rc = opir_answer_batch(srv.gpu_db(), B, ids.data(), q.data(), r.data());
-      if (rc != OPIR_OK) err = opir_last_error(srv.gpu_scheme()->ctx());
+      std::lock_guard<std::mutex> g(*me.ctx_mu);
+      rc = opir_answer_batch(me.db, B, ids.data(), q.data(), r.data());
+      if (rc != OPIR_OK) err = opir_last_error(me.ctx);
       if ((rc == OPIR_EINVAL || rc == OPIR_ERANGE) && B > 1) {
         // one request (an unknown client, say) failed the whole batch: answer
         // them one by one so that only the bad ones carry the error
         rc = OPIR_OK;
         for (size_t i = 0; i < B; i++) {
-          rcs[i] = opir_answer_batch(srv.gpu_db(), 1, &ids[i], &q[i * 2 * QN], &r[i * 2 * N]);
-          if (rcs[i] != OPIR_OK) errs[i] = opir_last_error(srv.gpu_scheme()->ctx());
+          rcs[i] = opir_answer_batch(me.db, 1, &ids[i], &q[i * 2 * QN], &r[i * 2 * N]);
+          if (rcs[i] != OPIR_OK) errs[i] = opir_last_error(me.ctx);
         }
       } else if (rc != OPIR_OK) {
         std::fill(rcs.begin(), rcs.end(), rc);

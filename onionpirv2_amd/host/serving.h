@@ -3,13 +3,17 @@
 // The reference's PirServer answers one query per make_query call
 // (src/server.cpp:895-947) and has no serving layer. This loop is the system
 // form on top of the same answer path: requests from many clients arrive at
-// any time for any of several databases; each database has a worker thread
-// that takes whatever is queued for it (up to its admitted batch size) as one
-// fused batch (opir_answer_batch), so requests that arrive while a batch is
-// on the GPU join the next one. Databases built on their own GpuScheme have
-// their own HIP stream, so batches of different databases overlap on the
-// device. Batch sizes are admitted against an HBM budget for client keys and
-// per-batch workspaces (expanded queries, selectors, first-dimension
+// any time for any of several databases. Each database has `lanes` answer
+// lanes -- its own context and database handle, and for every further lane a
+// context clone with a view of the same resident database (opir_ctx_clone,
+// opir_db_view) -- each with its own worker thread, HIP streams and
+// workspace. A free lane takes whatever is queued for its database (up to the
+// admitted batch size) as one fused batch (opir_answer_batch); while it runs,
+// requests that arrive are taken by the next free lane, so consecutive batches
+// of one database overlap on the device (one batch's expansion beside the
+// previous one's scan and MUX tree), and batches of different databases
+// overlap too. Batch sizes are admitted against an HBM budget for client keys
+// and the lanes' workspaces (expanded queries, selectors, first-dimension
 // candidates), which are reserved once up front.
 #pragma once
 #include <chrono>
@@ -31,6 +35,7 @@ struct ServeConfig {
   size_t max_batch = 64;      // upper bound per database and batch
   size_t hbm_budget = 0;      // bytes for keys + workspaces; 0 = bounded by max_batch only
   size_t keyed_clients = 0;   // clients per context counted against the budget; 0 = the keys resident at start
+  size_t lanes = 2;           // concurrent answer lanes per database (1 = one batch at a time)
 };
 
 struct ServeResult {
@@ -73,13 +78,19 @@ class ServingLoop {
     std::vector<u64> query;
     double t_submit;
   };
-  struct DbLane {
+  struct DbLane {              // one database: its queue and its answer lanes
     PirServer *srv = nullptr;
-    std::mutex *ctx_mu = nullptr;  // serialises databases that share one context
     std::deque<Req> queue;
-    std::thread th;
+    struct Lane {
+      opir_ctx *ctx = nullptr;
+      opir_db *db = nullptr;
+      bool clone = false;          // ctx / db are a clone and a view owned by the loop
+      std::mutex *ctx_mu = nullptr;  // serialises lanes that share one context
+      std::thread th;
+    };
+    std::vector<Lane> lanes;
   };
-  void worker(size_t d);
+  void worker(size_t d, size_t lane);
 
   std::vector<std::unique_ptr<DbLane>> lanes_;
   std::vector<std::unique_ptr<std::mutex>> ctx_mus_;

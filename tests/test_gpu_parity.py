@@ -505,3 +505,37 @@ def test_bv_keyswitch_matches_oracle(O, comp_params, gpu_ctx, k):
     assert np.array_equal(gpu_ctx.bv_ks(ct, k, key), want)
     with pytest.raises(InvalidArgument):
         gpu_ctx.bv_ks(ct, 2, key)
+
+
+def test_clone_and_view_answer_like_the_original(O, comp_params, gpu_ctx):
+    """opir_ctx_clone + opir_db_view: the clone reads the parent's keys, the
+    view shares the resident database; answers from two host threads, one on
+    each, equal the original's; key setters on the clone are refused."""
+    import threading
+    from onionpirv2_amd.gpu import InvalidArgument
+    say("[gpu] context clone and database view")
+    s = Setup(O, comp_params, (700, 8, True), seed=7171, gpu_ctx=gpu_ctx, cid=71)
+    p = s.p
+    qs = np.stack([s.client.fast_generate_query(p, i) for i in (3, 99, 700 - 1, 350)])
+    ids = [s.cid] * len(qs)
+    want = s.db.answer_batch(ids, qs)
+    clone = gpu_ctx.clone()
+    view = s.db.view(clone)
+    got = {}
+
+    def run(name, db):
+        got[name] = [db.answer_batch(ids, qs) for _ in range(5)]
+
+    th = [threading.Thread(target=run, args=("orig", s.db)), threading.Thread(target=run, args=("view", view))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for name in ("orig", "view"):
+        for b in got[name]:
+            assert np.array_equal(b, want), name
+    with pytest.raises(InvalidArgument):
+        clone.set_client_keys(72, s.bv, s.gsw)
+    view.close()
+    clone.close()
+    assert np.array_equal(s.db.answer(s.cid, qs[0]), want[0])

@@ -180,3 +180,36 @@ def test_bad_request_fails_alone(served):
             assert r["error"] == "", r["error"]
             assert np.array_equal(r["resp"], alone[tickets[r["ticket"]]])
     assert max(r["batch"] for r in res) > 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_lanes_overlap_batches_of_one_database(served, lanes):
+    """With several answer lanes per database (context clones sharing the
+    parent's keys, views sharing the resident database), consecutive batches
+    of one database run at the same time on the device; every response still
+    equals its answer alone."""
+    H, servers, clients, pool, alone = served
+    print(f"[gpu] serving with {lanes} lanes", flush=True)
+    loop = H.ServingLoop(servers[:1], max_batch=2, lanes=lanes)
+    try:
+        tickets = {}
+        own = [k for k, e in enumerate(pool) if e[0] == 0]
+        for rep in range(6):
+            for k in own:
+                d, cid, _, q = pool[k]
+                tickets[loop.submit(d, cid, q)] = k
+        loop.wait_idle()
+        res = loop.poll()
+    finally:
+        loop.stop()
+    assert len(res) == len(tickets)
+    for r in res:
+        assert r["error"] == ""
+        assert np.array_equal(r["resp"], alone[tickets[r["ticket"]]])
+    spans = sorted({(r["t_start"], r["t_done"]) for r in res})
+    overlapping = sum(1 for a, b in zip(spans, spans[1:]) if b[0] < a[1])
+    if lanes == 1:
+        assert overlapping == 0
+    else:
+        assert overlapping > 0, spans
