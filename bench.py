@@ -52,10 +52,13 @@ def parse():
     ap.add_argument("--serve-k2-cus", type=int, default=224,
                     help="serving: compute units the n=4096 database's context may use during the serving leg "
                          "(0 = all); the rest stay free for the n=2048 databases' short kernels")
-    ap.add_argument("--serve-k2-entries", type=int, default=2,
-                    help="pool entries for the n=4096 database in the serving mix (of 128 + this many)")
+    ap.add_argument("--serve-k2-entries", type=int, default=1,
+                    help="0 leaves the n=4096 database out of the serving mix")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per rate")
-    ap.add_argument("--serve-clients", type=int, default=512)
+    ap.add_argument("--serve-clients", type=int, default=512,
+                    help="seeded clients spread over the serving databases (the n=4096 one gets the k2 leg's)")
+    ap.add_argument("--serve-lanes", type=int, default=1,
+                    help="concurrent answer lanes per database in the serving loop")
     ap.add_argument("--serve-budget-gib", type=float, default=96.0,
                     help="HBM budget for client keys and batch workspaces of the serving loop")
     ap.add_argument("--no-k2", action="store_true", help="skip the CONFIG_N4096_K2_MP leg")
@@ -160,42 +163,36 @@ def graph_leg(args, G, ctx_h, answer, torch, B, d_r, server, queries, log):
 def serving(args, H, params, server, log, k2=None):
     """Continuous batching over resident databases: the benchmark DB
     (256 x 1024, n=2048), a 256 x 256 one (1 GiB NTT form) and, unless
-    --no-k2, the n=4096 two-prime DB (512 x 4096, 128 GiB NTT form); many
-    clients with their own keys on the device, Poisson arrivals swept over
-    rates. Every response is checked against the same query answered alone."""
+    --no-k2, the n=4096 two-prime DB (512 x 4096, 128 GiB NTT form).
+    --serve-clients seeded clients are spread over the three: the n=4096 one
+    gets the k2 leg's clients, the rest split evenly between the n=2048 ones,
+    each keyed only on its database (keys generated on the device from the
+    client seeds); every client has one query in the pool, and Poisson
+    arrivals draw uniformly from the pool, so each database's share of the
+    traffic is its share of the clients. Every response is checked against
+    the same query answered alone."""
     import numpy as np
-    from concurrent.futures import ThreadPoolExecutor
     p2 = H.PirParams().with_layout(65536, 9, True)
     s2 = H.PirServer(p2)
-    s2.gen_data([], args.seed + 11)
+    s2.gen_data_device([], args.seed + 11)
     servers, plist = [server, s2], [params, p2]
-    nc = args.serve_clients
-    t0 = time.time()
-
-    def mk(j):
-        c = H.PirClient(params, 5000 + j, args.seed + 1000003 * (j + 1))
-        return c, c.create_bv_galois_keys(), c.generate_gsw_from_key()
-
-    with ThreadPoolExecutor(8) as ex:
-        cl = list(ex.map(mk, range(nc)))
-    for j, (c, bv, gsw) in enumerate(cl):
-        for s in servers:
-            s.set_client_bv_galois_key(5000 + j, bv)
-            s.set_client_gsw_key(5000 + j, gsw)
-    per_db = 64
-    entries = [(d, (d * per_db + k) % nc) for d in range(2) for k in range(per_db)]
-
-    def mq(e):
-        d, j = e
-        idx = splitmix64(args.seed ^ (d * 1000 + j)) % plist[d].target_num_pt
-        return cl[j][0].fast_generate_query(plist[d], idx)
-
-    with ThreadPoolExecutor(8) as ex:
-        qs = list(ex.map(mq, entries))
-    ids = [5000 + j for _, j in entries]
     names = ["256x1024 n=2048 (4 GiB NTT form)", "256x256 n=2048 (1 GiB NTT form)"]
+    nk2 = len(k2["queries"]) if (k2 is not None and args.serve_k2_entries > 0) else 0
+    nc = args.serve_clients - nk2
+    t0 = time.time()
+    entries, ids, qs = [], [], []
+    for d in range(2):
+        cids = [5000 + 100000 * d + j for j in range(d, nc, 2)]
+        keys = [(args.seed + 7919 * c).to_bytes(8, "little") * 4 for c in cids]
+        servers[d].client_keygen_device(cids, keys)
+        for c, k in zip(cids, keys):
+            cl = H.PirClient(plist[d], c, k)
+            idx = splitmix64(args.seed ^ c) % plist[d].target_num_pt
+            entries.append(d)
+            ids.append(c)
+            qs.append(cl.fast_generate_query(plist[d], idx))
     k2_masked = None
-    if k2 is not None and args.serve_k2_entries > 0:
+    if nk2:
         if args.serve_k2_cus > 0:
             import ctypes
             from onionpirv2_amd import gpu as G
@@ -207,25 +204,26 @@ def serving(args, H, params, server, log, k2=None):
             assert G.ctx_call_raw("opir_ctx_set_cu_mask", k2_masked, arr, len(words)) == 0
         servers.append(k2["server"])
         names.append("512x4096 n=4096 K=2 (128 GiB NTT form)")
-        for j in range(min(args.serve_k2_entries, len(k2["queries"]))):
-            entries.append((2, j))
+        for j in range(nk2):
+            entries.append(2)
             ids.append(7000 + j)
             qs.append(k2["queries"][j])
-    alone = [servers[d].make_query(cid, q.copy()) for (d, _), cid, q in zip(entries, ids, qs)]
-    log(f"[serve] {nc} clients keyed on the n=2048 databases, {len(entries)}-query pool over {len(servers)} "
-        f"databases answered alone, {time.time() - t0:.1f}s")
-    loop = H.ServingLoop(servers, max_batch=64, hbm_budget=int(args.serve_budget_gib * 2**30), keyed_clients=0)
+    alone = [servers[d].make_query(cid, q.copy()) for d, cid, q in zip(entries, ids, qs)]
+    log(f"[serve] {nc} n=2048 clients keyed on the device ({time.time() - t0:.1f}s) and {nk2} n=4096 clients; "
+        f"{len(entries)}-query pool over {len(servers)} databases answered alone")
+    loop = H.ServingLoop(servers, max_batch=64, hbm_budget=int(args.serve_budget_gib * 2**30), keyed_clients=0,
+                         lanes=args.serve_lanes)
     rows = []
     try:
         for lam in [float(x) for x in args.serve_lambdas.split(",") if x]:
             # queries of different schemes differ in length: the loop takes a list
-            run = loop.run_poisson([d for d, _ in entries], ids, qs, lam, args.serve_seconds, args.seed + int(lam))
+            run = loop.run_poisson(entries, ids, qs, lam, args.serve_seconds, args.seed + int(lam))
             n = len(run["pool_index"])
             exact = sum(np.array_equal(run["resp"][i], alone[run["pool_index"][i]]) for i in range(n))
             lat = run["latency_ms"].tolist()
             per_db = {}
             for d in range(len(servers)):
-                ld = [lat[i] for i in range(n) if entries[run["pool_index"][i]][0] == d]
+                ld = [lat[i] for i in range(n) if entries[run["pool_index"][i]] == d]
                 if ld:
                     per_db[names[d]] = {"requests": len(ld), "p50_ms": round(pct(ld, 50), 3),
                                         "p99_ms": round(pct(ld, 99), 3)}
@@ -242,8 +240,10 @@ def serving(args, H, params, server, log, k2=None):
         loop.stop()
         if k2_masked is not None:  # the n=4096 leg runs unrestricted
             assert G.ctx_call_raw("opir_ctx_set_cu_mask", k2_masked, None, 0) == 0
-    return {"databases": names, "clients_n2048": nc, "n4096_cus": args.serve_k2_cus if k2_masked else None,
-            "clients_n4096": min(args.serve_k2_entries, len(k2["queries"])) if k2 is not None else 0,
+    return {"databases": names, "clients": {names[d]: entries.count(d) for d in range(len(servers))},
+            "client_keys": "seeded clients; n=2048 keys generated on the device, n=4096 keys on the host",
+            "lanes_per_database": args.serve_lanes,
+            "n4096_cus": args.serve_k2_cus if k2_masked else None,
             "hbm_budget_gib": args.serve_budget_gib, "batch_limits": limits, "rates": rows}
 
 

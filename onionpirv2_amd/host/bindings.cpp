@@ -294,6 +294,12 @@ PYBIND11_MODULE(_opir_host, m) {
              }
              return ct_arr(r);
            })
+      .def("client_keygen_device",
+           [](PirServer &s, std::vector<size_t> ids, std::vector<py::bytes> keys) {
+             std::vector<prg::Seed> k;
+             for (auto &b : keys) k.push_back(to_seed(b));
+             s.client_keygen_device(ids, k);
+           })
       .def("set_client_keys_wire",
            [](PirServer &s, size_t id, py::bytes w) { s.set_client_keys_wire(id, std::string(w)); })
       .def("make_query_wire",

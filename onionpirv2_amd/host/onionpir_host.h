@@ -268,6 +268,9 @@ class PirServer {
   // device regenerates (they live on the device only), and a query whose c1
   // is regenerated from its seed.
   void set_client_keys_wire(size_t client_id, const std::string &wire);
+  // Keys of seeded clients (PirClient(params, id, key)) generated on the GPU
+  // (opir_client_keygen_device), for load generation; device-only like wire keys.
+  void client_keygen_device(const std::vector<size_t> &client_ids, const std::vector<std::array<uint8_t, 32>> &keys);
   RlweCt make_query_wire(size_t client_id, const std::string &wire);
   SharedPirSessionKeys client_session_keys(size_t client_id) const;
   RlwePt direct_get_original_plaintext(size_t index) const;

@@ -238,6 +238,20 @@ void PirServer::set_client_keys_wire(size_t id, const std::string &wire) {
   wire_keyed_.insert(id);
 }
 
+void PirServer::client_keygen_device(const std::vector<size_t> &ids, const std::vector<std::array<uint8_t, 32>> &keys) {
+  if (ids.size() != keys.size()) throw std::invalid_argument("one key seed per client");
+  std::vector<uint64_t> id64(ids.begin(), ids.end());
+  std::vector<uint8_t> kb(32 * keys.size());
+  for (size_t i = 0; i < keys.size(); i++) std::memcpy(&kb[32 * i], keys[i].data(), 32);
+  scheme_->check(opir_client_keygen_device(scheme_->ctx(), ids.size(), id64.data(), kb.data(), params_.get_expan_height()));
+  for (size_t id : ids) {
+    bv_keys_.erase(id);
+    gsw_keys_.erase(id);
+    sessions_.erase(id);
+    wire_keyed_.insert(id);
+  }
+}
+
 RlweCt PirServer::make_query_wire(size_t id, const std::string &wire) {
   RlweCt q = codec::decode_query(params_, wire);
   return make_query(id, q);
