@@ -233,6 +233,27 @@ int opir_db_tuned_scan_variant(const opir_db *db, size_t B, int *out);
  * autotuned run without a graph. Off by default. */
 int opir_ctx_set_graphs(opir_ctx *ctx, int on);
 
+/* ---- Tree PIR on the same kernels (32-bit-limb schemes) ----
+ * The server side of the reference's full-binary-tree PIR layer for a tree of
+ * 2^L leaves with first dimension N0 = 2^a (TreePirParams from
+ * make_tree_pir_params_for_scheme, src/tree_query.cpp:44-61, ell_beta =
+ * ell_gamma = l_ep; L >= 11 + a at N = 2048).
+ * load_level: level l's R_l packed plaintexts [R_l][N] (< t), the layout of
+ *   pack_tree_level (src/tree_select.cpp:52-74), R_l = 2^max(l - 11, 0).
+ * select: one packed tree query [2][K*N] (make_tree_query,
+ *   src/tree_query.cpp:140-196) of a client whose keys reach expansion height
+ *   h_q; out receives, for every level l = 0..L, select_level's ciphertext
+ *   [2][K*N] in coefficient form (unpack_tree_query + build_alpha_pyramid +
+ *   select_level, src/tree_query.cpp:198-240, src/tree_select.cpp:121-238).
+ * shape: the packed width w, expansion height h_q, N0 and selector count
+ *   b + r of the tree's query. */
+typedef struct opir_tree opir_tree;
+int opir_tree_create(opir_ctx *ctx, size_t L, size_t a, opir_tree **out);
+void opir_tree_destroy(opir_tree *tree);
+int opir_tree_shape(const opir_tree *tree, size_t *w, size_t *h_q, size_t *N0, size_t *nsel);
+int opir_tree_load_level(opir_tree *tree, size_t level, const uint64_t *coeffs);
+int opir_tree_select(opir_tree *tree, uint64_t client_id, const uint64_t *query, uint64_t *out);
+
 /* ---- Stage entry points (host buffers), one per reference stage ---- */
 
 /* utils::ntt_fwd / ntt_inv over Q (src/utils.cpp:84-90), in place on

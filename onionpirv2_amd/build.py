@@ -26,8 +26,8 @@ HIP_FLAGS = [
 ]
 GPU_SRC = ["csrc/opir_kernels.hip", "csrc/opir_tree.hip", "csrc/opir_scan.hip", "csrc/opir_k2.hip",
            "csrc/opir_capi.hip", "csrc/opir_capi_k2.hip", "csrc/opir_noalloc.hip",
-           "csrc/opir_seed.hip", "csrc/opir_clientgen.hip"]
-HOST_SRC = ["host/params.cpp", "host/ring.cpp", "host/prg.cpp", "host/codec.cpp", "host/client.cpp", "host/server.cpp",
+           "csrc/opir_seed.hip", "csrc/opir_clientgen.hip", "csrc/opir_tree_pir.hip"]
+HOST_SRC = ["host/params.cpp", "host/ring.cpp", "host/prg.cpp", "host/codec.cpp", "host/client.cpp", "host/server.cpp", "host/tree.cpp",
             "host/serving.cpp", "host/bindings.cpp"]
 
 

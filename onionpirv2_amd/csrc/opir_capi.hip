@@ -33,6 +33,10 @@ inline void poly_to_rns(const opir_ctx *ctx, const uint64_t *src, u32 *dst) {
   }
 }
 
+}  // namespace
+
+namespace capi {
+
 int upload_k1(opir_ctx *ctx, const uint64_t *ids, size_t B, BatchPtrs &P) {
   const void *const *t = nullptr;
   const int rc = upload_key_ptrs(ctx, ids, B, &t);
@@ -102,6 +106,10 @@ u32 *run_expand(opir_db *db, size_t B, const BatchPtrs &P) {
   }
   return in;
 }
+
+}  // namespace capi
+
+namespace {
 
 // MUX tree (server.cpp:599-649) over NTT-form candidates in mid_a.
 // Returns the buffer whose slot 0 (per query) holds the result.

@@ -354,6 +354,15 @@ struct DevBuf {
 };
 #define DEV_ALLOC(ctx, buf, bytes) HIP_TRY(ctx, hipMalloc(&(buf).p, (bytes)))
 
+// Composite / MP scheme answer-path pieces shared with the tree PIR entry
+// points (opir_capi.hip): key-pointer upload for B queries, the answer
+// workspace of db for B queries, and the expansion of the root cts in
+// db->exp_a (returns the buffer holding the leaves).
+int upload_k1(opir_ctx *ctx, const uint64_t *ids, size_t B, BatchPtrs &P);
+size_t ws_bytes(const opir_db *db, size_t B);
+int ensure_ws(opir_db *db, size_t B);
+u32 *run_expand(opir_db *db, size_t B, const BatchPtrs &P);
+
 // Resolves ctx->scratch for ctx->device (the device must be current); once
 // per device, at context creation (opir_noalloc.hip).
 int noalloc_init(opir_ctx *ctx);

@@ -24,6 +24,7 @@ SCAN_AUTOTUNE = -2
 EXPORTED = [
     "opir_ctx_create", "opir_ctx_destroy", "opir_last_error", "opir_ctx_set_stream", "opir_ctx_synchronize",
     "opir_ctx_clone", "opir_db_view",
+    "opir_tree_create", "opir_tree_destroy", "opir_tree_shape", "opir_tree_load_level", "opir_tree_select",
     "opir_db_create", "opir_db_destroy", "opir_db_load_plain", "opir_db_load_ntt", "opir_db_load_plain_u64", "opir_db_gen_random",
     "opir_client_set_keys", "opir_client_set_keys_seeded", "opir_client_remove", "opir_client_get_keys",
     "opir_client_keygen_device", "opir_query_gen", "opir_query_gen_device",
@@ -81,6 +82,11 @@ def load_library():
         "opir_ctx_synchronize": ([P], ctypes.c_int),
         "opir_ctx_clone": ([P, ctypes.POINTER(P)], ctypes.c_int),
         "opir_db_view": ([P, P, ctypes.POINTER(P)], ctypes.c_int),
+        "opir_tree_create": ([P, sz, sz, ctypes.POINTER(P)], ctypes.c_int),
+        "opir_tree_destroy": ([P], None),
+        "opir_tree_shape": ([P] + [ctypes.POINTER(sz)] * 4, ctypes.c_int),
+        "opir_tree_load_level": ([P, sz, P], ctypes.c_int),
+        "opir_tree_select": ([P, ctypes.c_uint64, P, P], ctypes.c_int),
         "opir_db_create": ([P, sz, sz, sz, sz, ctypes.POINTER(P)], ctypes.c_int),
         "opir_db_destroy": ([P], None),
         "opir_db_load_plain": ([P, P, sz, sz], ctypes.c_int),
@@ -463,6 +469,46 @@ class GpuDatabase:
         s = _u64(sel, (ns, 2 * self.ctx.scheme["l_ep"], 2 * self.KN)) if ns else np.zeros(1, dtype=np.uint64)
         out = np.zeros((2, self.KN), dtype=np.uint64)
         self.ctx.check(self.lib.opir_other_dim(self.h, _ptr(m), _ptr(s), _ptr(out)))
+        return out
+
+
+class GpuTree:
+    """One opir_tree: the per-level packed plaintexts of a tree PIR database."""
+
+    def __init__(self, ctx: GpuContext, L, a):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.L, self.a = L, a
+        h = ctypes.c_void_p()
+        ctx.check(self.lib.opir_tree_create(ctx.h, L, a, ctypes.byref(h)))
+        self.h = h
+        vals = [ctypes.c_size_t() for _ in range(4)]
+        ctx.check(self.lib.opir_tree_shape(h, *[ctypes.byref(v) for v in vals]))
+        self.w, self.h_q, self.N0, self.nsel = (v.value for v in vals)
+        ctx._dbs.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.opir_tree_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_level(self, level, plaintexts):
+        a = _u64(plaintexts)
+        if a.size % self.ctx.N:
+            raise InvalidArgument("plaintexts must hold whole polynomials")
+        self.ctx.check(self.lib.opir_tree_load_level(self.h, level, _ptr(a)))
+
+    def select(self, client_id, query):
+        """Every level's selected ciphertext, [L + 1, 2, K*N], coefficient form."""
+        q = _u64(query, (2, self.ctx.KN))
+        out = np.zeros((self.L + 1, 2, self.ctx.KN), np.uint64)
+        self.ctx.check(self.lib.opir_tree_select(self.h, client_id, _ptr(q), _ptr(out)))
         return out
 
 

@@ -14,6 +14,9 @@ try:
     from _opir_host import (GpuScheme, PirClient, PirParams, PirServer, ServingLoop, chacha20_block,  # noqa: F401
                             decode_keys, decode_query, encode_keys, encode_query, expand_keys, gaussian_cdt,
                             keys_wire_bytes, pack_bits, plan_batch_limits, prg_gaussian, prg_ternary, prg_uniform,
-                            query_seed, query_wire_bytes, unpack_bits)
+                            query_seed, query_wire_bytes, unpack_bits, TreePirParams, TreePirServer,
+                            make_tree_pir_params_for_scheme, tree_query_expansion_params, client_index,
+                            build_level_plans, synthetic_tree_node_value, pack_tree_level_synthetic,
+                            make_tree_query)
 except ImportError as e:  # pragma: no cover
     raise ImportError(f"onionpirv2_amd native host module is not built in {_LIB}: {e}") from e

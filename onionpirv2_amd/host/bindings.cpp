@@ -11,6 +11,7 @@
 #include "codec.h"
 #include "onionpir_host.h"
 #include "serving.h"
+#include "tree.h"
 
 namespace py = pybind11;
 using namespace onionpir;
@@ -81,6 +82,74 @@ static py::bytes seed_bytes(const prg::Seed &s) { return py::bytes(reinterpret_c
 
 PYBIND11_MODULE(_opir_host, m) {
   m.doc() = "Host API of the MI355X OnionPIRv2 server (PirParams, PirClient, PirServer)";
+  // tree PIR (tree.h)
+  py::class_<TreePirParams>(m, "TreePirParams")
+      .def_readonly("N", &TreePirParams::N).def_readonly("L", &TreePirParams::L).def_readonly("n", &TreePirParams::n)
+      .def_readonly("r", &TreePirParams::r).def_readonly("N0", &TreePirParams::N0).def_readonly("a", &TreePirParams::a)
+      .def_readonly("P", &TreePirParams::P).def_readonly("B", &TreePirParams::B).def_readonly("b", &TreePirParams::b)
+      .def_readonly("w", &TreePirParams::w).def_readonly("W", &TreePirParams::W).def_readonly("h_q", &TreePirParams::h_q)
+      .def_readonly("t", &TreePirParams::t);
+  m.def("make_tree_pir_params_for_scheme", &make_tree_pir_params_for_scheme);
+  m.def("tree_query_expansion_params", &tree_query_expansion_params);
+  m.def("client_index", [](size_t leaf, const TreePirParams &p) {
+    const ClientCoordinates c = client_index(leaf, p);
+    return py::make_tuple(c.alpha, std::vector<int>(c.beta_bits_le.begin(), c.beta_bits_le.end()),
+                          std::vector<int>(c.gamma_bits_le.begin(), c.gamma_bits_le.end()));
+  });
+  m.def("build_level_plans", [](const TreePirParams &p) {
+    py::list out;
+    for (const LevelPlan &pl : build_level_plans(p)) {
+      py::dict d;
+      d["level"] = pl.level; d["R"] = pl.R; d["case"] = (int)pl.select_case; d["coarsen"] = pl.coarsen_count;
+      d["beta_begin"] = pl.beta_begin; d["beta_count"] = pl.beta_count;
+      out.append(d);
+    }
+    return out;
+  });
+  m.def("synthetic_tree_node_value", &synthetic_tree_node_value);
+  m.def("pack_tree_level_synthetic", [](size_t level, const TreePirParams &p) {
+    const auto d = pack_tree_level(level, p, [&](size_t l, size_t i) { return synthetic_tree_node_value(l, i, p.t); });
+    Arr a({(py::ssize_t)d.size(), (py::ssize_t)p.n});
+    for (size_t i = 0; i < d.size(); i++) std::memcpy(a.mutable_data() + i * p.n, d[i].data(), p.n * 8);
+    return a;
+  });
+  m.def("make_tree_query", [](PirClient &c, const PirParams &scheme, const TreePirParams &t, size_t leaf) {
+    return ct_arr(make_tree_query(c, scheme, t, leaf));
+  });
+  py::class_<TreePirServer>(m, "TreePirServer")
+      .def(py::init([](const PirParams &scheme, const TreePirParams &t) {
+             return std::make_unique<TreePirServer>(scheme, t, nullptr);
+           }))
+      .def("preprocess_synthetic",
+           [](TreePirServer &s) {
+             const uint64_t t = s.tree().t;
+             s.preprocess([t](size_t l, size_t i) { return synthetic_tree_node_value(l, i, t); });
+           })
+      .def("load_level",
+           [](TreePirServer &s, size_t level, Arr pts) {
+             std::vector<std::vector<uint64_t>> v(pts.shape(0));
+             for (size_t p = 0; p < v.size(); p++) v[p].assign(pts.data() + p * pts.shape(1), pts.data() + (p + 1) * pts.shape(1));
+             s.load_level(level, v);
+           })
+      .def("set_client_keys",
+           [](TreePirServer &s, size_t id, Arr bv, Arr g) {
+             s.gpu_scheme()->set_client_keys(id, arr_bv(bv, s.tree().n), arr_gsw(g));
+           })
+      .def("select_path", [](TreePirServer &s, size_t id, Arr q) {
+        std::vector<RlweCt> r;
+        {
+          RlweCt ct = arr_ct(q);
+          py::gil_scoped_release nogil;
+          r = s.select_path(id, ct);
+        }
+        const size_t KN = r.empty() ? 0 : r[0].c0.size();
+        Arr a({(py::ssize_t)r.size(), (py::ssize_t)2, (py::ssize_t)KN});
+        for (size_t l = 0; l < r.size(); l++) {
+          std::memcpy(a.mutable_data() + l * 2 * KN, r[l].c0.data(), KN * 8);
+          std::memcpy(a.mutable_data() + l * 2 * KN + KN, r[l].c1.data(), KN * 8);
+        }
+        return a;
+      });
   // counter-based sampler (prg.h)
   m.def("chacha20_block", [](py::bytes key, uint32_t counter, std::array<uint32_t, 3> nonce) {
     uint32_t w[16];

@@ -178,15 +178,28 @@ std::vector<size_t> PirClient::get_query_indices(const PirParams &qp, size_t pt_
 
 RlweCt PirClient::fast_generate_query(size_t pt_idx) { return fast_generate_query(params_, pt_idx); }
 
+RlweCt PirClient::fresh_zero_ct() {
+  RlweCt ct;
+  encrypt_zero(ct);
+  return ct;
+}
+
 RlweCt PirClient::fast_generate_query(const PirParams &qp, size_t pt_idx) {
-  // client.cpp:107-166: Delta * (2^h)^-1 (mod t rounding) at BitRev(column)
+  // client.cpp:107-166
   if (!params_.scheme_compatible(qp))
     throw std::invalid_argument("PirClient query layout is not scheme-compatible with its session");
   if (pt_idx >= qp.get_num_pt()) throw std::out_of_range("PIR plaintext index exceeds the runtime layout");
+  return generate_query_for_indices(qp, get_query_indices(qp, pt_idx));
+}
+
+RlweCt PirClient::generate_query_for_indices(const PirParams &qp, const std::vector<size_t> &idx) {
+  // Delta * (2^h)^-1 (mod t rounding) at BitRev(column), then the selector rows
+  if (!params_.scheme_compatible(qp))
+    throw std::invalid_argument("PirClient query layout is not scheme-compatible with its session");
   const size_t N = qp.get_poly_degree(), K = qs_.size();
   const u64 t = qp.get_plain_mod();
-  const auto idx = get_query_indices(qp, pt_idx);
   const size_t h = qp.get_expan_height();
+  if (idx.empty() || idx[0] >= (size_t{1} << h)) throw std::invalid_argument("query column is outside the capacity");
   u64 inv = 0;
   utils::try_invert_uint_mod(size_t{1} << h, t, inv);
   const size_t slot = utils::bit_reverse(idx[0], h);

@@ -193,6 +193,11 @@ class PirClient {
   RlweCt fast_generate_query(size_t pt_idx);
   RlweCt fast_generate_query(const PirParams &query_params, size_t pt_idx);
   void add_gsw_to_query(const PirParams &qp, RlweCt &query, const std::vector<size_t> &idx);
+  // fast_generate_query with explicit query indices: {first-dim column,
+  // selector bits...} of layout qp (what get_query_indices yields for a
+  // plaintext index; tree queries pass alpha, beta and gamma bits).
+  RlweCt generate_query_for_indices(const PirParams &qp, const std::vector<size_t> &idx);
+  RlweCt fresh_zero_ct();  // client.cpp:290-299
   bvks::BvGaloisKeys create_bv_galois_keys();
   GSWCt generate_gsw_from_key();
   SharedPirSessionKeys create_session_keys();

@@ -252,4 +252,35 @@ struct PirServer {
 
 size_t small_q_width(u64 small_q);  // ceil(log2(small_q)), server.cpp:963-964
 
+// ---------------------------------------------------------------------------
+// Tree PIR. src/tree_index.cpp, src/tree_query.cpp, src/tree_select.cpp
+// ---------------------------------------------------------------------------
+struct TreeParams {
+  size_t L = 0, N = 0, n = 0, r = 0, a = 0, N0 = 0, P = 0, B = 0, b = 0, ell = 0, w = 0, W = 0, h_q = 0;
+  u64 t = 0;
+};
+struct TreeCoords {
+  size_t alpha = 0;
+  std::vector<uint8_t> beta, gamma;  // little-endian bits
+};
+struct TreePlan {
+  size_t level = 0, R = 0;
+  int select_case = 0;  // 0 Single, 1 CoarsenedAlpha, 2 AlphaBeta
+  size_t coarsen = 0, beta_begin = 0, beta_count = 0;
+};
+struct TreeLevelOracle {
+  size_t node = 0, plaintext = 0, position = 0;
+};
+TreeParams make_tree_params(size_t L, size_t a, const PirParams &scheme);   // tree_query.cpp:44-61
+TreeCoords tree_client_index(size_t leaf, const TreeParams &p);            // tree_index.cpp:150-170
+std::vector<TreePlan> tree_level_plans(const TreeParams &p);               // tree_index.cpp:172-215
+u64 synthetic_tree_node(size_t level, size_t index, u64 t);                // tree_select.cpp:41-50
+std::vector<std::vector<u64>> pack_tree_level(size_t level, const TreeParams &p);  // tree_select.cpp:52-74
+TreeLevelOracle tree_level_oracle(size_t leaf, size_t level, const TreeParams &p); // tree_index.cpp:217-241
+RlweCt make_tree_query(PirClient &c, const TreeParams &tp, const PirParams &qview, size_t leaf); // tree_query.cpp:140-196
+RlweCt tree_pt_ct_mul(const std::vector<u64> &pt, const RlweCt &ct, const PirParams &p);        // tree_select.cpp:86-119
+// unpack_tree_query + build_alpha_pyramid + select_level for every level 0..L
+std::vector<RlweCt> tree_select_all(PirServer &srv, const TreeParams &tp, size_t id, const RlweCt &query,
+                                    const std::vector<std::vector<std::vector<u64>>> &levels);
+
 }  // namespace oracle

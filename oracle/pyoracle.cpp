@@ -308,6 +308,54 @@ PYBIND11_MODULE(onionpir_oracle, m) {
     return ct_to_arr(c);
   });
 
+  // tree PIR
+  py::class_<TreeParams>(m, "TreeParams")
+      .def(py::init(&make_tree_params), py::arg("L"), py::arg("a"), py::arg("scheme"))
+      .def_readonly("L", &TreeParams::L).def_readonly("N", &TreeParams::N).def_readonly("n", &TreeParams::n)
+      .def_readonly("r", &TreeParams::r).def_readonly("a", &TreeParams::a).def_readonly("N0", &TreeParams::N0)
+      .def_readonly("P", &TreeParams::P).def_readonly("B", &TreeParams::B).def_readonly("b", &TreeParams::b)
+      .def_readonly("ell", &TreeParams::ell).def_readonly("w", &TreeParams::w).def_readonly("W", &TreeParams::W)
+      .def_readonly("h_q", &TreeParams::h_q).def_readonly("t", &TreeParams::t);
+  m.def("tree_client_index", [](size_t leaf, const TreeParams &p) {
+    const TreeCoords c = tree_client_index(leaf, p);
+    return py::make_tuple(c.alpha, std::vector<int>(c.beta.begin(), c.beta.end()),
+                          std::vector<int>(c.gamma.begin(), c.gamma.end()));
+  });
+  m.def("tree_level_plans", [](const TreeParams &p) {
+    py::list out;
+    for (const TreePlan &pl : tree_level_plans(p)) {
+      py::dict d;
+      d["level"] = pl.level; d["R"] = pl.R; d["case"] = pl.select_case; d["coarsen"] = pl.coarsen;
+      d["beta_begin"] = pl.beta_begin; d["beta_count"] = pl.beta_count;
+      out.append(d);
+    }
+    return out;
+  });
+  m.def("synthetic_tree_node", &synthetic_tree_node);
+  m.def("pack_tree_level", [](size_t level, const TreeParams &p) {
+    const auto d = pack_tree_level(level, p);
+    U64Arr a({(py::ssize_t)d.size(), (py::ssize_t)p.n});
+    for (size_t i = 0; i < d.size(); i++) std::memcpy(a.mutable_data() + i * p.n, d[i].data(), p.n * 8);
+    return a;
+  });
+  m.def("tree_level_oracle", [](size_t leaf, size_t level, const TreeParams &p) {
+    const TreeLevelOracle o = tree_level_oracle(leaf, level, p);
+    return py::make_tuple(o.node, o.plaintext, o.position);
+  });
+  m.def("make_tree_query", [](PirClient &c, const TreeParams &tp, const PirParams &qview, size_t leaf) {
+    return ct_to_arr(make_tree_query(c, tp, qview, leaf));
+  });
+  m.def("tree_pt_ct_mul", [](U64Arr pt, U64Arr ct, const PirParams &p) {
+    return ct_to_arr(tree_pt_ct_mul(arr_to_vec(pt), arr_to_ct(ct), p));
+  });
+  m.def("tree_select_all", [](PirServer &srv, const TreeParams &tp, size_t id, U64Arr q) {
+    std::vector<std::vector<std::vector<u64>>> levels;
+    for (size_t l = 0; l <= tp.L; l++) levels.push_back(pack_tree_level(l, tp));
+    py::list out;
+    for (const RlweCt &ct : tree_select_all(srv, tp, id, arr_to_ct(q), levels)) out.append(ct_to_arr(ct));
+    return out;
+  });
+
   py::class_<PirClient>(m, "PirClient")
       .def(py::init<const PirParams &, size_t, u64>(), py::arg("params"), py::arg("client_id"), py::arg("seed"))
       .def_readonly("client_id", &PirClient::client_id)
