@@ -33,7 +33,10 @@
 //  - LDS-shared (MFMA-bound, large batches): CB column tiles; per level and k
 //    block the tiles' B fragments are staged once in LDS (double-buffered)
 //    and shared by the 4 waves, each of which multiplies its A fragments
-//    (prefetched one k block ahead) into CB x (2P-1) accumulator tiles.
+//    (prefetched one k block ahead) into CB x (2P-1) accumulator tiles;
+//  - register-blocked direct (large batches): each wave multiplies R row
+//    tiles by C column tiles from its own registers, R * C * P^2 MFMAs per
+//    R * P + C * P fragments loaded.
 // The shape is picked per batch size from a table (launch_scan).
 #include <type_traits>
 
@@ -272,7 +275,7 @@ constexpr int kLevelsPerWG = 16;
 // With W = 16 / sizeof(Res) dividing LV, as 16-byte stores, consecutive
 // lanes taking consecutive pieces of a run (its index, a multiple of W,
 // makes each piece 16-byte aligned); rows past Nrest are not stored.
-template <int LV, int SW, typename Res>
+template <int LV, bool NTS = true, int SW, typename Res>
 __device__ __forceinline__ void flush_stage(const Res (*st)[16][SW], int lane, int ncols, int rt, int col0, int limb,
                                             int lvo, Res *mid, size_t mid_stride, size_t ct_vals, int nlv,
                                             size_t Nrest) {
@@ -291,8 +294,12 @@ __device__ __forceinline__ void flush_stage(const Res (*st)[16][SW], int lane, i
       v4i x;
       __builtin_memcpy(&x, v, 16);
       // nontemporal: the results are read by the next kernel, not by this one
-      __builtin_nontemporal_store(x, reinterpret_cast<v4i *>(mid + qi * mid_stride + row * ct_vals +
-                                                             (size_t)(pp * 2 + limb) * nlv + lvo + ch * W));
+      v4i *dst = reinterpret_cast<v4i *>(mid + qi * mid_stride + row * ct_vals + (size_t)(pp * 2 + limb) * nlv +
+                                         lvo + ch * W);
+      if (NTS)
+        __builtin_nontemporal_store(x, dst);
+      else
+        *dst = x;
     }
   } else {
     for (int e = lane; e < 16 * ncols; e += 64) {
@@ -302,14 +309,19 @@ __device__ __forceinline__ void flush_stage(const Res (*st)[16][SW], int lane, i
       const int c = col0 + nn, qi = c >> 1, pp = c & 1;
       Res *dst = mid + qi * mid_stride + row * ct_vals + (size_t)(pp * 2 + limb) * nlv + lvo;
 #pragma unroll
-      for (int li = 0; li < LV; li++) __builtin_nontemporal_store(st[li][rr][nn], dst + li);
+      for (int li = 0; li < LV; li++) {
+        if (NTS)
+          __builtin_nontemporal_store(st[li][rr][nn], dst + li);
+        else
+          dst[li] = st[li][rr][nn];
+      }
     }
   }
 }
 
 // NT: nontemporal A loads, for grids where each DB tile has one reader
 // (one column block); with several, the tile's later readers find it in L2.
-template <int CB, int LV, bool NT, typename Epi, typename SchemeT>
+template <int CB, int LV, bool NT, typename Epi, typename SchemeT, int LW = kLevelsPerWG, bool NTS = true>
 __global__ __launch_bounds__(256) void k_scan(ScanArgs<Epi> A, SchemeT S) {
   constexpr int P = Epi::P, NU = 2 * P - 1;
   constexpr int BCHUNK = CB * P * 1024;  // bytes of one k block's B fragments
@@ -326,7 +338,7 @@ __global__ __launch_bounds__(256) void k_scan(ScanArgs<Epi> A, SchemeT S) {
   unsigned rest = Lg / A.ncb;
   const int rb = rest % A.nrb;
   rest /= A.nrb;
-  const int nlvb = g.nlv / kLevelsPerWG;
+  const int nlvb = g.nlv / LW;
   const int lvb = rest % nlvb;
   const int limb = rest / nlvb;
   Epi epi;
@@ -367,10 +379,10 @@ __global__ __launch_bounds__(256) void k_scan(ScanArgs<Epi> A, SchemeT S) {
     }
   };
 
-  for (int lsub = 0; lsub < kLevelsPerWG; lsub += LV) {
+  for (int lsub = 0; lsub < LW; lsub += LV) {
 #pragma unroll 1
     for (int li = 0; li < LV; li++) {
-      const int lv = lvb * kLevelsPerWG + lsub + li;
+      const int lv = lvb * LW + lsub + li;
       v4i acc[CB][NU];
 #pragma unroll
       for (int cb = 0; cb < CB; cb++)
@@ -422,8 +434,8 @@ __global__ __launch_bounds__(256) void k_scan(ScanArgs<Epi> A, SchemeT S) {
     }
     __syncthreads();
     // (row, column) pairs of this wave's row tile: LV consecutive levels each
-    flush_stage<LV>(stage[wave], lane, min(CB * 16, 2 * A.B - ct0 * 16), rt, ct0 * 16, limb,
-                    lvb * kLevelsPerWG + lsub, A.mid, A.mid_stride, A.ct_vals, g.nlv, g.Nrest);
+    flush_stage<LV, NTS>(stage[wave], lane, min(CB * 16, 2 * A.B - ct0 * 16), rt, ct0 * 16, limb,
+                         lvb * LW + lsub, A.mid, A.mid_stride, A.ct_vals, g.nlv, g.Nrest);
   }
 }
 
@@ -443,7 +455,8 @@ __global__ __launch_bounds__(256) void k_scan(ScanArgs<Epi> A, SchemeT S) {
 // (across level boundaries, so a level's epilogue overlaps the next level's
 // loads; the registers stay live through the epilogue). Otherwise each batch
 // is loaded at the top of its own iteration.
-template <int KC, int LV, int SC, bool NT, bool XPF, typename Epi, typename SchemeT>
+template <int KC, int LV, int SC, bool NT, bool XPF, typename Epi, typename SchemeT, bool NTS = true,
+          int LW = kLevelsPerWG>
 __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S) {
   constexpr int P = Epi::P, NU = 2 * P - 1;
   typedef typename Epi::Res Res;
@@ -456,7 +469,7 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
   unsigned rest = Lg / A.ncb;
   const int rb = rest % A.nrb;
   rest /= A.nrb;
-  const int nlvb = g.nlv / kLevelsPerWG;
+  const int nlvb = g.nlv / LW;
   const int lvb = rest % nlvb;
   const int limb = rest / nlvb;
   Epi epi;
@@ -469,7 +482,7 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
   const bool has_col = n < qg.ncm;
   const size_t qtile = (size_t)(P * 4 * qg.ncm * 16);
   const int nkb = (int)g.nkb;
-  const int lv0 = lvb * kLevelsPerWG;
+  const int lv0 = lvb * LW;
   const int8_t *abase = A.db + (((size_t)limb * g.nlv + lv0) * g.nrt + rta) * g.nkb * (size_t)(P * 1024) + lane * 16;
   const size_t astride = (size_t)g.nrt * g.nkb * (P * 1024);  // bytes between levels
   const int8_t *bbase = A.qfrag + (((size_t)limb * g.nlv + lv0) * qg.nct + ct) * qg.nkb * qtile +
@@ -496,7 +509,7 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
         b[j][t] = *reinterpret_cast<const v4i *>(pb + (size_t)j * qtile + (size_t)t * 4 * qg.ncm * 16);
   };
   if (XPF) load(0, 0);
-  for (int lsub = 0; lsub < kLevelsPerWG; lsub += LV) {
+  for (int lsub = 0; lsub < LW; lsub += LV) {
 #pragma unroll 1
     for (int li = 0; li < LV; li++) {
       v4i acc[NU];
@@ -515,7 +528,7 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
         if (XPF) {
           const bool last = ch + 1 == nch;
           const int nl = lsub + li + (last ? 1 : 0);
-          if (nl < kLevelsPerWG) load(nl, last ? 0 : ch + 1);
+          if (nl < LW) load(nl, last ? 0 : ch + 1);
         }
       }
 #pragma unroll
@@ -528,7 +541,7 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
       }
     }
     __syncthreads();
-    flush_stage<LV>(stage[wave], lane, min(16, 2 * A.B - ct * 16), rt, ct * 16, limb, lv0 + lsub, A.mid,
+    flush_stage<LV, NTS>(stage[wave], lane, min(16, 2 * A.B - ct * 16), rt, ct * 16, limb, lv0 + lsub, A.mid,
                     A.mid_stride, A.ct_vals, g.nlv, g.Nrest);
     __syncthreads();
   }
@@ -539,7 +552,7 @@ __global__ __launch_bounds__(256) void k_scan_direct(ScanArgs<Epi> A, SchemeT S)
 // loads of step k + 1 into the other buffer, then runs its MFMAs, so the
 // wave's next loads are in flight while it multiplies and runs a level's
 // epilogue (the direct shape with XPF issues them only after its MFMAs).
-template <int KC, int LV, int SC, bool NT, typename Epi, typename SchemeT>
+template <int KC, int LV, int SC, bool NT, typename Epi, typename SchemeT, bool NTS = true, int LW = kLevelsPerWG>
 __global__ __launch_bounds__(256) void k_scan_pp(ScanArgs<Epi> A, SchemeT S) {
   constexpr int P = Epi::P, NU = 2 * P - 1;
   typedef typename Epi::Res Res;
@@ -552,7 +565,7 @@ __global__ __launch_bounds__(256) void k_scan_pp(ScanArgs<Epi> A, SchemeT S) {
   unsigned rest = Lg / A.ncb;
   const int rb = rest % A.nrb;
   rest /= A.nrb;
-  const int nlvb = g.nlv / kLevelsPerWG;
+  const int nlvb = g.nlv / LW;
   const int lvb = rest % nlvb;
   const int limb = rest / nlvb;
   Epi epi;
@@ -564,14 +577,14 @@ __global__ __launch_bounds__(256) void k_scan_pp(ScanArgs<Epi> A, SchemeT S) {
   const bool has_col = n < qg.ncm;
   const size_t qtile = (size_t)(P * 4 * qg.ncm * 16);
   const int nkb = (int)g.nkb;
-  const int lv0 = lvb * kLevelsPerWG;
+  const int lv0 = lvb * LW;
   const int8_t *abase = A.db + (((size_t)limb * g.nlv + lv0) * g.nrt + rta) * g.nkb * (size_t)(P * 1024) + lane * 16;
   const size_t astride = (size_t)g.nrt * g.nkb * (P * 1024);
   const int8_t *bbase = A.qfrag + (((size_t)limb * g.nlv + lv0) * qg.nct + ct) * qg.nkb * qtile +
                         ((size_t)grp * qg.ncm + (has_col ? n : 0)) * 16;
   const size_t bstride = (size_t)qg.nct * qg.nkb * qtile;
   const int nch = nkb / KC;                 // batches per level
-  const int nsteps = kLevelsPerWG * nch;    // even: kLevelsPerWG is
+  const int nsteps = LW * nch;              // even: LW is
   typedef v4i Buf[KC][P];
   auto load = [&](int k, Buf &a, Buf &b) {
     const int lrel = k / nch, ch = k - lrel * nch;
@@ -622,8 +635,8 @@ __global__ __launch_bounds__(256) void k_scan_pp(ScanArgs<Epi> A, SchemeT S) {
     if (li != LV - 1) return;
     const int lsub = lrel - li;
     __syncthreads();
-    flush_stage<LV>(stage[wave], lane, ncols, rt, ct * 16, limb, lv0 + lsub, A.mid, A.mid_stride, A.ct_vals, g.nlv,
-                    g.Nrest);
+    flush_stage<LV, NTS>(stage[wave], lane, ncols, rt, ct * 16, limb, lv0 + lsub, A.mid, A.mid_stride, A.ct_vals,
+                         g.nlv, g.Nrest);
     __syncthreads();
   };
   Buf a0, b0, a1, b1;
@@ -637,6 +650,136 @@ __global__ __launch_bounds__(256) void k_scan_pp(ScanArgs<Epi> A, SchemeT S) {
   }
   step(next, nsteps - 2, a0, b0, a1, b1);
   step(last, nsteps - 1, a1, b1, a0, b0);
+}
+
+// Register-blocked direct variant for large batches: each wave multiplies R
+// consecutive row tiles by C consecutive column tiles, so one k block's
+// loads (R * P A fragments, C * P B fragments) feed R * C * P^2 MFMAs
+// instead of P^2 (1 KiB per MFMA in the direct shape, 256 B at R = C = 2).
+// The 4 waves form a (4 / WC) x WC grid over row and column blocks: a
+// workgroup covers 4R / WC row tiles and WC * C column tiles, and waves of
+// one row (column) block meet in L1 on the same A (B) fragments. The
+// workgroups of one row block (column blocks innermost) are neighbours on
+// one XCD, so the database tile passes through HBM once and is re-read from
+// L2. Row or column tiles past the last one re-read the last tile and store
+// nothing. Loads of the next (level, batch) are issued right after the
+// current batch's MFMAs. Each workgroup takes LW consecutive levels, staged
+// LV at a time: with cached stores (NTS false) the 4 * LV-byte pieces of one
+// output run meet in L2 and leave it as whole lines.
+template <int R, int C, int WC, int KC, int LV, int LW, bool NT, typename Epi, typename SchemeT, bool NTS>
+__global__ __launch_bounds__(256) void k_scan_rb(ScanArgs<Epi> A, SchemeT S) {
+  constexpr int P = Epi::P, NU = 2 * P - 1;
+  constexpr int WR = 4 / WC;
+  static_assert(WR * WC == 4 && LW % LV == 0, "4 waves; whole flushes");
+  typedef typename Epi::Res Res;
+  __shared__ Res stage[4][R][LV][16][16 * C + 1];
+  const DbGeom &g = A.g;
+  const QfragGeom &qg = A.qg;
+  const unsigned W = gridDim.x, pblk = blockIdx.x;
+  const unsigned Lg = (W % 8 == 0) ? (pblk % 8) * (W / 8) + pblk / 8 : pblk;
+  const int cbk = Lg % A.ncb;
+  unsigned rest = Lg / A.ncb;
+  const int rb = rest % A.nrb;
+  rest /= A.nrb;
+  const int nlvb = g.nlv / LW;
+  const int lvb = rest % nlvb;
+  const int limb = rest / nlvb;
+  Epi epi;
+  epi.init(S, limb);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rt0 = (rb * WR + wave / WC) * R;  // first row tile of this wave
+  const int ct0 = (cbk * WC + wave % WC) * C;  // first column tile of this wave
+  const int n = lane & 15, grp = lane >> 4;
+  const bool has_col = n < qg.ncm;
+  const size_t qtile = (size_t)(P * 4 * qg.ncm * 16);
+  const int lv0 = lvb * LW;
+  const size_t astride = (size_t)g.nrt * g.nkb * (P * 1024);  // bytes between levels
+  const size_t bstride = (size_t)qg.nct * qg.nkb * qtile;
+  const int8_t *abase[R];
+  const int8_t *bbase[C];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const size_t rta = rt0 + r < (int)g.nrt ? (size_t)(rt0 + r) : g.nrt - 1;
+    abase[r] = A.db + (((size_t)limb * g.nlv + lv0) * g.nrt + rta) * g.nkb * (size_t)(P * 1024) + lane * 16;
+  }
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const size_t cta = ct0 + c < qg.nct ? (size_t)(ct0 + c) : (size_t)qg.nct - 1;
+    bbase[c] = A.qfrag + (((size_t)limb * g.nlv + lv0) * qg.nct + cta) * qg.nkb * qtile +
+               ((size_t)grp * qg.ncm + (has_col ? n : 0)) * 16;
+  }
+  const int nch = (int)g.nkb / KC;
+  v4i a[KC][R][P], b[KC][C][P];
+  auto load = [&](int lrel, int ch) {
+#pragma unroll
+    for (int j = 0; j < KC; j++)
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int s = 0; s < P; s++) {
+          const v4i *q = reinterpret_cast<const v4i *>(abase[r] + (size_t)lrel * astride +
+                                                       (size_t)(ch * KC + j) * (P * 1024) + s * 1024);
+          a[j][r][s] = NT ? __builtin_nontemporal_load(q) : *q;
+        }
+#pragma unroll
+    for (int j = 0; j < KC; j++)
+#pragma unroll
+      for (int c = 0; c < C; c++)
+#pragma unroll
+        for (int t = 0; t < P; t++)
+          b[j][c][t] = *reinterpret_cast<const v4i *>(bbase[c] + (size_t)lrel * bstride +
+                                                      (size_t)(ch * KC + j) * qtile + (size_t)t * 4 * qg.ncm * 16);
+  };
+  load(0, 0);
+  for (int lsub = 0; lsub < LW; lsub += LV) {
+#pragma unroll 1
+    for (int li = 0; li < LV; li++) {
+      v4i acc[R][C][NU];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int c = 0; c < C; c++)
+#pragma unroll
+          for (int u = 0; u < NU; u++) acc[r][c][u] = (v4i){0, 0, 0, 0};
+#pragma unroll 1
+      for (int ch = 0; ch < nch; ch++) {
+#pragma unroll
+        for (int j = 0; j < KC; j++)
+#pragma unroll
+          for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int c = 0; c < C; c++)
+#pragma unroll
+              for (int s = 0; s < P; s++)
+#pragma unroll
+                for (int t = 0; t < P; t++)
+                  acc[r][c][s + t] =
+                      __builtin_amdgcn_mfma_i32_16x16x64_i8(a[j][r][s], b[j][c][t], acc[r][c][s + t], 0, 0, 0);
+        const bool last = ch + 1 == nch;
+        const int nl = lsub + li + (last ? 1 : 0);
+        if (nl < LW) load(nl, last ? 0 : ch + 1);
+      }
+      // D row = 4 * (lane / 16) + q, column = lane % 16
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int c = 0; c < C; c++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            int gs[NU];
+#pragma unroll
+            for (int u = 0; u < NU; u++) gs[u] = acc[r][c][u][q];
+            stage[wave][r][li][grp * 4 + q][c * 16 + n] = epi.combine(gs);
+          }
+    }
+    __syncthreads();
+    const int ncols = min(C * 16, 2 * A.B - ct0 * 16);  // <= 0: a column block past the last
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      flush_stage<LV, NTS>(stage[wave][r], lane, ncols, rt0 + r, ct0 * 16, limb, lv0 + lsub, A.mid, A.mid_stride,
+                      A.ct_vals, g.nlv, g.Nrest);
+    __syncthreads();
+  }
 }
 
 }  // namespace
@@ -703,8 +846,8 @@ void launch_qfrag64(const u64 *qntt, int8_t *qfrag, size_t q_stride, int B, cons
 // Scan variants: (kind, column tiles per workgroup CB, k blocks per load
 // batch KC, levels staged per flush LV, staged columns SC, next loads issued
 // early). A direct variant (k_scan_direct, or k_scan_pp with two register
-// buffers) takes one column tile per workgroup, needs nkb % KC == 0 and
-// stages min(16, 2B) <= SC columns.
+// buffers, one column tile per workgroup; k_scan_rb, CB column tiles) needs
+// nkb % KC == 0 and stages min(16, 2B) <= SC columns.
 struct ScanShape {
   bool direct;
   int CB, KC, LV, SC;
@@ -725,38 +868,52 @@ static ScanArgs<Epi> scan_args(const int8_t *db, const int8_t *qfrag, typename E
 }
 
 template <typename Epi>
-static unsigned scan_grid(const ScanArgs<Epi> &A) {
-  return (unsigned)((size_t)2 * (A.g.nlv / kLevelsPerWG) * A.nrb * A.ncb);
+static unsigned scan_grid(const ScanArgs<Epi> &A, int LW = kLevelsPerWG) {
+  return (unsigned)((size_t)2 * (A.g.nlv / LW) * A.nrb * A.ncb);
 }
 
-template <int CB, int LV, typename Epi, typename SchemeT>
+template <int CB, int LV, typename Epi, typename SchemeT, int LW = kLevelsPerWG, bool NTS = true>
 static void run_lds(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
                     const DbGeom &g, const SchemeT &S, hipStream_t st) {
   const ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, CB);
   if (A.ncb == 1)
-    k_scan<CB, LV, true, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+    k_scan<CB, LV, true, Epi, SchemeT, LW, NTS><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
   else
-    k_scan<CB, LV, false, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+    k_scan<CB, LV, false, Epi, SchemeT, LW, NTS><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
 }
 
-template <int KC, int LV, int SC, bool XPF, typename Epi, typename SchemeT>
+template <int KC, int LV, int SC, bool XPF, typename Epi, typename SchemeT, bool NTS = true, int LW = kLevelsPerWG>
 static void run_direct(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
                        const DbGeom &g, const SchemeT &S, hipStream_t st) {
   const ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, 1);
   if (A.ncb == 1)
-    k_scan_direct<KC, LV, SC, true, XPF, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+    k_scan_direct<KC, LV, SC, true, XPF, Epi, SchemeT, NTS, LW><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
   else
-    k_scan_direct<KC, LV, SC, false, XPF, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+    k_scan_direct<KC, LV, SC, false, XPF, Epi, SchemeT, NTS, LW><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
 }
 
-template <int KC, int LV, int SC, typename Epi, typename SchemeT>
+template <int KC, int LV, int SC, typename Epi, typename SchemeT, bool NTS = true, int LW = kLevelsPerWG>
 static void run_pp(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
                    const DbGeom &g, const SchemeT &S, hipStream_t st) {
   const ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, 1);
   if (A.ncb == 1)
-    k_scan_pp<KC, LV, SC, true, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+    k_scan_pp<KC, LV, SC, true, Epi, SchemeT, NTS, LW><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
   else
-    k_scan_pp<KC, LV, SC, false, Epi, SchemeT><<<scan_grid(A), 256, 0, st>>>(A, S);
+    k_scan_pp<KC, LV, SC, false, Epi, SchemeT, NTS, LW><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
+}
+
+// NT A loads only where each A fragment has one reader (one column block,
+// one wave per row block).
+template <int R, int C, int WC, int KC, int LV, int LW, typename Epi, typename SchemeT, bool NTS>
+static void run_rb(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
+                   const DbGeom &g, const SchemeT &S, hipStream_t st) {
+  ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, C * WC);
+  A.nrb = (int)cdivu(g.nrt, 4 / WC * R);
+  const unsigned grid = (unsigned)((size_t)2 * (g.nlv / LW) * A.nrb * A.ncb);
+  if (A.ncb == 1 && WC == 1)
+    k_scan_rb<R, C, WC, KC, LV, LW, true, Epi, SchemeT, NTS><<<grid, 256, 0, st>>>(A, S);
+  else
+    k_scan_rb<R, C, WC, KC, LV, LW, false, Epi, SchemeT, NTS><<<grid, 256, 0, st>>>(A, S);
 }
 
 template <typename Epi, typename SchemeT>
@@ -780,6 +937,7 @@ static const ScanEntry<EpiK1, SchemeConst> kScanK1[] = {
     {{false, 2, 1, 4, 32, false}, run_lds<2, 4, EpiK1, SchemeConst>},
     {{false, 2, 1, 2, 32, false}, run_lds<2, 2, EpiK1, SchemeConst>},
     {{true, 1, 2, 16, 2, true}, run_pp<2, 16, 2, EpiK1, SchemeConst>},
+    {{true, 4, 4, 16, 16, true}, run_rb<1, 1, 4, 4, 16, 32, EpiK1, SchemeConst, false>},
 };
 
 static const ScanEntry<EpiK2, S64> kScanK2[] = {
@@ -792,6 +950,7 @@ static const ScanEntry<EpiK2, S64> kScanK2[] = {
     {{false, 2, 1, 1, 32, false}, run_lds<2, 1, EpiK2, S64>},
     {{true, 1, 1, 16, 2, true}, run_direct<1, 16, 2, true, EpiK2, S64>},
     {{true, 1, 1, 16, 2, true}, run_pp<1, 16, 2, EpiK2, S64>},
+    {{true, 4, 1, 8, 16, true}, run_rb<1, 1, 4, 1, 8, 16, EpiK2, S64, false>},
 };
 
 int scan_variant_count(int P) { return P == 4 ? (int)std::size(kScanK1) : (int)std::size(kScanK2); }
@@ -803,8 +962,10 @@ static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
 
 // Default shapes per batch-size class (2B <= 2, <= 16, larger), in order of
 // preference, the fastest measured on MI355X at the 256 x 1024 and
-// 512 x 4096 layouts (B = 1, 8 and 64 / 32).
-static const int kPrefK1[3][4] = {{0, 3, 5, -1}, {1, 3, 5, -1}, {1, 6, 5, -1}};
+// 512 x 4096 layouts (B = 1, 8 and 64 / 32). At 64 queries the
+// register-blocked shape (one row tile x four column tiles per workgroup)
+// takes 3.37 ms against the direct shape's 3.70.
+static const int kPrefK1[3][4] = {{0, 3, 5, -1}, {1, 3, 5, -1}, {9, 1, 6, 5}};
 static const int kPrefK2[3][4] = {{8, 1, 3, 4}, {3, 4, -1, -1}, {3, 4, 5, -1}};
 
 template <typename Epi, typename SchemeT, size_t NV>
