@@ -210,8 +210,6 @@ extern "C" {
 const char *opir_last_error(const opir_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
 static int ctx_create_k2(const opir_scheme *sch, int device, opir_ctx **out) {
-  if (sch->poly_degree != (uint32_t)k2::kN)
-    return set_err(nullptr, OPIR_EINVAL, "the two-prime scheme runs at N=4096 (CONFIG_N4096_K2_MP)");
   if (int rc = k2_validate_scheme(sch)) return rc;
   opir_ctx *ctx = new (std::nothrow) opir_ctx();
   if (!ctx) return set_err(nullptr, OPIR_ENOMEM, "out of host memory");
@@ -246,7 +244,7 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   *out = nullptr;
   // two RNS primes below 2^30 at N = 2048 (CONFIG_N2048_K2_MP) run on the
   // composite scheme's 32-bit kernels with the MP gadgets and mod switch
-  const bool mp = !sch->composite && sch->poly_degree == (uint32_t)kN && sch->q1 < (1ULL << 30) &&
+  const bool mp = !sch->composite && sch->q2 && sch->poly_degree == (uint32_t)kN && sch->q1 < (1ULL << 30) &&
                   sch->q2 < (1ULL << 30);
   if (!sch->composite && !mp) return ctx_create_k2(sch, device, out);
   if (sch->poly_degree != (uint32_t)kN)
@@ -770,7 +768,7 @@ int opir_db_reserve(opir_db *db, size_t B) {
   int rc = ctx->k2 ? k2_ensure_ws(db, B) : ensure_ws(db, B);
   if (rc) return rc;
   // the host staging of opir_answer_batch and the key-pointer ring
-  const size_t vals = (ctx->k2 || ctx->S.mp ? 4 : 2) * ctx->N;
+  const size_t vals = 2 * (size_t)ctx->K * ctx->N;
   if ((rc = ensure_pin(ctx, B * vals * sizeof(u64)))) return rc;
   if ((rc = ensure_ring(ctx, B))) return rc;
   if (ctx->scan_variant == -2) {
@@ -788,7 +786,7 @@ int opir_db_reserve(opir_db *db, size_t B) {
 
 int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out) {
   if (!ctx || !out) return OPIR_EINVAL;
-  *out = ctx->k2 ? (h * ctx->G64.l_ks + 2 * (size_t)ctx->G64.l_key) * k2::kCt * sizeof(u64)
+  *out = ctx->k2 ? (h * ctx->G64.l_ks + 2 * (size_t)ctx->G64.l_key) * k2::ct_values(ctx->S64) * sizeof(u64)
                  : (h * ctx->G.l_ks + 2 * (size_t)ctx->G.l_key) * kCtWords * sizeof(u32);
   return OPIR_OK;
 }
@@ -815,7 +813,7 @@ int opir_answer_batch(opir_db *db, size_t B, const uint64_t *ids, const uint64_t
   if (!db || !ids || !queries || !resps) return OPIR_EINVAL;
   if (B == 0) return OPIR_OK;
   opir_ctx *ctx = db->ctx;
-  const size_t N = ctx->N, qvals = (ctx->k2 || ctx->S.mp ? 4 : 2) * N, rvals = 2 * N;
+  const size_t N = ctx->N, qvals = 2 * (size_t)ctx->K * N, rvals = 2 * N;
   int rc = ctx->k2 ? k2_validate_rns(ctx, queries, B * 2, "query") : validate_polys(ctx, queries, B * 2, "query");
   if (rc) return rc;
   for (size_t i = 0; i < B; i++)

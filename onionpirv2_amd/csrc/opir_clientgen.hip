@@ -485,11 +485,12 @@ int opir_client_get_keys(opir_ctx *ctx, uint64_t id, uint64_t *bv, size_t h, uin
   auto export_rows = [&](const void *dev, size_t nrows, uint64_t *out, bool swap) -> int {
     if (!nrows) return OPIR_OK;
     if (ctx->k2) {
-      std::vector<u64> h64(nrows * k2::kCt);
+      const size_t ct = k2::ct_values(ctx->S64);
+      std::vector<u64> h64(nrows * ct);
       HIP_TRY(ctx, hipMemcpy(h64.data(), dev, h64.size() * sizeof(u64), hipMemcpyDeviceToHost));
       for (size_t r = 0; r < nrows; r++)
         for (int p = 0; p < 2; p++)
-          std::memcpy(out + (r * 2 + (swap ? 1 - p : p)) * KN, &h64[r * k2::kCt + (size_t)p * KN], KN * sizeof(u64));
+          std::memcpy(out + (r * 2 + (swap ? 1 - p : p)) * KN, &h64[r * ct + (size_t)p * KN], KN * sizeof(u64));
       return OPIR_OK;
     }
     std::vector<u32> h32(nrows * kCtWords);

@@ -71,14 +71,16 @@ void launch_ntt_rns(const u32 *in, u32 *out, size_t npoly_rns, bool inverse, con
 struct DbGeom {
   size_t N0 = 0, Nrest = 0, nrt = 0, nkb = 0;
   int P = 4, nlv = kN;
+  int nl = 2;  // RNS limbs (one scan per limb)
 };
 struct QfragGeom {
   int N0 = 0, nkb = 0, nct = 0, ncm = 0;  // column tiles of 16, stored columns per tile
   int P = 4, nlv = kN;
+  int nl = 2;
 };
-DbGeom make_db_geom(size_t N0, size_t Nrest, int P = 4, int nlv = kN);
+DbGeom make_db_geom(size_t N0, size_t Nrest, int P = 4, int nlv = kN, int nl = 2);
 size_t db_bytes(const DbGeom &g);
-QfragGeom make_qfrag_geom(size_t N0, int B, int P = 4, int nlv = kN);
+QfragGeom make_qfrag_geom(size_t N0, int B, int P = 4, int nlv = kN, int nl = 2);
 size_t qfrag_bytes(const QfragGeom &q);
 
 // Seeded synthetic plaintexts [count][n]: coefficient c of plaintext i is

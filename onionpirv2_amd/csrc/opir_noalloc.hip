@@ -56,16 +56,16 @@ extern "C" int opir_ext_prod_mux_device_noalloc(opir_ctx *ctx, const uint64_t *d
     if (l == 0 || l > kMaxL2)
       return set_err(ctx, OPIR_EINVAL, "selector gadget of " + std::to_string(l) + " rows exceeds the " +
                                            std::to_string(kMaxL2) + "-row scratch limit");
-    const size_t ct = k2::kCt;  // u64 values per two-limb ciphertext
+    const size_t ct = k2::ct_values(ctx->S64);  // u64 values per ciphertext
     u64 *diff = reinterpret_cast<u64 *>(s);
     u64 *o = diff + ct;
-    static_assert(2 * k2::kCt * sizeof(u64) <= kScratchBytes, "scratch too small for the two-prime layout");
-    const u64 q0 = ctx->S64.limb[0].q, q1 = ctx->S64.limb[1].q;
+    static_assert(2 * k2::kMaxCt * sizeof(u64) <= kScratchBytes, "scratch too small for the 64-bit-limb layouts");
+    const u64 q0 = ctx->S64.limb[0].q, q1 = ctx->K == 2 ? ctx->S64.limb[1].q : q0;
     // y - x, external product (NTT form), INTT, + x (server.cpp:696-721)
-    launch_addsub_mod(d_y, d_x, diff, ct, k2::kN, 2, q0, q1, true, st);
+    launch_addsub_mod(d_y, d_x, diff, ct, ctx->N, ctx->K, q0, q1, true, st);
     k2::launch_k2_ext_prod(d_gsw, (int)l, (int)b, diff, o, ctx->S64, st);
     k2::launch_k2_ntt(o, 2, true, ctx->S64, st);
-    launch_addsub_mod(d_x, o, d_out, ct, k2::kN, 2, q0, q1, false, st);
+    launch_addsub_mod(d_x, o, d_out, ct, ctx->N, ctx->K, q0, q1, false, st);
     LAUNCH_CHECK(ctx);
     return OPIR_OK;
   }

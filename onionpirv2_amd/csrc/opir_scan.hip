@@ -123,7 +123,7 @@ __global__ void k_db_from_ref(const u32 *__restrict__ ref, int8_t *__restrict__ 
   store_digits<4>(db, g, (int)limb, (int)lv, pt / g.N0, pt % g.N0, ref[i]);
 }
 
-// Query columns -> B fragments. qntt [B][N0][2 polys][2 limbs][nlv] (NTT form)
+// Query columns -> B fragments. qntt [B][N0][2 polys][nl limbs][nlv] (NTT form)
 // -> qfrag [limb][level][ct][kb][plane][grp][n < ncm][16 bytes], ct = column
 // tile of 16 query columns (c = 2*query + poly), ncm = min(16, 2B); the 16
 // bytes of (grp, n) are digit `plane` of k = 64*kb + 16*grp + j, j < 16
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_qfrag(const V *__restrict__ qntt, int8_
     const int c = ct * 16 + n, k = kg * 16 + j;
     V x = 0;
     if (c < c_end && k < qg.N0)
-      x = qntt[(size_t)(c >> 1) * q_stride + (((size_t)k * 2 + (c & 1)) * 2 + limb) * qg.nlv + lv0 + lv];
+      x = qntt[(size_t)(c >> 1) * q_stride + (((size_t)k * 2 + (c & 1)) * qg.nl + limb) * qg.nlv + lv0 + lv];
     tile[n * kQfNs + j * (kQfLv + 1) + lv] = x;
   }
   __syncthreads();
@@ -258,7 +258,7 @@ struct ScanArgs {
   const int8_t *qfrag;
   typename Epi::Res *mid;
   size_t mid_stride;  // values between queries
-  size_t ct_vals;     // values per output ciphertext (4 * nlv)
+  size_t ct_vals;     // values per output ciphertext (2 polys x nl limbs x nlv)
   DbGeom g;
   QfragGeom qg;
   int nrb, ncb, B;
@@ -268,7 +268,8 @@ constexpr int kLevelsPerWG = 16;
 
 // Write one wave's staged results: for each (row rr < 16, column nn < ncols)
 // of its tile, the LV consecutive levels st[0..LV)[rr][nn] go to
-// mid[q][row][poly][limb][lvo ..], c = col0 + nn = 2q + poly, row = 16 rt + rr.
+// mid[q][row][poly][limb][lvo ..] (ct_vals / 2 values per poly), c = col0 + nn = 2q + poly,
+// row = 16 rt + rr.
 // The stores are nontemporal: the next kernel reads the results, and
 // keeping them out of the L2 leaves it to the database stream (the plain
 // scan ran 0.707 against 0.734 ms with cached stores).
@@ -294,8 +295,8 @@ __device__ __forceinline__ void flush_stage(const Res (*st)[16][SW], int lane, i
       v4i x;
       __builtin_memcpy(&x, v, 16);
       // nontemporal: the results are read by the next kernel, not by this one
-      v4i *dst = reinterpret_cast<v4i *>(mid + qi * mid_stride + row * ct_vals + (size_t)(pp * 2 + limb) * nlv +
-                                         lvo + ch * W);
+      v4i *dst = reinterpret_cast<v4i *>(mid + qi * mid_stride + row * ct_vals + (size_t)pp * (ct_vals >> 1) +
+                                         (size_t)limb * nlv + lvo + ch * W);
       if (NTS)
         __builtin_nontemporal_store(x, dst);
       else
@@ -307,7 +308,7 @@ __device__ __forceinline__ void flush_stage(const Res (*st)[16][SW], int lane, i
       const size_t row = (size_t)rt * 16 + rr;
       if (row >= Nrest) continue;
       const int c = col0 + nn, qi = c >> 1, pp = c & 1;
-      Res *dst = mid + qi * mid_stride + row * ct_vals + (size_t)(pp * 2 + limb) * nlv + lvo;
+      Res *dst = mid + qi * mid_stride + row * ct_vals + (size_t)pp * (ct_vals >> 1) + (size_t)limb * nlv + lvo;
 #pragma unroll
       for (int li = 0; li < LV; li++) {
         if (NTS)
@@ -786,8 +787,9 @@ __global__ __launch_bounds__(256) void k_scan_rb(ScanArgs<Epi> A, SchemeT S) {
 
 static inline unsigned cdivu(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
-DbGeom make_db_geom(size_t N0, size_t Nrest, int P, int nlv) {
+DbGeom make_db_geom(size_t N0, size_t Nrest, int P, int nlv, int nl) {
   DbGeom g;
+  g.nl = nl;
   g.N0 = N0;
   g.Nrest = Nrest;
   g.nrt = (Nrest + 15) / 16;
@@ -797,10 +799,11 @@ DbGeom make_db_geom(size_t N0, size_t Nrest, int P, int nlv) {
   return g;
 }
 
-size_t db_bytes(const DbGeom &g) { return 2 * (size_t)g.nlv * g.nrt * g.nkb * (size_t)g.P * 1024; }
+size_t db_bytes(const DbGeom &g) { return (size_t)g.nl * g.nlv * g.nrt * g.nkb * (size_t)g.P * 1024; }
 
-QfragGeom make_qfrag_geom(size_t N0, int B, int P, int nlv) {
+QfragGeom make_qfrag_geom(size_t N0, int B, int P, int nlv, int nl) {
   QfragGeom q;
+  q.nl = nl;
   q.N0 = (int)N0;
   q.nkb = (int)((N0 + 63) / 64);
   q.nct = (2 * B + 15) / 16;
@@ -810,7 +813,7 @@ QfragGeom make_qfrag_geom(size_t N0, int B, int P, int nlv) {
   return q;
 }
 
-size_t qfrag_bytes(const QfragGeom &q) { return 2 * (size_t)q.nlv * q.nct * q.nkb * (size_t)(q.P * 4 * q.ncm * 16); }
+size_t qfrag_bytes(const QfragGeom &q) { return (size_t)q.nl * q.nlv * q.nct * q.nkb * (size_t)(q.P * 4 * q.ncm * 16); }
 
 void launch_db_scatter(const u32 *tmp, int8_t *db, size_t first_pt, size_t count, const DbGeom &g, hipStream_t st) {
   if (!count) return;
@@ -821,7 +824,7 @@ void launch_db_scatter(const u32 *tmp, int8_t *db, size_t first_pt, size_t count
 void launch_db_scatter64(const u64 *tmp, int8_t *db, size_t first_pt, size_t count, const DbGeom &g,
                          hipStream_t st) {
   if (!count) return;
-  dim3 grid(cdivu(count, 64), g.nlv / 32, 2);
+  dim3 grid(cdivu(count, 64), g.nlv / 32, g.nl);
   k_db_scatter<8, u64><<<grid, 256, 0, st>>>(tmp, db, first_pt, count, g);
 }
 
@@ -838,8 +841,8 @@ void launch_qfrag(const u32 *qntt, int8_t *qfrag, size_t q_stride, int B, const 
 }
 
 void launch_qfrag64(const u64 *qntt, int8_t *qfrag, size_t q_stride, int B, const DbGeom &g, hipStream_t st) {
-  const QfragGeom qg = make_qfrag_geom(g.N0, B, 8, g.nlv);
-  dim3 pg(qg.nct * qg.nkb * 4, g.nlv / qf_levels<u64>(), 2);
+  const QfragGeom qg = make_qfrag_geom(g.N0, B, 8, g.nlv, g.nl);
+  dim3 pg(qg.nct * qg.nkb * 4, g.nlv / qf_levels<u64>(), g.nl);
   k_qfrag<8, u64><<<pg, 256, 0, st>>>(qntt, qfrag, q_stride, qg, 2 * B);
 }
 
@@ -858,9 +861,9 @@ template <typename Epi>
 static ScanArgs<Epi> scan_args(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride,
                                int B, const DbGeom &g, int CB) {
   ScanArgs<Epi> A;
-  A.db = db; A.qfrag = qfrag; A.mid = mid; A.mid_stride = mid_stride; A.ct_vals = 4 * (size_t)g.nlv;
+  A.db = db; A.qfrag = qfrag; A.mid = mid; A.mid_stride = mid_stride; A.ct_vals = 2 * (size_t)g.nl * g.nlv;
   A.g = g;
-  A.qg = make_qfrag_geom(g.N0, B, Epi::P, g.nlv);
+  A.qg = make_qfrag_geom(g.N0, B, Epi::P, g.nlv, g.nl);
   A.nrb = (int)cdivu(g.nrt, 4);
   A.ncb = (int)cdivu(A.qg.nct, CB);
   A.B = B;
@@ -869,7 +872,7 @@ static ScanArgs<Epi> scan_args(const int8_t *db, const int8_t *qfrag, typename E
 
 template <typename Epi>
 static unsigned scan_grid(const ScanArgs<Epi> &A, int LW = kLevelsPerWG) {
-  return (unsigned)((size_t)2 * (A.g.nlv / LW) * A.nrb * A.ncb);
+  return (unsigned)((size_t)A.g.nl * (A.g.nlv / LW) * A.nrb * A.ncb);
 }
 
 template <int CB, int LV, typename Epi, typename SchemeT, int LW = kLevelsPerWG, bool NTS = true>
@@ -909,7 +912,7 @@ static void run_rb(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid
                    const DbGeom &g, const SchemeT &S, hipStream_t st) {
   ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, C * WC);
   A.nrb = (int)cdivu(g.nrt, 4 / WC * R);
-  const unsigned grid = (unsigned)((size_t)2 * (g.nlv / LW) * A.nrb * A.ncb);
+  const unsigned grid = (unsigned)((size_t)g.nl * (g.nlv / LW) * A.nrb * A.ncb);
   if (A.ncb == 1 && WC == 1)
     k_scan_rb<R, C, WC, KC, LV, LW, true, Epi, SchemeT, NTS><<<grid, 256, 0, st>>>(A, S);
   else

@@ -57,10 +57,10 @@ __global__ void k_seed_uniform_u64(u64 *out, size_t stride, prg::Key key, uint32
                                    k2::Scheme64 S) {
   const int p = blockIdx.y, limb = blockIdx.z;
   const uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blk >= (uint32_t)(k2::kN / 4)) return;
+  if (blk >= (uint32_t)(S.N / 4)) return;
   uint32_t w[16];
   prg::chacha20_block(key, blk, domain, i0 + p, limb, w);
-  u64 *o = out + (size_t)p * stride + (size_t)limb * k2::kN;
+  u64 *o = out + (size_t)p * stride + (size_t)limb * S.N;
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     u64 hi, lo;
@@ -82,7 +82,7 @@ prg::Key key_words(const uint8_t *seed) {
 int fill_uniform(opir_ctx *ctx, void *dev, size_t stride, size_t npoly, const prg::Key &key, uint32_t domain) {
   if (!npoly) return OPIR_OK;
   if (ctx->k2) {
-    dim3 grid((unsigned)((k2::kN / 4 + 255) / 256), (unsigned)npoly, 2);
+    dim3 grid((unsigned)((ctx->N / 4 + 255) / 256), (unsigned)npoly, (unsigned)ctx->K);
     k_seed_uniform_u64<<<grid, 256, 0, ctx->stream>>>((u64 *)dev, stride, key, domain, 0, ctx->S64);
   } else {
     dim3 grid((unsigned)((kN / 4 + 255) / 256), (unsigned)npoly, ctx->S.mp ? 2 : 1);
@@ -113,7 +113,7 @@ extern "C" int opir_client_set_keys_seeded(opir_ctx *ctx, uint64_t id, const uin
   const prg::Key key = key_words(seed);
   // device key rows: [b or c0 (-> c0), a or c1 (-> c1)][2 limbs][N]; the
   // uniform half is the second polynomial of each row
-  const size_t row = ctx->k2 ? k2::kCt : kCtWords, half = row / 2;
+  const size_t row = ctx->k2 ? k2::ct_values(ctx->S64) : kCtWords, half = row / 2;
   if (ctx->k2) {
     if ((rc = fill_uniform(ctx, (u64 *)k.bv + half, row, h * L, key, prg::kBvA))) return rc;
     if ((rc = fill_uniform(ctx, (u64 *)k.gsw + half, row, 2 * lk, key, prg::kGswA))) return rc;

@@ -160,7 +160,9 @@ class GpuContext:
         self.lib = load_library()
         self.scheme = dict(scheme)
         self.N = scheme["poly_degree"]
-        self.K = 1 if scheme.get("composite", 1) else 2  # RNS limbs per polynomial at the boundary
+        # RNS limbs per polynomial at the boundary: 1 for the composite scheme and
+        # for one prime (q2 = 0), else 2
+        self.K = 1 if scheme.get("composite", 1) or not scheme.get("q2", 0) else 2
         self.KN = self.K * self.N
         h = ctypes.c_void_p()
         if _parent is None:

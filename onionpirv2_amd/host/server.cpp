@@ -21,15 +21,21 @@ GpuScheme::GpuScheme(const PirParams &p, int device) : params_(p) {
     s.q1 = p.q1();
     s.q2 = p.q2();
     s.ntt_root = p.ntt_root();
-  } else if (p.K() == 2) {  // CONFIG_N4096_K2_MP: two RNS limbs
+  } else if (p.K() == 2) {  // CONFIG_N2048_K2_MP / CONFIG_N4096_K2_MP: two RNS limbs
     const size_t N = p.get_poly_degree();
     s.composite = 0;
     s.q1 = p.get_rns_mods()[0];
     s.q2 = p.get_rns_mods()[1];
     s.ntt_root = utils::minimal_primitive_root(2 * N, s.q1);
     s.ntt_root2 = utils::minimal_primitive_root(2 * N, s.q2);
+  } else if (p.K() == 1) {  // CONFIG_N2048_K1: one prime, q2 = 0
+    s.composite = 0;
+    s.q1 = p.get_rns_mods()[0];
+    s.q2 = 0;
+    s.ntt_root = utils::minimal_primitive_root(2 * p.get_poly_degree(), s.q1);
   } else {
-    throw std::invalid_argument("the GPU path runs CONFIG_N2048_K1_COMP, CONFIG_N2048_K2_MP and CONFIG_N4096_K2_MP");
+    throw std::invalid_argument("the GPU path runs CONFIG_N2048_K1_COMP, CONFIG_N2048_K1, CONFIG_N2048_K2_MP and "
+                                "CONFIG_N4096_K2_MP");
   }
   s.plain_mod = p.get_plain_mod();
   s.small_q = p.get_small_q();

@@ -1,7 +1,7 @@
 """Write the golden answer fixtures of tests/golden/ with the CPU oracle.
 
 Each fixture is one seeded PIR exchange under one parameter set
-(CONFIG_N2048_K1_COMP, CONFIG_N2048_K2_MP or CONFIG_N4096_K2_MP):
+(CONFIG_N2048_K1_COMP, CONFIG_N2048_K1, CONFIG_N2048_K2_MP or CONFIG_N4096_K2_MP):
   database   plaintext i, coefficient c = mt19937_64(db_seed) draw i*N + c mod t
              (the oracle's "random" loader, src/server.cpp:80-203 contract)
   client     PirClient(params, client_id, client_seed)
@@ -10,7 +10,7 @@ Each fixture is one seeded PIR exchange under one parameter set
   wire       save_resp_to_stream(response)           ceil(2N log2(small_q) / 8) bytes
 
 Run from the repository root after building the oracle (make -C oracle):
-  python tests/golden/make_golden.py
+  python tests/golden/make_golden.py [fixture names to write; default all]
 """
 import os
 import sys
@@ -30,6 +30,8 @@ CASES = [
     (43, 5, True, 4302, 8, 94, 40, "CONFIG_N2048_K2_MP"),
     (700, 8, True, 7002, 9, 95, 699, "CONFIG_N2048_K2_MP"),
     (43, 5, True, 4303, 10, 96, 17, "CONFIG_N4096_K2_MP"),
+    (43, 5, True, 4304, 11, 97, 41, "CONFIG_N2048_K1"),
+    (700, 8, True, 7004, 12, 98, 512, "CONFIG_N2048_K1"),
 ]
 
 
@@ -61,7 +63,10 @@ def make(O, case):
 
 def main():
     import onionpir_oracle as O
+    only = set(sys.argv[1:])
     for case in CASES:
+        if only and name(case) not in only:
+            continue
         d = make(O, case)
         np.savez_compressed(os.path.join(HERE, name(case)), **d)
         print("wrote", name(case))

@@ -65,3 +65,32 @@ def test_k2_scheme_validation_without_gpu(O):
     assert create(l_ep=33, base_log2=3) == gpu.OPIR_EINVAL
     assert create(l_key=33, base_log2_key=3) == gpu.OPIR_EINVAL
     assert create(l_ep=32, base_log2=3) in (gpu.OPIR_OK, gpu.OPIR_EHIP)
+
+
+def test_single_prime_scheme_validation_without_gpu(O):
+    """CONFIG_N2048_K1 (one ~60-bit prime, q2 = 0) passes host-side
+    validation; malformed variants are rejected before any device call."""
+    from onionpirv2_amd import gpu
+    lib = gpu.load_library()
+    p = O.PirParams("CONFIG_N2048_K1")
+    (q,) = p.rns_mods
+    good = dict(poly_degree=p.N, composite=0, q1=q, q2=0, ntt_root=O.ntt_root(p.N, q), plain_mod=p.plain_mod,
+                small_q=p.small_q, l_ep=p.l_ep, base_log2=p.base_log2, l_key=p.l_key, base_log2_key=p.base_log2_key,
+                l_ks=p.l_ks, bv_base_log2=p.bv_base_log2)
+
+    def create(**over):
+        s = gpu.OpirScheme(**{**good, **over})
+        h = ctypes.c_void_p()
+        rc = lib.opir_ctx_create(ctypes.byref(s), 0, ctypes.byref(h))
+        if rc == gpu.OPIR_OK:
+            lib.opir_ctx_destroy(h)
+        return rc
+
+    assert create() in (gpu.OPIR_OK, gpu.OPIR_EHIP)     # valid: fails only for want of a device
+    assert create(ntt_root=2) == gpu.OPIR_EINVAL         # not a primitive 2N-th root
+    assert create(poly_degree=4096) == gpu.OPIR_EINVAL   # one prime runs at N = 2048
+    assert create(small_q=q) == gpu.OPIR_EINVAL
+    # balanced digits over a centred int64: the last offset 2^(b(l-1)-1) must fit
+    assert create(base_log2=16) == gpu.OPIR_EINVAL       # 16 * 4 = 64 > 62
+    assert create(base_log2=15) in (gpu.OPIR_OK, gpu.OPIR_EHIP)
+    assert create(bv_base_log2=9) == gpu.OPIR_EINVAL     # 9 * 7 = 63 > 62

@@ -31,7 +31,7 @@ def db_plaintexts(O, f, N, t):
 
 def test_fixtures_present():
     assert len(FIXTURES) >= 6
-    assert {load(f)["config"] for f in FIXTURES} == {"CONFIG_N2048_K1_COMP", "CONFIG_N2048_K2_MP",
+    assert {load(f)["config"] for f in FIXTURES} == {"CONFIG_N2048_K1_COMP", "CONFIG_N2048_K1", "CONFIG_N2048_K2_MP",
                                                      "CONFIG_N4096_K2_MP"}
 
 

@@ -1,9 +1,12 @@
-"""GPU parity of the two-prime schemes against the CPU oracle, bit-exact,
-through the C ABI. Every test runs for both:
+"""GPU parity of the RNS-limb schemes against the CPU oracle, bit-exact,
+through the C ABI. Every test runs for all three:
   CONFIG_N4096_K2_MP: N = 4096, two ~60-bit RNS limbs (u64 kernels);
   CONFIG_N2048_K2_MP: N = 2048, two 29-bit RNS limbs (the 32-bit kernels of
     the composite scheme with the MP gadgets and the drop-then-rescale mod
-    switch, database_constants.h:96-109).
+    switch, database_constants.h:96-109);
+  CONFIG_N2048_K1: N = 2048, one ~60-bit prime (the u64 kernels with the
+    single-modulus balanced gadgets and the direct rescale,
+    database_constants.h:61-75).
 
 Layouts (with_layout, K = 2 gadgets l_ep = 5):
   (43, 5)        N0=16,  dims=3, Nrest=3   ragged deepest level (offset 1)
@@ -27,7 +30,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-K2_CONFIGS = ["CONFIG_N4096_K2_MP", "CONFIG_N2048_K2_MP"]
+K2_CONFIGS = ["CONFIG_N4096_K2_MP", "CONFIG_N2048_K2_MP", "CONFIG_N2048_K1"]
 
 
 @pytest.fixture(scope="module", params=K2_CONFIGS)
@@ -41,8 +44,9 @@ def k2p(O, k2name):
 
 
 def scheme_k2(O, p):
-    q0, q1 = p.rns_mods
-    return dict(poly_degree=p.N, composite=0, q1=q0, q2=q1, ntt_root=O.ntt_root(p.N, q0), ntt_root2=O.ntt_root(p.N, q1),
+    q0, q1 = (p.rns_mods + [0])[:2]  # one prime: q2 = 0
+    return dict(poly_degree=p.N, composite=0, q1=q0, q2=q1, ntt_root=O.ntt_root(p.N, q0),
+                ntt_root2=O.ntt_root(p.N, q1) if q1 else 0,
                 plain_mod=p.plain_mod, small_q=p.small_q, l_ep=p.l_ep, base_log2=p.base_log2, l_key=p.l_key,
                 base_log2_key=p.base_log2_key, l_ks=p.l_ks, bv_base_log2=p.bv_base_log2)
 
@@ -50,15 +54,15 @@ def scheme_k2(O, p):
 @pytest.fixture(scope="module")
 def ctx2(O, k2p, k2name):
     from onionpirv2_amd.gpu import GpuContext
-    say(f"[gpu] creating K=2 context ({k2name})")
+    say(f"[gpu] creating RNS-limb context ({k2name})")
     ctx = GpuContext(scheme_k2(O, k2p))
     yield ctx
     ctx.close()
 
 
 def rand_rns(rng, p, shape):
-    """Random polynomials [..., 2 limbs, N], each limb reduced mod its prime."""
-    out = np.empty(shape + (2, p.N), dtype=np.uint64)
+    """Random polynomials [..., K limbs, N], each limb reduced mod its prime."""
+    out = np.empty(shape + (len(p.rns_mods), p.N), dtype=np.uint64)
     for m, q in enumerate(p.rns_mods):
         out[..., m, :] = rng.integers(0, q, size=shape + (p.N,), dtype=np.uint64)
     return out
@@ -85,7 +89,7 @@ class Setup:
 
 @pytest.mark.parametrize("nq", [1, 3])
 def test_k2_ntt_matches_oracle(O, k2p, ctx2, nq):
-    say(f"[gpu] K=2 ntt nq={nq}")
+    say(f"[gpu] ntt nq={nq}")
     rng = np.random.default_rng(50 + nq)
     polys = rand_rns(rng, k2p, (nq,))
     exp = polys.copy()
@@ -98,22 +102,22 @@ def test_k2_ntt_matches_oracle(O, k2p, ctx2, nq):
 
 
 def test_k2_ntt_kat_product(k2p, ctx2):
-    """tests/test_hexl_ntt.cpp:41-59 on both limbs: (1+2x+3x^2)(4+5x)."""
-    N = k2p.N
-    a = np.zeros((2, N), np.uint64); a[:, :3] = [1, 2, 3]
-    b = np.zeros((2, N), np.uint64); b[:, :2] = [4, 5]
-    fa = ctx2.ntt_fwd(a.reshape(1, -1)).reshape(2, N)
-    fb = ctx2.ntt_fwd(b.reshape(1, -1)).reshape(2, N)
+    """tests/test_hexl_ntt.cpp:41-59 on every limb: (1+2x+3x^2)(4+5x)."""
+    N, K = k2p.N, len(k2p.rns_mods)
+    a = np.zeros((K, N), np.uint64); a[:, :3] = [1, 2, 3]
+    b = np.zeros((K, N), np.uint64); b[:, :2] = [4, 5]
+    fa = ctx2.ntt_fwd(a.reshape(1, -1)).reshape(K, N)
+    fb = ctx2.ntt_fwd(b.reshape(1, -1)).reshape(K, N)
     prod = np.array([[(int(x) * int(y)) % q for x, y in zip(fa[m], fb[m])] for m, q in enumerate(k2p.rns_mods)],
                     dtype=np.uint64)
-    c = ctx2.ntt_inv(prod.reshape(1, -1)).reshape(2, N)
-    for m in range(2):
+    c = ctx2.ntt_inv(prod.reshape(1, -1)).reshape(K, N)
+    for m in range(K):
         assert list(c[m, :5]) == [4, 13, 22, 15, 0]
 
 
 @pytest.mark.parametrize("which", ["key", "data"])
 def test_k2_external_product_matches_oracle(O, k2p, ctx2, which):
-    say(f"[gpu] K=2 external product ({which})")
+    say(f"[gpu] external product ({which})")
     p = k2p
     l, b = (p.l_key, p.base_log2_key) if which == "key" else (p.l_ep, p.base_log2)
     sk, _ = O.gen_sk_and_encrypt(p, np.zeros(p.N, np.uint64), 5)
@@ -129,7 +133,7 @@ def test_k2_ext_prod_mux_matches_oracle(O, k2p, ctx2, k2name, sel):
     """PirServer::ext_prod_mux (src/server.cpp:652-722) with two 60-bit limbs,
     through the C ABI and the host PirServer, bit-exact against the oracle."""
     import onionpirv2_amd.host as H
-    say(f"[gpu] K=2 ext_prod_mux ({sel} selector)")
+    say(f"[gpu] ext_prod_mux ({sel} selector)")
     p = k2p
     l, b = p.l_ep, p.base_log2
     rng = np.random.default_rng(23)
@@ -141,11 +145,11 @@ def test_k2_ext_prod_mux_matches_oracle(O, k2p, ctx2, k2name, sel):
         gsw = O.plain_to_gsw(p, one, sk, l, b, 9)
     x = rand_rns(rng, p, (2,))
     y = rand_rns(rng, p, (2,))
-    q0, q1 = p.rns_mods
+    q0, q1 = p.rns_mods[0], p.rns_mods[-1]
     x[0, 0, :3] = [0, q0 - 1, 7]
     y[0, 0, :3] = [q0 - 1, 0, 7]
-    x[1, 1, :2] = [q1 - 1, 0]
-    y[1, 1, :2] = [q1 - 1, q1 - 1]
+    x[1, -1, :2] = [q1 - 1, 0]
+    y[1, -1, :2] = [q1 - 1, q1 - 1]
     x, y = x.reshape(2, -1), y.reshape(2, -1)
     exp = O.PirServer(p.with_layout(43, 5)).ext_prod_mux(x.copy(), y.copy(), gsw)
     assert np.array_equal(ctx2.ext_prod_mux(gsw, l, b, x, y), exp)
@@ -154,12 +158,11 @@ def test_k2_ext_prod_mux_matches_oracle(O, k2p, ctx2, k2name, sel):
 
 
 def test_k2_mod_switch_matches_oracle(O, k2p, ctx2):
-    say("[gpu] K=2 mod switch")
+    say("[gpu] mod switch")
     p = k2p
     ct = rand_rns(np.random.default_rng(3), p, (2,))
-    q0, q1 = p.rns_mods
-    ct[0, 0, :4] = [0, 1, q0 // 2, q0 - 1]
-    ct[0, 1, :4] = [0, 1, q1 // 2, q1 - 1]
+    for m, q in enumerate(p.rns_mods):
+        ct[0, m, :5] = [0, 1, q // 2, q // 2 + 1, q - 1]
     ct = ct.reshape(2, -1)
     srv = O.PirServer(p.with_layout(1, 0))
     assert np.array_equal(ctx2.mod_switch(ct), srv.mod_switch(ct))
@@ -167,7 +170,7 @@ def test_k2_mod_switch_matches_oracle(O, k2p, ctx2):
 
 @pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: "x".join(map(str, l)))
 def test_k2_stages_match_oracle(O, k2p, ctx2, layout):
-    say(f"[gpu] K=2 stages layout={layout}")
+    say(f"[gpu] stages layout={layout}")
     s = Setup(O, k2p, layout, seed=3000 + layout[0], ctx=ctx2)
     p = s.p
     query = s.client.fast_generate_query(p, s.indices[-1])
@@ -193,7 +196,7 @@ def test_k2_stages_match_oracle(O, k2p, ctx2, layout):
 
 @pytest.mark.parametrize("layout", LAYOUTS, ids=lambda l: "x".join(map(str, l)))
 def test_k2_answer_bit_exact_and_decrypts(O, k2p, ctx2, layout):
-    say(f"[gpu] K=2 answer layout={layout}")
+    say(f"[gpu] answer layout={layout}")
     s = Setup(O, k2p, layout, seed=4000 + layout[0], ctx=ctx2)
     p = s.p
     for idx in s.indices:
@@ -208,7 +211,7 @@ def test_k2_answer_bit_exact_and_decrypts(O, k2p, ctx2, layout):
 
 
 def test_k2_batch_equals_single(O, k2p, ctx2):
-    say("[gpu] K=2 batch vs single")
+    say("[gpu] batch vs single")
     s = Setup(O, k2p, (1024, 8, True), seed=88, ctx=ctx2, cid=31)
     other = O.PirClient(s.p, 32, 89)
     ctx2.set_client_keys(32, other.create_bv_galois_keys(), other.generate_gsw_from_key())
@@ -227,7 +230,7 @@ def test_k2_graph_replays_match_eager(O, k2p, ctx2):
     """The two-prime answer captured as a HIP graph and replayed: every
     replay equals the eager answer, with eager calls in between and a
     changing client order in the batch."""
-    say("[gpu] K=2 graph replays")
+    say("[gpu] graph replays")
     s = Setup(O, k2p, (1024, 8, True), seed=91, ctx=ctx2, cid=35)
     other = O.PirClient(s.p, 36, 92)
     ctx2.set_client_keys(36, other.create_bv_galois_keys(), other.generate_gsw_from_key())
@@ -255,7 +258,7 @@ def test_k2_every_scan_shape_matches_oracle(O, k2p, ctx2, layout):
     """Every tile shape of the two-prime scan gives the oracle's first
     dimension (one query) and the same 9-query batch answers."""
     from test_gpu_parity import _each_scan_shape
-    say(f"[gpu] K=2 scan shapes layout={layout}")
+    say(f"[gpu] scan shapes layout={layout}")
     s = Setup(O, k2p, layout, seed=4000 + layout[0], ctx=ctx2, cid=41)
     p = s.p
     query = s.client.fast_generate_query(p, s.indices[-1])
@@ -283,13 +286,9 @@ def test_k2_bv_keyswitch_matches_oracle(O, k2p, ctx2, kexpr):
     p = k2p
     N = p.N
     k = eval(kexpr, {"N": N})
-    say(f"[gpu] K=2 BV keyswitch k={k}")
-    q0, q1 = p.rns_mods
+    say(f"[gpu] BV keyswitch k={k}")
     sk, _ = O.gen_sk_and_encrypt(p, np.zeros(N, np.uint64), 25)
     key = O.gen_bv_ks_key(p, sk, k, 26 + k)[0]
-    rng = np.random.default_rng(k)
-    ct = np.empty((2, 2 * N), np.uint64)
-    ct[:, :N] = rng.integers(0, q0, size=(2, N), dtype=np.uint64)
-    ct[:, N:] = rng.integers(0, q1, size=(2, N), dtype=np.uint64)
+    ct = rand_rns(np.random.default_rng(k), p, (2,)).reshape(2, -1)
     want = O.bv_apply(p, ct, k, key[None])
     assert np.array_equal(ctx2.bv_ks(ct, k, key), want)

@@ -10,7 +10,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-CONFIGS = ["CONFIG_N2048_K1_COMP", "CONFIG_N2048_K2_MP", "CONFIG_N4096_K2_MP"]
+CONFIGS = ["CONFIG_N2048_K1_COMP", "CONFIG_N2048_K1", "CONFIG_N2048_K2_MP", "CONFIG_N4096_K2_MP"]
 
 
 def say(*a):

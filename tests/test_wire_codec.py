@@ -7,7 +7,7 @@ query and keys decrypts to the stored plaintext.
 import numpy as np
 import pytest
 
-CONFIGS = ["CONFIG_N2048_K1_COMP", "CONFIG_N2048_K2_MP", "CONFIG_N4096_K2_MP"]
+CONFIGS = ["CONFIG_N2048_K1_COMP", "CONFIG_N2048_K1", "CONFIG_N2048_K2_MP", "CONFIG_N4096_K2_MP"]
 
 
 def _seeded(H, cfg, layout=(43, 5, True), cid=3, tag=0):

@@ -124,3 +124,25 @@ def test_noalloc_mux_matches_ext_prod_mux_two_primes(O):
             assert np.array_equal(_noalloc(ctx, gsw, l, b, x, y), want), (l, b)
     finally:
         ctx.close()
+
+
+def test_noalloc_mux_matches_ext_prod_mux_single_prime(O):
+    from onionpirv2_amd.gpu import GpuContext
+    say("[gpu] no-alloc ext_prod_mux, single-prime scheme")
+    p = O.PirParams("CONFIG_N2048_K1")
+    (q,) = p.rns_mods
+    sch = dict(poly_degree=p.N, composite=0, q1=q, q2=0, ntt_root=O.ntt_root(p.N, q), plain_mod=p.plain_mod,
+               small_q=p.small_q, l_ep=p.l_ep, base_log2=p.base_log2, l_key=p.l_key, base_log2_key=p.base_log2_key,
+               l_ks=p.l_ks, bv_base_log2=p.bv_base_log2)
+    ctx = GpuContext(sch)
+    try:
+        rng = np.random.default_rng(73)
+        for l, b in [(p.l_key, p.base_log2_key), (p.l_ep, p.base_log2)]:
+            gsw = rng.integers(0, q, size=(2 * l, 2 * p.N), dtype=np.uint64)
+            x, y = _random_ct(rng, [q], p.N), _random_ct(rng, [q], p.N)
+            x[0, :3] = [0, q - 1, 7]
+            y[0, :3] = [q - 1, 0, 7]
+            want = ctx.ext_prod_mux(gsw, l, b, x, y)
+            assert np.array_equal(_noalloc(ctx, gsw, l, b, x, y), want), (l, b)
+    finally:
+        ctx.close()
