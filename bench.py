@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--serve-budget-gib", type=float, default=96.0,
                     help="HBM budget for client keys and batch workspaces of the serving loop")
     ap.add_argument("--no-k2", action="store_true", help="skip the CONFIG_N4096_K2_MP leg")
+    ap.add_argument("--other-schemes", default="CONFIG_N2048_K1,CONFIG_N2048_K2_MP",
+                    help="n=2048 parameter sets also measured on the headline layout (empty: none)")
     ap.add_argument("--no-graphs", action="store_true", help="skip the HIP-graph replay leg")
     ap.add_argument("--no-paper-row", action="store_true",
                     help="skip the with_layout({349526,10}) leg (the reference's 1 GB row, 5.3 GiB NTT form)")
@@ -437,6 +439,109 @@ def k2_leg(args, H, G, torch, log, k2):
     return res
 
 
+def scheme_leg(args, H, G, torch, log, cfg):
+    """Another n=2048 parameter set (CONFIG_N2048_K1: one ~60-bit prime, u64
+    kernels; CONFIG_N2048_K2_MP: two 29-bit RNS limbs, MP gadgets) on the
+    headline layout with_layout({262144, 9}) = 256 x 1024 plaintexts, data
+    generated in HBM from a seed: plain single-query path with its stage
+    split and scan rate, and one fused batch of 64."""
+    import ctypes
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    p = H.PirParams(cfg).with_layout(args.target, args.height, True)
+    B = 64
+    K = len(p.rns_mods)
+    t0 = time.time()
+    server = H.PirServer(p)
+    idxs = distinct_indices(B, p.num_pt, args.seed + 11)
+    idxs = [i % p.target_num_pt for i in idxs]
+    server.gen_data_device(idxs[:2], args.seed + 11)
+
+    def mk(j):
+        c = H.PirClient(p, 11000 + j, args.seed + 13 * j + 5)
+        return c, c.create_bv_galois_keys(), c.generate_gsw_from_key(), c.fast_generate_query(p, idxs[j])
+
+    with ThreadPoolExecutor(8) as ex:
+        cl = list(ex.map(mk, range(B)))
+    for j, (c, bv, gsw, _) in enumerate(cl):
+        server.set_client_bv_galois_key(11000 + j, bv)
+        server.set_client_gsw_key(11000 + j, gsw)
+    queries = np.stack([x[3] for x in cl])
+    tag = cfg.lower().replace("config_", "")
+    log(f"[{tag}] layout N0={p.fst_dim_sz} Nrest={p.other_dim_sz} dims={p.num_dims}, {K} RNS limb(s), {B} clients, "
+        f"setup {time.time() - t0:.1f}s")
+    for j in range(2):
+        pt, budget = cl[j][0].decrypt_mod_q(server.make_query(11000 + j, queries[j].copy()))
+        assert np.array_equal(pt, server.direct_get_original_plaintext(idxs[j])), f"wrong {cfg} PIR answer"
+    db_h, ctx_h = server.db_handle(), server.ctx_handle()
+    stream = torch.cuda.current_stream()
+    assert G.ctx_call_raw("opir_ctx_set_stream", ctx_h, ctypes.c_void_p(stream.cuda_stream)) == 0
+    assert G.ctx_call_raw("opir_ctx_set_scan_variant", ctx_h, scan_shape_arg(args)) == 0
+    dev = torch.device("cuda:0")
+    d_q = torch.from_numpy(queries.view(np.int64)).to(dev)
+    d_r = torch.empty((B, 2, p.N), dtype=torch.int64, device=dev)
+    ids = np.arange(11000, 11000 + B, dtype=np.uint64)
+
+    def answer(b, off=0):
+        rc = G.answer_batch_device_raw(db_h, ids[off:off + b], d_q[off:off + b].data_ptr(), d_r[off:off + b].data_ptr())
+        if rc:
+            raise RuntimeError(G.last_error_raw(ctx_h))
+
+    def timed(b, steps):
+        for i in range(2):
+            answer(b)
+        torch.cuda.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        evs[0].record()
+        for i in range(steps):
+            answer(b, (i * b) % B if b < B else 0)
+            evs[i + 1].record()
+        torch.cuda.synchronize()
+        per = [evs[i].elapsed_time(evs[i + 1]) for i in range(steps)]
+        return sum(per) / steps, per
+
+    ms_plain, per = timed(1, args.steps)
+    G.ctx_call_raw("opir_ctx_set_profiling", ctx_h, 1)
+    acc = []
+    for i in range(5):
+        answer(1, i)
+        out = (ctypes.c_float * 6)()
+        assert G.ctx_call_raw("opir_last_stage_ms", ctx_h, out) == 0
+        acc.append(list(out))
+    G.ctx_call_raw("opir_ctx_set_profiling", ctx_h, 0)
+    names = ["Expand", "Convert", "First dim", "Other dim", "Modulus switching"]
+    stages = {k: round(sum(a[i] for a in acc) / len(acc), 4) for i, k in enumerate(names)}
+    scan_ms = sum(a[5] for a in acc) / len(acc)
+    # DB (K limbs of 8 B for the 60-bit prime, of 4 B for 29-bit limbs) + query columns + results
+    wb = 8 if p.rns_mods[0] >= 1 << 32 else 4
+    scan_bytes = p.num_pt * p.N * K * wb + p.fst_dim_sz * 2 * K * p.N * wb + p.other_dim_sz * 2 * K * p.N * wb
+    ms_b, per_b = timed(B, 4)
+    answer(B)
+    torch.cuda.synchronize()
+    rb = d_r.cpu().numpy().view(np.uint64)
+    for j in (0, B - 1):
+        assert np.array_equal(rb[j], server.make_query(11000 + j, queries[j].copy())), f"{cfg} batch != plain"
+    gib = p.target_num_pt * p.pt_size / 2**30
+    log(f"[{tag}] plain {ms_plain:.3f} ms -> {gib / (ms_plain * 1e-3):.0f} DB GiB/s, stages {stages}, scan "
+        f"{scan_ms:.3f} ms -> {scan_bytes / scan_ms / 1e6:.0f} GB/s; batch {B}: {ms_b:.2f} ms -> "
+        f"{B * 1e3 / ms_b:.0f} q/s ({B * ms_plain / ms_b:.1f}x plain)")
+    res = {"workload": f"OnionPIRv2 make_query, {cfg}, with_layout({{{args.target},{args.height}}}) = "
+                       f"{p.fst_dim_sz} x {p.other_dim_sz} plaintexts, synthetic seeded data generated in HBM",
+           "rns_limbs": K, "plain_mod": p.plain_mod,
+           "db_gib_per_s_plain": round(gib / (ms_plain * 1e-3), 2), "ms_per_query_plain": round(ms_plain, 4),
+           "queries_per_s_plain": round(1e3 / ms_plain, 2),
+           "latency_ms_plain": {"p50": round(pct(per, 50), 4), "p99": round(pct(per, 99), 4)},
+           "stage_ms_plain": stages,
+           "scan": {"kernel_ms": round(scan_ms, 4), "algorithmic_bytes": scan_bytes,
+                    "achieved_gbs": round(scan_bytes / scan_ms / 1e6, 1),
+                    "frac_of_hbm_peak": round(scan_bytes / scan_ms / 1e6 / HBM_PEAK_GBS, 4)},
+           "batched": {"batch": B, "ms_per_batch": round(ms_b, 3), "queries_per_s": round(B * 1e3 / ms_b, 2),
+                       "speedup_vs_plain_qps": round(B * ms_plain / ms_b, 3)}}
+    del d_q, d_r, server
+    torch.cuda.synchronize()
+    return res
+
+
 def host_cpu():
     """CPU model (as lscpu reports it, from /proc/cpuinfo) and the cores this
     process may use (nproc)."""
@@ -674,6 +779,9 @@ def main():
         result["graphs"] = graph_leg(args, G, ctx_h, answer, torch, B, d_r, server, queries, log)
     if not args.no_paper_row:
         result["paper_1gb_row"] = paper_row(args, H, G, torch, log)
+    others = [c for c in args.other_schemes.split(",") if c]
+    if others:
+        result["other_schemes"] = {c: scheme_leg(args, H, G, torch, log, c) for c in others}
     k2 = k2_setup(args, H, log) if not args.no_k2 else None
     if not args.no_serve:
         torch.cuda.synchronize()

@@ -10,8 +10,9 @@
  *   others      -> std::runtime_error.
  *
  * Data formats at this boundary are the reference's own, with K RNS limbs
- * (K = 1 logical limb mod Q = q1*q2 for CONFIG_N2048_K1_COMP, K = 2 limbs
- * mod q1 and q2 for CONFIG_N4096_K2_MP):
+ * (K = 1 logical limb mod Q = q1*q2 for CONFIG_N2048_K1_COMP, K = 1 limb mod
+ * one ~60-bit prime for CONFIG_N2048_K1, K = 2 limbs mod q1 and q2 for
+ * CONFIG_N2048_K2_MP and CONFIG_N4096_K2_MP):
  *   - ciphertext: c0 then c1, each K*N uint64 values, limb-major
  *     ([limb 0 N][limb 1 N] when K = 2), src/includes/rlwe.h:17-31;
  *   - NTT-form values use the reference's NTT evaluation order;
@@ -49,17 +50,19 @@ typedef struct opir_db opir_db;
  * PirParams-derived values (src/includes/database_constants.h:77-94,
  * src/pir.cpp:14-98), as a runtime value. */
 typedef struct opir_scheme {
-  uint32_t poly_degree;     /* N: 2048 (composite) or 4096 (two primes) */
+  uint32_t poly_degree;     /* N: 2048 or 4096 (CONFIG_N4096_K2_MP) */
   uint32_t composite;       /* 1: Q = q1*q2 composite single limb (CONFIG_N2048_K1_COMP);
-                               0: two RNS limbs q1, q2 (CONFIG_N4096_K2_MP, database_constants.h:111-124) */
-  uint64_t q1, q2;          /* RNS primes of Q */
-  uint64_t ntt_root;        /* composite: primitive 2N-th root mod Q (w_crt); two primes: root mod q1 */
+                               0: the RNS limbs q1, q2 of the ciphertext (CONFIG_N2048_K2_MP,
+                               CONFIG_N4096_K2_MP, database_constants.h:96-124), or one prime q1
+                               when q2 = 0 (CONFIG_N2048_K1, database_constants.h:61-75) */
+  uint64_t q1, q2;          /* RNS primes of Q (q2 = 0: one prime) */
+  uint64_t ntt_root;        /* composite: primitive 2N-th root mod Q (w_crt); otherwise root mod q1 */
   uint64_t plain_mod;       /* t */
   uint64_t small_q;         /* response modulus q' */
   uint32_t l_ep, base_log2;         /* data selector gadget */
   uint32_t l_key, base_log2_key;    /* RGSW(s) completion gadget */
   uint32_t l_ks, bv_base_log2;      /* BV keyswitch gadget */
-  uint64_t ntt_root2;       /* two primes: primitive 2N-th root mod q2 (ignored when composite) */
+  uint64_t ntt_root2;       /* two primes: primitive 2N-th root mod q2 (ignored otherwise) */
   double noise_std_dev;     /* error standard deviation of device-generated keys and queries
                                (DBConsts::NoiseStdDev, database_constants.h:48); 0 -> 2.55 */
 } opir_scheme;
@@ -157,7 +160,8 @@ int opir_client_get_keys(opir_ctx *ctx, uint64_t client_id, uint64_t *bv, size_t
  * PirClient::create_bv_galois_keys / generate_gsw_from_key
  * (src/bv_keyswitch.cpp:191-276, src/client.cpp:25-38) and
  * fast_generate_query (src/client.cpp:107-225). The 32-bit-limb schemes
- * (N = 2048) only; a two-prime N = 4096 context returns OPIR_EINVAL.
+ * (CONFIG_N2048_K1_COMP, CONFIG_N2048_K2_MP) only; a 64-bit-limb context
+ * (CONFIG_N4096_K2_MP, CONFIG_N2048_K1) returns OPIR_EINVAL.
  * keygen: n clients (keys [n][32] bytes), h BV levels each, installed as
  * resident keys under ids[i] (no key material crosses PCIe).
  * query_gen: B queries [B][2][K*N] of clients keys[i] with query counter
@@ -290,8 +294,8 @@ int opir_ext_prod_mux(opir_ctx *ctx, const uint64_t *gsw, size_t l, size_t base_
  * select of opir_ext_prod_mux with every operand in device memory (d_gsw
  * [2l][2][K*N] NTT form; d_x, d_y, d_out [2][K*N] coefficient form, values
  * reduced; d_out may alias d_x). It calls no allocator: its intermediates use
- * a fixed 2 MiB device array per GPU, so l is limited to 16 rows (composite
- * scheme) or 32 (two primes), larger -> OPIR_EINVAL. The work is queued on
+ * a fixed 2 MiB device array per GPU, so l is limited to 16 rows (32-bit-limb
+ * schemes) or 32 (64-bit-limb schemes), larger -> OPIR_EINVAL. The work is queued on
  * the context stream and the call returns without waiting. All calls on one
  * GPU share that array: calls that may overlap on the device (other streams,
  * other contexts on the same GPU) race. */

@@ -14,8 +14,8 @@ using namespace capi;
 namespace {
 
 constexpr size_t kScratchBytes = size_t{2} << 20;
-constexpr size_t kMaxL1 = 16;  // composite scheme: gadget rows of the selector RGSW
-constexpr size_t kMaxL2 = 32;  // two-prime scheme (the MAC sums bound it, as in scheme validation)
+constexpr size_t kMaxL1 = 16;  // 32-bit-limb schemes: gadget rows of the selector RGSW
+constexpr size_t kMaxL2 = 32;  // 64-bit-limb schemes (the MAC sums bound it, as in scheme validation)
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
