@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per rate")
     ap.add_argument("--serve-clients", type=int, default=512,
                     help="seeded clients spread over the serving databases (the n=4096 one gets the k2 leg's)")
+    ap.add_argument("--serve-replan", type=float, default=1.0,
+                    help="serving: re-plan batch limits, trimming lanes idle this many seconds (< 0: fixed plan)")
     ap.add_argument("--serve-lanes", type=int, default=1,
                     help="concurrent answer lanes per database in the serving loop")
     ap.add_argument("--serve-budget-gib", type=float, default=96.0,
@@ -213,8 +215,10 @@ def serving(args, H, params, server, log, k2=None):
     alone = [servers[d].make_query(cid, q.copy()) for d, cid, q in zip(entries, ids, qs)]
     log(f"[serve] {nc} n=2048 clients keyed on the device ({time.time() - t0:.1f}s) and {nk2} n=4096 clients; "
         f"{len(entries)}-query pool over {len(servers)} databases answered alone")
+    # the n=4096 database's batches are bounded by its own measured batch size
+    caps = [64] * (len(servers) - (1 if nk2 else 0)) + ([args.k2_batch] if nk2 else [])
     loop = H.ServingLoop(servers, max_batch=64, hbm_budget=int(args.serve_budget_gib * 2**30), keyed_clients=0,
-                         lanes=args.serve_lanes)
+                         lanes=args.serve_lanes, shrink_after_s=args.serve_replan, max_batch_db=caps)
     rows = []
     try:
         for lam in [float(x) for x in args.serve_lambdas.split(",") if x]:
@@ -238,6 +242,8 @@ def serving(args, H, params, server, log, k2=None):
                 f"per db {[(k.split()[0], v['requests'], v['p50_ms'], v['p99_ms']) for k, v in per_db.items()]}")
             assert exact == n and run["errors"] == 0, "served response differs from the query answered alone"
         limits = list(loop.batch_limits)
+        replan = {"peak_batch": [loop.peak_batch(d) for d in range(len(servers))], "grows": loop.grows,
+                  "trims": loop.trims} if args.serve_replan >= 0 else None
     finally:
         loop.stop()
         if k2_masked is not None:  # the n=4096 leg runs unrestricted
@@ -245,8 +251,9 @@ def serving(args, H, params, server, log, k2=None):
     return {"databases": names, "clients": {names[d]: entries.count(d) for d in range(len(servers))},
             "client_keys": "seeded clients; n=2048 keys generated on the device, n=4096 keys on the host",
             "lanes_per_database": args.serve_lanes,
+            "replan_idle_s": args.serve_replan if args.serve_replan >= 0 else None,
             "n4096_cus": args.serve_k2_cus if k2_masked else None,
-            "hbm_budget_gib": args.serve_budget_gib, "batch_limits": limits, "rates": rows}
+            "hbm_budget_gib": args.serve_budget_gib, "batch_limits": limits, "replanned": replan, "rates": rows}
 
 
 def paper_row(args, H, G, torch, log):

@@ -198,6 +198,12 @@ int opir_answer_batch_device(opir_db *db, size_t B, const uint64_t *client_ids, 
  * client's keys at expansion height h. */
 int opir_db_workspace_bytes(const opir_db *db, size_t B, size_t *out);
 int opir_db_reserve(opir_db *db, size_t B);
+/* Gives back the part of db's answer workspace beyond B queries (it is
+ * reallocated for B; B = 0 frees it), after the work queued on the context
+ * has finished; a later answer of more queries allocates again. A serving
+ * loop that re-plans its batch limits returns an idle database's extra
+ * workspace to the budget this way. */
+int opir_db_trim(opir_db *db, size_t B);
 int opir_client_key_bytes(const opir_ctx *ctx, size_t h, size_t *out);
 /* Device bytes of all client keys resident on ctx now. */
 int opir_ctx_resident_key_bytes(const opir_ctx *ctx, size_t *out);

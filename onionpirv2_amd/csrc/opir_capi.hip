@@ -56,11 +56,8 @@ size_t ws_bytes(const opir_db *db, size_t B) {
          kMuxSplitMaxBlocks * ct;
 }
 
-int ensure_ws(opir_db *db, size_t B) {
-  opir_ctx *ctx = db->ctx;
-  if (db->ws_B >= B) return OPIR_OK;
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // queued work may still use the old buffers
-  drop_graphs(db);
+// Frees the answer workspace (the caller has drained the stream).
+void release_ws(opir_db *db) {
   u32 **bufs[] = {&db->exp_a, &db->exp_b, &db->qntt, &db->sel, &db->mid_a, &db->mid_b, &db->part};
   for (auto b : bufs) {
     if (*b) (void)hipFree(*b);
@@ -72,6 +69,14 @@ int ensure_ws(opir_db *db, size_t B) {
   db->q_u64 = db->r_u64 = nullptr;
   db->qfrag = nullptr;
   db->ws_B = 0;
+}
+
+int ensure_ws(opir_db *db, size_t B) {
+  opir_ctx *ctx = db->ctx;
+  if (db->ws_B >= B) return OPIR_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // queued work may still use the old buffers
+  drop_graphs(db);
+  release_ws(db);
   HIP_TRY(ctx, hipMalloc((void **)&db->qfrag, qfrag_bytes(make_qfrag_geom(db->N0, (int)B))));
   const size_t ct = kCtWords * sizeof(u32);
   const size_t nsel = db->dims - 1;
@@ -759,6 +764,19 @@ int opir_db_workspace_bytes(const opir_db *db, size_t B, size_t *out) {
   if (!db || !out) return OPIR_EINVAL;
   *out = db->ctx->k2 ? k2_ws_bytes(db, B) : ws_bytes(db, B);
   return OPIR_OK;
+}
+
+int opir_db_trim(opir_db *db, size_t B) {
+  if (!db) return OPIR_EINVAL;
+  opir_ctx *ctx = db->ctx;
+  if (db->ws_B <= B) return OPIR_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // queued work may still use the buffers
+  drop_graphs(db);
+  if (ctx->k2)
+    k2_release_ws(db);
+  else
+    release_ws(db);
+  return B ? (ctx->k2 ? k2_ensure_ws(db, B) : ensure_ws(db, B)) : OPIR_OK;
 }
 
 int opir_db_reserve(opir_db *db, size_t B) {

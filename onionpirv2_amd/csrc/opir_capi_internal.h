@@ -361,6 +361,7 @@ struct DevBuf {
 int upload_k1(opir_ctx *ctx, const uint64_t *ids, size_t B, BatchPtrs &P);
 size_t ws_bytes(const opir_db *db, size_t B);
 int ensure_ws(opir_db *db, size_t B);
+void release_ws(opir_db *db);
 u32 *run_expand(opir_db *db, size_t B, const BatchPtrs &P);
 
 // Resolves ctx->scratch for ctx->device (the device must be current); once
@@ -375,6 +376,7 @@ void k2_ctx_free(opir_ctx *ctx);
 int k2_db_init(opir_db *db);
 size_t k2_ws_bytes(const opir_db *db, size_t B);
 int k2_ensure_ws(opir_db *db, size_t B);
+void k2_release_ws(opir_db *db);
 int k2_tune_scan(opir_db *db, size_t B);  // settle the autotuned scan shape for B queries
 void k2_db_free(opir_db *db);
 int k2_ingest(opir_db *db, const u64 *d_pts, u64 *d_tmp, size_t first_pt, size_t n);

@@ -201,7 +201,7 @@ size_t k2_ws_bytes(const opir_db *db, size_t B) {
          B * 2 * ctx->N * sizeof(u64) + k2::kSplitTargetBlocks64 * ct;
 }
 
-static void free_ws(opir_db *db) {
+void k2_release_ws(opir_db *db) {
   for (u64 **b : {&db->e64_a, &db->e64_b, &db->q64, &db->sel64, &db->m64_a, &db->m64_b, &db->q_u64, &db->r_u64,
                   &db->part64}) {
     if (*b) (void)hipFree(*b);
@@ -217,7 +217,7 @@ int k2_ensure_ws(opir_db *db, size_t B) {
   if (db->ws_B >= B) return OPIR_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // queued work may still use the old buffers
   drop_graphs(db);
-  free_ws(db);
+  k2_release_ws(db);
   const size_t ct = ctv(ctx) * sizeof(u64), nsel = db->dims - 1;
   HIP_TRY(ctx, hipMalloc((void **)&db->e64_a, B * db->exp_cts * ct));
   HIP_TRY(ctx, hipMalloc((void **)&db->e64_b, B * db->exp_cts * ct));
@@ -235,7 +235,7 @@ int k2_ensure_ws(opir_db *db, size_t B) {
 
 void k2_db_free(opir_db *db) {
   if (!db->ctx->k2) return;
-  free_ws(db);
+  k2_release_ws(db);
   if (db->d_db && db->owns_db) (void)hipFree(db->d_db);
   db->d_db = nullptr;
 }

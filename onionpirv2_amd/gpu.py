@@ -29,7 +29,7 @@ EXPORTED = [
     "opir_client_set_keys", "opir_client_set_keys_seeded", "opir_client_remove", "opir_client_get_keys",
     "opir_client_keygen_device", "opir_query_gen", "opir_query_gen_device",
     "opir_answer", "opir_answer_batch", "opir_answer_batch_device",
-    "opir_db_workspace_bytes", "opir_db_reserve", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
+    "opir_db_workspace_bytes", "opir_db_reserve", "opir_db_trim", "opir_client_key_bytes", "opir_ctx_resident_key_bytes",
     "opir_ctx_set_profiling", "opir_ctx_set_cu_mask", "opir_last_stage_ms", "opir_ctx_set_scan_variant", "opir_ctx_set_graphs", "opir_scan_variant_count", "opir_db_tuned_scan_variant",
     "opir_ntt_fwd", "opir_ntt_inv", "opir_expand", "opir_complete", "opir_first_dim", "opir_other_dim",
     "opir_ext_prod", "opir_ext_prod_mux", "opir_mod_switch", "opir_bv_ks", "opir_ext_prod_mux_device_noalloc",
@@ -105,6 +105,7 @@ def load_library():
         "opir_answer_batch_device": ([P, sz, P, P, P], ctypes.c_int),
         "opir_db_workspace_bytes": ([P, sz, ctypes.POINTER(sz)], ctypes.c_int),
         "opir_db_reserve": ([P, sz], ctypes.c_int),
+        "opir_db_trim": ([P, sz], ctypes.c_int),
         "opir_client_key_bytes": ([P, sz, ctypes.POINTER(sz)], ctypes.c_int),
         "opir_ctx_resident_key_bytes": ([P, ctypes.POINTER(sz)], ctypes.c_int),
         "opir_ctx_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
@@ -439,6 +440,16 @@ class GpuDatabase:
         """Prepare answers of up to `batch` queries: workspace, host staging
         and, in autotune mode, the scan shape of every batch-size class."""
         self.ctx.check(self.lib.opir_db_reserve(self.h, batch))
+
+    def trim(self, batch):
+        """Give back the workspace beyond `batch` queries (0: all of it)."""
+        self.ctx.check(self.lib.opir_db_trim(self.h, batch))
+
+    def workspace_bytes(self, batch):
+        """Device bytes of the answer workspace for `batch` queries."""
+        out = ctypes.c_size_t()
+        self.ctx.check(self.lib.opir_db_workspace_bytes(self.h, batch, ctypes.byref(out)))
+        return out.value
 
     def tuned_scan_variant(self, batch):
         """Scan shape the autotuner picked for this batch size (-1: not tuned)."""

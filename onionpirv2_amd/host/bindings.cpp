@@ -434,16 +434,19 @@ PYBIND11_MODULE(_opir_host, m) {
 
   py::class_<ServingLoop>(m, "ServingLoop")
       .def(py::init([](std::vector<PirServer *> dbs, size_t max_batch, size_t hbm_budget, size_t keyed_clients,
-                       size_t lanes) {
+                       size_t lanes, double shrink_after_s, std::vector<size_t> max_batch_db) {
              ServeConfig c;
              c.max_batch = max_batch;
+             c.max_batch_db = std::move(max_batch_db);
              c.hbm_budget = hbm_budget;
              c.keyed_clients = keyed_clients;
              c.lanes = lanes;
+             c.shrink_after_s = shrink_after_s;
              return std::make_unique<ServingLoop>(std::move(dbs), c);
            }),
            py::arg("dbs"), py::arg("max_batch") = 64, py::arg("hbm_budget") = 0, py::arg("keyed_clients") = 0,
-           py::arg("lanes") = 2, py::keep_alive<1, 2>())
+           py::arg("lanes") = 2, py::arg("shrink_after_s") = -1.0,
+           py::arg("max_batch_db") = std::vector<size_t>{}, py::keep_alive<1, 2>())
       .def("submit",
            [](ServingLoop &l, size_t db, size_t cid, Arr q) {
              std::vector<u64> v(q.data(), q.data() + q.size());
@@ -475,6 +478,10 @@ PYBIND11_MODULE(_opir_host, m) {
       .def("wait_idle", &ServingLoop::wait_idle, py::call_guard<py::gil_scoped_release>())
       .def("stop", &ServingLoop::stop, py::call_guard<py::gil_scoped_release>())
       .def_property_readonly("batch_limits", &ServingLoop::batch_limits)
+      .def("reserved_batch", &ServingLoop::reserved_batch, py::arg("db"), py::arg("lane") = 0)
+      .def("peak_batch", &ServingLoop::peak_batch, py::arg("db"))
+      .def_property_readonly("grows", &ServingLoop::grows)
+      .def_property_readonly("trims", &ServingLoop::trims)
       .def("run_poisson",
            // qs: one query [2, K*N] per pool entry (a list, or a [P, 2, K*N] array when all
            // databases share a scheme). resp comes back as a list of [2, N] arrays.

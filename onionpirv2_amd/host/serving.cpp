@@ -97,8 +97,20 @@ ServingLoop::ServingLoop(std::vector<PirServer *> dbs, const ServeConfig &cfg) {
       lanes_[d]->srv->gpu_scheme()->check(opir_db_workspace_bytes(lanes_[d]->srv->gpu_db(), B, &b));
       return b * cfg.lanes;
     };
+    if (!cfg.max_batch_db.empty() && cfg.max_batch_db.size() != lanes_.size())
+      throw std::invalid_argument("max_batch_db needs one bound per database");
+    caps_.assign(lanes_.size(), cfg.max_batch);
+    for (size_t d = 0; d < cfg.max_batch_db.size(); d++) {
+      if (cfg.max_batch_db[d] == 0) throw std::invalid_argument("max_batch_db bounds must be positive");
+      caps_[d] = std::min(cfg.max_batch, cfg.max_batch_db[d]);
+    }
     limits_ = plan_batch_limits(lanes_.size(), ws, fixed, cfg.hbm_budget, cfg.max_batch);
-    for (size_t d = 0; d < lanes_.size(); d++)  // reserve every workspace before the first request
+    for (size_t d = 0; d < lanes_.size(); d++) limits_[d] = std::min(limits_[d], caps_[d]);
+    budget_ = cfg.hbm_budget;
+    fixed_ = fixed;
+    shrink_after_ = cfg.hbm_budget ? cfg.shrink_after_s : -1;  // no budget: every lane holds max_batch
+    for (size_t d = 0; d < lanes_.size(); d++) {  // reserve every workspace before the first request
+      lanes_[d]->peak_B = limits_[d];
       for (auto &l : lanes_[d]->lanes) {
         std::lock_guard<std::mutex> g(*l.ctx_mu);
         const int rc = opir_db_reserve(l.db, limits_[d]);
@@ -107,7 +119,11 @@ ServingLoop::ServingLoop(std::vector<PirServer *> dbs, const ServeConfig &cfg) {
           if (rc == OPIR_EINVAL) throw std::invalid_argument(msg);
           throw std::runtime_error(msg);
         }
+        l.res_B = limits_[d];
+        l.res_bytes = lane_ws(d, l.res_B);
+        reserved_total_ += l.res_bytes;
       }
+    }
   } catch (...) {
     release();
     throw;
@@ -118,6 +134,103 @@ ServingLoop::ServingLoop(std::vector<PirServer *> dbs, const ServeConfig &cfg) {
 }
 
 ServingLoop::~ServingLoop() { stop(); }
+
+size_t ServingLoop::lane_ws(size_t d, size_t B) const {
+  size_t b = 0;
+  lanes_[d]->srv->gpu_scheme()->check(opir_db_workspace_bytes(lanes_[d]->srv->gpu_db(), B, &b));
+  return b;
+}
+
+size_t ServingLoop::reserved_batch(size_t db, size_t lane) const {
+  std::lock_guard<std::mutex> g(bmu_);
+  return lanes_.at(db)->lanes.at(lane).res_B;
+}
+size_t ServingLoop::peak_batch(size_t db) const {
+  std::lock_guard<std::mutex> g(bmu_);
+  return lanes_.at(db)->peak_B;
+}
+size_t ServingLoop::grows() const {
+  std::lock_guard<std::mutex> g(bmu_);
+  return grows_;
+}
+size_t ServingLoop::trims() const {
+  std::lock_guard<std::mutex> g(bmu_);
+  return trims_;
+}
+
+// Grow lane li of database d toward `want` queries with budget no lane holds:
+// the largest B in (res_B, want] whose extra workspace fits, and not past its
+// share while another lane reclaims. The bytes are committed before the
+// allocation and returned if it fails. A lane that stays below min(want, its
+// share) flags a reclaim.
+void ServingLoop::try_grow(size_t d, size_t li, size_t want) {
+  DbLane::Lane &me = lanes_[d]->lanes[li];
+  want = std::min(want, caps_[d]);
+  size_t target = 0, bytes = 0, old_bytes = 0;
+  {
+    std::lock_guard<std::mutex> g(bmu_);
+    const size_t cap = budget_ > fixed_ + reserved_total_ ? budget_ - fixed_ - reserved_total_ : 0;
+    size_t lo = me.res_B, hi = want;
+    if (others_reclaim(d, li)) hi = std::min(hi, std::max(me.res_B, limits_[d]));  // not past its share
+    while (lo < hi) {  // ws is non-decreasing in B
+      const size_t mid = lo + (hi - lo + 1) / 2;
+      if (lane_ws(d, mid) - me.res_bytes <= cap)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    me.reclaim = lo < std::min(want, limits_[d]);
+    if (lo <= me.res_B) return;
+    target = lo;
+    bytes = lane_ws(d, target);
+    old_bytes = me.res_bytes;
+    reserved_total_ += bytes - old_bytes;
+  }
+  int rc;
+  {
+    std::lock_guard<std::mutex> g(*me.ctx_mu);
+    rc = opir_db_reserve(me.db, target);
+    if (rc != OPIR_OK) (void)opir_db_trim(me.db, me.res_B);  // back to what the plan holds
+  }
+  std::lock_guard<std::mutex> g(bmu_);
+  if (rc != OPIR_OK) {
+    reserved_total_ -= bytes - old_bytes;
+    me.reclaim = me.res_B < std::min(want, limits_[d]);
+    return;
+  }
+  me.res_B = target;
+  me.res_bytes = bytes;
+  lanes_[d]->peak_B = std::max(lanes_[d]->peak_B, target);
+  grows_++;
+}
+
+// Trim lane li of database d to B queries' workspace (when it holds more).
+void ServingLoop::trim_to(size_t d, size_t li, size_t B) {
+  DbLane::Lane &me = lanes_[d]->lanes[li];
+  {
+    std::lock_guard<std::mutex> g(bmu_);
+    if (me.res_B <= B) return;
+  }
+  int rc;
+  {
+    std::lock_guard<std::mutex> g(*me.ctx_mu);
+    rc = opir_db_trim(me.db, B);
+  }
+  if (rc != OPIR_OK) return;  // the workspace stays as it was
+  const size_t bytes = lane_ws(d, B);
+  std::lock_guard<std::mutex> g(bmu_);
+  reserved_total_ -= me.res_bytes - bytes;
+  me.res_B = B;
+  me.res_bytes = bytes;
+  trims_++;
+}
+
+bool ServingLoop::others_reclaim(size_t d, size_t li) const {
+  for (size_t e = 0; e < lanes_.size(); e++)
+    for (size_t k = 0; k < lanes_[e]->lanes.size(); k++)
+      if ((e != d || k != li) && lanes_[e]->lanes[k].reclaim) return true;
+  return false;
+}
 
 double ServingLoop::now() const {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
@@ -179,20 +292,48 @@ void ServingLoop::worker(size_t d, size_t li) {
   PirServer &srv = *lane.srv;
   DbLane::Lane &me = lane.lanes[li];
   const size_t N = srv.get_params().get_poly_degree(), QN = N * srv.get_params().K();  // response / query values per poly
-  const size_t limit = limits_[d];
+  const bool replan = shrink_after_ >= 0;
+  const auto idle_wait = std::chrono::duration<double>(std::max(shrink_after_, 0.0));
   std::vector<uint64_t> ids, q, r;
   for (;;) {
     std::vector<Req> batch;
+    size_t want = 0;
     {
       std::unique_lock<std::mutex> lk(mu_);
-      cv_work_.wait(lk, [&] { return stop_ || !lane.queue.empty(); });
+      const auto has_work = [&] { return stop_ || !lane.queue.empty(); };
+      if (replan && reserved_batch(d, li) > 1) {
+        if (!cv_work_.wait_for(lk, idle_wait, has_work)) {  // idle: give the workspace back
+          lk.unlock();
+          {
+            std::lock_guard<std::mutex> g(bmu_);
+            me.reclaim = false;
+          }
+          trim_to(d, li, 1);
+          continue;
+        }
+      } else {
+        cv_work_.wait(lk, has_work);
+      }
       if (lane.queue.empty()) return;  // stopping, nothing left for this database
-      const size_t b = std::min(lane.queue.size(), limit);
+      want = lane.queue.size();
+    }
+    if (replan) {
+      if (want > reserved_batch(d, li)) {
+        try_grow(d, li, want);  // flags a reclaim if it stays below its share
+      } else {
+        std::lock_guard<std::mutex> g(bmu_);
+        me.reclaim = false;
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      const size_t b = std::min(lane.queue.size(), reserved_batch(d, li));
       for (size_t i = 0; i < b; i++) {
         batch.push_back(std::move(lane.queue.front()));
         lane.queue.pop_front();
       }
     }
+    if (batch.empty()) continue;  // another lane took them
     const size_t B = batch.size();
     ids.resize(B);
     q.resize(B * 2 * QN);
@@ -239,10 +380,20 @@ void ServingLoop::worker(size_t d, size_t li) {
       else
         x.error = errs[i];
     }
-    std::lock_guard<std::mutex> g(mu_);
-    for (auto &x : res) done_.push_back(std::move(x));
-    completed_ += B;
-    cv_done_.notify_all();
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      for (auto &x : res) done_.push_back(std::move(x));
+      completed_ += B;
+      cv_done_.notify_all();
+    }
+    if (replan) {  // above its share while another lane waits below its own: give the extra back
+      bool give = false;
+      {
+        std::lock_guard<std::mutex> g(bmu_);
+        give = me.res_B > limits_[d] && others_reclaim(d, li);
+      }
+      if (give) trim_to(d, li, limits_[d]);
+    }
   }
 }
 

@@ -14,7 +14,15 @@
 // previous one's scan and MUX tree), and batches of different databases
 // overlap too. Batch sizes are admitted against an HBM budget for client keys
 // and the lanes' workspaces (expanded queries, selectors, first-dimension
-// candidates), which are reserved once up front.
+// candidates). The plan is an equal share per database, reserved up front.
+// With shrink_after_s >= 0 it is re-planned as the load moves: a lane idle for
+// shrink_after_s gives its workspace back down to one query's
+// (opir_db_trim); a lane whose queue holds more than it may take grows into
+// budget no lane holds; a lane with work that holds less than its share and
+// cannot grow to it flags a reclaim, and lanes above their share trim back to
+// it at their next batch boundary (and do not grow past it while the flag
+// stands). Each grow or trim reallocates a workspace, which waits for the
+// lane's queued work.
 #pragma once
 #include <chrono>
 #include <condition_variable>
@@ -33,9 +41,11 @@ namespace onionpir {
 
 struct ServeConfig {
   size_t max_batch = 64;      // upper bound per database and batch
+  std::vector<size_t> max_batch_db;  // optional tighter bound per database (empty: max_batch for all)
   size_t hbm_budget = 0;      // bytes for keys + workspaces; 0 = bounded by max_batch only
   size_t keyed_clients = 0;   // clients per context counted against the budget; 0 = the keys resident at start
   size_t lanes = 2;           // concurrent answer lanes per database (1 = one batch at a time)
+  double shrink_after_s = -1;  // re-planning: idle time after which a lane trims to one query; < 0: fixed plan
 };
 
 struct ServeResult {
@@ -68,7 +78,15 @@ class ServingLoop {
   // Blocks until every submitted request has completed.
   void wait_idle();
   void stop();
+  // The equal-share plan every lane starts from.
   const std::vector<size_t> &batch_limits() const { return limits_; }
+  // Re-planning: the batch size lane `lane` of database `db` may take now,
+  // the largest one any lane of `db` has held, and how often lanes grew and
+  // trimmed their workspaces.
+  size_t reserved_batch(size_t db, size_t lane) const;
+  size_t peak_batch(size_t db) const;
+  size_t grows() const;
+  size_t trims() const;
   double now() const;
 
  private:
@@ -87,14 +105,25 @@ class ServingLoop {
       bool clone = false;          // ctx / db are a clone and a view owned by the loop
       std::mutex *ctx_mu = nullptr;  // serialises lanes that share one context
       std::thread th;
+      size_t res_B = 0, res_bytes = 0;  // reserved batch and its workspace bytes (under bmu_)
+      bool reclaim = false;             // holds less than its share while it has work (under bmu_)
     };
     std::vector<Lane> lanes;
+    size_t peak_B = 0;         // under bmu_
   };
   void worker(size_t d, size_t lane);
+  size_t lane_ws(size_t d, size_t B) const;  // one lane's workspace bytes for B queries
+  void try_grow(size_t d, size_t li, size_t want);
+  void trim_to(size_t d, size_t li, size_t B);
+  bool others_reclaim(size_t d, size_t li) const;  // under bmu_
 
   std::vector<std::unique_ptr<DbLane>> lanes_;
   std::vector<std::unique_ptr<std::mutex>> ctx_mus_;
   std::vector<size_t> limits_;
+  size_t budget_ = 0, fixed_ = 0, reserved_total_ = 0, grows_ = 0, trims_ = 0;
+  std::vector<size_t> caps_;   // largest batch per database
+  double shrink_after_ = -1;
+  mutable std::mutex bmu_;     // budget accounting: reserved_total_, lane res_*, peaks, counters
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
   std::vector<ServeResult> done_;

@@ -213,3 +213,69 @@ def test_lanes_overlap_batches_of_one_database(served, lanes):
         assert overlapping == 0
     else:
         assert overlapping > 0, spans
+
+
+def _ws(server, batch):
+    """opir_db_workspace_bytes of a host PirServer's database."""
+    import ctypes
+    from onionpirv2_amd.gpu import load_library
+    out = ctypes.c_size_t()
+    assert load_library().opir_db_workspace_bytes(ctypes.c_void_p(server.db_handle()), batch, ctypes.byref(out)) == 0
+    return out.value
+
+
+def _resident_keys(server):
+    import ctypes
+    from onionpirv2_amd.gpu import load_library
+    out = ctypes.c_size_t()
+    assert load_library().opir_ctx_resident_key_bytes(ctypes.c_void_p(server.ctx_handle()), ctypes.byref(out)) == 0
+    return out.value
+
+
+@pytest.mark.gpu
+def test_replanning_moves_budget_to_the_busy_database(served):
+    """Admission re-planned as the load moves: with a budget whose equal
+    share admits a few queries per database, an idle database trims to one
+    query's workspace and a burst on the other grows past its share into the
+    freed budget; a later burst on both reclaims the shares. Every response
+    still equals its answer alone."""
+    import time
+    H, servers, clients, pool, alone = served
+    print("[gpu] serving: re-planned admission", flush=True)
+    keys = sum(_resident_keys(s) for s in servers)
+    budget = keys + _ws(servers[0], 8) + _ws(servers[1], 1)
+    loop = H.ServingLoop(servers, max_batch=8, hbm_budget=budget, lanes=1, shrink_after_s=0.1)
+    try:
+        share = loop.batch_limits
+        assert share[0] < 8, share
+        time.sleep(0.5)  # both lanes idle past shrink_after_s: one query's workspace each
+        assert loop.reserved_batch(0) == 1 and loop.reserved_batch(1) == 1
+        own = [k for k, e in enumerate(pool) if e[0] == 0]
+        tickets = {}
+        for rep in range(8):
+            for k in own:
+                d, cid, _, q = pool[k]
+                tickets[loop.submit(d, cid, q)] = k
+        loop.wait_idle()
+        res = loop.poll()
+        assert loop.peak_batch(0) > share[0], (loop.peak_batch(0), share)
+        assert max(r["batch"] for r in res) > share[0]
+        for r in res:
+            assert r["error"] == ""
+            assert np.array_equal(r["resp"], alone[tickets[r["ticket"]]])
+        # both databases busy: the second reclaims its share from the first
+        tickets = {}
+        for rep in range(8):
+            for k, (d, cid, _, q) in enumerate(pool):
+                tickets[loop.submit(d, cid, q)] = k
+        loop.wait_idle()
+        res = loop.poll()
+        for r in res:
+            assert r["error"] == ""
+            assert np.array_equal(r["resp"], alone[tickets[r["ticket"]]])
+        assert max(r["batch"] for r in res if r["db"] == 1) > 1
+        assert loop.grows > 0 and loop.trims > 0
+        time.sleep(0.5)
+        assert loop.reserved_batch(0) == 1 and loop.reserved_batch(1) == 1
+    finally:
+        loop.stop()
