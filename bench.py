@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--serve-budget-gib", type=float, default=96.0,
                     help="HBM budget for client keys and batch workspaces of the serving loop")
     ap.add_argument("--no-k2", action="store_true", help="skip the CONFIG_N4096_K2_MP leg")
+    ap.add_argument("--tree-L", type=int, default=20, help="tree PIR leg: tree height (0: skip the leg)")
+    ap.add_argument("--tree-a", type=int, default=4, help="tree PIR leg: first-dimension bits")
     ap.add_argument("--other-schemes", default="CONFIG_N2048_K1,CONFIG_N2048_K2_MP",
                     help="n=2048 parameter sets also measured on the headline layout (empty: none)")
     ap.add_argument("--no-graphs", action="store_true", help="skip the HIP-graph replay leg")
@@ -549,6 +551,46 @@ def scheme_leg(args, H, G, torch, log, cfg):
     return res
 
 
+def tree_leg(args, H, log):
+    """Tree PIR on the answer path's kernels (the reference's tree layer,
+    src/tree_query.cpp:198-240, src/tree_select.cpp:121-238): a tree of 2^L
+    leaves (CONFIG_N2048_K1_COMP, first dimension 2^a), every level's packed
+    plaintexts resident on the device; one query selects one node on every
+    level. Timed through the host TreePirServer (query upload and the L + 1
+    level ciphertexts' download included); parity is in tests/test_gpu_tree.py."""
+    import numpy as np
+    hs = H.PirParams()
+    L, a = args.tree_L, args.tree_a
+    ht = H.make_tree_pir_params_for_scheme(L, a, hs)
+    t0 = time.time()
+    srv = H.TreePirServer(hs, ht)
+    srv.preprocess_synthetic()
+    hq = H.tree_query_expansion_params(ht, hs)
+    c = H.PirClient(hq, 8, args.seed + 99)
+    srv.set_client_keys(8, c.create_bv_galois_keys(), c.generate_gsw_from_key())
+    leaves = [(args.seed * (j + 1) * 2654435761) % ht.N for j in range(8)]
+    qs = [H.make_tree_query(c, hs, ht, leaf) for leaf in leaves]
+    setup = time.time() - t0
+    first = srv.select_path(8, qs[0])
+    srv.select_path(8, qs[1])
+    per = []
+    for j in range(max(10, args.steps)):
+        t1 = time.perf_counter()
+        srv.select_path(8, qs[j % len(qs)])
+        per.append((time.perf_counter() - t1) * 1e3)
+    assert np.array_equal(srv.select_path(8, qs[0]), first), "tree select is not deterministic"
+    npt = sum(1 << max(lv - (ht.n.bit_length() - 1), 0) for lv in range(L + 1))
+    ms = sum(per) / len(per)
+    log(f"[tree] L={L} a={a}: {1 << L} leaves, {npt} packed plaintexts over {L + 1} levels (setup {setup:.1f}s); "
+        f"select {ms:.3f} ms per query (p50 {pct(per, 50):.3f} p99 {pct(per, 99):.3f})")
+    return {"workload": f"tree PIR, CONFIG_N2048_K1_COMP, 2^{L} leaves, first dimension 2^{a}, synthetic node "
+                        f"values; one query selects a node on each of the {L + 1} levels",
+            "packed_plaintexts": npt, "packed_width": ht.w, "query_expansion_height": ht.h_q,
+            "ms_per_query": round(ms, 4), "queries_per_s": round(1e3 / ms, 2),
+            "latency_ms": {"p50": round(pct(per, 50), 4), "p99": round(pct(per, 99), 4)},
+            "timing": "host TreePirServer::select_path, query upload and level downloads included"}
+
+
 def host_cpu():
     """CPU model (as lscpu reports it, from /proc/cpuinfo) and the cores this
     process may use (nproc)."""
@@ -786,6 +828,8 @@ def main():
         result["graphs"] = graph_leg(args, G, ctx_h, answer, torch, B, d_r, server, queries, log)
     if not args.no_paper_row:
         result["paper_1gb_row"] = paper_row(args, H, G, torch, log)
+    if args.tree_L > 0:
+        result["tree_pir"] = tree_leg(args, H, log)
     others = [c for c in args.other_schemes.split(",") if c]
     if others:
         result["other_schemes"] = {c: scheme_leg(args, H, G, torch, log, c) for c in others}
