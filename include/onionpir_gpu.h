@@ -212,7 +212,9 @@ int opir_ctx_resident_key_bytes(const opir_ctx *ctx, size_t *out);
  * names (src/includes/logging.h:122-134): [0] Expand, [1] Convert,
  * [2] First dim, [3] Other dim, [4] Modulus switching, [5] scan kernel only.
  * Convert (selector completion) runs on a second stream concurrently with
- * First dim; both are timed from the end of Expand.
+ * First dim; both are timed from the end of Expand. For N = 4096 two-prime
+ * batches whose completion has a workgroup per compute unit or more, it runs
+ * ahead of the scan on the answer stream instead, and First dim includes it.
  * Requires opir_ctx_set_profiling(ctx, 1) before the answer call. */
 int opir_ctx_set_profiling(opir_ctx *ctx, int on);
 int opir_last_stage_ms(opir_ctx *ctx, float *out6);

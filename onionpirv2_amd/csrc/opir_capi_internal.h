@@ -82,6 +82,7 @@ struct opir_ctx {
   // this device's copy of the allocation-free entry point's scratch array
   void *scratch = nullptr;
   std::vector<uint32_t> cu_mask;  // compute-unit mask of the own streams (empty: all CUs)
+  int dev_cus = 0;                // compute units of the device
 };
 
 struct opir_db {
@@ -176,6 +177,25 @@ inline int set_err(opir_ctx *ctx, int code, const std::string &msg) {
 
 // Fork: work queued on ctx->aux_stream after this runs after everything
 // queued on ctx->stream so far. Join: ctx->stream waits for the aux work.
+// Compute units the context's streams run on.
+inline int ctx_cus(const opir_ctx *ctx) {
+  if (ctx->cu_mask.empty()) return ctx->dev_cus > 0 ? ctx->dev_cus : 1;
+  int n = 0;
+  for (uint32_t w : ctx->cu_mask) n += __builtin_popcount(w);
+  return n;
+}
+
+// Whether the selector completion runs beside the first-dimension scan on
+// the auxiliary stream, or ahead of it on the answer stream. Beside the scan
+// it fills CUs the scan leaves idle. A completion whose workgroups each hold
+// most of a CU's LDS (whole_cu) and whose grid has a workgroup per CU or
+// more waits for whole CUs that the scan's workgroups keep taking. It then
+// runs ahead of the scan, which costs the batch nothing and keeps its own
+// time at what it does alone.
+inline bool complete_beside_scan(const opir_ctx *ctx, size_t rows, size_t B, bool whole_cu) {
+  return !whole_cu || rows * B < (size_t)ctx_cus(ctx);
+}
+
 inline void fork_aux(opir_ctx *ctx) {
   (void)hipEventRecord(ctx->ev_fork, ctx->stream);
   (void)hipStreamWaitEvent(ctx->aux_stream, ctx->ev_fork, 0);

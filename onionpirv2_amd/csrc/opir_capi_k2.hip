@@ -332,14 +332,17 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
     u64 *leaves = run_expand(db, B, P);
     LAUNCH_CHECK(ctx);
     rec(ctx, 1);
-    // selector completion on the auxiliary stream, concurrent with the scan
-    fork_aux(ctx);
+    // selector completion: beside the scan on the auxiliary stream, or ahead
+    // of it (complete_beside_scan); at N = 4096 with two limbs its
+    // 1024-thread workgroups hold 130 KiB of LDS, one per CU
     const size_t nsel = db->dims - 1;
+    const bool beside = complete_beside_scan(ctx, nsel * ctx->G64.l_ep, B, ctx->K == 2);
+    if (beside) fork_aux(ctx);
     const size_t sel_stride = nsel * 2 * (size_t)ctx->G64.l_ep * ctv(ctx);
     k2::launch_k2_complete(leaves + db->N0 * ctv(ctx), db->sel64, exp_stride, sel_stride, (int)(nsel * ctx->G64.l_ep),
-                           (int)B, P, ctx->G64, ctx->S64, ctx->aux_stream);
+                           (int)B, P, ctx->G64, ctx->S64, beside ? ctx->aux_stream : ctx->stream);
     LAUNCH_CHECK(ctx);
-    rec(ctx, 2, true);
+    rec(ctx, 2, beside);
     k2::launch_k2_ct_ntt(leaves, db->q64, exp_stride, db->N0 * ctv(ctx), (int)db->N0, (int)B, ctx->S64, ctx->stream);
     launch_qfrag64(db->q64, db->qfrag, db->N0 * ctv(ctx), (int)B, db->geom, ctx->stream);
     rec(ctx, 5);
@@ -351,7 +354,7 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
   LAUNCH_CHECK(ctx);
   rec(ctx, 6);
   rec(ctx, 3);
-  join_aux(ctx);
+  if (beside) join_aux(ctx);
   u64 *res = run_other_dim(db, B, db->sel64);
   LAUNCH_CHECK(ctx);
   rec(ctx, 4);

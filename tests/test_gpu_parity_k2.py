@@ -253,6 +253,36 @@ def test_k2_graph_replays_match_eager(O, k2p, ctx2):
         ctx2.remove_client(36)
 
 
+def test_k2_completion_ahead_of_scan_matches(O, k2p, ctx2):
+    """On 8 compute units a 4-query batch has 60 completion workgroups, at
+    least one per CU, so the N = 4096 two-prime completion runs ahead of the
+    scan on the answer stream instead of beside it. The answers, eager and
+    replayed as graphs, equal those of the unrestricted context."""
+    say("[gpu] completion ahead of the scan (8-CU mask)")
+    s = Setup(O, k2p, (1024, 8, True), seed=95, ctx=ctx2, cid=37)
+    other = O.PirClient(s.p, 38, 96)
+    ctx2.set_client_keys(38, other.create_bv_galois_keys(), other.generate_gsw_from_key())
+    ids = [37, 38, 37, 38]
+    qs = np.stack([(s.client if c == 37 else other).fast_generate_query(s.p, i)
+                   for c, i in zip(ids, [3, 900, 511, 64])])
+    want = s.db.answer_batch(ids, qs)
+    assert np.array_equal(want[0], s.server.make_query(37, qs[0]))
+    try:
+        ctx2.set_cu_mask(range(8))
+        assert np.array_equal(s.db.answer_batch(ids, qs), want)
+        assert np.array_equal(s.db.answer(38, qs[1]), want[1])
+        ctx2.set_graphs(True)
+        for r in range(3):
+            assert np.array_equal(s.db.answer_batch(ids, qs), want), r
+    finally:
+        ctx2.set_graphs(False)
+        ctx2.set_cu_mask(None)
+    try:
+        assert np.array_equal(s.db.answer_batch(ids, qs), want)  # unrestricted again
+    finally:
+        ctx2.remove_client(38)
+
+
 @pytest.mark.parametrize("layout", LAYOUTS[:4], ids=lambda l: "x".join(map(str, l)))
 def test_k2_every_scan_shape_matches_oracle(O, k2p, ctx2, layout):
     """Every tile shape of the two-prime scan gives the oracle's first
