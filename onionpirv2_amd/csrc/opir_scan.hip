@@ -145,14 +145,30 @@ __global__ __launch_bounds__(256) void k_qfrag(const V *__restrict__ qntt, int8_
   const int kb = kg >> 2, grp = kg & 3;
   const int lv0 = blockIdx.y * kQfLv, limb = blockIdx.z;
   const int ncols = qg.ncm;  // columns stored per tile; c_end = 2B columns in all
-  // only the stored columns are staged (the writes below read no others)
-  for (int e = threadIdx.x; e < ncols * 16 * kQfLv; e += 256) {
-    const int lv = e % kQfLv, row = e / kQfLv, n = row >> 4, j = row & 15;
+  // Only the stored columns are staged (the writes below read no others), as
+  // 16-byte pieces of each (column, k) level run, all of a thread's loads in
+  // flight before the first LDS store.
+  constexpr int VL = 16 / (int)sizeof(V);    // levels per piece
+  constexpr int NCH = kQfLv / VL;            // pieces per run
+  constexpr int NIT = 16 * 16 * NCH / 256;   // pieces per thread (16 columns)
+  uint4 piece[NIT];
+#pragma unroll
+  for (int i = 0; i < NIT; i++) {
+    const int e = threadIdx.x + 256 * i, ch = e % NCH, row = e / NCH, n = row >> 4, j = row & 15;
     const int c = ct * 16 + n, k = kg * 16 + j;
-    V x = 0;
-    if (c < c_end && k < qg.N0)
-      x = qntt[(size_t)(c >> 1) * q_stride + (((size_t)k * 2 + (c & 1)) * qg.nl + limb) * qg.nlv + lv0 + lv];
-    tile[n * kQfNs + j * (kQfLv + 1) + lv] = x;
+    piece[i] = make_uint4(0, 0, 0, 0);
+    if (n < ncols && c < c_end && k < qg.N0)
+      piece[i] = *reinterpret_cast<const uint4 *>(
+          qntt + (size_t)(c >> 1) * q_stride + (((size_t)k * 2 + (c & 1)) * qg.nl + limb) * qg.nlv + lv0 + ch * VL);
+  }
+#pragma unroll
+  for (int i = 0; i < NIT; i++) {
+    const int e = threadIdx.x + 256 * i, ch = e % NCH, row = e / NCH, n = row >> 4, j = row & 15;
+    if (n >= ncols) continue;
+    V x[VL];
+    __builtin_memcpy(x, &piece[i], 16);
+#pragma unroll
+    for (int v = 0; v < VL; v++) tile[n * kQfNs + j * (kQfLv + 1) + ch * VL + v] = x[v];
   }
   __syncthreads();
   const size_t blk = (((size_t)limb * qg.nlv + lv0) * qg.nct + ct) * qg.nkb + kb;  // level lv0
