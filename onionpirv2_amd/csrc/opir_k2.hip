@@ -167,32 +167,80 @@ __device__ __forceinline__ void inv8(u64 v[8], unsigned b, const Limb64 &L) {
   }
 }
 
-// LDS exchange between rounds. t is passed by reference and made opaque
+// LDS exchanges between rounds. t is passed by reference and made opaque
 // afterwards, so the next round's twiddle addresses (functions of t) cannot be
 // computed and loaded before the exchange: hoisting all rounds' twiddles
 // exceeds the 128-register budget of a 1024-thread workgroup.
+//
+// In maps 1, 2 and 3 the 64 threads of wave w (t in [64w, 64w + 64)) hold
+// exactly indices [512w, 512w + 512), for N = 4096 and N = 2048 alike, so
+// only the exchanges from and to map 0 move values between waves. The others
+// go through the wave's own region of the buffer with no workgroup barrier:
+// the hardware runs a wave's LDS instructions in order, and the wavefront
+// fences only keep the compiler from reordering them.
+__device__ __forceinline__ void wave_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 template <int N>
-__device__ __forceinline__ void exch(u64 v[8], u64 *lds, int &t, int from, int to) {
+__device__ __forceinline__ void put(const u64 v[8], u64 *lds, int t, int r) {
 #pragma unroll
-  for (int c = 0; c < 8; c++) lds[pidx(r_index<N>(from, t, c))] = v[c];
+  for (int c = 0; c < 8; c++) lds[pidx(r_index<N>(r, t, c))] = v[c];
+}
+template <int N>
+__device__ __forceinline__ void get(u64 v[8], const u64 *lds, int t, int r) {
+#pragma unroll
+  for (int c = 0; c < 8; c++) v[c] = lds[pidx(r_index<N>(r, t, c))];
+}
+// Between maps 1, 2 and 3, inside each wave.
+template <int N>
+__device__ __forceinline__ void exch_wave(u64 v[8], u64 *lds, int &t, int from, int to) {
+  wave_order();
+  put<N>(v, lds, t, from);
+  wave_order();
+  get<N>(v, lds, t, to);
+  asm volatile("" : "+v"(t));
+}
+// Map 0 -> map 1 (forward transforms): the first barrier lets every wave
+// finish its reads of the buffer, whatever they were, before the scattered
+// writes; the second publishes them. Afterwards each wave reads only its own
+// region.
+template <int N>
+__device__ __forceinline__ void exch_from0(u64 v[8], u64 *lds, int &t) {
   __syncthreads();
-#pragma unroll
-  for (int c = 0; c < 8; c++) v[c] = lds[pidx(r_index<N>(to, t, c))];
+  put<N>(v, lds, t, 0);
+  __syncthreads();
+  get<N>(v, lds, t, 1);
+  asm volatile("" : "+v"(t));
+}
+// Map 1 -> map 0 (inverse transforms): each wave writes its own region (no
+// other wave reads the buffer on entry), the barrier publishes them, and the
+// last barrier lets every wave finish its scattered reads before the buffer
+// is written again.
+template <int N>
+__device__ __forceinline__ void exch_to0(u64 v[8], u64 *lds, int &t) {
+  wave_order();
+  put<N>(v, lds, t, 1);
+  __syncthreads();
+  get<N>(v, lds, t, 0);
   __syncthreads();
   asm volatile("" : "+v"(t));
 }
 
-// map0 [0, q) in -> map3 out, [0, q) when reduce, else [0, 4q).
+// map0 [0, q) in -> map3 out, [0, q) when reduce, else [0, 4q). Two
+// workgroup barriers, both before the first round's values cross waves; on
+// return no wave reads another wave's region of lds.
 template <int N>
 __device__ __forceinline__ void ntt_fwd(u64 v[8], u64 *lds, int t, const Limb64 &L, bool reduce = true) {
   constexpr int LB = N / 8;
   asm volatile("" : "+v"(t));
   fwd8(v, 1, L);
-  exch<N>(v, lds, t, 0, 1);
+  exch_from0<N>(v, lds, t);
   fwd8(v, 8 + t / (LB / 8), L);
-  exch<N>(v, lds, t, 1, 2);
+  exch_wave<N>(v, lds, t, 1, 2);
   fwd8(v, 64 + t / (LB / 64), L);
-  exch<N>(v, lds, t, 2, 3);
+  exch_wave<N>(v, lds, t, 2, 3);
   if constexpr (N == 4096)
     fwd8(v, 512 + t, L);
   else
@@ -203,7 +251,10 @@ __device__ __forceinline__ void ntt_fwd(u64 v[8], u64 *lds, int t, const Limb64 
   }
 }
 
-// map3 [0, q) in -> map0 out in [0, q).
+// map3 [0, q) in -> map0 out in [0, q). On entry no other wave may still
+// read this wave's region of lds (every transform and every staging use of
+// lds in these kernels leaves it so); two workgroup barriers, the last after
+// the scattered reads, so lds is free for any writes on return.
 template <int N>
 __device__ __forceinline__ void ntt_inv(u64 v[8], u64 *lds, int t, const Limb64 &L) {
   constexpr int LB = N / 8;
@@ -212,11 +263,11 @@ __device__ __forceinline__ void ntt_inv(u64 v[8], u64 *lds, int t, const Limb64 
     inv8(v, 512 + t, L);
   else
     inv4x2(v, 256 + t, L);
-  exch<N>(v, lds, t, 3, 2);
+  exch_wave<N>(v, lds, t, 3, 2);
   inv8(v, 64 + t / (LB / 64), L);
-  exch<N>(v, lds, t, 2, 1);
+  exch_wave<N>(v, lds, t, 2, 1);
   inv8(v, 8 + t / (LB / 8), L);
-  exch<N>(v, lds, t, 1, 0);
+  exch_to0<N>(v, lds, t);
   inv8(v, 1, L);
 #pragma unroll
   for (int c = 0; c < 8; c++) v[c] = csub(shoup(v[c], L.ninv, L.ninv_s, L.q), L.q);
