@@ -35,11 +35,11 @@ def full(O):
     p = O.PirParams("CONFIG_N2048_K1_COMP").with_layout(*LAYOUT)
     assert (p.fst_dim_sz, p.other_dim_sz, p.num_dims) == (256, 1024, 11)
     t0 = time.time()
-    say("[gpu] cfg1: oracle database (uniform NTT residues, 4 GiB)")
+    say("[gpu] 256x1024: oracle database (uniform NTT residues, 4 GiB)")
     ref = O.PirServer(p)
     ref.fill_random_ntt(0x5EED)
     lo, hi = ref.db_lo(), ref.db_hi()
-    say(f"[gpu] cfg1: loading it through opir_db_load_ntt ({time.time() - t0:.1f} s)")
+    say(f"[gpu] 256x1024: loading it through opir_db_load_ntt ({time.time() - t0:.1f} s)")
     ctx = GpuContext(scheme_from_oracle(p))
     db = GpuDatabase(ctx, p.fst_dim_sz, p.other_dim_sz, p.num_dims, p.expansion_height)
     db.load_ntt(lo, hi)
@@ -51,7 +51,7 @@ def full(O):
         bv, gsw = c.create_bv_galois_keys(), c.generate_gsw_from_key()
         ctx.set_client_keys(1000 + j, bv, gsw)
         clients.append((c, bv, gsw))
-    say(f"[gpu] cfg1: 128 clients keyed ({time.time() - t0:.1f} s)")
+    say(f"[gpu] 256x1024: 128 clients keyed ({time.time() - t0:.1f} s)")
     yield p, hp, ref, ctx, db, clients
     db.close()
     ctx.close()
@@ -62,47 +62,47 @@ def _indices(n, num_pt, seed):
     return [int(x) for x in rng.choice(num_pt, size=n, replace=False)]
 
 
-def test_cfg1_plain_answer_bit_exact_vs_oracle(full):
+def test_256x1024_plain_answer_bit_exact_vs_oracle(full):
     p, hp, ref, ctx, db, clients = full
     c, bv, gsw = clients[0]
     idx = p.target_num_pt - 1
     q = c.fast_generate_query(hp, idx)
-    say("[gpu] cfg1: plain answer, last plaintext")
+    say("[gpu] 256x1024: plain answer, last plaintext")
     got = db.answer(1000, q)
     ref.set_client_keys(1000, bv, gsw)
-    say("[gpu] cfg1: oracle make_query")
+    say("[gpu] 256x1024: oracle make_query")
     want = ref.make_query(1000, q)
     assert np.array_equal(got, want)
 
 
 @pytest.mark.parametrize("B", [64, 128])
-def test_cfg2_batch_of_distinct_clients_equals_single(full, B):
-    """BASELINE's batching form: B queries of B distinct clients (their own
+def test_256x1024_batch_of_distinct_clients_equals_single(full, B):
+    """The batched form: B queries of B distinct clients (their own
     key sets) in one fused pass; every response equals the query answered
     alone, and two equal the oracle's make_query."""
     p, hp, ref, ctx, db, clients = full
     idxs = _indices(B, p.target_num_pt, B)
     ids = [1000 + j for j in range(B)]
     qs = np.stack([clients[j][0].fast_generate_query(hp, idxs[j]) for j in range(B)])
-    say(f"[gpu] cfg2: batch of {B}")
+    say(f"[gpu] 256x1024 batch: batch of {B}")
     batch = db.answer_batch(ids, qs)
-    say(f"[gpu] cfg2: {B} single answers")
+    say(f"[gpu] 256x1024 batch: {B} single answers")
     for j in range(B):
         assert np.array_equal(batch[j], db.answer(ids[j], qs[j])), j
     for j in (1, B - 1):
         _, bv, gsw = clients[j]
         ref.set_client_keys(ids[j], bv, gsw)
-        say(f"[gpu] cfg2: oracle make_query for client {ids[j]}")
+        say(f"[gpu] 256x1024 batch: oracle make_query for client {ids[j]}")
         assert np.array_equal(batch[j], ref.make_query(ids[j], qs[j]))
 
 
-def test_cfg1_device_generated_database_decrypts(full):
+def test_256x1024_device_generated_database_decrypts(full):
     """The same layout through the host PirServer with plaintexts generated
     on the device: recorded entries (first, last, one inside) decrypt."""
     import onionpirv2_amd.host as H
     p, hp, ref, ctx, db, clients = full
     rec = [0, 123457, hp.target_num_pt - 1]
-    say("[gpu] cfg1: gen_data_device on a second 4 GiB database")
+    say("[gpu] 256x1024: gen_data_device on a second 4 GiB database")
     srv = H.PirServer(hp)
     srv.gen_data_device(rec, 777)
     c, bv, gsw = clients[5]

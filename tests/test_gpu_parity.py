@@ -5,7 +5,7 @@ Layouts are small enough for the oracle to finish in seconds; each covers a
 shape feature of the reference's evaluate_other_dim (src/server.cpp:599-649):
   (43, 5)        N0=16,  dims=3, Nrest=3   ragged deepest level (offset 1)
   (700, 8)       N0=128, dims=4, Nrest=6   ragged with offset 2
-  (4096, 9)      N0=256, dims=5, Nrest=16  cfg0-like, perfect tree
+  (4096, 9)      N0=256, dims=5, Nrest=16  perfect tree (the 4096-plaintext workload)
   (43, 5, False) N0=26,  dims=2, Nrest=2   non-power-of-two first dimension
   (1, 0)         singleton: no expansion, no selectors
 """
