@@ -55,7 +55,9 @@ typedef struct opir_scheme {
                                0: the RNS limbs q1, q2 of the ciphertext (CONFIG_N2048_K2_MP,
                                CONFIG_N4096_K2_MP, database_constants.h:96-124), or one prime q1
                                when q2 = 0 (CONFIG_N2048_K1, database_constants.h:61-75) */
-  uint64_t q1, q2;          /* RNS primes of Q (q2 = 0: one prime) */
+  uint64_t q1, q2;          /* RNS primes of Q (q2 = 0: one prime); at N = 2048 with two
+                               primes (CONFIG_N2048_K1_COMP, CONFIG_N2048_K2_MP) each is
+                               below 2^29, as the reference's 29-bit primes are */
   uint64_t ntt_root;        /* composite: primitive 2N-th root mod Q (w_crt); otherwise root mod q1 */
   uint64_t plain_mod;       /* t */
   uint64_t small_q;         /* response modulus q' */

@@ -253,15 +253,18 @@ int opir_ctx_create(const opir_scheme *sch, int device, opir_ctx **out) {
   *out = nullptr;
   // two RNS primes below 2^30 at N = 2048 (CONFIG_N2048_K2_MP) run on the
   // composite scheme's 32-bit kernels with the MP gadgets and mod switch
+  // (which take primes below 2^29; larger ones are rejected there)
   const bool mp = !sch->composite && sch->q2 && sch->poly_degree == (uint32_t)kN && sch->q1 < (1ULL << 30) &&
                   sch->q2 < (1ULL << 30);
   if (!sch->composite && !mp) return ctx_create_k2(sch, device, out);
   if (sch->poly_degree != (uint32_t)kN)
     return set_err(nullptr, OPIR_EINVAL, "the composite scheme runs at N=2048 (CONFIG_N2048_K1_COMP)");
   const u64 q1 = sch->q1, q2 = sch->q2;
-  if (q1 >= (1ULL << 30) || q2 >= (1ULL << 30) || q1 < 3 || q2 < 3 || q1 >= 2 * q2 || q2 >= 2 * q1 ||
+  // below 2^29: the forward transforms let values grow to 8q < 2^32 between
+  // reductions (ntt_fwd_2048)
+  if (q1 >= (1ULL << 29) || q2 >= (1ULL << 29) || q1 < 3 || q2 < 3 || q1 >= 2 * q2 || q2 >= 2 * q1 ||
       (q1 - 1) % (2 * kN) || (q2 - 1) % (2 * kN))
-    return set_err(nullptr, OPIR_EINVAL, "limb primes must be < 2^30, within a factor 2, and 1 mod 2N");
+    return set_err(nullptr, OPIR_EINVAL, "limb primes must be < 2^29, within a factor 2, and 1 mod 2N");
   if (sch->l_ep == 0 || sch->l_key == 0 || sch->l_ks == 0 || sch->base_log2 == 0 || sch->base_log2 > 32 ||
       sch->base_log2_key == 0 || sch->base_log2_key > 32 || sch->bv_base_log2 == 0 || sch->bv_base_log2 > 32)
     return set_err(nullptr, OPIR_EINVAL, "invalid gadget parameters");

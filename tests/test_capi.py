@@ -35,6 +35,25 @@ def test_ctx_create_rejects_bad_scheme_without_gpu():
     assert lib.opir_answer(None, 0, None, None) == gpu.OPIR_EINVAL
 
 
+def test_32bit_limb_primes_above_2_29_rejected_without_gpu(O):
+    """The 32-bit kernels take limb primes below 2^29 (the forward transform
+    lets values grow to 8q < 2^32): two primes 1 mod 2N in [2^29, 2^30) are
+    rejected before any device call, composite or as two RNS limbs."""
+    from onionpirv2_amd import gpu
+    from conftest import scheme_from_oracle
+    from test_oracle_pins_k2 import _is_prime
+    lib = gpu.load_library()
+    p = O.PirParams("CONFIG_N2048_K1_COMP")
+    m = 2 * p.N
+    big = [q for q in range((1 << 29) // m * m + 1, 1 << 30, m) if q > (1 << 29) and _is_prime(q)][:2]
+    assert len(big) == 2 and all((1 << 29) < q < (1 << 30) and q % m == 1 for q in big)
+    for composite in (1, 0):
+        s = gpu.OpirScheme(**{**scheme_from_oracle(p), "composite": composite, "q1": big[0], "q2": big[1]})
+        h = ctypes.c_void_p()
+        assert lib.opir_ctx_create(ctypes.byref(s), 0, ctypes.byref(h)) == gpu.OPIR_EINVAL
+        assert b"2^29" in lib.opir_last_error(None)
+
+
 def test_k2_scheme_validation_without_gpu(O):
     """The two-prime scheme of CONFIG_N4096_K2_MP passes host-side validation
     (it then needs a device); malformed variants are rejected first."""
