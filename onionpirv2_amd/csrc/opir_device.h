@@ -138,15 +138,16 @@ __device__ __forceinline__ int lds_idx(int i) { return i + ((i >> 5) << 2); }
 // x + 2q - t). The conditional subtractions applied to X first depend on
 // the bound X has on entry (every value of a stage has the same bound):
 //   kBfC: X < 4q, subtract 2q if X >= 2q             -> outputs < 4q
+//   kBfH: X < 8q, subtract 4q if X >= 4q             -> outputs < 6q
 //   kBfD: X < 8q, subtract 4q, then 2q, if at least  -> outputs < 4q
 //   kBfN: none                                       -> outputs < bound + 2q
 // With q < 2^29 every bound below stays under 8q < 2^32.
-enum { kBfN, kBfC, kBfD };
+enum { kBfN, kBfC, kBfH, kBfD };
 template <int kKind>
 __device__ __forceinline__ void bf_fwd(u32 &X, u32 &Y, u32 w, u32 ws, const LimbConst &L) {
   u32 x = X;
-  if constexpr (kKind == kBfD) x = csub(x, 2 * L.q2);
-  if constexpr (kKind != kBfN) x = csub(x, L.q2);
+  if constexpr (kKind == kBfD || kKind == kBfH) x = csub(x, 2 * L.q2);
+  if constexpr (kKind == kBfD || kKind == kBfC) x = csub(x, L.q2);
   const u32 hi = __umulhi(Y, ws);
   const u32 tn = hi * L.q - Y * w;
   X = x - tn;
@@ -275,16 +276,16 @@ __device__ __forceinline__ void lds_exchange(u32 v[8], u32 *lds, int buf, int t,
 
 // kReduce = false leaves outputs lazily in [0, 4q) (< 2^31), for callers that
 // only multiply-accumulate them into 64-bit sums. Inputs in [0, q) (any
-// values below 2q are fine as well). Value bounds through the 11 stages:
-// 3q, 5q, 7q | 4q, 6q, 8q | 4q, 6q, 8q | 4q, 4q, so one stage per round
-// reduces (kBfD) and the others add without a conditional subtraction.
+// values below 2q are fine as well). Value bounds (for inputs below q)
+// through the 11 stages: 3q, 5q, 7q | 6q, 8q, 6q | 8q, 6q, 8q | 6q, 4q: six
+// conditional subtractions per value where a reduction every stage takes 11.
 template <bool kReduce = true, typename Bar>
 __device__ __forceinline__ void ntt_fwd_2048(u32 v[8], u32 *lds, int t, const LimbConst &L, Bar bar) {
   fwd_round8<kBfN, kBfN, kBfN>(v, 1, 0, L);     // stages 0-2 (map0, g = 0)
   lds_exchange<0, 1>(v, lds, 0, t, bar);
-  fwd_round8<kBfD, kBfN, kBfN>(v, 8, t / 32, L);   // stages 3-5
+  fwd_round8<kBfH, kBfN, kBfH>(v, 8, t / 32, L);   // stages 3-5
   lds_exchange<1, 2>(v, lds, 1, t, bar);
-  fwd_round8<kBfD, kBfN, kBfN>(v, 64, t / 4, L);   // stages 6-8
+  fwd_round8<kBfN, kBfH, kBfN>(v, 64, t / 4, L);   // stages 6-8
   lds_exchange<2, 3>(v, lds, 2, t, bar);
   // stages 9-10 on 8 consecutive values: twiddles w[512 + 2t + h], w[1024 + 4t + h]
   {
@@ -293,14 +294,14 @@ __device__ __forceinline__ void ntt_fwd_2048(u32 v[8], u32 *lds, int t, const Li
     const uint2 s2 = ld_off<uint2>(L.psi_s, (512 * 4) + (u << 3));
     const uint4 w4 = ld_off<uint4>(L.psi, (1024 * 4) + (u << 4));
     const uint4 s4 = ld_off<uint4>(L.psi_s, (1024 * 4) + (u << 4));
-    bf_fwd<kBfD>(v[0], v[2], w2.x, s2.x, L);
-    bf_fwd<kBfD>(v[1], v[3], w2.x, s2.x, L);
-    bf_fwd<kBfD>(v[4], v[6], w2.y, s2.y, L);
-    bf_fwd<kBfD>(v[5], v[7], w2.y, s2.y, L);
-    bf_fwd<kBfC>(v[0], v[1], w4.x, s4.x, L);
-    bf_fwd<kBfC>(v[2], v[3], w4.y, s4.y, L);
-    bf_fwd<kBfC>(v[4], v[5], w4.z, s4.z, L);
-    bf_fwd<kBfC>(v[6], v[7], w4.w, s4.w, L);
+    bf_fwd<kBfH>(v[0], v[2], w2.x, s2.x, L);
+    bf_fwd<kBfH>(v[1], v[3], w2.x, s2.x, L);
+    bf_fwd<kBfH>(v[4], v[6], w2.y, s2.y, L);
+    bf_fwd<kBfH>(v[5], v[7], w2.y, s2.y, L);
+    bf_fwd<kBfD>(v[0], v[1], w4.x, s4.x, L);
+    bf_fwd<kBfD>(v[2], v[3], w4.y, s4.y, L);
+    bf_fwd<kBfD>(v[4], v[5], w4.z, s4.z, L);
+    bf_fwd<kBfD>(v[6], v[7], w4.w, s4.w, L);
   }
 #pragma unroll
   for (int c = 0; c < 8; c++) v[c] = kReduce ? reduce_4q(v[c], L.q, L.q2) : v[c];

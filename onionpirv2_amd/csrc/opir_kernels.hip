@@ -119,7 +119,7 @@ __global__ void k_addsub_mod(const u64 *a, const u64 *b, u64 *out, size_t n, siz
 __global__ __launch_bounds__(512) void k_ntt_u64(u64 *data, int inverse, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 res[2][kN];
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int sb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // sb: wave-uniform
   const LimbConst &L = S.limb[sb];
   u64 *poly = data + (size_t)blockIdx.x * (S.mp ? 2 : 1) * kN;
   u32 v[8];
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(512) void k_ntt_u64(u64 *data, int inverse, SchemeC
 // RNS polys [npoly][2 limbs][N]: coef (map0) <-> NTT (map3).
 __global__ __launch_bounds__(512) void k_ntt_rns(const u32 *in, u32 *out, int inverse, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int sb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // sb: wave-uniform
   const LimbConst &L = S.limb[sb];
   const size_t off = ((size_t)blockIdx.x * 2 + sb) * kN;
   u32 v[8];
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(512) void k_ntt_rns(const u32 *in, u32 *out, int in
 __global__ __launch_bounds__(512) void k_db_ntt_plain(const uint16_t *__restrict__ pts, u32 *__restrict__ tmp,
                                                       SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int sb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // sb: wave-uniform
   const LimbConst &L = S.limb[sb];
   const uint16_t *p = pts + (size_t)blockIdx.x * kN;
   u32 v[8];
@@ -218,7 +218,7 @@ void launch_gen_plain_u64(u64 *out, size_t first_pt, size_t count, int n, u64 se
 __global__ __launch_bounds__(512) void k_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride,
                                                 SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int sb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // sb: wave-uniform
   const LimbConst &L = S.limb[sb];
   const size_t poly = blockIdx.x;  // ct*2 + p
   const size_t off = (poly * 2 + sb) * kN;
@@ -276,7 +276,7 @@ __device__ __forceinline__ u64 drop_and_rescale(u32 r0, u32 r1, const SchemeCons
 __global__ __launch_bounds__(512) void k_final(const u32 *res, u64 *out, size_t res_stride, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 sh[2][kN];
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int sb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // sb: wave-uniform
   const LimbConst &L = S.limb[sb];
   const int p = blockIdx.x, qi = blockIdx.y;
   u32 v[8];
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(512) void k_final(const u32 *res, u64 *out, size_t 
 __global__ __launch_bounds__(512) void k_ct_to_coef_u64(const u32 *in, u64 *out, SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
   __shared__ u32 sh[2][kN];
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int sb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // sb: wave-uniform
   const LimbConst &L = S.limb[sb];
   const size_t poly = blockIdx.x;
   u32 v[8];
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(512) void k_ct_to_coef_u64(const u32 *in, u64 *out,
 __global__ __launch_bounds__(512) void k_ext_prod(const u32 *gsw, int l, int b, const u32 *ct, u32 *out,
                                                   SchemeConst S) {
   __shared__ __align__(16) u32 nlds[2][kLdsPoly];
-  const int sb = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int sb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;  // sb: wave-uniform
   const LimbConst &L = S.limb[sb];
   i64 D[2][8];
 #pragma unroll
