@@ -108,8 +108,19 @@ __device__ __forceinline__ int r_index(int r, int t, int c) {
 template <int N>
 __device__ __forceinline__ int map0(int t, int c) { return t + (N / 8) * c; }
 
+// Forward butterfly (X, Y) -> (X + Y*w, X - Y*w) with t = Y*w mod q in
+// [0, 2q) (Shoup, any Y < 2^64). Conditional subtractions on X first, by
+// the bound X has on entry (every value of a stage has the same bound):
+//   kFwN: none                                     -> outputs < bound + 2q
+//   kFwH: X < 16q, subtract 8q                     -> outputs < 10q
+//   kFwD: X < 16q, subtract 8q, then 4q, then 2q   -> outputs < 4q
+// With q < 2^60 every bound stays under 16q < 2^64.
+enum { kFwN, kFwH, kFwD };
+template <int kKind>
 __device__ __forceinline__ void bf_fwd(u64 &X, u64 &Y, u64 w, u64 ws, const Limb64 &L) {
-  const u64 x = csub(X, L.q2);
+  u64 x = X;
+  if constexpr (kKind != kFwN) x = csub(x, 4 * L.q2);
+  if constexpr (kKind == kFwD) x = csub(csub(x, 2 * L.q2), L.q2);
   const u64 t = shoup(Y, w, ws, L.q);
   X = x + t;
   Y = x - t + L.q2;
@@ -120,32 +131,35 @@ __device__ __forceinline__ void bf_inv(u64 &X, u64 &Y, u64 w, u64 ws, const Limb
   Y = shoup(x - y + L.q2, w, ws, L.q);
 }
 
-// Radix-8 group b: stages with twiddles psi[b], psi[2b + h], psi[4b + h].
+// Radix-8 group b: stages with twiddles psi[b], psi[2b + h], psi[4b + h];
+// K1, K2, K3: the butterfly kind of each stage.
+template <int K1, int K2, int K3>
 __device__ __forceinline__ void fwd8(u64 v[8], unsigned b, const Limb64 &L) {
   {
     const u64 w = L.psi[b], ws = L.psi_s[b];
 #pragma unroll
-    for (int c = 0; c < 4; c++) bf_fwd(v[c], v[c + 4], w, ws, L);
+    for (int c = 0; c < 4; c++) bf_fwd<K1>(v[c], v[c + 4], w, ws, L);
   }
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const u64 w = L.psi[2 * b + h], ws = L.psi_s[2 * b + h];
-    bf_fwd(v[4 * h + 0], v[4 * h + 2], w, ws, L);
-    bf_fwd(v[4 * h + 1], v[4 * h + 3], w, ws, L);
+    bf_fwd<K2>(v[4 * h + 0], v[4 * h + 2], w, ws, L);
+    bf_fwd<K2>(v[4 * h + 1], v[4 * h + 3], w, ws, L);
   }
 #pragma unroll
-  for (int h = 0; h < 4; h++) bf_fwd(v[2 * h], v[2 * h + 1], L.psi[4 * b + h], L.psi_s[4 * b + h], L);
+  for (int h = 0; h < 4; h++) bf_fwd<K3>(v[2 * h], v[2 * h + 1], L.psi[4 * b + h], L.psi_s[4 * b + h], L);
 }
 // The last two stages of fwd8 only: two radix-4 groups (N = 2048's round 3).
+template <int K2, int K3>
 __device__ __forceinline__ void fwd4x2(u64 v[8], unsigned b, const Limb64 &L) {
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const u64 w = L.psi[2 * b + h], ws = L.psi_s[2 * b + h];
-    bf_fwd(v[4 * h + 0], v[4 * h + 2], w, ws, L);
-    bf_fwd(v[4 * h + 1], v[4 * h + 3], w, ws, L);
+    bf_fwd<K2>(v[4 * h + 0], v[4 * h + 2], w, ws, L);
+    bf_fwd<K2>(v[4 * h + 1], v[4 * h + 3], w, ws, L);
   }
 #pragma unroll
-  for (int h = 0; h < 4; h++) bf_fwd(v[2 * h], v[2 * h + 1], L.psi[4 * b + h], L.psi_s[4 * b + h], L);
+  for (int h = 0; h < 4; h++) bf_fwd<K3>(v[2 * h], v[2 * h + 1], L.psi[4 * b + h], L.psi_s[4 * b + h], L);
 }
 
 __device__ __forceinline__ void inv4x2(u64 v[8], unsigned b, const Limb64 &L) {
@@ -230,21 +244,24 @@ __device__ __forceinline__ void exch_to0(u64 v[8], u64 *lds, int &t) {
 
 // map0 [0, q) in -> map3 out, [0, q) when reduce, else [0, 4q). Two
 // workgroup barriers, both before the first round's values cross waves; on
-// return no wave reads another wave's region of lds.
+// return no wave reads another wave's region of lds. Value bounds: six
+// stages without a subtraction (13q), then 10q, 12q, 14q | 10q, 12q, 4q at
+// N = 4096 (10q, 4q for the last two at N = 2048): five conditional
+// subtractions per value where one per stage takes 11-12.
 template <int N>
 __device__ __forceinline__ void ntt_fwd(u64 v[8], u64 *lds, int t, const Limb64 &L, bool reduce = true) {
   constexpr int LB = N / 8;
   asm volatile("" : "+v"(t));
-  fwd8(v, 1, L);
+  fwd8<kFwN, kFwN, kFwN>(v, 1, L);
   exch_from0<N>(v, lds, t);
-  fwd8(v, 8 + t / (LB / 8), L);
+  fwd8<kFwN, kFwN, kFwN>(v, 8 + t / (LB / 8), L);
   exch_wave<N>(v, lds, t, 1, 2);
-  fwd8(v, 64 + t / (LB / 64), L);
+  fwd8<kFwH, kFwN, kFwN>(v, 64 + t / (LB / 64), L);
   exch_wave<N>(v, lds, t, 2, 3);
   if constexpr (N == 4096)
-    fwd8(v, 512 + t, L);
+    fwd8<kFwH, kFwN, kFwD>(v, 512 + t, L);
   else
-    fwd4x2(v, 256 + t, L);
+    fwd4x2<kFwH, kFwD>(v, 256 + t, L);
   if (reduce) {
 #pragma unroll
     for (int c = 0; c < 8; c++) v[c] = csub(csub(v[c], L.q2), L.q);
