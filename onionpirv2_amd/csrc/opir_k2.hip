@@ -50,6 +50,11 @@ struct Cfg {
   static constexpr int WG = K * LB;              // threads per tree-node workgroup
   static constexpr size_t Ct = 2 * (size_t)K * N;  // u64 values per ciphertext
   static constexpr int Lds = N + N / 32;         // padded u64 words of one exchange buffer
+  // Waves per SIMD the tree kernels are compiled for. One prime: three
+  // 256-thread workgroups (168 registers, a few spilled), which ran the
+  // 64-query batch in 18.9 ms against 19.7-20.2 ms at two (its plain query
+  // unchanged). Two primes: one 1024-thread workgroup per CU, bound by LDS.
+  static constexpr int Waves = K == 1 ? 3 : 1;
 };
 typedef Cfg<4096, 2> CfgK2;
 typedef Cfg<2048, 1> CfgK1;
@@ -512,7 +517,7 @@ __device__ __forceinline__ int lane_limb() {
 // Kernels
 // ---------------------------------------------------------------------------
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_ntt(u64 *data, int inverse, Scheme64 S) {
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_ntt(u64 *data, int inverse, Scheme64 S) {
   __shared__ u64 lds[C::K][C::Lds];
   const int limb = lane_limb<C>(), t = threadIdx.x % C::LB;
   const Limb64 &L = S.limb[limb];
@@ -541,7 +546,7 @@ struct ExpandArgs64 {
 
 // Expansion level (server.cpp:786-810, bv_keyswitch.cpp:296-481).
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_expand_level(ExpandArgs64 A, Scheme64 S) {
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_expand_level(ExpandArgs64 A, Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u128 xs[K == 2 ? N : 1];  // K = 2: recombined coefficients being decomposed
   __shared__ u64 lds[K][C::Lds];
@@ -602,7 +607,7 @@ struct CompleteArgs64 {
 
 // Selector completion (server.cpp:823-855, gsw.cpp:317-357).
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_complete(CompleteArgs64 A, Scheme64 S) {
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_complete(CompleteArgs64 A, Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u128 xs[K == 2 ? N : 1];
   __shared__ u64 lds[K][C::Lds];
@@ -635,7 +640,7 @@ __global__ __launch_bounds__(C::WG) void k2_complete(CompleteArgs64 A, Scheme64 
 }
 
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_ct_ntt(const u64 *in, u64 *out, size_t in_stride, size_t out_stride,
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_ct_ntt(const u64 *in, u64 *out, size_t in_stride, size_t out_stride,
                                                    Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u64 lds[K][C::Lds];
@@ -659,7 +664,7 @@ struct MuxArgs64 {
 
 // MUX (server.cpp:599-722) on NTT-form candidates: res = x + EP(sel, INTT(y - x)).
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_mux(MuxArgs64 A, Scheme64 S) {
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_mux(MuxArgs64 A, Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u128 xs[K == 2 ? N : 1];
   __shared__ u64 lds[K][C::Lds];
@@ -716,7 +721,7 @@ __global__ __launch_bounds__(C::WG) void k2_mux(MuxArgs64 A, Scheme64 S) {
 
 // INTT (when ntt_in), then the mod switch per coefficient. grid (2 polys, B).
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_final(const u64 *res, u64 *out, size_t res_stride, int ntt_in,
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_final(const u64 *res, u64 *out, size_t res_stride, int ntt_in,
                                                   Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u64 lds[K][C::Lds];
@@ -744,7 +749,7 @@ __global__ __launch_bounds__(C::WG) void k2_final(const u64 *res, u64 *out, size
 }
 
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_db_load(const u64 *pts, u64 *db, size_t num_pt, size_t first,
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_db_load(const u64 *pts, u64 *db, size_t num_pt, size_t first,
                                                     Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u64 lds[K][C::Lds];
@@ -758,7 +763,7 @@ __global__ __launch_bounds__(C::WG) void k2_db_load(const u64 *pts, u64 *db, siz
 }
 
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_ext_prod(const u64 *gsw, int l, int b, const u64 *ct, u64 *out,
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_ext_prod(const u64 *gsw, int l, int b, const u64 *ct, u64 *out,
                                                      Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u128 xs[K == 2 ? N : 1];
@@ -803,7 +808,7 @@ __device__ __forceinline__ void mux_operands64(const MuxArgs64 &A, int m, int &x
 }
 
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_mux_part(MuxArgs64 A, u64 *__restrict__ part, int G, Scheme64 S) {
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_mux_part(MuxArgs64 A, u64 *__restrict__ part, int G, Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u128 xs[K == 2 ? N : 1];
   __shared__ u64 lds[K][C::Lds];
@@ -876,7 +881,7 @@ __global__ __launch_bounds__(256) void k2_mux_fin(MuxArgs64 A, const u64 *__rest
 }
 
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_expand_part(ExpandArgs64 A, u64 *__restrict__ part, int G, Scheme64 S) {
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_expand_part(ExpandArgs64 A, u64 *__restrict__ part, int G, Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u128 xs[K == 2 ? N : 1];
   __shared__ u64 lds[K][C::Lds];
@@ -906,7 +911,7 @@ __global__ __launch_bounds__(C::WG) void k2_expand_part(ExpandArgs64 A, u64 *__r
 // Workgroup (node, h), lane block = limb: delta_h = INTT(sum of the G
 // partials), then child polynomials h as in k2_expand_level.
 template <typename C>
-__global__ __launch_bounds__(C::WG) void k2_expand_fin(ExpandArgs64 A, const u64 *__restrict__ part, int G,
+__global__ __launch_bounds__(C::WG) __attribute__((amdgpu_waves_per_eu(C::Waves))) void k2_expand_fin(ExpandArgs64 A, const u64 *__restrict__ part, int G,
                                                        int nnodes, Scheme64 S) {
   constexpr int N = C::N, K = C::K;
   __shared__ u64 lds[K][C::Lds];
