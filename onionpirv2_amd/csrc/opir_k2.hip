@@ -50,8 +50,8 @@ struct Cfg {
   static constexpr int WG = K * LB;              // threads per tree-node workgroup
   static constexpr size_t Ct = 2 * (size_t)K * N;  // u64 values per ciphertext
   static constexpr int Lds = N + N / 32;         // padded u64 words of one exchange buffer
-  // Waves per SIMD the tree kernels are compiled for. One prime: three
-  // 256-thread workgroups (168 registers, a few spilled), which ran the
+  // Waves per SIMD the tree kernels are compiled for. One prime: three, i.e.
+  // three 256-thread workgroups per CU (168 registers, a few spilled): the
   // 64-query batch in 18.9 ms against 19.7-20.2 ms at two (its plain query
   // unchanged). Two primes: one 1024-thread workgroup per CU, bound by LDS.
   static constexpr int Waves = K == 1 ? 3 : 1;
