@@ -80,7 +80,6 @@ __device__ __forceinline__ u128 Q128(const Scheme64 &S) { return ((u128)S.Q_hi <
 __device__ __forceinline__ u128 hQ128(const Scheme64 &S) { return ((u128)S.hQ_hi << 64) | S.hQ_lo; }
 
 // Signed residue r (|r| < 2^63) mod q.
-__device__ __forceinline__ u64 sres(int64_t r, u64 q) { return r >= 0 ? (u64)r : q - (u64)(-r); }
 
 // floor(num / d) for num < 2^121 by Barrett: m = floor(2^k / d) in
 // [2^63, 2^64) (k = 63 + bit length of d). floor(num * m / 2^k), keeping only
@@ -247,11 +246,11 @@ __device__ __forceinline__ void exch_to0(u64 v[8], u64 *lds, int &t) {
   asm volatile("" : "+v"(t));
 }
 
-// map0 [0, q) in -> map3 out, [0, q) when reduce, else [0, 4q). Two
-// workgroup barriers, both before the first round's values cross waves; on
-// return no wave reads another wave's region of lds. Value bounds: six
-// stages without a subtraction (13q), then 10q, 12q, 14q | 10q, 12q, 4q at
-// N = 4096 (10q, 4q for the last two at N = 2048): five conditional
+// map0 values below 2q in -> map3 out, [0, q) when reduce, else [0, 4q).
+// Two workgroup barriers, both before the first round's values cross waves;
+// on return no wave reads another wave's region of lds. Value bounds: six
+// stages without a subtraction (below 14q), then 10q, 12q, 14q | 10q, 12q,
+// 4q at N = 4096 (10q, 4q for the last two at N = 2048): five conditional
 // subtractions per value where one per stage takes 11-12.
 template <int N>
 __device__ __forceinline__ void ntt_fwd(u64 v[8], u64 *lds, int t, const Limb64 &L, bool reduce = true) {
@@ -387,11 +386,12 @@ struct Coef<C, 1> {
 };
 
 // Balanced digit i (base 2^b) of centred D with the running offset H = H_i,
-// as a residue mod q (K = 1 gadgets, gsw.cpp:285-293, bv_keyswitch.cpp:32-39).
+// as r + q, a representative mod q (K = 1 gadgets, gsw.cpp:285-293,
+// bv_keyswitch.cpp:32-39).
 __device__ __forceinline__ u64 bal_digit(int64_t D, int64_t H, int i, int b, u64 q) {
   const int64_t x = (D + H) >> (b * i);
   const int64_t r = (int64_t)((u64)x << (64 - b)) >> (64 - b);
-  return r < 0 ? (u64)(r + (int64_t)q) : (u64)r;
+  return (u64)(r + (int64_t)q);  // in (0, 2q): the forward transform takes inputs below 2q
 }
 // H_i = sum_{j=1..i} 2^(b*j - 1)
 __device__ __forceinline__ int64_t bal_offset(int i, int b) {
@@ -452,7 +452,7 @@ __device__ __forceinline__ void bv_mac(const Coef<C> &X, const u64 *key, int d0,
       for (int cc = 0; cc < 8; cc++) {
         const i128 di = ((i128)X.xs[map0<N>(t, cc)] + H) >> (b * i);
         const int64_t r = (int64_t)(((u64)(di + h)) & (((u64)1 << b) - 1)) - (int64_t)h;
-        d[cc] = sres(r, L.q);
+        d[cc] = (u64)(r + (int64_t)L.q);  // in (0, 2q), taken as is by the forward transform
       }
       H += h << (b * i);
       ntt_fwd<N>(d, lds, t, L, false);

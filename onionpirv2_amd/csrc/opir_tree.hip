@@ -42,9 +42,11 @@ __device__ __forceinline__ i64 digit_offset(int i, int b, i64 delta) {
   for (int j = 0; j < i; j++) H += (((i64)1 << (b - 1)) - delta) << (b * j);
   return H;
 }
+// The digit r (|r| < q for every gadget base) as r + q in (0, 2q): the
+// forward transform it goes into takes inputs below 2q.
 __device__ __forceinline__ u32 digit_residue(i64 D, i64 H, int shift, int b, u32 q, int delta) {
   const int r = __builtin_amdgcn_sbfe((int)(u32)((D + H) >> shift) - delta, 0, b) + delta;
-  return (u32)r + (q & (u32)(r >> 31));
+  return (u32)r + q;
 }
 
 // Per-gadget digit rule: the composite scheme centres and takes balanced
