@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 HBM_PEAK_GBS = 8000.0
+INT8_MFMA_PEAK_TOPS = 5000.0  # dense int8 MFMA, MI355X
 
 
 def parse():
@@ -111,6 +112,35 @@ def tuned_shape(G, db_h, b):
     out = ctypes.c_int(-1)
     G.ctx_call_raw("opir_db_tuned_scan_variant", db_h, ctypes.c_size_t(b), ctypes.byref(out))
     return out.value
+
+
+def batched_scan_leg(G, ctx_h, answer, B, params, log):
+    """The first-dimension scan of a B-query batch on its own: HIP-event time
+    of the scan launch (profiling slot 5), against both bounds it has at this
+    size: HBM bytes (database + query columns + results) and the int8 MFMA
+    work of the digit-plane products (16 x 16 x 64 tiles over the padded
+    rows, columns and k)."""
+    import ctypes
+    G.ctx_call_raw("opir_ctx_set_profiling", ctx_h, 1)
+    ms = []
+    for _ in range(3):
+        answer(B)
+        out = (ctypes.c_float * 6)()
+        assert G.ctx_call_raw("opir_last_stage_ms", ctx_h, out) == 0
+        ms.append(out[5])
+    G.ctx_call_raw("opir_ctx_set_profiling", ctx_h, 0)
+    scan = sorted(ms)[1]
+    N, N0, Nr = params.N, params.fst_dim_sz, params.other_dim_sz
+    nbytes = params.num_pt * N * 8 + B * 2 * N0 * N * 8 + B * Nr * 2 * N * 8
+    rows, cols, ks = -(-Nr // 16) * 16, -(-2 * B // 16) * 16, -(-N0 // 64) * 64
+    mfma_ops = 2 * rows * cols * ks * 16 * 2 * N  # 4 x 4 digit planes, 2 limbs x N levels
+    r = {"kernel_ms": round(scan, 4), "algorithmic_bytes": nbytes,
+         "achieved_gbs": round(nbytes / scan / 1e6, 1), "frac_of_hbm_peak": round(nbytes / scan / 1e6 / HBM_PEAK_GBS, 4),
+         "int8_mfma_ops": mfma_ops, "achieved_tops": round(mfma_ops / scan / 1e9, 1),
+         "frac_of_int8_mfma_peak": round(mfma_ops / scan / 1e9 / INT8_MFMA_PEAK_TOPS, 4)}
+    log(f"[batch] scan {scan:.3f} ms: {r['achieved_gbs']:.0f} GB/s ({r['frac_of_hbm_peak']:.2f} of HBM peak), "
+        f"{r['achieved_tops']:.0f} int8 TOP/s ({r['frac_of_int8_mfma_peak']:.2f} of the dense MFMA peak)")
+    return r
 
 
 def pct(xs, p):
@@ -806,6 +836,7 @@ def main():
                              "latency_ms": {"p50": round(pct(per_b, 50), 3), "p99": round(pct(per_b, 99), 3)},
                              "scan_shape": tuned_shape(G, db_h, B)}
         log(f"[batch] B={B}: {ms_b:.2f} ms/batch -> {B * 1e3 / ms_b:.1f} q/s")
+        result["batched"]["scan"] = batched_scan_leg(G, ctx_h, answer, B, params, log)
         # batched answers must equal the plain ones
         answer(B)
         torch.cuda.synchronize()

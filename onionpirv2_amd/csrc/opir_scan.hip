@@ -37,7 +37,11 @@
 //  - register-blocked direct (large batches): each wave multiplies R row
 //    tiles by C column tiles from its own registers, R * C * P^2 MFMAs per
 //    R * P + C * P fragments loaded.
-// The shape is picked per batch size from a table (launch_scan).
+// Beside them, for the 29-bit limbs, a VALU form (k_scan_valu: the
+// reference's u32 x u32 -> u64 multiply-accumulate on the same layouts), so
+// that the autotuner measures where the matrix cores pay off.
+// The shape is picked per batch size from a table (launch_scan), or timed on
+// the live operands in autotune mode.
 #include <type_traits>
 
 #include "opir_k2.h"
@@ -799,6 +803,169 @@ __global__ __launch_bounds__(256) void k_scan_rb(ScanArgs<Epi> A, SchemeT S) {
   }
 }
 
+// Residues of 4 consecutive k from one 32-bit word of each digit plane
+// (P = 4): y = a + 0x808080 has the base-256 digits d_s + 128 = byte ^ 0x80
+// (s < 3) and the top digit d_3 (balanced_digit's identity), so the bytes of
+// the four words regroup into y with one flip per word.
+__device__ __forceinline__ void residues4(u32 w0, u32 w1, u32 w2, u32 w3, u32 (&a)[4]) {
+  w0 ^= 0x80808080u;
+  w1 ^= 0x80808080u;
+  w2 ^= 0x80808080u;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const u32 y = ((w0 >> (8 * j)) & 255u) | (((w1 >> (8 * j)) & 255u) << 8) | (((w2 >> (8 * j)) & 255u) << 16) |
+                  (((w3 >> (8 * j)) & 255u) << 24);
+    a[j] = y - 0x808080u;
+  }
+}
+
+// VALU form of the first-dimension scan for the 29-bit limbs: the
+// reference's u32 x u32 -> u64 multiply-accumulate (matrix.cpp:24-151), one
+// thread per candidate row, on the same database and query-fragment layouts
+// as the MFMA shapes (residues rebuilt from their digit planes). The
+// autotuner times it beside the MFMA shapes, so the matrix cores are used
+// only for the batch sizes where they are faster.
+// Workgroup: 256 rows x C query columns x kLevelsPerWG levels of one limb.
+// Per level the C query columns are rebuilt into LDS (Q[k][c], read as
+// broadcasts); each thread accumulates its row's C sums in u64, folding the
+// high half back every 8 products, and keeps the C x 16 results in registers
+// until they are written as 64-byte level runs.
+// Bounds: residues a, b < 2^30, so each product is < 2^60; a fold leaves
+// (x >> 32) * (2^32 mod q) + (x mod 2^32) < 2^61 + 2^32, and 8 more products
+// keep the sum below 2^64. Columns past the batch's 2B read as zero and are
+// not stored; rows past Nrest re-read the last row and are not stored.
+template <int C>
+__global__ __launch_bounds__(256) void k_scan_valu(ScanArgs<EpiK1> A, SchemeConst S) {
+  constexpr int P = 4, LW = kLevelsPerWG;
+  static_assert(C == 2 || C % 4 == 0, "query columns read as uint2 or uint4");
+  static_assert(16 % C == 0, "a column block lies inside one column tile");
+  extern __shared__ u32 qs[];  // [nkb * 64][C]
+  const DbGeom &g = A.g;
+  const QfragGeom &qg = A.qg;
+  const unsigned W = gridDim.x, pblk = blockIdx.x;
+  const unsigned Lg = (W % 8 == 0) ? (pblk % 8) * (W / 8) + pblk / 8 : pblk;
+  const int cb = Lg % A.ncb;
+  unsigned rest = Lg / A.ncb;
+  const int rb = rest % A.nrb;
+  rest /= A.nrb;
+  const int nlvb = g.nlv / LW;
+  const int lvb = rest % nlvb;
+  const int limb = rest / nlvb;
+  const LimbConst &L = S.limb[limb];
+  const int t = threadIdx.x;
+  const size_t row = (size_t)rb * 256 + t;
+  const size_t rowa = row < g.nrt * 16 ? row : g.nrt * 16 - 1;
+  const size_t rt = rowa >> 4;
+  const int lr = (int)(rowa & 15);
+  const int c0 = cb * C, ct = c0 >> 4, n0 = c0 & 15;
+  const int nkb = (int)g.nkb;
+  const size_t qtile = (size_t)(P * 4 * qg.ncm * 16);
+  const size_t astride = (size_t)g.nrt * g.nkb * (P * 1024);  // bytes between levels
+  const int8_t *abase = A.db + (((size_t)limb * g.nlv + (size_t)lvb * LW) * g.nrt + rt) * g.nkb * (size_t)(P * 1024);
+  u32 res[LW][C];
+  // unrolled so that res is indexed statically and stays in registers (each
+  // level's body still loops over its k chunks)
+#pragma unroll
+  for (int li = 0; li < LW; li++) {
+    const int lv = lvb * LW + li;
+    __syncthreads();  // the previous level's reads of qs are done
+    // rebuild Q[k][c] of this level: unit (kb, grp, cc) = 16 consecutive k of one column
+    const int8_t *qb = A.qfrag + (((size_t)limb * g.nlv + lv) * qg.nct + ct) * qg.nkb * qtile;
+    for (int u = t; u < nkb * 4 * C; u += 256) {
+      const int cc = u % C, grp = (u / C) & 3, kb = u / (4 * C);
+      const int n = n0 + cc;
+      u32 v[16];
+      if (n < qg.ncm) {
+        const int8_t *p = qb + (size_t)kb * qtile + ((size_t)grp * qg.ncm + n) * 16;
+        uint4 d[P];
+#pragma unroll
+        for (int s = 0; s < P; s++) d[s] = *reinterpret_cast<const uint4 *>(p + (size_t)s * 4 * qg.ncm * 16);
+        const u32 w[P][4] = {{d[0].x, d[0].y, d[0].z, d[0].w}, {d[1].x, d[1].y, d[1].z, d[1].w},
+                             {d[2].x, d[2].y, d[2].z, d[2].w}, {d[3].x, d[3].y, d[3].z, d[3].w}};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          u32 a4[4];
+          residues4(w[0][i], w[1][i], w[2][i], w[3][i], a4);
+#pragma unroll
+          for (int j = 0; j < 4; j++) v[4 * i + j] = a4[j];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = 0;
+      }
+#pragma unroll
+      for (int j = 0; j < 16; j++) qs[(kb * 64 + grp * 16 + j) * C + cc] = v[j];
+    }
+    __syncthreads();
+    u64 acc[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) acc[c] = 0;
+    const int8_t *pa = abase + (size_t)li * astride + (size_t)lr * 16;
+    // 16-k chunks (k block, group of 16): the next chunk's four plane words
+    // are loaded while the current one multiplies
+    const int nch = nkb * 4;
+    auto ldchunk = [&](int ch, v4i(&d)[P]) {
+      const int8_t *p = pa + (size_t)(ch >> 2) * (P * 1024) + (ch & 3) * 256;
+#pragma unroll
+      for (int s = 0; s < P; s++) d[s] = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(p + s * 1024));
+    };
+    v4i dc[P];
+    ldchunk(0, dc);
+#pragma unroll 1
+    for (int ch = 0; ch < nch; ch++) {
+      v4i dn[P];
+      ldchunk(ch + 1 < nch ? ch + 1 : ch, dn);
+      const u32 *qk0 = qs + ch * 16 * C;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        // keeps the query-column reads of later steps from being hoisted
+        // (they would hold 16 x C registers)
+        asm volatile("" ::: "memory");
+        u32 a4[4];
+        residues4((u32)dc[0][i], (u32)dc[1][i], (u32)dc[2][i], (u32)dc[3][i], a4);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          u32 qv[C];
+          if constexpr (C % 4 == 0) {
+#pragma unroll
+            for (int h = 0; h < C / 4; h++) {
+              const uint4 x = reinterpret_cast<const uint4 *>(qk0 + (4 * i + j) * C)[h];
+              qv[4 * h] = x.x; qv[4 * h + 1] = x.y; qv[4 * h + 2] = x.z; qv[4 * h + 3] = x.w;
+            }
+          } else {
+            const uint2 x = *reinterpret_cast<const uint2 *>(qk0 + (4 * i + j) * C);
+            qv[0] = x.x; qv[1] = x.y;
+          }
+#pragma unroll
+          for (int c = 0; c < C; c++) acc[c] += (u64)a4[j] * qv[c];
+        }
+        if (i & 1) {  // 8 products since the last fold
+#pragma unroll
+          for (int c = 0; c < C; c++) acc[c] = (acc[c] >> 32) * L.r32 + (u32)acc[c];
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < P; s++) dc[s] = dn[s];
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) res[li][c] = reduce_u64(acc[c], L);
+  }
+  if (row >= g.Nrest) return;
+  const int ncols = min(C, 2 * A.B - c0);
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    if (c >= ncols) break;
+    const int col = c0 + c, qi = col >> 1, pp = col & 1;
+    v4i *dst = reinterpret_cast<v4i *>(A.mid + qi * A.mid_stride + row * A.ct_vals + (size_t)pp * (A.ct_vals >> 1) +
+                                       (size_t)limb * g.nlv + (size_t)lvb * LW);
+#pragma unroll
+    for (int h = 0; h < LW / 4; h++) {
+      const v4i x = {(int)res[4 * h][c], (int)res[4 * h + 1][c], (int)res[4 * h + 2][c], (int)res[4 * h + 3][c]};
+      __builtin_nontemporal_store(x, dst + h);
+    }
+  }
+}
+
 }  // namespace
 
 static inline unsigned cdivu(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
@@ -871,6 +1038,7 @@ struct ScanShape {
   bool direct;
   int CB, KC, LV, SC;
   bool xpf;
+  bool valu = false;  // k_scan_valu with CB query columns per workgroup
 };
 
 template <typename Epi>
@@ -921,6 +1089,19 @@ static void run_pp(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid
     k_scan_pp<KC, LV, SC, false, Epi, SchemeT, NTS, LW><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
 }
 
+// VALU scan: 256-row blocks x C-column blocks (column blocks innermost, so
+// the workgroups re-reading a row block's database bytes share an XCD's L2).
+template <int C>
+static void run_valu(const int8_t *db, const int8_t *qfrag, u32 *mid, size_t mid_stride, int B, const DbGeom &g,
+                     const SchemeConst &S, hipStream_t st) {
+  ScanArgs<EpiK1> A = scan_args<EpiK1>(db, qfrag, mid, mid_stride, B, g, 1);
+  A.nrb = (int)cdivu(g.Nrest, 256);
+  A.ncb = (int)cdivu(2 * (size_t)B, C);
+  const unsigned grid = (unsigned)((size_t)g.nl * (g.nlv / kLevelsPerWG) * A.nrb * A.ncb);
+  const size_t lds = g.nkb * 64 * C * sizeof(u32);
+  k_scan_valu<C><<<grid, 256, lds, st>>>(A, S);
+}
+
 // NT A loads only where each A fragment has one reader (one column block,
 // one wave per row block).
 template <int R, int C, int WC, int KC, int LV, int LW, typename Epi, typename SchemeT, bool NTS>
@@ -957,6 +1138,9 @@ static const ScanEntry<EpiK1, SchemeConst> kScanK1[] = {
     {{false, 2, 1, 2, 32, false}, run_lds<2, 2, EpiK1, SchemeConst>},
     {{true, 1, 2, 16, 2, true}, run_pp<2, 16, 2, EpiK1, SchemeConst>},
     {{true, 4, 4, 16, 16, true}, run_rb<1, 1, 4, 4, 16, 32, EpiK1, SchemeConst, false>},
+    {{false, 2, 0, 16, 0, false, true}, run_valu<2>},
+    {{false, 4, 0, 16, 0, false, true}, run_valu<4>},
+    {{false, 8, 0, 16, 0, false, true}, run_valu<8>},
 };
 
 static const ScanEntry<EpiK2, S64> kScanK2[] = {
@@ -975,6 +1159,7 @@ static const ScanEntry<EpiK2, S64> kScanK2[] = {
 int scan_variant_count(int P) { return P == 4 ? (int)std::size(kScanK1) : (int)std::size(kScanK2); }
 
 static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
+  if (s.valu) return g.nkb * s.CB <= 256;  // LDS of the rebuilt query columns <= 64 KiB
   if (!s.direct) return true;
   return g.nkb % s.KC == 0 && (2 * B < 16 ? 2 * B : 16) <= s.SC;
 }
