@@ -185,7 +185,11 @@ int opir_answer(opir_db *db, uint64_t client_id, const uint64_t *query, uint64_t
 int opir_answer_batch(opir_db *db, size_t B, const uint64_t *client_ids, const uint64_t *queries,
                       uint64_t *resps);
 /* Same as opir_answer_batch with queries/resps in device memory; work is
- * queued on the context stream and this call returns without waiting. */
+ * queued on the context stream and this call returns without waiting.
+ * Client ids and keys are checked as in opir_answer_batch; the query values
+ * are not (they are never read on the host): the caller guarantees every
+ * value is reduced mod Q (per limb for the two-prime schemes). An unreduced
+ * value yields a wrong response, never an access outside the buffers. */
 int opir_answer_batch_device(opir_db *db, size_t B, const uint64_t *client_ids, const uint64_t *d_queries,
                              uint64_t *d_resps);
 
