@@ -1159,7 +1159,10 @@ static const ScanEntry<EpiK2, S64> kScanK2[] = {
 int scan_variant_count(int P) { return P == 4 ? (int)std::size(kScanK1) : (int)std::size(kScanK2); }
 
 static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
-  if (s.valu) return g.nkb * s.CB <= 256;  // LDS of the rebuilt query columns <= 64 KiB
+  // VALU: LDS of the rebuilt query columns <= 64 KiB, and at most 16 column
+  // blocks re-reading each row block (more only lengthen autotuning: 18 and
+  // 33 ms at 32 and 64 blocks against 11 ms at 16 for 64 queries)
+  if (s.valu) return g.nkb * s.CB <= 256 && (2 * B + s.CB - 1) / s.CB <= 16;
   if (!s.direct) return true;
   return g.nkb % s.KC == 0 && (2 * B < 16 ? 2 * B : 16) <= s.SC;
 }
