@@ -1141,8 +1141,10 @@ static const ScanEntry<EpiK1, SchemeConst> kScanK1[] = {
     {{false, 2, 0, 16, 0, false, true}, run_valu<2>},
     {{false, 4, 0, 16, 0, false, true}, run_valu<4>},
     {{false, 8, 0, 16, 0, false, true}, run_valu<8>},
-    {{true, 1, 4, 16, 2, true}, run_direct<4, 16, 2, true, EpiK1, SchemeConst, true, 32>},
+    // one query: more levels per workgroup, staged at once (longer result runs)
     {{true, 1, 4, 32, 2, true}, run_direct<4, 32, 2, true, EpiK1, SchemeConst, true, 32>},
+    {{true, 1, 4, 64, 2, true}, run_direct<4, 64, 2, true, EpiK1, SchemeConst, true, 64>},
+    {{true, 1, 4, 64, 2, true}, run_direct<4, 64, 2, true, EpiK1, SchemeConst, true, 128>},
 };
 
 static const ScanEntry<EpiK2, S64> kScanK2[] = {
@@ -1173,10 +1175,15 @@ static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
 // preference, the fastest measured on MI355X at the 256 x 1024 and
 // 512 x 4096 layouts (B = 1, 8 and 64 / 32). At 64 queries the
 // register-blocked shape (one row tile x four column tiles per workgroup)
-// takes 3.37 ms against the direct shape's 3.70. For one query, 32 levels per
-// workgroup staged at once (128-byte result runs) take 0.700-0.703 ms against
-// 0.706-0.708 for 16 (and 0.721-0.723 for 32 levels staged 16 at a time).
-static const int kPrefK1[3][4] = {{14, 0, 3, 5}, {1, 3, 5, -1}, {9, 1, 6, 5}};
+// takes 3.37 ms against the direct shape's 3.70. For one query the direct
+// shape gains from more levels per workgroup, staged at once: 16 / 32 / 64
+// levels (64- / 128- / 256-byte result runs) take 0.706 / 0.700 / 0.678 ms,
+// and 128 levels per workgroup staged 64 at a time (512 workgroups, two per
+// CU, one round) 0.671 ms. Staging fewer levels than a workgroup takes
+// (16 of 32: 0.722 ms; 32 of 128: 0.690 ms) loses. The two-prime plain scan
+// (u64 results, 128-byte runs at 16 levels) gains nothing from 32 of 64 or
+// 128 (21.55-21.96 against 21.55 ms).
+static const int kPrefK1[3][4] = {{15, 14, 13, 0}, {1, 3, 5, -1}, {9, 1, 6, 5}};
 static const int kPrefK2[3][4] = {{8, 1, 3, 4}, {3, 4, -1, -1}, {3, 4, 5, -1}};
 
 template <typename Epi, typename SchemeT, size_t NV>
