@@ -1183,7 +1183,7 @@ static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
 // (16 of 32: 0.722 ms; 32 of 128: 0.690 ms) loses. The two-prime plain scan
 // (u64 results, 128-byte runs at 16 levels) gains nothing from 32 of 64 or
 // 128 (21.55-21.96 against 21.55 ms).
-static const int kPrefK1[3][4] = {{15, 14, 13, 0}, {1, 3, 5, -1}, {9, 1, 6, 5}};
+static const int kPrefK1[3][4] = {{15, 14, 3, 5}, {1, 3, 5, -1}, {9, 1, 6, 5}};
 static const int kPrefK2[3][4] = {{8, 1, 3, 4}, {3, 4, -1, -1}, {3, 4, 5, -1}};
 
 template <typename Epi, typename SchemeT, size_t NV>
