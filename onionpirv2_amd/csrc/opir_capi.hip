@@ -173,7 +173,7 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
     // selector completion on the auxiliary stream, concurrent with the scan
     // (two of its workgroups share a CU)
     const size_t nsel = db->dims - 1;
-    const bool beside = complete_beside_scan(ctx, nsel * ctx->G.l_ep, B, false);
+    const bool beside = complete_beside_scan(nsel * ctx->G.l_ep, B, false, 1024);
     if (beside) fork_aux(ctx);
     const size_t sel_stride = nsel * 2 * (size_t)ctx->G.l_ep * kCtWords;
     launch_complete(leaves + db->N0 * kCtWords, db->sel, exp_stride, sel_stride, (int)(nsel * ctx->G.l_ep), (int)B, P,

@@ -186,14 +186,25 @@ inline int ctx_cus(const opir_ctx *ctx) {
 }
 
 // Whether the selector completion runs beside the first-dimension scan on
-// the auxiliary stream, or ahead of it on the answer stream. Beside the scan
-// it fills CUs the scan leaves idle. A completion whose workgroups each hold
-// most of a CU's LDS (whole_cu) and whose grid has a workgroup per CU or
-// more waits for whole CUs that the scan's workgroups keep taking. It then
-// runs ahead of the scan, which costs the batch nothing and keeps its own
-// time at what it does alone.
-inline bool complete_beside_scan(const opir_ctx *ctx, size_t rows, size_t B, bool whole_cu) {
-  return !whole_cu || rows * B < (size_t)ctx_cus(ctx);
+// the auxiliary stream, or ahead of it on the answer stream. Measured on
+// MI355X (256 x 1024 layout): for one query the completion alone takes
+// 0.048 ms ahead of the scan, and the scan then takes 0.641-0.645 ms; beside
+// it the completion's workgroups hold CUs that the scan's one round of
+// workgroups needs, and the scan takes 0.669-0.672 ms, so the answer is
+// 1.177-1.179 ms ahead against 1.195-1.197 ms beside. At 8 queries ahead is
+// also faster (2.63 against 2.68 ms); from about 32 queries on the 32-bit
+// completion (two 512-thread workgroups per CU) beside the scan ties or wins
+// (64 queries: 13.36 ms either way; earlier builds 14.60 beside against
+// 14.70 ahead). The 64-bit limbs' completion (whole_cu: each workgroup holds
+// most of a CU's LDS) runs ahead at every batch size: beside the scan it
+// waits for whole CUs that the scan's workgroups keep taking (27.02 against
+// 27.18 ms for one n = 4096 query; 75 ms stretched beside a 32-query scan).
+// The single-prime 64-bit completion (256-thread workgroups) takes 0.099 ms
+// alone for one query, and ahead of the scan the answer took 1.540 against
+// 1.503 ms beside it, so it stays beside (ahead_below = 0).
+// ahead_below: row-queries (rows x B) below which the completion runs ahead.
+inline bool complete_beside_scan(size_t rows, size_t B, bool whole_cu, size_t ahead_below) {
+  return !whole_cu && rows * B >= ahead_below;
 }
 
 inline void fork_aux(opir_ctx *ctx) {

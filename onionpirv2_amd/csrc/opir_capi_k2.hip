@@ -336,7 +336,7 @@ int k2_answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q,
     // of it (complete_beside_scan); at N = 4096 with two limbs its
     // 1024-thread workgroups hold 130 KiB of LDS, one per CU
     const size_t nsel = db->dims - 1;
-    const bool beside = complete_beside_scan(ctx, nsel * ctx->G64.l_ep, B, ctx->K == 2);
+    const bool beside = complete_beside_scan(nsel * ctx->G64.l_ep, B, ctx->K == 2, 0);
     if (beside) fork_aux(ctx);
     const size_t sel_stride = nsel * 2 * (size_t)ctx->G64.l_ep * ctv(ctx);
     k2::launch_k2_complete(leaves + db->N0 * ctv(ctx), db->sel64, exp_stride, sel_stride, (int)(nsel * ctx->G64.l_ep),

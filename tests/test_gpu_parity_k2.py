@@ -254,10 +254,10 @@ def test_k2_graph_replays_match_eager(O, k2p, ctx2):
 
 
 def test_k2_completion_ahead_of_scan_matches(O, k2p, ctx2):
-    """On 8 compute units a 4-query batch has 60 completion workgroups, at
-    least one per CU, so the N = 4096 two-prime completion runs ahead of the
-    scan on the answer stream instead of beside it. The answers, eager and
-    replayed as graphs, equal those of the unrestricted context."""
+    """The N = 4096 two-prime completion (one 1024-thread workgroup per CU)
+    runs ahead of the scan on the answer stream. On 8 compute units, eager
+    and replayed as graphs, a 4-query batch (60 completion workgroups) and a
+    single query answer as on the unrestricted context."""
     say("[gpu] completion ahead of the scan (8-CU mask)")
     s = Setup(O, k2p, (1024, 8, True), seed=95, ctx=ctx2, cid=37)
     other = O.PirClient(s.p, 38, 96)
