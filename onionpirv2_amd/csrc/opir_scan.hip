@@ -1069,14 +1069,20 @@ static void run_lds(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mi
     k_scan<CB, LV, false, Epi, SchemeT, LW, NTS><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
 }
 
-template <int KC, int LV, int SC, bool XPF, typename Epi, typename SchemeT, bool NTS = true, int LW = kLevelsPerWG>
+// PAD: KiB of unused dynamic LDS per workgroup, to cap the workgroups a CU
+// may hold (the staged results plus PAD above a third of the CU's 160 KiB
+// leave room for two), so a grid of two workgroups per CU cannot land three
+// on some CUs and one on others.
+template <int KC, int LV, int SC, bool XPF, typename Epi, typename SchemeT, bool NTS = true, int LW = kLevelsPerWG,
+          int PAD = 0>
 static void run_direct(const int8_t *db, const int8_t *qfrag, typename Epi::Res *mid, size_t mid_stride, int B,
                        const DbGeom &g, const SchemeT &S, hipStream_t st) {
   const ScanArgs<Epi> A = scan_args<Epi>(db, qfrag, mid, mid_stride, B, g, 1);
+  const size_t lds = (size_t)PAD * 1024;
   if (A.ncb == 1)
-    k_scan_direct<KC, LV, SC, true, XPF, Epi, SchemeT, NTS, LW><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
+    k_scan_direct<KC, LV, SC, true, XPF, Epi, SchemeT, NTS, LW><<<scan_grid(A, LW), 256, lds, st>>>(A, S);
   else
-    k_scan_direct<KC, LV, SC, false, XPF, Epi, SchemeT, NTS, LW><<<scan_grid(A, LW), 256, 0, st>>>(A, S);
+    k_scan_direct<KC, LV, SC, false, XPF, Epi, SchemeT, NTS, LW><<<scan_grid(A, LW), 256, lds, st>>>(A, S);
 }
 
 template <int KC, int LV, int SC, typename Epi, typename SchemeT, bool NTS = true, int LW = kLevelsPerWG>
@@ -1144,7 +1150,9 @@ static const ScanEntry<EpiK1, SchemeConst> kScanK1[] = {
     // one query: more levels per workgroup, staged at once (longer result runs)
     {{true, 1, 4, 32, 2, true}, run_direct<4, 32, 2, true, EpiK1, SchemeConst, true, 32>},
     {{true, 1, 4, 64, 2, true}, run_direct<4, 64, 2, true, EpiK1, SchemeConst, true, 64>},
-    {{true, 1, 4, 64, 2, true}, run_direct<4, 64, 2, true, EpiK1, SchemeConst, true, 128>},
+    // 512 workgroups for the 256 x 1024 layout, two per CU in one round (48 KiB
+    // of staged results + 8 KiB of padding: a CU cannot take a third)
+    {{true, 1, 4, 64, 2, true}, run_direct<4, 64, 2, true, EpiK1, SchemeConst, true, 128, 8>},
 };
 
 static const ScanEntry<EpiK2, S64> kScanK2[] = {
