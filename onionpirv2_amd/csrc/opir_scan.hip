@@ -1155,8 +1155,6 @@ static const ScanEntry<EpiK1, SchemeConst> kScanK1[] = {
     {{true, 1, 4, 64, 2, true}, run_direct<4, 64, 2, true, EpiK1, SchemeConst, true, 128, 8>},
     {{true, 1, 4, 16, 16, true}, run_direct<4, 16, 16, true, EpiK1, SchemeConst, true, 32>},
     {{true, 1, 4, 16, 16, true}, run_direct<4, 16, 16, true, EpiK1, SchemeConst, true, 64>},
-    {{true, 4, 4, 16, 16, true}, run_rb<1, 1, 4, 4, 16, 64, EpiK1, SchemeConst, false>},
-    {{true, 4, 4, 16, 16, true}, run_rb<1, 1, 4, 4, 16, 128, EpiK1, SchemeConst, false>},
 };
 
 static const ScanEntry<EpiK2, S64> kScanK2[] = {
@@ -1194,11 +1192,13 @@ static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
 // CU, one round) 0.671 ms. Staging fewer levels than a workgroup takes
 // (16 of 32: 0.722 ms; 32 of 128: 0.690 ms) loses. The two-prime plain scan
 // (u64 results, 128-byte runs at 16 levels) gains nothing from 32 of 64 or
-// 128 (21.55-21.96 against 21.55 ms). Longer runs per workgroup give the
-// batched shapes 1-2%: 8 queries 0.993 against 1.009 ms (32 levels per
-// workgroup), 64 queries 3.50 against 3.57 ms (register-blocked, 128 levels
-// per workgroup staged 16 at a time), in one autotuning pass.
-static const int kPrefK1[3][4] = {{15, 14, 3, 5}, {16, 1, 3, 5}, {19, 9, 1, 6}};
+// 128 (21.55-21.96 against 21.55 ms). For 8 queries 32 levels per
+// workgroup take 0.993 against 1.009 ms. For 64 queries the register-blocked
+// shape with 128 levels per workgroup timed 3.50 against 3.57 ms alone, but
+// inside the batch, where the selector completion runs beside the scan, it
+// took 3.50 against 3.35 ms: with fewer workgroups the scan cannot balance
+// around the CUs the completion holds, so only the 32-level one is kept.
+static const int kPrefK1[3][4] = {{15, 14, 3, 5}, {16, 1, 3, 5}, {9, 1, 6, 5}};
 static const int kPrefK2[3][4] = {{8, 1, 3, 4}, {3, 4, -1, -1}, {3, 4, 5, -1}};
 
 template <typename Epi, typename SchemeT, size_t NV>
