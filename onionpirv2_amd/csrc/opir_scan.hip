@@ -1039,6 +1039,10 @@ struct ScanShape {
   int CB, KC, LV, SC;
   bool xpf;
   bool valu = false;  // k_scan_valu with CB query columns per workgroup
+  // Largest batch offered (0: any). Shapes with few workgroups time well
+  // alone but lose inside large batches, whose selector completion runs
+  // beside the scan and holds CUs they would need.
+  int max_b = 0;
 };
 
 template <typename Epi>
@@ -1153,8 +1157,8 @@ static const ScanEntry<EpiK1, SchemeConst> kScanK1[] = {
     // 512 workgroups for the 256 x 1024 layout, two per CU in one round (48 KiB
     // of staged results + 8 KiB of padding: a CU cannot take a third)
     {{true, 1, 4, 64, 2, true}, run_direct<4, 64, 2, true, EpiK1, SchemeConst, true, 128, 8>},
-    {{true, 1, 4, 16, 16, true}, run_direct<4, 16, 16, true, EpiK1, SchemeConst, true, 32>},
-    {{true, 1, 4, 16, 16, true}, run_direct<4, 16, 16, true, EpiK1, SchemeConst, true, 64>},
+    {{true, 1, 4, 16, 16, true, false, 8}, run_direct<4, 16, 16, true, EpiK1, SchemeConst, true, 32>},
+    {{true, 1, 4, 16, 16, true, false, 8}, run_direct<4, 16, 16, true, EpiK1, SchemeConst, true, 64>},
 };
 
 static const ScanEntry<EpiK2, S64> kScanK2[] = {
@@ -1173,6 +1177,7 @@ static const ScanEntry<EpiK2, S64> kScanK2[] = {
 int scan_variant_count(int P) { return P == 4 ? (int)std::size(kScanK1) : (int)std::size(kScanK2); }
 
 static bool applicable(const ScanShape &s, int B, const DbGeom &g) {
+  if (s.max_b > 0 && B > s.max_b) return false;
   // VALU: LDS of the rebuilt query columns <= 64 KiB, and at most 16 column
   // blocks re-reading each row block (more only lengthen autotuning: 18 and
   // 33 ms at 32 and 64 blocks against 11 ms at 16 for 64 queries)
