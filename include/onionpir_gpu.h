@@ -233,7 +233,9 @@ int opir_last_stage_ms(opir_ctx *ctx, float *out6);
  * fastest; 0 .. count-1 forces one, for tuning and for tests that check
  * every shape against the same answers. A forced shape that cannot take a
  * batch size makes the answer call fail with OPIR_EINVAL. Results are
- * identical for every shape. */
+ * identical for every shape. For the 29-bit limbs the shapes include a VALU
+ * form (u32 x u32 -> u64 multiply-accumulate) beside the int8 MFMA ones, so
+ * autotuning measures where the matrix cores pay off. */
 #define OPIR_SCAN_DEFAULT (-1)
 #define OPIR_SCAN_AUTOTUNE (-2)
 int opir_ctx_set_scan_variant(opir_ctx *ctx, int variant);
