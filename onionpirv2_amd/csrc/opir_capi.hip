@@ -177,7 +177,7 @@ int answer_device(opir_db *db, size_t B, const uint64_t *ids, const u64 *d_q, u6
     if (beside) fork_aux(ctx);
     const size_t sel_stride = nsel * 2 * (size_t)ctx->G.l_ep * kCtWords;
     launch_complete(leaves + db->N0 * kCtWords, db->sel, exp_stride, sel_stride, (int)(nsel * ctx->G.l_ep), (int)B, P,
-                    ctx->G, ctx->S, beside ? ctx->aux_stream : ctx->stream);
+                    ctx->G, ctx->S, beside ? ctx->aux_stream : ctx->stream, beside ? nullptr : db->part);
     LAUNCH_CHECK(ctx);
     rec(ctx, 2, beside);
     launch_ct_ntt(leaves, db->qntt, exp_stride, db->N0 * kCtWords, (int)db->N0, (int)B, ctx->S, ctx->stream);

@@ -112,8 +112,11 @@ void launch_bv_apply(const u32 *in, u32 *out, const u32 *const *key_ptr, int kin
 void launch_ct_ntt(const u32 *in, u32 *out, size_t in_stride, size_t out_stride, int count, int B,
                    const SchemeConst &S, hipStream_t st);
 // Selector completion: rows = nsel * l_ep expanded leaves (coef form) -> sel [nsel][2 l_ep][2][2][N].
+// part (kMuxSplitMaxBlocks cts, or null): scratch for spreading the rows of
+// a small batch over digit groups.
 void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
-                     const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st);
+                     const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st,
+                     u32 *part = nullptr);
 // First-dim scan in two launches: launch_qfrag turns the selectors qntt
 // [B][N0][2][2][N] into MFMA B fragments (qfrag_bytes(make_qfrag_geom(N0, B))
 // bytes), launch_scan multiplies the DB by them -> mid [B][Nrest][2][2][N]

@@ -421,6 +421,76 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
   store_map3(bot + (1 * 2 + limb) * kN, o1, t);
 }
 
+// Split completion for small batches, whose completion runs ahead of the
+// scan with its latency on the answer's path: workgroup (row, poly p, digit
+// group j) of two lane blocks (limbs) runs digits [j*dg, (j+1)*dg) of poly
+// p and writes its partial bottom row (both outputs, both limbs, reduced) to
+// part block qi * gridDim.x + blockIdx.x; the last group, which has the
+// fewest digits, also writes the top row NTT(poly p). k_complete_fin adds
+// the 2G partials of a row. Chains of ceil(l_key / G) + 1 transforms instead
+// of l_key + 1.
+template <bool MP>
+__global__ __launch_bounds__(512) void k_complete_part(CompleteArgs A, u32 *__restrict__ part, int G, SchemeConst S) {
+  __shared__ __align__(16) u32 nlds[2][kLdsPoly];
+  __shared__ u32 sh[2][kN];
+  const int limb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 8), t = threadIdx.x & 255;
+  const LimbConst &L = S.limb[limb];
+  const int j = blockIdx.x % G, p = (blockIdx.x / G) & 1, g = blockIdx.x / (2 * G), qi = blockIdx.y;
+  const int a = g / A.l_ep, r = g % A.l_ep;
+  const u32 *row = A.leaves + qi * A.leaf_stride + (size_t)g * kCtWords;
+  u32 v[8];
+  load_map0(v, row + (p * 2 + limb) * kN, t);
+  store_map0(sh[limb], v, t);
+  __syncthreads();
+  i64 D[8];
+  compose_centre8(sh[0], sh[1], t, S, D, !MP);
+  u64 acc0[8], acc1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) acc0[c] = acc1[c] = 0;
+  const int dg = (A.l_key + G - 1) / G, i0 = j * dg, i1 = min(A.l_key, i0 + dg);
+  digit_mac<true>(D, i0, i1, A.l_key, A.b_key, true, p * A.l_key, A.gsw_keys[qi], limb, t, L, nlds[limb], acc0, acc1,
+                  sel_delta<MP>(A.b_key));
+  if (j == G - 1) {  // top row: NTT of poly p, on the same scratch rotation
+    ntt_fwd_2048(v, nlds[limb], t, L, Barrier());
+    u32 *selb = A.sel + qi * A.sel_stride + (size_t)a * (2 * A.l_ep) * kCtWords;
+    store_map3(selb + (size_t)r * kCtWords + (p * 2 + limb) * kN, v, t);
+  }
+  u32 o0[8], o1[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    o0[c] = reduce_u64(acc0[c], L);
+    o1[c] = reduce_u64(acc1[c], L);
+  }
+  u32 *dst = part + ((size_t)qi * gridDim.x + blockIdx.x) * kCtWords;
+  store_map3(dst + (0 * 2 + limb) * kN, o0, t);
+  store_map3(dst + (1 * 2 + limb) * kN, o1, t);
+}
+
+// Bottom row r of a split completion: the sum of its 2G partials (blocks
+// [g * 2G, g * 2G + 2G) of the query). Eight workgroups per row, one 16-byte
+// word per thread, all partials in flight at once (G <= kSplitMaxG).
+__global__ __launch_bounds__(256) void k_complete_fin(CompleteArgs A, const u32 *__restrict__ part, int G, int nrows,
+                                                      SchemeConst S) {
+  const int g = blockIdx.x >> 3, qi = blockIdx.y;
+  const int e = (blockIdx.x & 7) * 256 + threadIdx.x;  // 16-byte word of the ciphertext, < kCtWords / 4
+  const int a = g / A.l_ep, r = g % A.l_ep;
+  const uint4 *pp =
+      reinterpret_cast<const uint4 *>(part + ((size_t)qi * nrows * 2 * G + (size_t)g * 2 * G) * kCtWords);
+  uint4 b[2 * kSplitMaxG];
+#pragma unroll
+  for (int k = 0; k < 2 * kSplitMaxG; k++)
+    if (k < 2 * G) b[k] = pp[(size_t)k * (kCtWords / 4) + e];
+  u64 s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+  for (int k = 0; k < 2 * kSplitMaxG; k++)
+    if (k < 2 * G) {
+      s0 += b[k].x; s1 += b[k].y; s2 += b[k].z; s3 += b[k].w;
+    }
+  const LimbConst &L = S.limb[((4 * e) / kN) & 1];  // polynomial index = output * 2 + limb
+  uint4 *dst = reinterpret_cast<uint4 *>(A.sel + qi * A.sel_stride + ((size_t)a * 2 * A.l_ep + A.l_ep + r) * kCtWords);
+  dst[e] = make_uint4(reduce_u64(s0, L), reduce_u64(s1, L), reduce_u64(s2, L), reduce_u64(s3, L));
+}
+
 // ---------------------------------------------------------------------------
 // MUX level (server.cpp:599-722) on NTT-form candidates:
 //   res = x + EP(sel, INTT(y - x))   (EP output in NTT form)
@@ -614,11 +684,19 @@ void launch_bv_apply(const u32 *in, u32 *out, const u32 *const *key_ptr, int kin
 }
 
 void launch_complete(const u32 *leaves, u32 *sel, size_t leaf_stride, size_t sel_stride, int nrows, int B,
-                     const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st) {
+                     const BatchPtrs &P, const GadgetCfg &G, const SchemeConst &S, hipStream_t st, u32 *part) {
   if (nrows <= 0) return;
   CompleteArgs A;
   A.leaves = leaves; A.sel = sel; A.leaf_stride = leaf_stride; A.sel_stride = sel_stride;
   A.gsw_keys = P.gsw_keys; A.l_ep = G.l_ep; A.l_key = G.l_key; A.b_key = G.b_key;
+  // small batches: digit groups of about a third of l_key each
+  const int Gs = std::min(kSplitMaxG, std::max(1, (G.l_key + 3) / 4));
+  if (part && (size_t)nrows * B < 256 && (size_t)nrows * 2 * Gs * B <= kMuxSplitMaxBlocks) {
+    if (S.mp) k_complete_part<true><<<dim3(nrows * 2 * Gs, B), 512, 0, st>>>(A, part, Gs, S);
+    else k_complete_part<false><<<dim3(nrows * 2 * Gs, B), 512, 0, st>>>(A, part, Gs, S);
+    k_complete_fin<<<dim3(nrows * 8, B), 256, 0, st>>>(A, part, Gs, nrows, S);
+    return;
+  }
   dim3 grid(nrows, B);
   if ((size_t)nrows * B >= 256)  // enough rows to fill the device twice over
     if (S.mp) k_complete_serial<true><<<grid, 512, 0, st>>>(A, S);
